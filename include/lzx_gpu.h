@@ -1,0 +1,227 @@
+/*
+ * lzx_gpu.h -- C ABI of the MI355X (gfx950) lattice-commitment hot path.
+ *
+ * Drop-in boundary for three Rust items of idrees2516/lzx (paths relative to
+ * the reference's crates/):
+ *   - the negacyclic NTT            lattice-ring/src/ntt.rs, ring.rs
+ *   - the Ajtai commitment A*s      lattice-commitment/src/ajtai.rs
+ *   - the Goldilocks sum-check      lattice-sumcheck/src/sumcheck.rs with the
+ *     prover and its Fiat-Shamir     lattice-core/src/transcript.rs, keccak.rs
+ *
+ * Conventions
+ *   - Every entry point returns lzx_status; values 1..29 map one-to-one onto
+ *     the reference's error enums (noted per code), 30.. are ABI/device codes.
+ *     Nothing unwinds across the ABI.
+ *   - Pointers named *_dev are device (HBM) pointers; *_host are host pointers.
+ *     `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Device entry points are stream-ordered and return before completion.
+ *   - Residues are canonical: u32 in [0, q), Goldilocks u64 in [0, p).
+ *   - Handles are immutable after creation and may be shared across host
+ *     threads; calls on one handle from several threads only read it.
+ */
+#ifndef LZX_GPU_H
+#define LZX_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t lzx_status;
+
+enum {
+    LZX_OK = 0,
+    LZX_NTT_LENGTH_MISMATCH = 1,        /* NttError::LengthMismatch        ntt.rs:40 */
+    LZX_NTT_ADICITY_TOO_SMALL = 2,      /* NttError::AdicityTooSmall       ntt.rs:41 */
+    LZX_RING_LENGTH_MISMATCH = 3,       /* RingError::LengthMismatch       ring.rs:20 */
+    LZX_AJTAI_DIMENSION_MISMATCH = 4,   /* AjtaiError::DimensionMismatch   ajtai.rs:45 */
+    LZX_AJTAI_NORM_EXCEEDED = 5,        /* AjtaiError::NormExceeded        ajtai.rs:46 */
+    LZX_AJTAI_COMMITMENT_MISMATCH = 6,  /* AjtaiError::CommitmentMismatch  ajtai.rs:47 */
+    LZX_SUMCHECK_BAD_ROUND_SHAPE = 10,  /* SumcheckError::BadRoundShape    sumcheck.rs:34 */
+    LZX_SUMCHECK_ROUND_CHECK_FAILED = 11, /* SumcheckError::RoundCheckFailed sumcheck.rs:39 */
+    LZX_SUMCHECK_FINAL_CHECK_FAILED = 12, /* SumcheckError::FinalCheckFailed sumcheck.rs:43 */
+    LZX_SUMCHECK_CLAIM_MISMATCH = 13,   /* SumcheckError::ClaimMismatch    sumcheck.rs:45 */
+    LZX_SUMCHECK_VIRTUAL_POLY = 14,     /* SumcheckError::VirtualPoly      sumcheck.rs:30 */
+    LZX_TRANSCRIPT_REJECTION_BUDGET = 20, /* TranscriptError::RejectionBudgetExceeded transcript.rs:16 */
+    LZX_TRANSCRIPT_MESSAGE_TOO_LARGE = 21, /* TranscriptError::MessageTooLarge transcript.rs:18 */
+    LZX_INVALID_ARGUMENT = 30,          /* null handle/pointer or malformed shape */
+    LZX_UNSUPPORTED = 31,               /* shape outside what the kernels implement */
+    LZX_DEVICE_ERROR = 100,             /* HIP launch/runtime failure */
+    LZX_DEVICE_OUT_OF_MEMORY = 101,
+    LZX_NO_DEVICE = 102
+};
+
+/* Human-readable name of a status code (static storage). */
+const char *lzx_status_string(lzx_status s);
+
+/* Text of the most recent HIP error seen by this thread ("" if none). */
+const char *lzx_last_device_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Negacyclic NTT over Z_q[X]/(X^n + 1), n = 2^log_n, q an odd prime < 2^32.
+ * Forward output is natural order: out[k] = sum_i a_i * psi^(i(2k+1)) with
+ * psi = generator^((q-1) >> (log_n+1)), exactly NttTables::forward.
+ * ------------------------------------------------------------------------- */
+typedef struct lzx_ntt_plan lzx_ntt_plan;
+
+/* Replaces NttTables::new(Modulus32::new(q, two_adicity, generator), log_n)
+ * (ntt.rs:46, modulus.rs:37).  LZX_NTT_ADICITY_TOO_SMALL when
+ * log_n + 1 > two_adicity.  Uploads the twiddle tables to the current device. */
+lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n,
+                               lzx_ntt_plan **out);
+void lzx_ntt_plan_destroy(lzx_ntt_plan *plan);
+
+/* psi and n^{-1} of the plan (NttTables fields, ntt.rs:52-58). */
+lzx_status lzx_ntt_plan_constants(const lzx_ntt_plan *plan, uint32_t *psi, uint32_t *n_inv);
+
+/* In-place batched transforms of len/n polynomials laid out [poly][n] in
+ * HBM.  Replace NttTables::forward/forward_fast (ntt.rs:153,118) and
+ * inverse/inverse_fast (ntt.rs:220,187) applied to each polynomial.
+ * LZX_NTT_LENGTH_MISMATCH when len is not a multiple of n. */
+lzx_status lzx_ntt_forward_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len, void *stream);
+lzx_status lzx_ntt_inverse_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len, void *stream);
+
+/* Incomplete NTT (ntt.rs:257 forward_partial, ntt.rs:291 complete_partial),
+ * batched.  LZX_NTT_LENGTH_MISMATCH when levels/already > log_n. */
+lzx_status lzx_ntt_forward_partial_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len, uint32_t levels,
+                                         void *stream);
+lzx_status lzx_ntt_complete_partial_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len,
+                                          uint32_t already, void *stream);
+
+/* CRT batch: nplans moduli of one degree in ONE launch.  data_dev is laid
+ * out [plan][poly][n] with polys_per_plan polynomials per modulus (the
+ * reference applies NttTables::forward per element per modulus). */
+lzx_status lzx_ntt_forward_multi(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data_dev,
+                                 size_t polys_per_plan, void *stream);
+lzx_status lzx_ntt_inverse_multi(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data_dev,
+                                 size_t polys_per_plan, void *stream);
+
+/* Synchronous host-buffer forms with the exact reference signatures
+ * forward(&self, &mut [u32]) / inverse(&self, &mut [u32]): len must equal n
+ * times the number of polynomials. */
+lzx_status lzx_ntt_forward_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t len);
+lzx_status lzx_ntt_inverse_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t len);
+
+/* Negacyclic ring products out = a * b for len/n pairs (RingElement::mul,
+ * ring.rs:309-340).  Device buffers, [poly][n]. */
+lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                              uint32_t *out_dev, size_t len, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Ajtai commitment t = A * s over R_q (lattice-commitment/src/ajtai.rs).
+ * The key keeps the NTT-domain matrix A^ resident in HBM ([i][j][n],
+ * row-major k x m) and, for the int8 path, the signed 8-bit limb planes of
+ * the centred coefficient matrix.
+ * ------------------------------------------------------------------------- */
+typedef struct lzx_ajtai_key lzx_ajtai_key;
+
+enum { LZX_COMMIT_AUTO = 0, LZX_COMMIT_NTT = 1, LZX_COMMIT_INT8 = 2 };
+
+/* Replaces AjtaiPublicKey::from_seed(AjtaiParams{ring, k, m, norm_bound},
+ * seed) (ajtai.rs:87-103): A[i*m+j] = uniform_from_seed("ajtai-A", seed,
+ * i*m+j) (ring.rs:102-180), A^ = forward NTT of every entry. */
+lzx_status lzx_ajtai_key_from_seed(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n, size_t k,
+                                   size_t m, uint32_t norm_bound, const uint8_t seed[32], lzx_ajtai_key **out);
+
+/* The same key from an explicit coefficient matrix A (host, [k*m][n]). */
+lzx_status lzx_ajtai_key_from_matrix(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n,
+                                     size_t k, size_t m, uint32_t norm_bound, const uint32_t *A_host,
+                                     lzx_ajtai_key **out);
+void lzx_ajtai_key_destroy(lzx_ajtai_key *key);
+
+/* Copy A (coefficient domain) or A^ (NTT domain) of the key to the host,
+ * [k*m][n] each (AjtaiPublicKey::entry, ajtai.rs:106-112). */
+lzx_status lzx_ajtai_key_matrix(const lzx_ajtai_key *key, uint32_t *A_host);
+lzx_status lzx_ajtai_key_matrix_ntt(const lzx_ajtai_key *key, uint32_t *Ahat_host);
+
+/* Batched commit of W witnesses (AjtaiPublicKey::commit, ajtai.rs:118-153):
+ * s_dev [W][m][n] canonical residues, out_dev [W][k][n].  s_len is the
+ * per-witness ring-element count and must equal m (DimensionMismatch). */
+lzx_status lzx_ajtai_commit_batch(const lzx_ajtai_key *key, const uint32_t *s_dev, size_t s_len, size_t W,
+                                  uint32_t *out_dev, void *stream);
+
+/* Batched commit of small-digit witnesses: digits_dev [W][m][n] int8 with
+ * |digit| <= 127 (coefficient c represents c mod q, ring.rs:232-243
+ * from_signed).  path selects the NTT-domain kernels, the int8 MFMA
+ * coefficient-domain kernels, or AUTO (per-shape choice).  Results are
+ * identical on every path. */
+lzx_status lzx_ajtai_commit_digits_batch(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t s_len,
+                                         size_t W, uint32_t *out_dev, int32_t path, void *stream);
+
+/* Synchronous host forms: commit(&self, &[RingElement]) (ajtai.rs:118) and
+ * verify_opening(&self, &AjtaiCommitment, &[RingElement]) (ajtai.rs:210):
+ * norm check (NormExceeded) then recommit and compare (CommitmentMismatch). */
+lzx_status lzx_ajtai_commit_host(const lzx_ajtai_key *key, const uint32_t *s_host, size_t s_len,
+                                 uint32_t *out_host);
+lzx_status lzx_ajtai_verify_opening_host(const lzx_ajtai_key *key, const uint32_t *t_host, const uint32_t *s_host,
+                                         size_t s_len);
+
+/* ---------------------------------------------------------------------------
+ * Fiat-Shamir transcript (lattice-core/src/transcript.rs), host side.
+ * ------------------------------------------------------------------------- */
+typedef struct lzx_transcript lzx_transcript;
+
+/* Transcript::new(protocol, budget) (transcript.rs:46). */
+lzx_status lzx_transcript_new(const uint8_t *protocol, size_t len, uint64_t query_budget, lzx_transcript **out);
+lzx_status lzx_transcript_clone(const lzx_transcript *t, lzx_transcript **out);
+void lzx_transcript_destroy(lzx_transcript *t);
+/* append_message (transcript.rs:67); MessageTooLarge above 2^26 bytes. */
+lzx_status lzx_transcript_append_message(lzx_transcript *t, const uint8_t *label, size_t label_len,
+                                         const uint8_t *msg, size_t msg_len);
+/* append_field_slice (transcript.rs:90). */
+lzx_status lzx_transcript_append_field_slice(lzx_transcript *t, const uint8_t *label, size_t label_len,
+                                             const uint64_t *values, size_t n);
+/* challenge_fields (transcript.rs:109); RejectionBudgetExceeded past the budget. */
+lzx_status lzx_transcript_challenge_fields(lzx_transcript *t, const uint8_t *label, size_t label_len, size_t n,
+                                           uint64_t *out);
+uint64_t lzx_transcript_query_count(const lzx_transcript *t);
+
+/* ---------------------------------------------------------------------------
+ * Sum-check over Goldilocks (lattice-sumcheck/src/sumcheck.rs).
+ * A virtual polynomial P = sum_t c_t * prod_{f in ids_t} F_f is given as
+ * nfactors device tables of 2^num_vars canonical u64 (variable 0 = most
+ * significant index bit, mle.rs:1-7) and terms in CSR form: term t has
+ * coefficient coeffs[t] and factor ids term_ids[term_off[t] .. term_off[t+1]).
+ * ------------------------------------------------------------------------- */
+
+/* sumcheck::prove(&vp, claim, &mut transcript) (sumcheck.rs:62-162).  The
+ * factor tables are read, never written (the reference clones them).
+ * Outputs (host): rounds_out [num_vars][degree+1], challenges_out
+ * [num_vars], *final_claim, factor_claims_out [nfactors], *degree_out.
+ * degree is the maximum term length, or 1 for the zero polynomial. */
+lzx_status lzx_sumcheck_prove(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev, size_t nterms,
+                              const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids,
+                              uint64_t claim, lzx_transcript *transcript, uint64_t *rounds_out,
+                              uint64_t *challenges_out, uint64_t *final_claim, uint64_t *factor_claims_out,
+                              size_t *degree_out, void *stream);
+
+/* Same prover with the challenges supplied by the caller instead of a
+ * transcript (kernel-level parity and timing; no Fiat-Shamir). */
+lzx_status lzx_sumcheck_prove_with_challenges(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev,
+                                              size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
+                                              const uint32_t *term_ids, uint64_t claim,
+                                              const uint64_t *challenges, uint64_t *rounds_out,
+                                              uint64_t *final_claim, uint64_t *factor_claims_out,
+                                              size_t *degree_out, void *stream);
+
+/* SumcheckProof::verify (sumcheck.rs:324-371), host only.  rounds is
+ * num_rounds rows of round_len values; expected_final is used when
+ * has_expected != 0.  Writes point_out [num_rounds] and *final_claim. */
+lzx_status lzx_sumcheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_len, size_t num_vars,
+                               size_t max_degree, uint64_t claim, lzx_transcript *transcript, int32_t has_expected,
+                               uint64_t expected_final, uint64_t *point_out, uint64_t *final_claim);
+
+/* Multilinear helpers on device tables (field_simd.rs:391 eq_table,
+ * virtual_poly.rs:107 sum_over_hypercube). */
+lzx_status lzx_eq_table(const uint64_t *point_host, size_t num_vars, uint64_t *out_dev, void *stream);
+lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev,
+                                     size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
+                                     const uint32_t *term_ids, uint64_t *sum_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LZX_GPU_H */

@@ -1,0 +1,220 @@
+// Modular arithmetic for the gfx950 kernels (u32 NTT primes and Goldilocks).
+//
+// Every routine returns the canonical residue, so results are bit-identical
+// to the reference's Barrett arithmetic (lattice-ring/src/modulus.rs:65-112)
+// and Goldilocks field (lattice-core/src/field.rs:28-105): canonical modular
+// values do not depend on the reduction method.
+//
+// Two constant-multiplication schemes are used for the u32 primes:
+//   - q < 2^31: Shoup.  w' = floor(w * 2^32 / q); r = a*w - mulhi(a, w')*q is
+//     in [0, 2q) for every 32-bit a, and 2q < 2^32, so one conditional
+//     subtraction gives the canonical value (3 multiplies).
+//   - q > 2^31 (e.g. 3*2^30+1): Montgomery with R = 2^32 and the constant in
+//     Montgomery form w*R mod q.  (a*wR - m*q)/R = hi(a*wR) - hi(m*q) lies in
+//     (-q, q); one conditional add gives the canonical value (4 multiplies).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LZX_HD __host__ __device__ __forceinline__
+
+namespace lzx {
+
+LZX_HD uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+LZX_HD uint64_t mulhi64(uint64_t a, uint64_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// Constants of one u32 prime as the kernels see them.
+struct Mod32 {
+    uint32_t q;
+    uint32_t qinv;   // q^{-1} mod 2^32 (Montgomery)
+    uint64_t barrett; // floor((2^64-1)/q), modulus.rs:15-17
+};
+
+// ---- canonical add / sub ---------------------------------------------------
+
+// q < 2^31: a + b < 2^32.  min(s, s-q) selects s-q exactly when s >= q.
+LZX_HD uint32_t add_small(uint32_t a, uint32_t b, uint32_t q)
+{
+    uint32_t s = a + b;
+    uint32_t t = s - q;
+    return t < s ? t : s;
+}
+
+LZX_HD uint32_t sub_small(uint32_t a, uint32_t b, uint32_t q)
+{
+    uint32_t d = a - b;
+    uint32_t e = d + q;
+    return e < d ? e : d;
+}
+
+// any q < 2^32.
+LZX_HD uint32_t add_big(uint32_t a, uint32_t b, uint32_t q)
+{
+    uint64_t s = (uint64_t)a + b;
+    return s >= q ? (uint32_t)(s - q) : (uint32_t)s;
+}
+
+LZX_HD uint32_t sub_big(uint32_t a, uint32_t b, uint32_t q)
+{
+    return a >= b ? a - b : a + (q - b);
+}
+
+template <bool BIGQ>
+LZX_HD uint32_t addm(uint32_t a, uint32_t b, uint32_t q)
+{
+    return BIGQ ? add_big(a, b, q) : add_small(a, b, q);
+}
+
+template <bool BIGQ>
+LZX_HD uint32_t subm(uint32_t a, uint32_t b, uint32_t q)
+{
+    return BIGQ ? sub_big(a, b, q) : sub_small(a, b, q);
+}
+
+// ---- multiplication by a precomputed constant -----------------------------
+
+LZX_HD uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t wp, uint32_t q)
+{
+    uint32_t qh = mulhi32(a, wp);
+    uint32_t r = a * w - qh * q; // in [0, 2q)
+    uint32_t t = r - q;
+    return t < r ? t : r;
+}
+
+// a * b * 2^{-32} mod q, canonical.  Requires min(a, b) < q.
+LZX_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t q, uint32_t qinv)
+{
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * qinv;
+    uint32_t mh = mulhi32(m, q);
+    uint32_t th = (uint32_t)(t >> 32);
+    uint32_t r = th - mh;
+    return th < mh ? r + q : r;
+}
+
+// a * w mod q with (w, wp) from the plan's constant tables: Shoup pair for
+// q < 2^31, (w*R mod q, unused) for q > 2^31.
+template <bool BIGQ>
+LZX_HD uint32_t mulc(uint32_t a, uint32_t w, uint32_t wp, uint32_t q, uint32_t qinv)
+{
+    if (BIGQ) {
+        (void)wp;
+        return mont_mul(a, w, q, qinv);
+    } else {
+        (void)qinv;
+        return shoup_mul(a, w, wp, q);
+    }
+}
+
+// Barrett reduction of any u64 (modulus.rs:65-76 restated with a 64-bit high
+// product): qhat in {floor(x/q)-1, floor(x/q)}, so <= 2 corrections.
+LZX_HD uint32_t reduce_u64(uint64_t x, uint32_t q, uint64_t barrett)
+{
+    uint64_t qhat = mulhi64(x, barrett);
+    uint64_t r = x - qhat * q;
+    if (r >= q) r -= q;
+    if (r >= q) r -= q;
+    return (uint32_t)r;
+}
+
+// ---- host-side constant helpers -------------------------------------------
+
+inline uint32_t host_pow(uint32_t b, uint64_t e, uint32_t q)
+{
+    uint64_t acc = 1 % q, base = b % q;
+    while (e) {
+        if (e & 1) acc = acc * base % q;
+        base = base * base % q;
+        e >>= 1;
+    }
+    return (uint32_t)acc;
+}
+
+inline uint32_t host_inv32(uint32_t q) // q^{-1} mod 2^32 by Newton iteration
+{
+    uint32_t x = q; // correct to 3 bits for odd q
+    for (int i = 0; i < 5; i++) x *= 2u - q * x;
+    return x;
+}
+
+inline uint32_t shoup_companion(uint32_t w, uint32_t q) { return (uint32_t)(((uint64_t)w << 32) / q); }
+
+inline uint32_t to_mont(uint32_t w, uint32_t q) { return (uint32_t)(((uint64_t)w << 32) % q); }
+
+// ---- Goldilocks p = 2^64 - 2^32 + 1 ---------------------------------------
+
+constexpr uint64_t GL_P = 0xFFFFFFFF00000001ULL;
+constexpr uint64_t GL_EPS = 0xFFFFFFFFULL; // 2^64 mod p
+
+LZX_HD uint64_t gl_canon(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
+
+LZX_HD uint64_t gl_add(uint64_t a, uint64_t b) // field.rs:70-77
+{
+    uint64_t s = a + b;
+    if (s < a) s += GL_EPS; // a wrapped 2^64 is worth 2^32 - 1
+    return gl_canon(s);
+}
+
+LZX_HD uint64_t gl_sub(uint64_t a, uint64_t b) // field.rs:82-89
+{
+    uint64_t d = a - b;
+    if (a < b) d -= GL_EPS;
+    return gl_canon(d);
+}
+
+// Reduce hi*2^64 + lo with 2^64 = 2^32 - 1 and 2^96 = -1 (mod p).
+LZX_HD uint64_t gl_reduce128(uint64_t hi, uint64_t lo)
+{
+    uint64_t hh = hi >> 32, hl = hi & GL_EPS;
+    uint64_t t0 = lo - hh;
+    if (lo < hh) t0 -= GL_EPS; // borrowed 2^64 = p + eps
+    uint64_t t1 = (hl << 32) - hl; // hl * (2^32 - 1)
+    uint64_t r = t0 + t1;
+    if (r < t1) r += GL_EPS;
+    return gl_canon(r);
+}
+
+LZX_HD uint64_t gl_mul(uint64_t a, uint64_t b)
+{
+    return gl_reduce128(mulhi64(a, b), a * b);
+}
+
+// Lazy sum: true value = acc + carries * 2^64.
+struct GlAcc {
+    uint64_t acc;
+    uint32_t carries;
+};
+
+LZX_HD void gl_acc_add(GlAcc &s, uint64_t v)
+{
+    uint64_t n = s.acc + v;
+    s.carries += n < v;
+    s.acc = n;
+}
+
+LZX_HD uint64_t gl_acc_finish(const GlAcc &s)
+{
+    // carries * 2^64 + acc = carries*(2^32-1) + acc (mod p); fits 128 bits.
+    uint64_t c = (uint64_t)s.carries * GL_EPS;
+    uint64_t r = s.acc + c;
+    uint64_t hi = r < c ? 1 : 0;
+    return gl_reduce128(hi, r);
+}
+
+} // namespace lzx

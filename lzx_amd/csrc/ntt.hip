@@ -1,0 +1,549 @@
+// Batched negacyclic NTT over Z_q[X]/(X^n+1) for gfx950.
+//
+// Reference semantics (crates/lattice-ring/src/ntt.rs):
+//   forward  (ntt.rs:118-182): out[k] = sum_i a_i psi^(i(2k+1)), natural order,
+//            psi = g^((q-1) >> (log_n+1)) (modulus.rs:149-161);
+//   inverse  (ntt.rs:187-250): the exact inverse;
+//   forward_partial / complete_partial (ntt.rs:257-316): the reference's own
+//            level structure (prescale by psi^i, bit-reverse, CT levels with
+//            omega = psi^2), whose intermediate state is part of the contract.
+//
+// Kernels
+//   ntt_reg_kernel  n in [4, 64]: one polynomial per lane, all log n levels in
+//       VGPRs (no cross-lane traffic), twiddles wave-uniform (scalar loads).
+//       A 64-polynomial tile is staged through LDS (padded row stride n+1, so
+//       the per-lane row reads are bank-conflict free) so HBM sees 16-byte
+//       coalesced loads/stores.  psi is folded into the twiddles; the forward
+//       permutes to natural order on the way back to LDS and the inverse folds
+//       n^{-1} into its last level.
+//   ntt_lds_kernel  any n up to 2^13 and the incomplete transforms: a tile of
+//       polynomials in LDS, one butterfly per thread per level.
+#include <algorithm>
+#include <cstring>
+
+#include "ntt.hpp"
+
+namespace lzx {
+
+__host__ __device__ constexpr uint32_t brv_bits(uint32_t x, uint32_t bits)
+{
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < bits; i++) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t brv_dev(uint32_t x, uint32_t bits)
+{
+    return bits == 0 ? 0 : (__brev(x) >> (32 - bits));
+}
+
+template <int LOG_N, bool BIGQ, int MODE>
+__global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
+                                                     uint32_t blocks_per_plan)
+{
+    constexpr uint32_t N = 1u << LOG_N;
+    constexpr uint32_t S = N + 1;
+    __shared__ uint32_t tile[64 * S];
+
+    const uint32_t plan = blockIdx.x / blocks_per_plan;
+    const uint32_t blk = blockIdx.x - plan * blocks_per_plan;
+    const uint32_t *__restrict__ tab = tabs.p[plan];
+    const uint32_t p0 = blk * 64;
+    const uint32_t np = min(64u, polys - p0);
+    const uint32_t tid = threadIdx.x;
+    uint32_t *__restrict__ g = data + ((size_t)plan * polys + p0) * N;
+    const uint32_t nvec = np * N / 4;
+
+    {
+        const uint4 *__restrict__ g4 = reinterpret_cast<const uint4 *>(g);
+        for (uint32_t i = tid; i < nvec; i += 64) {
+            const uint4 v = g4[i];
+            const uint32_t w = 4 * i;
+            uint32_t *t = tile + (w / N) * S + (w % N);
+            t[0] = v.x;
+            t[1] = v.y;
+            t[2] = v.z;
+            t[3] = v.w;
+        }
+    }
+    __syncthreads();
+
+    const uint32_t q = tab[0], qi = tab[1];
+    uint32_t a[N];
+    uint32_t *row = tile + tid * S;
+    if (MODE == NTT_FWD) {
+#pragma unroll
+        for (uint32_t c = 0; c < N; c++) a[c] = row[c];
+#pragma unroll
+        for (uint32_t len = N / 2; len >= 1; len >>= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                const uint32_t w = tab[TabLayout::fwd_w(N) + k];
+                const uint32_t wp = tab[TabLayout::fwd_wp(N) + k];
+#pragma unroll
+                for (uint32_t j = s; j < s + len; j++) {
+                    const uint32_t t = mulc<BIGQ>(a[j + len], w, wp, q, qi);
+                    const uint32_t u = a[j];
+                    a[j] = addm<BIGQ>(u, t, q);
+                    a[j + len] = subm<BIGQ>(u, t, q);
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < N; k++) row[k] = a[brv_bits(k, LOG_N)];
+    } else {
+#pragma unroll
+        for (uint32_t c = 0; c < N; c++) a[brv_bits(c, LOG_N)] = row[c];
+#pragma unroll
+        for (uint32_t len = 1; len <= N / 2; len <<= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                if (len == N / 2) {
+                    const uint32_t nw = tab[8], nwp = tab[9], lw = tab[10], lwp = tab[11];
+#pragma unroll
+                    for (uint32_t j = s; j < s + len; j++) {
+                        const uint32_t u = a[j], v = a[j + len];
+                        a[j] = mulc<BIGQ>(addm<BIGQ>(u, v, q), nw, nwp, q, qi);
+                        a[j + len] = mulc<BIGQ>(subm<BIGQ>(u, v, q), lw, lwp, q, qi);
+                    }
+                } else {
+                    const uint32_t w = tab[TabLayout::inv_w(N) + k];
+                    const uint32_t wp = tab[TabLayout::inv_wp(N) + k];
+#pragma unroll
+                    for (uint32_t j = s; j < s + len; j++) {
+                        const uint32_t u = a[j], v = a[j + len];
+                        a[j] = addm<BIGQ>(u, v, q);
+                        a[j + len] = mulc<BIGQ>(subm<BIGQ>(u, v, q), w, wp, q, qi);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < N; c++) row[c] = a[c];
+    }
+    __syncthreads();
+
+    {
+        uint4 *__restrict__ g4 = reinterpret_cast<uint4 *>(g);
+        for (uint32_t i = tid; i < nvec; i += 64) {
+            const uint32_t w = 4 * i;
+            const uint32_t *t = tile + (w / N) * S + (w % N);
+            g4[i] = make_uint4(t[0], t[1], t[2], t[3]);
+        }
+    }
+}
+
+template <bool BIGQ, int MODE>
+__global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
+                                                      uint32_t blocks_per_plan, uint32_t log_n, uint32_t ppb,
+                                                      uint32_t levels)
+{
+    extern __shared__ uint32_t tile[];
+    const uint32_t n = 1u << log_n;
+    const uint32_t half = n >> 1;
+    const uint32_t plan = blockIdx.x / blocks_per_plan;
+    const uint32_t blk = blockIdx.x - plan * blocks_per_plan;
+    const uint32_t *__restrict__ tab = tabs.p[plan];
+    const uint32_t p0 = blk * ppb;
+    const uint32_t np = min(ppb, polys - p0);
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    uint32_t *__restrict__ g = data + ((size_t)plan * polys + p0) * n;
+    const uint32_t nwords = np * n;
+    const uint32_t q = tab[0], qi = tab[1];
+
+    for (uint32_t i = tid; i < nwords; i += nt) {
+        const uint32_t p = i >> log_n, c = i & (n - 1);
+        const uint32_t x = g[i];
+        if (MODE == NTT_INV) {
+            tile[p * n + brv_dev(c, log_n)] = x;
+        } else if (MODE == NTT_PARTIAL) {
+            // ntt.rs:262-265: pre-scale by psi^c, then bit-reverse
+            tile[p * n + brv_dev(c, log_n)] =
+                mulc<BIGQ>(x, tab[TabLayout::psi_w(n) + c], tab[TabLayout::psi_wp(n) + c], q, qi);
+        } else {
+            tile[p * n + c] = x;
+        }
+    }
+    __syncthreads();
+
+    const uint32_t nbf = np * half;
+    if (MODE == NTT_FWD) {
+        for (uint32_t len = half; len >= 1; len >>= 1) {
+            for (uint32_t b = tid; b < nbf; b += nt) {
+                const uint32_t p = b >> (log_n - 1), bi = b & (half - 1);
+                const uint32_t grp = bi / len, j = bi % len;
+                const uint32_t k = n / (2 * len) + grp;
+                const uint32_t i0 = p * n + grp * 2 * len + j, i1 = i0 + len;
+                const uint32_t t = mulc<BIGQ>(tile[i1], tab[TabLayout::fwd_w(n) + k],
+                                              tab[TabLayout::fwd_wp(n) + k], q, qi);
+                const uint32_t u = tile[i0];
+                tile[i0] = addm<BIGQ>(u, t, q);
+                tile[i1] = subm<BIGQ>(u, t, q);
+            }
+            __syncthreads();
+        }
+    } else if (MODE == NTT_INV) {
+        for (uint32_t len = 1; len <= half; len <<= 1) {
+            for (uint32_t b = tid; b < nbf; b += nt) {
+                const uint32_t p = b >> (log_n - 1), bi = b & (half - 1);
+                const uint32_t grp = bi / len, j = bi % len;
+                const uint32_t k = n / (2 * len) + grp;
+                const uint32_t i0 = p * n + grp * 2 * len + j, i1 = i0 + len;
+                const uint32_t u = tile[i0], v = tile[i1];
+                if (len == half) {
+                    tile[i0] = mulc<BIGQ>(addm<BIGQ>(u, v, q), tab[8], tab[9], q, qi);
+                    tile[i1] = mulc<BIGQ>(subm<BIGQ>(u, v, q), tab[10], tab[11], q, qi);
+                } else {
+                    tile[i0] = addm<BIGQ>(u, v, q);
+                    tile[i1] = mulc<BIGQ>(subm<BIGQ>(u, v, q), tab[TabLayout::inv_w(n) + k],
+                                          tab[TabLayout::inv_wp(n) + k], q, qi);
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+        // ntt.rs:267-284 (partial: len = 2 .. 2^levels) and ntt.rs:299-314
+        // (complete: len = 2^(levels+1) .. n), twiddle omega^((n/len) j).
+        uint32_t len = MODE == NTT_PARTIAL ? 2u : (1u << (levels + 1));
+        const uint32_t max_levels = MODE == NTT_PARTIAL ? levels : 64u;
+        for (uint32_t applied = 0; len <= n && applied < max_levels; len <<= 1, applied++) {
+            const uint32_t hl = len >> 1, stride = n / len;
+            for (uint32_t b = tid; b < nbf; b += nt) {
+                const uint32_t p = b >> (log_n - 1), bi = b & (half - 1);
+                const uint32_t grp = bi / hl, j = bi % hl;
+                const uint32_t i0 = p * n + grp * len + j, i1 = i0 + hl;
+                const uint32_t e = stride * j;
+                const uint32_t u = tile[i0];
+                const uint32_t v = mulc<BIGQ>(tile[i1], tab[TabLayout::omg_w(n) + e], tab[TabLayout::omg_wp(n) + e],
+                                              q, qi);
+                tile[i0] = addm<BIGQ>(u, v, q);
+                tile[i1] = subm<BIGQ>(u, v, q);
+            }
+            __syncthreads();
+        }
+    }
+
+    for (uint32_t i = tid; i < nwords; i += nt) {
+        const uint32_t p = i >> log_n, c = i & (n - 1);
+        g[i] = MODE == NTT_FWD ? tile[p * n + brv_dev(c, log_n)] : tile[p * n + c];
+    }
+}
+
+// out[i] = out[i] * b[i] mod q (general residues), RingElement::mul's
+// pointwise step (ring.rs:329-331).
+__global__ void pointwise_mul_kernel(uint32_t *__restrict__ out, const uint32_t *__restrict__ b, size_t len,
+                                     uint32_t q, uint64_t barrett)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = reduce_u64((uint64_t)out[i] * b[i], q, barrett);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+template <int LOG_N, bool BIGQ>
+static void launch_reg(NttMode mode, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys, uint32_t bpp,
+                       hipStream_t stream)
+{
+    if (mode == NTT_FWD)
+        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, BIGQ, NTT_FWD>), grid, dim3(64), 0, stream, data, ts, polys, bpp);
+    else
+        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, BIGQ, NTT_INV>), grid, dim3(64), 0, stream, data, ts, polys, bpp);
+}
+
+template <bool BIGQ>
+static void dispatch_reg(uint32_t log_n, NttMode mode, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys,
+                         uint32_t bpp, hipStream_t stream)
+{
+    switch (log_n) {
+    case 2: launch_reg<2, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 3: launch_reg<3, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 4: launch_reg<4, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 5: launch_reg<5, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
+    default: launch_reg<6, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
+    }
+}
+
+template <bool BIGQ>
+static void dispatch_lds(NttMode mode, dim3 grid, size_t shmem, uint32_t *data, const TabSet &ts, uint32_t polys,
+                         uint32_t bpp, uint32_t log_n, uint32_t ppb, uint32_t levels, hipStream_t stream)
+{
+    switch (mode) {
+    case NTT_FWD:
+        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_FWD>), grid, dim3(256), shmem, stream, data, ts, polys, bpp,
+                           log_n, ppb, levels);
+        break;
+    case NTT_INV:
+        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_INV>), grid, dim3(256), shmem, stream, data, ts, polys, bpp,
+                           log_n, ppb, levels);
+        break;
+    case NTT_PARTIAL:
+        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_PARTIAL>), grid, dim3(256), shmem, stream, data, ts, polys,
+                           bpp, log_n, ppb, levels);
+        break;
+    default:
+        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_COMPLETE>), grid, dim3(256), shmem, stream, data, ts, polys,
+                           bpp, log_n, ppb, levels);
+        break;
+    }
+}
+
+lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data, size_t polys, NttMode mode,
+                      uint32_t levels, hipStream_t stream)
+{
+    if (nplans == 0 || polys == 0) return LZX_OK;
+    if (!data) return LZX_INVALID_ARGUMENT;
+    const uint32_t log_n = plans[0]->log_n, n = plans[0]->n;
+    if (polys > 0xFFFFFFFFull / 2) return LZX_UNSUPPORTED;
+    const bool aligned = ((uintptr_t)data % 16) == 0;
+    size_t i = 0;
+    while (i < nplans) {
+        // a launch covers consecutive plans of one arithmetic class
+        const bool bigq = plans[i]->bigq;
+        size_t cnt = 0;
+        TabSet ts{};
+        while (i + cnt < nplans && cnt < (size_t)MAX_PLANS_PER_LAUNCH && plans[i + cnt]->bigq == bigq) {
+            if (plans[i + cnt]->n != n) return LZX_INVALID_ARGUMENT;
+            ts.p[cnt] = plans[i + cnt]->dev_tab;
+            cnt++;
+        }
+        uint32_t *chunk = data + i * polys * n;
+        const bool use_reg = (mode == NTT_FWD || mode == NTT_INV) && log_n >= 2 && log_n <= 6 && aligned;
+        if (use_reg) {
+            const uint32_t bpp = (uint32_t)((polys + 63) / 64);
+            const dim3 grid((uint32_t)(bpp * cnt));
+            if (bigq)
+                dispatch_reg<true>(log_n, mode, grid, chunk, ts, (uint32_t)polys, bpp, stream);
+            else
+                dispatch_reg<false>(log_n, mode, grid, chunk, ts, (uint32_t)polys, bpp, stream);
+        } else {
+            const uint32_t ppb = std::max<uint32_t>(1u, 2048u >> log_n);
+            const uint32_t bpp = (uint32_t)((polys + ppb - 1) / ppb);
+            const dim3 grid((uint32_t)(bpp * cnt));
+            const size_t shmem = (size_t)ppb * n * sizeof(uint32_t);
+            if (bigq)
+                dispatch_lds<true>(mode, grid, shmem, chunk, ts, (uint32_t)polys, bpp, log_n, ppb, levels, stream);
+            else
+                dispatch_lds<false>(mode, grid, shmem, chunk, ts, (uint32_t)polys, bpp, log_n, ppb, levels, stream);
+        }
+        LZX_LAUNCH_CHECK("ntt kernel launch");
+        i += cnt;
+    }
+    return LZX_OK;
+}
+
+// Host table construction restating NttTables::new (ntt.rs:46-106).
+static void build_tables(lzx_ntt_plan *p)
+{
+    const uint32_t q = p->q, n = p->n, log_n = p->log_n;
+    std::vector<uint32_t> &t = p->host_tab;
+    t.assign(TabLayout::words(n), 0);
+    auto put = [&](uint32_t off_w, uint32_t off_wp, uint32_t idx, uint32_t w) {
+        if (p->bigq) {
+            t[off_w + idx] = to_mont(w, q);
+            t[off_wp + idx] = 0;
+        } else {
+            t[off_w + idx] = w;
+            t[off_wp + idx] = shoup_companion(w, q);
+        }
+    };
+    t[0] = q;
+    t[1] = p->mod.qinv;
+    t[2] = n;
+    t[3] = log_n;
+    t[4] = p->bigq ? 1 : 0;
+    put(8, 9, 0, p->n_inv);
+    const uint32_t z1inv = host_pow(p->psi_inv, brv_bits(1, log_n), q);
+    put(10, 11, 0, (uint32_t)((uint64_t)z1inv * p->n_inv % q));
+    for (uint32_t k = 1; k < n; k++) {
+        const uint32_t e = brv_bits(k, log_n);
+        put(TabLayout::fwd_w(n), TabLayout::fwd_wp(n), k, host_pow(p->psi, e, q));
+        put(TabLayout::inv_w(n), TabLayout::inv_wp(n), k, host_pow(p->psi_inv, e, q));
+    }
+    uint64_t pw = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        put(TabLayout::psi_w(n), TabLayout::psi_wp(n), i, (uint32_t)pw);
+        pw = pw * p->psi % q;
+    }
+    const uint32_t omega = (uint32_t)((uint64_t)p->psi * p->psi % q);
+    uint64_t ow = 1;
+    for (uint32_t e = 0; e < (n > 1 ? n / 2 : 1); e++) {
+        put(TabLayout::omg_w(n), TabLayout::omg_wp(n), e, (uint32_t)ow);
+        ow = ow * omega % q;
+    }
+}
+
+static lzx_status run_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t len, NttMode mode, uint32_t levels)
+{
+    if (!plan || (!a_host && len)) return LZX_INVALID_ARGUMENT;
+    if (len % plan->n) return LZX_NTT_LENGTH_MISMATCH;
+    if (len == 0) return LZX_OK;
+    LZX_HIP(hipSetDevice(plan->device));
+    uint32_t *d = nullptr;
+    LZX_HIP(hipMalloc(&d, len * sizeof(uint32_t)));
+    lzx_status st = LZX_OK;
+    hipError_t e = hipMemcpy(d, a_host, len * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = device_fail(e, "hipMemcpy H2D");
+    if (st == LZX_OK) st = ntt_launch(&plan, 1, d, len / plan->n, mode, levels, nullptr);
+    if (st == LZX_OK) {
+        e = hipMemcpy(a_host, d, len * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = device_fail(e, "hipMemcpy D2H");
+    }
+    hipFree(d);
+    return st;
+}
+
+} // namespace lzx
+
+using namespace lzx;
+
+extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n,
+                                          lzx_ntt_plan **out)
+{
+    if (!out) return LZX_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (log_n + 1 > two_adicity) return LZX_NTT_ADICITY_TOO_SMALL; // ntt.rs:47-50
+    if (q < 3 || (q & 1) == 0) return LZX_INVALID_ARGUMENT;
+    if (log_n > NTT_MAX_LOG_N) return LZX_UNSUPPORTED;
+    lzx_ntt_plan *p = new lzx_ntt_plan();
+    p->q = q;
+    p->two_adicity = two_adicity;
+    p->generator = generator;
+    p->log_n = log_n;
+    p->n = 1u << log_n;
+    p->bigq = q >= (1u << 31);
+    p->mod.q = q;
+    p->mod.qinv = host_inv32(q);
+    p->mod.barrett = UINT64_MAX / q;
+    // modulus.rs:149-161 and ntt.rs:52-58
+    p->psi = host_pow(generator, ((uint64_t)q - 1) >> (log_n + 1), q);
+    p->psi_inv = host_pow(p->psi, (uint64_t)q - 2, q);
+    p->n_inv = host_pow(p->n, (uint64_t)q - 2, q);
+    build_tables(p);
+    p->dev_tab = nullptr;
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipMalloc(&p->dev_tab, p->host_tab.size() * sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = hipMemcpy(p->dev_tab, p->host_tab.data(), p->host_tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        lzx_status st = device_fail(e, "ntt plan upload");
+        if (p->dev_tab) hipFree(p->dev_tab);
+        delete p;
+        return st;
+    }
+    *out = p;
+    return LZX_OK;
+}
+
+extern "C" void lzx_ntt_plan_destroy(lzx_ntt_plan *plan)
+{
+    if (!plan) return;
+    if (plan->dev_tab) hipFree(plan->dev_tab);
+    delete plan;
+}
+
+extern "C" lzx_status lzx_ntt_plan_constants(const lzx_ntt_plan *plan, uint32_t *psi, uint32_t *n_inv)
+{
+    if (!plan) return LZX_INVALID_ARGUMENT;
+    if (psi) *psi = plan->psi;
+    if (n_inv) *n_inv = plan->n_inv;
+    return LZX_OK;
+}
+
+static lzx_status batch(const lzx_ntt_plan *plan, uint32_t *data, size_t len, NttMode mode, uint32_t levels,
+                        void *stream)
+{
+    if (!plan) return LZX_INVALID_ARGUMENT;
+    if (len % plan->n) return LZX_NTT_LENGTH_MISMATCH;
+    return ntt_launch(&plan, 1, data, len / plan->n, mode, levels, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_ntt_forward_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len, void *stream)
+{
+    return batch(plan, data_dev, len, NTT_FWD, 0, stream);
+}
+
+extern "C" lzx_status lzx_ntt_inverse_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len, void *stream)
+{
+    return batch(plan, data_dev, len, NTT_INV, 0, stream);
+}
+
+extern "C" lzx_status lzx_ntt_forward_partial_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len,
+                                                    uint32_t levels, void *stream)
+{
+    if (plan && levels > plan->log_n) return LZX_NTT_LENGTH_MISMATCH; // ntt.rs:258
+    return batch(plan, data_dev, len, NTT_PARTIAL, levels, stream);
+}
+
+extern "C" lzx_status lzx_ntt_complete_partial_batch(const lzx_ntt_plan *plan, uint32_t *data_dev, size_t len,
+                                                     uint32_t already, void *stream)
+{
+    if (plan && already > plan->log_n) return LZX_NTT_LENGTH_MISMATCH; // ntt.rs:292
+    return batch(plan, data_dev, len, NTT_COMPLETE, already, stream);
+}
+
+static lzx_status multi(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data, size_t polys, NttMode mode,
+                        void *stream)
+{
+    if (!plans) return LZX_INVALID_ARGUMENT;
+    for (size_t i = 0; i < nplans; i++)
+        if (!plans[i] || plans[i]->n != plans[0]->n) return LZX_INVALID_ARGUMENT;
+    return ntt_launch(plans, nplans, data, polys, mode, 0, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_ntt_forward_multi(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data_dev,
+                                            size_t polys_per_plan, void *stream)
+{
+    return multi(plans, nplans, data_dev, polys_per_plan, NTT_FWD, stream);
+}
+
+extern "C" lzx_status lzx_ntt_inverse_multi(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data_dev,
+                                            size_t polys_per_plan, void *stream)
+{
+    return multi(plans, nplans, data_dev, polys_per_plan, NTT_INV, stream);
+}
+
+extern "C" lzx_status lzx_ntt_forward_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t len)
+{
+    return run_host(plan, a_host, len, NTT_FWD, 0);
+}
+
+extern "C" lzx_status lzx_ntt_inverse_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t len)
+{
+    return run_host(plan, a_host, len, NTT_INV, 0);
+}
+
+extern "C" lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                                         uint32_t *out_dev, size_t len, void *stream)
+{
+    if (!plan || !a_dev || !b_dev || !out_dev) return LZX_INVALID_ARGUMENT;
+    if (len % plan->n) return LZX_RING_LENGTH_MISMATCH;
+    if (len == 0) return LZX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t *tmp = nullptr;
+    LZX_HIP(hipMallocAsync((void **)&tmp, len * sizeof(uint32_t), s));
+    lzx_status st = LZX_OK;
+    hipError_t e = hipMemcpyAsync(tmp, b_dev, len * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && out_dev != a_dev)
+        e = hipMemcpyAsync(out_dev, a_dev, len * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) st = device_fail(e, "ring mul staging copy");
+    const size_t polys = len / plan->n;
+    if (st == LZX_OK) st = ntt_launch(&plan, 1, out_dev, polys, NTT_FWD, 0, s);
+    if (st == LZX_OK) st = ntt_launch(&plan, 1, tmp, polys, NTT_FWD, 0, s);
+    if (st == LZX_OK) {
+        const uint32_t blocks = (uint32_t)std::min<size_t>((len + 255) / 256, 65536);
+        hipLaunchKernelGGL(pointwise_mul_kernel, dim3(blocks), dim3(256), 0, s, out_dev, tmp, len, plan->q,
+                           plan->mod.barrett);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) st = device_fail(le, "pointwise_mul_kernel");
+    }
+    if (st == LZX_OK) st = ntt_launch(&plan, 1, out_dev, polys, NTT_INV, 0, s);
+    hipFreeAsync(tmp, s);
+    return st;
+}

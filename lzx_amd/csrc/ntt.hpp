@@ -1,0 +1,68 @@
+// Negacyclic NTT plans and launch helpers shared by the NTT, ring and Ajtai
+// modules.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "modarith.hpp"
+#include "runtime.hpp"
+
+namespace lzx {
+
+// Layout of one plan's constant table in HBM (u32 words).  The forward and
+// inverse kernels use twiddles with psi folded in (Cooley-Tukey from natural
+// order, Gentleman-Sande back), and permute to the reference's natural-order
+// evaluations on store.  The ref* tables restate NttTables' pow_psi and
+// pow_omega (ntt.rs:60-79) for the incomplete transforms, which must follow
+// the reference's own level structure.
+//   [0] q  [1] qinv  [2] n  [3] log_n  [4] bigq
+//   [8] n^{-1} (w, w')   [10] z_1^{-1} n^{-1} (w, w')
+//   fwd zeta  z_k = psi^brv(k), k in [1, n):   FWD_W + k, FWD_WP + k
+//   inv zeta  z_k^{-1}:                         INV_W + k, INV_WP + k
+//   pow_psi[i], i < n:                          PSI_W + i, PSI_WP + i
+//   pow_omega[e], e < max(n/2, 1):              OMG_W + e, OMG_WP + e
+// (w, w') is the Shoup pair for q < 2^31 and (w*2^32 mod q, 0) otherwise.
+struct TabLayout {
+    static constexpr uint32_t HDR = 16;
+    static __host__ __device__ constexpr uint32_t fwd_w(uint32_t n) { return HDR; }
+    static __host__ __device__ constexpr uint32_t fwd_wp(uint32_t n) { return HDR + n; }
+    static __host__ __device__ constexpr uint32_t inv_w(uint32_t n) { return HDR + 2 * n; }
+    static __host__ __device__ constexpr uint32_t inv_wp(uint32_t n) { return HDR + 3 * n; }
+    static __host__ __device__ constexpr uint32_t psi_w(uint32_t n) { return HDR + 4 * n; }
+    static __host__ __device__ constexpr uint32_t psi_wp(uint32_t n) { return HDR + 5 * n; }
+    static __host__ __device__ constexpr uint32_t omg_w(uint32_t n) { return HDR + 6 * n; }
+    static __host__ __device__ constexpr uint32_t omg_wp(uint32_t n) { return HDR + 6 * n + (n > 1 ? n / 2 : 1); }
+    static __host__ __device__ constexpr uint32_t words(uint32_t n) { return HDR + 7 * n + 2; }
+};
+
+constexpr uint32_t NTT_MAX_LOG_N = 13; // LDS tile limit of the generic kernel
+constexpr int MAX_PLANS_PER_LAUNCH = 8;
+
+struct TabSet {
+    const uint32_t *p[MAX_PLANS_PER_LAUNCH];
+};
+
+} // namespace lzx
+
+struct lzx_ntt_plan {
+    uint32_t q, two_adicity, generator, log_n, n;
+    uint32_t psi, psi_inv, n_inv;
+    bool bigq;
+    lzx::Mod32 mod;
+    std::vector<uint32_t> host_tab; // same words as dev_tab
+    uint32_t *dev_tab;
+    int device;
+};
+
+namespace lzx {
+
+enum NttMode { NTT_FWD = 0, NTT_INV = 1, NTT_PARTIAL = 2, NTT_COMPLETE = 3 };
+
+// Launch one transform mode over nplans plans sharing n and bigq; data is
+// laid out [plan][poly][n] with `polys` polynomials per plan.
+lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data, size_t polys, NttMode mode,
+                      uint32_t levels, hipStream_t stream);
+
+} // namespace lzx

@@ -1,0 +1,28 @@
+// Shared host-side runtime pieces of liblzx_gpu: status handling, HIP error
+// capture and the per-device bookkeeping used by every module.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lzx_gpu.h"
+
+namespace lzx {
+
+// Records the HIP error text for lzx_last_device_error() and maps it to a
+// status code.
+lzx_status device_fail(hipError_t e, const char *what);
+
+} // namespace lzx
+
+#define LZX_HIP(call)                                                                                        \
+    do {                                                                                                     \
+        hipError_t lzx_e_ = (call);                                                                          \
+        if (lzx_e_ != hipSuccess) return ::lzx::device_fail(lzx_e_, #call);                                  \
+    } while (0)
+
+#define LZX_LAUNCH_CHECK(what)                                                                               \
+    do {                                                                                                     \
+        hipError_t lzx_e_ = hipGetLastError();                                                               \
+        if (lzx_e_ != hipSuccess) return ::lzx::device_fail(lzx_e_, what);                                   \
+    } while (0)

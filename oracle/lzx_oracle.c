@@ -1,0 +1,1012 @@
+/*
+ * lzx_oracle.c -- CPU restatement of the LZX lattice-commitment hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: the parity checker for
+ * the HIP product path and the timed scalar CPU baseline.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * never as the thing measured or shipped.
+ *
+ * Every function restates the algorithm of the Rust reference (idrees2516/lzx)
+ * at the cited file:line (paths relative to the reference's crates/).  The
+ * reference cannot be compiled here (no Rust toolchain), so it is pinned by:
+ *   - the reference's own known answers (Goldilocks KATs field.rs:268-313,
+ *     FIPS 202 KATs keccak.rs:416-446, reduce_i64 cases modulus.rs:254-265,
+ *     Lagrange KAT sumcheck.rs:503-509), held in tests/golden/;
+ *   - Python's hashlib SHA3/SHAKE (FIPS 202) for the Keccak permutation;
+ *   - closed forms computed independently in Python big integers: the direct
+ *     odd-power DFT for the NTT, schoolbook negacyclic convolution for ring
+ *     products and Ajtai commitments, hypercube enumeration for sum-check
+ *     round polynomials and the multilinear evaluation identity.
+ * Parity unpinned: the byte-exact Fiat-Shamir challenge VALUES of
+ * Transcript::challenge_fields (its squeeze is non-standard and the reference
+ * holds no vector for it); only its structure is pinned, by code reading and
+ * by self-consistency tests.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+enum {
+    OR_OK = 0,
+    OR_NTT_LENGTH_MISMATCH = 1,
+    OR_NTT_ADICITY_TOO_SMALL = 2,
+    OR_AJTAI_DIMENSION_MISMATCH = 4,
+    OR_AJTAI_NORM_EXCEEDED = 5,
+    OR_AJTAI_COMMITMENT_MISMATCH = 6,
+    OR_SUMCHECK_BAD_ROUND_SHAPE = 10,
+    OR_SUMCHECK_ROUND_CHECK_FAILED = 11,
+    OR_SUMCHECK_FINAL_CHECK_FAILED = 12,
+    OR_SUMCHECK_CLAIM_MISMATCH = 13,
+    OR_SUMCHECK_VIRTUAL_POLY = 14,
+    OR_TRANSCRIPT_REJECTION_BUDGET = 20,
+    OR_TRANSCRIPT_MESSAGE_TOO_LARGE = 21,
+    OR_BAD_ARGUMENT = 30,
+};
+
+/* ------------------------------------------------------------------------ */
+/* Modulus32: lattice-ring/src/modulus.rs:22-162                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t q, two_adicity, generator;
+    uint64_t barrett; /* floor((2^64-1)/q), modulus.rs:15-17 */
+} or_mod32;
+
+or_mod32 or_mod32_new(uint32_t q, uint32_t two_adicity, uint32_t generator)
+{
+    or_mod32 m;
+    m.q = q;
+    m.two_adicity = two_adicity;
+    m.generator = generator;
+    m.barrett = UINT64_MAX / (uint64_t)q;
+    return m;
+}
+
+/* modulus.rs:65-76: Barrett with u128 intermediate, <=2 corrections. */
+uint32_t or_reduce_u64(const or_mod32 *m, uint64_t x)
+{
+    u128 q = m->q;
+    u128 qhat = ((u128)x * (u128)m->barrett) >> 64;
+    u128 r = (u128)x - qhat * q;
+    if (r >= q) r -= q;
+    if (r >= q) r -= q;
+    return (uint32_t)r;
+}
+
+/* modulus.rs:80-83: rem_euclid. */
+uint32_t or_reduce_i64(const or_mod32 *m, int64_t x)
+{
+    int64_t q = (int64_t)m->q;
+    int64_t r = x % q;
+    if (r < 0) r += q;
+    return (uint32_t)r;
+}
+
+uint32_t or_mul(const or_mod32 *m, uint32_t a, uint32_t b) /* modulus.rs:87 */
+{
+    return or_reduce_u64(m, (uint64_t)a * (uint64_t)b);
+}
+
+uint32_t or_add(const or_mod32 *m, uint32_t a, uint32_t b) /* modulus.rs:93-100 */
+{
+    uint64_t s = (uint64_t)a + (uint64_t)b;
+    return s >= m->q ? (uint32_t)(s - m->q) : (uint32_t)s;
+}
+
+uint32_t or_sub(const or_mod32 *m, uint32_t a, uint32_t b) /* modulus.rs:104-112 */
+{
+    return a >= b ? a - b : a + (m->q - b);
+}
+
+uint32_t or_neg(const or_mod32 *m, uint32_t a) /* modulus.rs:116-122 */
+{
+    return a == 0 ? 0 : m->q - a;
+}
+
+uint32_t or_pow(const or_mod32 *m, uint32_t base, uint64_t e) /* modulus.rs:125-137 */
+{
+    uint32_t acc = 1;
+    uint32_t b = or_reduce_u64(m, base);
+    while (e > 0) {
+        if (e & 1) acc = or_mul(m, acc, b);
+        b = or_mul(m, b, b);
+        e >>= 1;
+    }
+    return acc;
+}
+
+/* modulus.rs:140-146: Fermat inverse; returns 0 for a == 0 (None). */
+uint32_t or_inv(const or_mod32 *m, uint32_t a)
+{
+    if (a == 0) return 0;
+    return or_pow(m, a, (uint64_t)m->q - 2);
+}
+
+/* modulus.rs:149-156: g^((q-1) >> k); returns 0 when k > two_adicity (None). */
+uint32_t or_root_of_unity(const or_mod32 *m, uint32_t k)
+{
+    if (k > m->two_adicity) return 0;
+    return or_pow(m, m->generator, ((uint64_t)m->q - 1) >> k);
+}
+
+/* ------------------------------------------------------------------------ */
+/* NttTables: lattice-ring/src/ntt.rs:16-342                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    or_mod32 mod;
+    uint32_t log_n;
+    uint32_t n;
+    uint32_t psi, n_inv;
+    uint32_t *pow_psi;       /* [n] */
+    uint32_t *pow_psi_inv;   /* [n] */
+    uint32_t *pow_omega;     /* [max(n/2,1)] */
+    uint32_t *pow_omega_inv; /* [max(n/2,1)] */
+} or_ntt;
+
+void or_ntt_free(or_ntt *t)
+{
+    if (!t) return;
+    free(t->pow_psi);
+    free(t->pow_psi_inv);
+    free(t->pow_omega);
+    free(t->pow_omega_inv);
+    free(t);
+}
+
+/* ntt.rs:46-106.  The per-level tables of ntt.rs:80-94 are views of
+ * pow_omega with stride n/len, so forward/forward_fast read pow_omega
+ * directly at index stride*t (identical values). */
+int or_ntt_new(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n, or_ntt **out)
+{
+    *out = NULL;
+    or_mod32 m = or_mod32_new(q, two_adicity, generator);
+    if (log_n + 1 > m.two_adicity) return OR_NTT_ADICITY_TOO_SMALL;
+    if (log_n > 24) return OR_BAD_ARGUMENT;
+    uint32_t n = 1u << log_n;
+    uint32_t psi = or_root_of_unity(&m, log_n + 1);
+    uint32_t psi_inv = or_inv(&m, psi);
+    uint32_t omega = or_mul(&m, psi, psi);
+    uint32_t omega_inv = or_inv(&m, omega);
+    uint32_t n_inv = or_pow(&m, n, (uint64_t)m.q - 2);
+    if (psi == 0 || psi_inv == 0 || omega_inv == 0) return OR_NTT_ADICITY_TOO_SMALL;
+    or_ntt *t = (or_ntt *)calloc(1, sizeof(or_ntt));
+    uint32_t half = n / 2 > 0 ? n / 2 : 1;
+    t->mod = m;
+    t->log_n = log_n;
+    t->n = n;
+    t->psi = psi;
+    t->n_inv = n_inv;
+    t->pow_psi = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    t->pow_psi_inv = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    t->pow_omega = (uint32_t *)malloc(sizeof(uint32_t) * half);
+    t->pow_omega_inv = (uint32_t *)malloc(sizeof(uint32_t) * half);
+    uint32_t p = 1, pi = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        t->pow_psi[i] = p;
+        t->pow_psi_inv[i] = pi;
+        p = or_mul(&m, p, psi);
+        pi = or_mul(&m, pi, psi_inv);
+    }
+    uint32_t w = 1, wi = 1;
+    for (uint32_t i = 0; i < half; i++) {
+        t->pow_omega[i] = w;
+        t->pow_omega_inv[i] = wi;
+        w = or_mul(&m, w, omega);
+        wi = or_mul(&m, wi, omega_inv);
+    }
+    *out = t;
+    return OR_OK;
+}
+
+uint32_t or_ntt_psi(const or_ntt *t) { return t->psi; }
+uint32_t or_ntt_n_inv(const or_ntt *t) { return t->n_inv; }
+
+/* ntt.rs:334-342 */
+static uint32_t reverse_bits(uint32_t x, uint32_t bits)
+{
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < bits; i++) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
+}
+
+/* ntt.rs:320-332 */
+static void bit_reverse(uint32_t *a, uint32_t n)
+{
+    if (n <= 1) return;
+    uint32_t bits = (uint32_t)__builtin_ctz(n);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t j = reverse_bits(i, bits);
+        if (i < j) {
+            uint32_t tmp = a[i];
+            a[i] = a[j];
+            a[j] = tmp;
+        }
+    }
+}
+
+/* Cooley-Tukey DIT levels len = 2^(first+1) .. 2^(last) (ntt.rs:165-180). */
+static void ct_levels(const or_ntt *t, uint32_t *a, uint32_t first_len, uint32_t max_levels)
+{
+    const or_mod32 *q = &t->mod;
+    uint32_t n = t->n, applied = 0;
+    for (uint32_t len = first_len; len <= n && applied < max_levels; len <<= 1, applied++) {
+        uint32_t stride = n / len;
+        for (uint32_t start = 0; start < n; start += len) {
+            for (uint32_t j = 0; j < len / 2; j++) {
+                uint32_t w = t->pow_omega[stride * j];
+                uint32_t u = a[start + j];
+                uint32_t v = or_mul(q, a[start + j + len / 2], w);
+                a[start + j] = or_add(q, u, v);
+                a[start + j + len / 2] = or_sub(q, u, v);
+            }
+        }
+    }
+}
+
+/* ntt.rs:118-146 (forward_fast) == ntt.rs:153-182 (forward).  Output is
+ * natural order: out[k] = sum_i a_i psi^(i(2k+1)). */
+int or_ntt_forward(const or_ntt *t, uint32_t *a, size_t len)
+{
+    if (len != t->n) return OR_NTT_LENGTH_MISMATCH;
+    for (uint32_t i = 0; i < t->n; i++) a[i] = or_mul(&t->mod, a[i], t->pow_psi[i]);
+    bit_reverse(a, t->n);
+    ct_levels(t, a, 2, 64);
+    return OR_OK;
+}
+
+/* ntt.rs:187-216 (inverse_fast) == ntt.rs:220-250 (inverse). */
+int or_ntt_inverse(const or_ntt *t, uint32_t *a, size_t len)
+{
+    if (len != t->n) return OR_NTT_LENGTH_MISMATCH;
+    const or_mod32 *q = &t->mod;
+    uint32_t n = t->n;
+    for (uint32_t l = n; l >= 2; l >>= 1) {
+        uint32_t stride = n / l;
+        for (uint32_t start = 0; start < n; start += l) {
+            for (uint32_t j = 0; j < l / 2; j++) {
+                uint32_t w = t->pow_omega_inv[stride * j];
+                uint32_t u = a[start + j];
+                uint32_t v = a[start + j + l / 2];
+                a[start + j] = or_add(q, u, v);
+                a[start + j + l / 2] = or_mul(q, or_sub(q, u, v), w);
+            }
+        }
+    }
+    bit_reverse(a, n);
+    for (uint32_t i = 0; i < n; i++) a[i] = or_mul(q, or_mul(q, a[i], t->pow_psi_inv[i]), t->n_inv);
+    return OR_OK;
+}
+
+/* ntt.rs:257-286 */
+int or_ntt_forward_partial(const or_ntt *t, uint32_t *a, size_t len, uint32_t levels)
+{
+    if (len != t->n || levels > t->log_n) return OR_NTT_LENGTH_MISMATCH;
+    for (uint32_t i = 0; i < t->n; i++) a[i] = or_mul(&t->mod, a[i], t->pow_psi[i]);
+    bit_reverse(a, t->n);
+    ct_levels(t, a, 2, levels);
+    return OR_OK;
+}
+
+/* ntt.rs:291-316 */
+int or_ntt_complete_partial(const or_ntt *t, uint32_t *a, size_t len, uint32_t already)
+{
+    if (len != t->n || already > t->log_n) return OR_NTT_LENGTH_MISMATCH;
+    if (already >= 31) return OR_OK;
+    ct_levels(t, a, 1u << (already + 1), 64);
+    return OR_OK;
+}
+
+/* Batched helpers (loops over the scalar transform; layout [poly][n]). */
+int or_ntt_forward_batch(const or_ntt *t, uint32_t *a, size_t count)
+{
+    for (size_t p = 0; p < count; p++) or_ntt_forward(t, a + p * t->n, t->n);
+    return OR_OK;
+}
+
+int or_ntt_inverse_batch(const or_ntt *t, uint32_t *a, size_t count)
+{
+    for (size_t p = 0; p < count; p++) or_ntt_inverse(t, a + p * t->n, t->n);
+    return OR_OK;
+}
+
+/* ring.rs:309-340: negacyclic product via two forwards, pointwise, inverse. */
+int or_ring_mul(const or_ntt *t, const uint32_t *a, const uint32_t *b, uint32_t *out)
+{
+    uint32_t n = t->n;
+    uint32_t *tb = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    memcpy(out, a, sizeof(uint32_t) * n);
+    memcpy(tb, b, sizeof(uint32_t) * n);
+    or_ntt_forward(t, out, n);
+    or_ntt_forward(t, tb, n);
+    for (uint32_t i = 0; i < n; i++) out[i] = or_mul(&t->mod, out[i], tb[i]);
+    or_ntt_inverse(t, out, n);
+    free(tb);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Keccak-f[1600] / SHA3 / SHAKE256: lattice-core/src/keccak.rs:9-373       */
+/* ------------------------------------------------------------------------ */
+
+static const uint64_t KRC[24] = {
+    0x0000000000000001ULL, 0x0000000000008082ULL, 0x800000000000808aULL, 0x8000000080008000ULL,
+    0x000000000000808bULL, 0x0000000080000001ULL, 0x8000000080008081ULL, 0x8000000000008009ULL,
+    0x000000000000008aULL, 0x0000000000000088ULL, 0x0000000080008009ULL, 0x000000008000000aULL,
+    0x000000008000808bULL, 0x800000000000008bULL, 0x8000000000008089ULL, 0x8000000000008003ULL,
+    0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
+    0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
+
+/* rho offsets by lane index x + 5y (FIPS 202 table). */
+static const unsigned KRHO[25] = {0,  1,  62, 28, 27, 36, 44, 6,  55, 20, 3,  10, 43,
+                                  25, 39, 41, 45, 15, 21, 8,  18, 2,  61, 56, 14};
+
+static inline uint64_t rotl64(uint64_t x, unsigned r) { return r ? (x << r) | (x >> (64 - r)) : x; }
+
+/* keccak.rs:52 (unrolled) == keccak.rs reference loop: theta, rho, pi, chi, iota. */
+void or_keccak_f1600(uint64_t st[25])
+{
+    for (int round = 0; round < 24; round++) {
+        uint64_t c[5], b[25];
+        for (int x = 0; x < 5; x++) c[x] = st[x] ^ st[x + 5] ^ st[x + 10] ^ st[x + 15] ^ st[x + 20];
+        for (int x = 0; x < 5; x++) {
+            uint64_t d = c[(x + 4) % 5] ^ rotl64(c[(x + 1) % 5], 1);
+            for (int y = 0; y < 5; y++) st[x + 5 * y] ^= d;
+        }
+        for (int x = 0; x < 5; x++)
+            for (int y = 0; y < 5; y++) {
+                int nx = y, ny = (2 * x + 3 * y) % 5;
+                b[nx + 5 * ny] = rotl64(st[x + 5 * y], KRHO[x + 5 * y]);
+            }
+        for (int y = 0; y < 5; y++)
+            for (int x = 0; x < 5; x++)
+                st[x + 5 * y] = b[x + 5 * y] ^ ((~b[(x + 1) % 5 + 5 * y]) & b[(x + 2) % 5 + 5 * y]);
+        st[0] ^= KRC[round];
+    }
+}
+
+typedef struct {
+    uint64_t st[25];
+    size_t pos;
+    size_t rate;
+    uint8_t suffix;
+} or_sponge;
+
+static void sponge_init(or_sponge *s, size_t rate, uint8_t suffix)
+{
+    memset(s, 0, sizeof(*s));
+    s->rate = rate;
+    s->suffix = suffix;
+}
+
+/* keccak.rs:322-331 */
+static void sponge_update(or_sponge *s, const uint8_t *data, size_t len)
+{
+    for (size_t i = 0; i < len; i++) {
+        s->st[s->pos / 8] ^= (uint64_t)data[i] << (8 * (s->pos % 8));
+        s->pos++;
+        if (s->pos == s->rate) {
+            or_keccak_f1600(s->st);
+            s->pos = 0;
+        }
+    }
+}
+
+/* keccak.rs:334-355 */
+static void sponge_finalize(or_sponge *s, uint8_t *out, size_t out_len)
+{
+    s->st[s->pos / 8] ^= (uint64_t)s->suffix << (8 * (s->pos % 8));
+    size_t last = s->rate - 1;
+    s->st[last / 8] ^= 0x80ULL << (8 * (last % 8));
+    or_keccak_f1600(s->st);
+    size_t done = 0;
+    while (done < out_len) {
+        size_t take = s->rate < out_len - done ? s->rate : out_len - done;
+        for (size_t i = 0; i < take; i++) out[done + i] = (uint8_t)(s->st[i / 8] >> (8 * (i % 8)));
+        done += take;
+        if (done < out_len) or_keccak_f1600(s->st);
+    }
+}
+
+void or_shake256(const uint8_t *in, size_t len, uint8_t *out, size_t out_len) /* keccak.rs:369 */
+{
+    or_sponge s;
+    sponge_init(&s, 136, 0x1f);
+    sponge_update(&s, in, len);
+    sponge_finalize(&s, out, out_len);
+}
+
+void or_sha3_256(const uint8_t *in, size_t len, uint8_t out[32]) /* keccak.rs:359 */
+{
+    or_sponge s;
+    sponge_init(&s, 136, 0x06);
+    sponge_update(&s, in, len);
+    sponge_finalize(&s, out, 32);
+}
+
+static void put_u32le(uint8_t *p, uint32_t v)
+{
+    for (int i = 0; i < 4; i++) p[i] = (uint8_t)(v >> (8 * i));
+}
+
+static void put_u64le(uint8_t *p, uint64_t v)
+{
+    for (int i = 0; i < 8; i++) p[i] = (uint8_t)(v >> (8 * i));
+}
+
+static uint32_t get_u32le(const uint8_t *p)
+{
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+static uint64_t get_u64le(const uint8_t *p)
+{
+    uint64_t v = 0;
+    for (int i = 7; i >= 0; i--) v = (v << 8) | p[i];
+    return v;
+}
+
+/* transcript.rs:229-237: SHAKE256("LZXX" || u32le|d| || d || u32le|m| || m). */
+void or_xof(const uint8_t *domain, size_t dlen, const uint8_t *msg, size_t mlen, uint8_t *out, size_t out_len)
+{
+    size_t blen = 12 + dlen + mlen;
+    uint8_t *buf = (uint8_t *)malloc(blen);
+    memcpy(buf, "LZXX", 4);
+    put_u32le(buf + 4, (uint32_t)dlen);
+    memcpy(buf + 8, domain, dlen);
+    put_u32le(buf + 8 + dlen, (uint32_t)mlen);
+    memcpy(buf + 12 + dlen, msg, mlen);
+    or_shake256(buf, blen, out, out_len);
+    free(buf);
+}
+
+/* ring.rs:102-180: uniform ring element by rejection over 4-byte LE windows
+ * of the XOF stream starting at offset 8. */
+void or_uniform_from_seed(uint32_t q, uint32_t log_n, const uint8_t *domain, size_t dlen, const uint8_t *seed,
+                          size_t slen, uint64_t index, uint32_t *coeffs)
+{
+    size_t n = (size_t)1 << log_n;
+    uint64_t limit = ((uint64_t)UINT32_MAX + 1) - (((uint64_t)UINT32_MAX + 1) % q);
+    size_t salt_len = dlen + slen + 8;
+    uint8_t *salt = (uint8_t *)malloc(salt_len);
+    memcpy(salt, domain, dlen);
+    memcpy(salt + dlen, seed, slen);
+    put_u64le(salt + dlen + slen, index);
+    size_t slack = n > 16 ? n : 16;
+    size_t stream_len = 8 + 4 * (n + slack);
+    uint8_t *stream = (uint8_t *)malloc(stream_len);
+    or_xof((const uint8_t *)"uniform", 7, salt, salt_len, stream, stream_len);
+    size_t got = 0;
+    for (size_t w = 0; w < n + slack && got < n; w++) {
+        uint64_t cand = get_u32le(stream + 8 + 4 * w);
+        if (cand < limit) coeffs[got++] = (uint32_t)(cand % q);
+    }
+    if (got < n) {
+        /* ring.rs:155-175: counter-indexed continuation (window `counter`
+         * is the last 4 bytes of an XOF of length 8 + 4*counter + 4). */
+        size_t counter = (stream_len - 8) / 4;
+        while (got < n) {
+            size_t blen = 8 + counter * 4 + 4;
+            uint8_t *bytes = (uint8_t *)malloc(blen);
+            or_xof((const uint8_t *)"uniform", 7, salt, salt_len, bytes, blen);
+            uint64_t cand = get_u32le(bytes + blen - 4);
+            if (cand < limit) coeffs[got++] = (uint32_t)(cand % q);
+            free(bytes);
+            counter++;
+        }
+    }
+    free(stream);
+    free(salt);
+}
+
+/* ajtai.rs:307-348: small secret with coefficients uniform in [-bound, bound]. */
+void or_sample_small_secret(uint32_t q, uint32_t log_n, size_t m, uint32_t bound, const uint8_t *seed, size_t slen,
+                            uint32_t *out /* m*n */)
+{
+    or_mod32 mod = or_mod32_new(q, 1, 1);
+    size_t n = (size_t)1 << log_n;
+    uint64_t span = 2ULL * bound + 1;
+    uint64_t limit = ((uint64_t)UINT32_MAX / span) * span;
+    size_t blen = n * 16 > 64 ? n * 16 : 64;
+    uint8_t *bytes = (uint8_t *)malloc(blen);
+    uint8_t *input = (uint8_t *)malloc(slen + 16);
+    memcpy(input, seed, slen);
+    for (size_t i = 0; i < m; i++) {
+        size_t got = 0;
+        uint64_t ctr = 0;
+        while (got < n) {
+            put_u64le(input + slen, (uint64_t)i);
+            put_u64le(input + slen + 8, ctr);
+            or_xof((const uint8_t *)"ajtai-secret", 12, input, slen + 16, bytes, blen);
+            for (size_t c = 0; c + 4 <= blen && got < n; c += 4) {
+                uint64_t v = get_u32le(bytes + c);
+                if (v < limit) out[i * n + got++] = or_reduce_i64(&mod, (int64_t)(v % span) - (int64_t)bound);
+            }
+            ctr++;
+        }
+    }
+    free(bytes);
+    free(input);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ajtai commitment: lattice-commitment/src/ajtai.rs:87-180                  */
+/* ------------------------------------------------------------------------ */
+
+/* ajtai.rs:87-103: A[idx] = uniform_from_seed("ajtai-A", seed, idx), idx = i*m + j;
+ * Ahat[idx] = forward NTT of A[idx].  Either output may be NULL. */
+int or_ajtai_from_seed(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n, size_t k, size_t m,
+                       const uint8_t seed[32], uint32_t *A, uint32_t *Ahat)
+{
+    or_ntt *t = NULL;
+    int st = or_ntt_new(q, two_adicity, generator, log_n, &t);
+    if (st) return st;
+    size_t n = t->n;
+    uint32_t *tmp = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (size_t idx = 0; idx < k * m; idx++) {
+        or_uniform_from_seed(q, log_n, (const uint8_t *)"ajtai-A", 7, seed, 32, (uint64_t)idx, tmp);
+        if (A) memcpy(A + idx * n, tmp, sizeof(uint32_t) * n);
+        if (Ahat) {
+            or_ntt_forward(t, tmp, n);
+            memcpy(Ahat + idx * n, tmp, sizeof(uint32_t) * n);
+        }
+    }
+    free(tmp);
+    or_ntt_free(t);
+    return OR_OK;
+}
+
+/* ajtai.rs:118-153: cached-NTT commit.  Ahat is k*m*n (row-major), s is m*n
+ * canonical coefficients, out is k*n.  All-zero s_j are skipped. */
+int or_ajtai_commit(const or_ntt *t, size_t k, size_t m, const uint32_t *Ahat, const uint32_t *s, size_t s_len,
+                    uint32_t *out)
+{
+    if (s_len != m) return OR_AJTAI_DIMENSION_MISMATCH;
+    size_t n = t->n;
+    const or_mod32 *q = &t->mod;
+    uint32_t *s_ntt = (uint32_t *)malloc(sizeof(uint32_t) * n * (m ? m : 1));
+    uint8_t *nz = (uint8_t *)malloc(m ? m : 1);
+    for (size_t j = 0; j < m; j++) {
+        nz[j] = 0;
+        for (size_t c = 0; c < n; c++) nz[j] |= s[j * n + c] != 0;
+        memcpy(s_ntt + j * n, s + j * n, sizeof(uint32_t) * n);
+        if (nz[j]) or_ntt_forward(t, s_ntt + j * n, n);
+    }
+    for (size_t i = 0; i < k; i++) {
+        uint32_t *acc = out + i * n;
+        memset(acc, 0, sizeof(uint32_t) * n);
+        for (size_t j = 0; j < m; j++) {
+            if (!nz[j]) continue;
+            const uint32_t *a = Ahat + (i * m + j) * n;
+            const uint32_t *b = s_ntt + j * n;
+            for (size_t c = 0; c < n; c++) acc[c] = or_add(q, acc[c], or_mul(q, a[c], b[c]));
+        }
+        or_ntt_inverse(t, acc, n);
+    }
+    free(nz);
+    free(s_ntt);
+    return OR_OK;
+}
+
+/* ajtai.rs:158-180: per-product ring multiplication (differential path). */
+int or_ajtai_commit_naive(const or_ntt *t, size_t k, size_t m, const uint32_t *A, const uint32_t *s, size_t s_len,
+                          uint32_t *out)
+{
+    if (s_len != m) return OR_AJTAI_DIMENSION_MISMATCH;
+    size_t n = t->n;
+    uint32_t *prod = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (size_t i = 0; i < k; i++) {
+        uint32_t *acc = out + i * n;
+        memset(acc, 0, sizeof(uint32_t) * n);
+        for (size_t j = 0; j < m; j++) {
+            int nzj = 0;
+            for (size_t c = 0; c < n; c++) nzj |= s[j * n + c] != 0;
+            if (!nzj) continue;
+            or_ring_mul(t, A + (i * m + j) * n, s + j * n, prod);
+            for (size_t c = 0; c < n; c++) acc[c] = or_add(&t->mod, acc[c], prod[c]);
+        }
+    }
+    free(prod);
+    return OR_OK;
+}
+
+/* Schoolbook negacyclic product (ntt.rs:390-403 shape), exact in u128. */
+void or_negacyclic_schoolbook(uint32_t q, uint32_t n, const uint32_t *a, const uint32_t *b, uint32_t *out)
+{
+    or_mod32 m = or_mod32_new(q, 1, 1);
+    for (uint32_t i = 0; i < n; i++) out[i] = 0;
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < n; j++) {
+            uint32_t v = or_mul(&m, a[i], b[j]);
+            if (i + j < n)
+                out[i + j] = or_add(&m, out[i + j], v);
+            else
+                out[i + j - n] = or_sub(&m, out[i + j - n], v);
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Goldilocks: lattice-core/src/field.rs:9-209                               */
+/* ------------------------------------------------------------------------ */
+
+#define GL_P 0xFFFFFFFF00000001ULL
+
+uint64_t or_gl_from_u64(uint64_t x) { return x >= GL_P ? x - GL_P : x; } /* field.rs:28-36 */
+
+uint64_t or_gl_from_u128(uint64_t hi, uint64_t lo) /* field.rs:42-55 */
+{
+    u128 acc = ((u128)hi << 64) | lo;
+    for (int i = 0; i < 6; i++) {
+        if ((acc >> 64) == 0) break;
+        uint64_t h = (uint64_t)(acc >> 64), l = (uint64_t)acc;
+        acc = (u128)h * 0xFFFFFFFFULL + (u128)l;
+    }
+    return or_gl_from_u64((uint64_t)acc);
+}
+
+uint64_t or_gl_add(uint64_t a, uint64_t b) /* field.rs:70-77 */
+{
+    uint64_t s = a + b;
+    if (s < a) s += 0xFFFFFFFFULL;
+    return or_gl_from_u64(s);
+}
+
+uint64_t or_gl_sub(uint64_t a, uint64_t b) /* field.rs:82-89 */
+{
+    uint64_t d = a - b;
+    if (a < b) d -= 0xFFFFFFFFULL;
+    return or_gl_from_u64(d);
+}
+
+uint64_t or_gl_neg(uint64_t a) { return a == 0 ? 0 : or_gl_from_u64(GL_P - a); } /* field.rs:93-99 */
+
+uint64_t or_gl_mul(uint64_t a, uint64_t b) /* field.rs:103-105 */
+{
+    u128 p = (u128)a * (u128)b;
+    return or_gl_from_u128((uint64_t)(p >> 64), (uint64_t)p);
+}
+
+uint64_t or_gl_pow(uint64_t a, uint64_t e) /* field.rs:119-130 (single word) */
+{
+    uint64_t acc = 1;
+    for (int i = 63; i >= 0; i--) {
+        acc = or_gl_mul(acc, acc);
+        if ((e >> i) & 1) acc = or_gl_mul(acc, a);
+    }
+    return acc;
+}
+
+uint64_t or_gl_inverse(uint64_t a) { return a == 0 ? 0 : or_gl_pow(a, GL_P - 2); } /* field.rs:137-143 */
+
+/* ------------------------------------------------------------------------ */
+/* Transcript: lattice-core/src/transcript.rs:36-238                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    or_sponge sponge;
+    uint64_t query_count;
+    uint64_t query_budget;
+} or_transcript;
+
+/* transcript.rs:46-59 */
+or_transcript *or_transcript_new(const uint8_t *protocol, size_t plen, uint64_t budget)
+{
+    or_transcript *t = (or_transcript *)calloc(1, sizeof(or_transcript));
+    sponge_init(&t->sponge, 136, 0x1f);
+    uint8_t head[8];
+    memcpy(head, "LZX1", 4);
+    put_u32le(head + 4, (uint32_t)plen);
+    sponge_update(&t->sponge, head, 8);
+    sponge_update(&t->sponge, protocol, plen);
+    t->query_budget = budget;
+    return t;
+}
+
+void or_transcript_free(or_transcript *t) { free(t); }
+
+or_transcript *or_transcript_clone(const or_transcript *t)
+{
+    or_transcript *c = (or_transcript *)malloc(sizeof(or_transcript));
+    *c = *t;
+    return c;
+}
+
+/* transcript.rs:67-78 */
+int or_transcript_append_message(or_transcript *t, const uint8_t *label, size_t llen, const uint8_t *msg, size_t mlen)
+{
+    if (mlen > ((size_t)1 << 26)) return OR_TRANSCRIPT_MESSAGE_TOO_LARGE;
+    uint8_t b4[4];
+    put_u32le(b4, (uint32_t)llen);
+    sponge_update(&t->sponge, b4, 4);
+    sponge_update(&t->sponge, label, llen);
+    put_u32le(b4, (uint32_t)mlen);
+    sponge_update(&t->sponge, b4, 4);
+    sponge_update(&t->sponge, msg, mlen);
+    return OR_OK;
+}
+
+/* transcript.rs:90-100 */
+int or_transcript_append_field_slice(or_transcript *t, const uint8_t *label, size_t llen, const uint64_t *v, size_t n)
+{
+    uint8_t *bytes = (uint8_t *)malloc(n * 8 + 1);
+    for (size_t i = 0; i < n; i++) put_u64le(bytes + 8 * i, v[i]);
+    int st = or_transcript_append_message(t, label, llen, bytes, n * 8);
+    free(bytes);
+    return st;
+}
+
+/* transcript.rs:180-205: squeeze from a COPY of the raw state; stream block i
+ * is available after i+1 permutations (no padding is applied). */
+static void transcript_squeeze(const or_transcript *t, size_t *offset, uint8_t *out, size_t n)
+{
+    const size_t RATE = 136;
+    uint64_t st[25];
+    memcpy(st, t->sponge.st, sizeof(st));
+    size_t block_idx = *offset / RATE;
+    for (size_t i = 0; i <= block_idx; i++) or_keccak_f1600(st);
+    size_t pos = *offset % RATE, done = 0;
+    while (done < n) {
+        size_t take = RATE - pos < n - done ? RATE - pos : n - done;
+        for (size_t i = 0; i < take; i++) out[done + i] = (uint8_t)(st[(pos + i) / 8] >> (8 * ((pos + i) % 8)));
+        done += take;
+        pos += take;
+        if (pos == RATE) {
+            or_keccak_f1600(st);
+            pos = 0;
+        }
+    }
+    *offset += n;
+}
+
+/* transcript.rs:109-148 */
+int or_transcript_challenge_fields(or_transcript *t, const uint8_t *label, size_t llen, size_t n, uint64_t *out)
+{
+    t->query_count++;
+    if (t->query_count > t->query_budget) return OR_TRANSCRIPT_REJECTION_BUDGET;
+    uint8_t b4[4];
+    sponge_update(&t->sponge, (const uint8_t *)"CHAL", 4);
+    put_u32le(b4, (uint32_t)llen);
+    sponge_update(&t->sponge, b4, 4);
+    sponge_update(&t->sponge, label, llen);
+    put_u32le(b4, (uint32_t)n);
+    sponge_update(&t->sponge, b4, 4);
+    size_t got = 0, offset = 0;
+    while (got < n) {
+        size_t need = (n - got) * 16;
+        if (need < 32) need = 32;
+        uint8_t *bytes = (uint8_t *)malloc(need);
+        transcript_squeeze(t, &offset, bytes, need);
+        for (size_t c = 0; c < need && got < n; c += 16) {
+            uint8_t chunk[16] = {0};
+            size_t cl = need - c < 16 ? need - c : 16;
+            memcpy(chunk, bytes + c, cl);
+            uint64_t cand = get_u64le(chunk);
+            if (cand < GL_P) out[got++] = cand;
+        }
+        free(bytes);
+    }
+    return OR_OK;
+}
+
+int or_transcript_challenge_field(or_transcript *t, const uint8_t *label, size_t llen, uint64_t *out)
+{
+    return or_transcript_challenge_fields(t, label, llen, 1, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multilinear pieces: lattice-core/src/mle.rs, field_simd.rs                */
+/* ------------------------------------------------------------------------ */
+
+/* field_simd.rs:391-407: eq table, variable 0 = most significant index bit. */
+void or_eq_table(const uint64_t *point, size_t m, uint64_t *out /* 2^m */)
+{
+    out[0] = 1;
+    size_t cur = 1;
+    for (size_t v = m; v-- > 0;) {
+        uint64_t p = point[v];
+        uint64_t omp = or_gl_sub(1, p);
+        for (size_t i = 0; i < cur; i++) {
+            out[cur + i] = or_gl_mul(out[i], p);
+            out[i] = or_gl_mul(out[i], omp);
+        }
+        cur *= 2;
+    }
+}
+
+/* field_simd.rs:297-313 / mle.rs:152-172: bind variable 0 in place. */
+void or_bind_first_half(uint64_t *evals, size_t len, uint64_t r)
+{
+    size_t half = len / 2;
+    for (size_t i = 0; i < half; i++) {
+        uint64_t a = evals[i], b = evals[i + half];
+        evals[i] = or_gl_add(a, or_gl_mul(or_gl_sub(b, a), r));
+    }
+}
+
+/* mle.rs:131-148 */
+uint64_t or_mle_evaluate(const uint64_t *evals, size_t num_vars, const uint64_t *point)
+{
+    size_t len = (size_t)1 << num_vars;
+    uint64_t *cur = (uint64_t *)malloc(sizeof(uint64_t) * len);
+    memcpy(cur, evals, sizeof(uint64_t) * len);
+    for (size_t v = 0; v < num_vars; v++) {
+        or_bind_first_half(cur, len, point[v]);
+        len /= 2;
+    }
+    uint64_t r = cur[0];
+    free(cur);
+    return r;
+}
+
+/* sumcheck.rs:286-306: Lagrange over nodes 0..n-1. */
+uint64_t or_interpolate_at(const uint64_t *evals, size_t n, uint64_t r)
+{
+    uint64_t acc = 0;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t weight = 1;
+        for (size_t j = 0; j < n; j++) {
+            if (i == j) continue;
+            uint64_t num = or_gl_sub(r, (uint64_t)j);
+            uint64_t den = or_gl_sub((uint64_t)i, (uint64_t)j);
+            weight = or_gl_mul(weight, or_gl_mul(num, or_gl_inverse(den)));
+        }
+        acc = or_gl_add(acc, or_gl_mul(evals[i], weight));
+    }
+    return acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sum-check: lattice-sumcheck/src/sumcheck.rs:62-371                        */
+/* A virtual polynomial is nfactors tables of 2^num_vars canonical u64 plus  */
+/* nterms terms (coeff, factor ids) in CSR form (term_off[nterms+1]).        */
+/* ------------------------------------------------------------------------ */
+
+/* virtual_poly.rs:98-104 */
+static size_t vp_max_degree(size_t nterms, const uint32_t *term_off)
+{
+    size_t d = 0;
+    for (size_t t = 0; t < nterms; t++) {
+        size_t len = term_off[t + 1] - term_off[t];
+        if (len > d) d = len;
+    }
+    return d;
+}
+
+size_t or_vp_max_degree(size_t nterms, const uint32_t *term_off) { return vp_max_degree(nterms, term_off); }
+
+/* virtual_poly.rs:107-121: sum of P over the hypercube. */
+uint64_t or_vp_sum_over_hypercube(size_t num_vars, size_t nfactors, const uint64_t *const *factors, size_t nterms,
+                                  const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids)
+{
+    (void)nfactors;
+    size_t len = (size_t)1 << num_vars;
+    uint64_t acc = 0;
+    for (size_t t = 0; t < nterms; t++)
+        for (size_t p = 0; p < len; p++) {
+            uint64_t prod = coeffs[t];
+            for (uint32_t k = term_off[t]; k < term_off[t + 1]; k++) prod = or_gl_mul(prod, factors[term_ids[k]][p]);
+            acc = or_gl_add(acc, prod);
+        }
+    return acc;
+}
+
+/* sumcheck.rs:172-226: g(t), t = 0..d, from the current bound tables.
+ * bind_half_slices (field_simd.rs:273-291) gives lo + (hi - lo)*t. */
+void or_sumcheck_round_evals(size_t rem_vars, size_t nfactors, uint64_t *const *bound, size_t nterms,
+                             const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids, size_t d,
+                             uint64_t *evals /* d+1 */)
+{
+    size_t points = (size_t)1 << (rem_vars - 1);
+    uint64_t *vals = (uint64_t *)malloc(sizeof(uint64_t) * (nfactors ? nfactors : 1));
+    for (size_t t = 0; t <= d; t++) {
+        uint64_t acc = 0;
+        for (size_t p = 0; p < points; p++) {
+            for (size_t f = 0; f < nfactors; f++) {
+                uint64_t lo = bound[f][p], hi = bound[f][p + points];
+                vals[f] = t == 0 ? lo : t == 1 ? hi : or_gl_add(lo, or_gl_mul(or_gl_sub(hi, lo), (uint64_t)t));
+            }
+            for (size_t tt = 0; tt < nterms; tt++) {
+                uint64_t prod = coeffs[tt];
+                for (uint32_t k = term_off[tt]; k < term_off[tt + 1]; k++) prod = or_gl_mul(prod, vals[term_ids[k]]);
+                acc = or_gl_add(acc, prod);
+            }
+        }
+        evals[t] = acc;
+    }
+    free(vals);
+}
+
+/* sumcheck.rs:62-162.  rounds_out is num_vars*(d+1) (d = max degree, or 1 for
+ * the zero polynomial); challenges_out num_vars; factor_claims_out nfactors.
+ * *d_out receives the round width minus one. */
+int or_sumcheck_prove(size_t num_vars, size_t nfactors, const uint64_t *const *factors, size_t nterms,
+                      const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids, uint64_t claim,
+                      or_transcript *tr, uint64_t *rounds_out, uint64_t *challenges_out, uint64_t *final_claim_out,
+                      uint64_t *factor_claims_out, size_t *d_out)
+{
+    static const uint8_t LROUND[] = "sumcheck-round";
+    static const uint8_t LCHAL[] = "sumcheck-challenge";
+    int st;
+    if (nterms == 0) {
+        if (claim != 0) return OR_SUMCHECK_CLAIM_MISMATCH;
+        *d_out = 1;
+        for (size_t r = 0; r < num_vars; r++) {
+            uint64_t ev[2] = {0, 0};
+            if ((st = or_transcript_append_field_slice(tr, LROUND, 14, ev, 2))) return st;
+            if ((st = or_transcript_challenge_field(tr, LCHAL, 18, &challenges_out[r]))) return st;
+            rounds_out[2 * r] = rounds_out[2 * r + 1] = 0;
+        }
+        *final_claim_out = 0;
+        return OR_OK;
+    }
+    size_t d = vp_max_degree(nterms, term_off);
+    *d_out = d;
+    size_t len = (size_t)1 << num_vars;
+    uint64_t **bound = (uint64_t **)malloc(sizeof(uint64_t *) * nfactors);
+    for (size_t f = 0; f < nfactors; f++) {
+        bound[f] = (uint64_t *)malloc(sizeof(uint64_t) * len);
+        memcpy(bound[f], factors[f], sizeof(uint64_t) * len);
+    }
+    uint64_t current = claim;
+    int status = OR_OK;
+    for (size_t r = 0; r < num_vars; r++) {
+        uint64_t *ev = rounds_out + r * (d + 1);
+        or_sumcheck_round_evals(num_vars - r, nfactors, bound, nterms, coeffs, term_off, term_ids, d, ev);
+        if ((status = or_transcript_append_field_slice(tr, LROUND, 14, ev, d + 1))) goto done;
+        if ((status = or_transcript_challenge_field(tr, LCHAL, 18, &challenges_out[r]))) goto done;
+        if (or_gl_add(ev[0], ev[1]) != current) {
+            status = OR_SUMCHECK_CLAIM_MISMATCH;
+            goto done;
+        }
+        current = or_interpolate_at(ev, d + 1, challenges_out[r]);
+        for (size_t f = 0; f < nfactors; f++) or_bind_first_half(bound[f], len, challenges_out[r]);
+        len /= 2;
+    }
+    {
+        uint64_t fin = 0;
+        for (size_t f = 0; f < nfactors; f++) factor_claims_out[f] = bound[f][0];
+        for (size_t t = 0; t < nterms; t++) {
+            uint64_t prod = coeffs[t];
+            for (uint32_t k = term_off[t]; k < term_off[t + 1]; k++) prod = or_gl_mul(prod, factor_claims_out[term_ids[k]]);
+            fin = or_gl_add(fin, prod);
+        }
+        *final_claim_out = fin;
+        if (fin != current) status = OR_SUMCHECK_FINAL_CHECK_FAILED;
+    }
+done:
+    for (size_t f = 0; f < nfactors; f++) free(bound[f]);
+    free(bound);
+    return status;
+}
+
+/* sumcheck.rs:324-371.  rounds is num_vars rows of width round_len.
+ * has_expected selects the Option<expected_final>. */
+int or_sumcheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_len, size_t num_vars,
+                       size_t max_degree, uint64_t claim, or_transcript *tr, int has_expected,
+                       uint64_t expected_final, uint64_t *point_out, uint64_t *final_claim_out)
+{
+    static const uint8_t LROUND[] = "sumcheck-round";
+    static const uint8_t LCHAL[] = "sumcheck-challenge";
+    if (num_rounds != num_vars) return OR_SUMCHECK_BAD_ROUND_SHAPE;
+    uint64_t current = claim;
+    int st;
+    for (size_t r = 0; r < num_rounds; r++) {
+        const uint64_t *ev = rounds + r * round_len;
+        /* The reference indexes evals[1]; a one-entry round cannot pass. */
+        if (round_len == 0 || round_len > max_degree + 1 || round_len < 2) return OR_SUMCHECK_BAD_ROUND_SHAPE;
+        if ((st = or_transcript_append_field_slice(tr, LROUND, 14, ev, round_len))) return st;
+        uint64_t c;
+        if ((st = or_transcript_challenge_field(tr, LCHAL, 18, &c))) return st;
+        if (or_gl_add(ev[0], ev[1]) != current) return OR_SUMCHECK_ROUND_CHECK_FAILED;
+        current = or_interpolate_at(ev, round_len, c);
+        point_out[r] = c;
+    }
+    if (has_expected && current != expected_final) return OR_SUMCHECK_FINAL_CHECK_FAILED;
+    *final_claim_out = current;
+    return OR_OK;
+}
