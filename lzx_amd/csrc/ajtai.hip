@@ -1,0 +1,356 @@
+// Ajtai commitment t = A * s over R_q = Z_q[X]/(X^n+1), NTT-domain path.
+//
+// Reference: crates/lattice-commitment/src/ajtai.rs
+//   from_seed (:87-103)  A[i*m+j] = uniform_from_seed("ajtai-A", seed, i*m+j),
+//                        A^ cached as the forward NTT of every entry;
+//   commit    (:118-153) forward NTT of each s_j, pointwise MAC of A^_ij * s^_j
+//                        into k accumulators, one inverse NTT per row.
+//
+// Device pipeline of a batched commit (W witnesses):
+//   1. s^ = forward NTT of every witness element (scratch, [W][m][n]);
+//   2. ajtai_mac_kernel: split-K over j.  For each coefficient t the product
+//      is a (W x m) * (m x k) matrix product mod q; a thread owns one t and a
+//      4 x 4 tile of (witness, row) pairs, accumulating the exact 64-bit
+//      products lazily (64-bit sum + carry count), reduced once per chunk;
+//   3. ajtai_reduce_kernel sums the chunk partials mod q;
+//   4. inverse NTT of the [W][k][n] accumulators.
+// All-zero witness elements contribute zero, which is exactly the
+// reference's zero-skip (:130-137).
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "ajtai.hpp"
+#include "seeded.hpp"
+
+namespace lzx {
+
+constexpr int MAC_TB = 64;  // coefficients per block
+constexpr int MAC_TILE = 4; // witnesses x rows per thread
+
+__global__ __launch_bounds__(1024) void ajtai_mac_kernel(const uint32_t *__restrict__ Ahat,
+                                                         const uint32_t *__restrict__ shat,
+                                                         uint32_t *__restrict__ partial, uint32_t W, uint32_t k,
+                                                         uint32_t m, uint32_t n, uint32_t tb, uint32_t wg,
+                                                         uint32_t ig, uint32_t jchunk, uint32_t q, uint64_t barrett,
+                                                         uint32_t r64)
+{
+    // blockIdx.x = t-block + (n/tb) * w-block ; blockIdx.y = row block ; blockIdx.z = chunk
+    const uint32_t tblocks = n / tb;
+    const uint32_t tblk = blockIdx.x % tblocks, wblk = blockIdx.x / tblocks;
+    const uint32_t tt = threadIdx.x % tb;
+    const uint32_t rest = threadIdx.x / tb;
+    const uint32_t wsub = rest % wg, isub = rest / wg;
+    const uint32_t t = tblk * tb + tt;
+    const uint32_t w0 = (wblk * wg + wsub) * MAC_TILE;
+    const uint32_t i0 = (blockIdx.y * ig + isub) * MAC_TILE;
+    const uint32_t j0 = blockIdx.z * jchunk;
+    const uint32_t j1 = min(m, j0 + jchunk);
+
+    uint64_t acc[MAC_TILE][MAC_TILE];
+    uint32_t cnt[MAC_TILE][MAC_TILE];
+#pragma unroll
+    for (int a = 0; a < MAC_TILE; a++)
+#pragma unroll
+        for (int b = 0; b < MAC_TILE; b++) {
+            acc[a][b] = 0;
+            cnt[a][b] = 0;
+        }
+    const size_t mn = (size_t)m * n;
+    for (uint32_t j = j0; j < j1; j++) {
+        uint32_t sv[MAC_TILE], av[MAC_TILE];
+#pragma unroll
+        for (int a = 0; a < MAC_TILE; a++)
+            sv[a] = (w0 + a < W) ? shat[(size_t)(w0 + a) * mn + (size_t)j * n + t] : 0u;
+#pragma unroll
+        for (int b = 0; b < MAC_TILE; b++)
+            av[b] = (i0 + b < k) ? Ahat[((size_t)(i0 + b) * m + j) * n + t] : 0u;
+#pragma unroll
+        for (int a = 0; a < MAC_TILE; a++)
+#pragma unroll
+            for (int b = 0; b < MAC_TILE; b++) {
+                const uint64_t p = (uint64_t)sv[a] * av[b];
+                const uint64_t s = acc[a][b] + p;
+                cnt[a][b] += s < p;
+                acc[a][b] = s;
+            }
+    }
+    const size_t plane = (size_t)W * k * n;
+#pragma unroll
+    for (int a = 0; a < MAC_TILE; a++)
+#pragma unroll
+        for (int b = 0; b < MAC_TILE; b++) {
+            if (w0 + a < W && i0 + b < k) {
+                // value = cnt * 2^64 + acc, 2^64 = r64 (mod q)
+                const uint32_t lo = reduce_u64(acc[a][b], q, barrett);
+                const uint32_t hi = reduce_u64((uint64_t)cnt[a][b] * r64, q, barrett);
+                partial[(size_t)blockIdx.z * plane + ((size_t)(w0 + a) * k + (i0 + b)) * n + t] = add_big(lo, hi, q);
+            }
+        }
+}
+
+__global__ void ajtai_reduce_kernel(const uint32_t *__restrict__ partial, uint32_t *__restrict__ out, size_t plane,
+                                    uint32_t chunks, uint32_t q, uint64_t barrett)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < plane; e += (size_t)gridDim.x * blockDim.x) {
+        uint64_t s = 0;
+        for (uint32_t c = 0; c < chunks; c++) s += partial[(size_t)c * plane + e];
+        out[e] = reduce_u64(s, q, barrett);
+    }
+}
+
+// int8 balanced digits -> canonical residues (ring.rs:232-243 from_signed).
+__global__ void digits_to_residues_kernel(const int8_t *__restrict__ d, uint32_t *__restrict__ out, size_t len,
+                                          uint32_t q)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (size_t)gridDim.x * blockDim.x) {
+        const int32_t v = d[e];
+        out[e] = v < 0 ? q - (uint32_t)(-v) : (uint32_t)v;
+    }
+}
+
+static uint32_t grid_for(size_t len, uint32_t block = 256)
+{
+    return (uint32_t)std::min<size_t>((len + block - 1) / block, 1u << 16);
+}
+
+lzx_status ajtai_commit_ntt(const lzx_ajtai_key *key, const uint32_t *s_dev, const int8_t *digits_dev, size_t W,
+                            uint32_t *out_dev, hipStream_t stream)
+{
+    if (W == 0) return LZX_OK;
+    const size_t n = key->n, m = key->m, k = key->k;
+    const uint32_t q = key->plan->q;
+    const uint64_t barrett = key->plan->mod.barrett;
+    const size_t slen = W * m * n;
+    if (W > 0xFFFFFFFFull / 4 || slen / n > 0x7FFFFFFFull) return LZX_UNSUPPORTED;
+
+    const uint32_t tb = (uint32_t)std::min<size_t>(n, MAC_TB);
+    const uint32_t wg = (uint32_t)std::min<size_t>(4, (W + MAC_TILE - 1) / MAC_TILE);
+    const uint32_t ig = (uint32_t)std::min<size_t>(4, (k + MAC_TILE - 1) / MAC_TILE);
+    const uint32_t wblocks = (uint32_t)((W + (size_t)wg * MAC_TILE - 1) / ((size_t)wg * MAC_TILE));
+    const uint32_t iblocks = (uint32_t)((k + (size_t)ig * MAC_TILE - 1) / ((size_t)ig * MAC_TILE));
+    const uint32_t xy = (uint32_t)(n / tb) * wblocks * iblocks;
+    // enough chunks for ~1024 blocks in flight, at least 8 ring elements per chunk
+    uint32_t chunks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((m + 7) / 8), (1024 + xy - 1) / xy));
+    chunks = std::min<uint32_t>(chunks, 65535);
+    const uint32_t jchunk = (uint32_t)((m + chunks - 1) / chunks);
+    chunks = (uint32_t)((m + jchunk - 1) / jchunk);
+    const size_t plane = W * k * n;
+
+    uint32_t *shat = nullptr, *partial = nullptr;
+    LZX_HIP(hipMallocAsync((void **)&shat, slen * sizeof(uint32_t), stream));
+    hipError_t e = hipMallocAsync((void **)&partial, (size_t)chunks * plane * sizeof(uint32_t), stream);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(shat, stream);
+        return device_fail(e, "ajtai partial alloc");
+    }
+    lzx_status st = LZX_OK;
+    if (digits_dev) {
+        hipLaunchKernelGGL(digits_to_residues_kernel, dim3(grid_for(slen)), dim3(256), 0, stream, digits_dev, shat,
+                           slen, q);
+        e = hipGetLastError();
+    } else {
+        e = hipMemcpyAsync(shat, s_dev, slen * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+    }
+    if (e != hipSuccess) st = device_fail(e, "ajtai witness staging");
+    if (st == LZX_OK) st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, shat, W * m, NTT_FWD, 0, stream);
+    if (st == LZX_OK) {
+        const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % q);
+        hipLaunchKernelGGL(ajtai_mac_kernel, dim3((uint32_t)(n / tb) * wblocks, iblocks, chunks), dim3(tb * wg * ig), 0, stream, key->Ahat_dev, shat,
+                           partial, (uint32_t)W, (uint32_t)k, (uint32_t)m, (uint32_t)n, tb, wg, ig, jchunk, q,
+                           barrett, r64);
+        e = hipGetLastError();
+        if (e != hipSuccess) st = device_fail(e, "ajtai_mac_kernel");
+    }
+    if (st == LZX_OK) {
+        hipLaunchKernelGGL(ajtai_reduce_kernel, dim3(grid_for(plane)), dim3(256), 0, stream, partial, out_dev, plane,
+                           chunks, q, barrett);
+        e = hipGetLastError();
+        if (e != hipSuccess) st = device_fail(e, "ajtai_reduce_kernel");
+    }
+    if (st == LZX_OK) st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, out_dev, W * k, NTT_INV, 0, stream);
+    (void)hipFreeAsync(partial, stream);
+    (void)hipFreeAsync(shat, stream);
+    return st;
+}
+
+static lzx_status key_finish(lzx_ajtai_key *key, const uint32_t *A_host)
+{
+    const size_t words = key->k * key->m * key->n;
+    LZX_HIP(hipMalloc(&key->A_dev, words * sizeof(uint32_t)));
+    LZX_HIP(hipMalloc(&key->Ahat_dev, words * sizeof(uint32_t)));
+    LZX_HIP(hipMemcpy(key->A_dev, A_host, words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    LZX_HIP(hipMemcpy(key->Ahat_dev, key->A_dev, words * sizeof(uint32_t), hipMemcpyDeviceToDevice));
+    lzx_status st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, key->Ahat_dev, key->k * key->m, NTT_FWD,
+                               0, nullptr);
+    if (st != LZX_OK) return st;
+    if (ajtai_int8_supported(key)) {
+        st = ajtai_build_limbs(key);
+        if (st != LZX_OK) return st;
+    }
+    LZX_HIP(hipDeviceSynchronize());
+    return LZX_OK;
+}
+
+static lzx_status key_new(uint32_t q, uint32_t adic, uint32_t g, uint32_t log_n, size_t k, size_t m,
+                          uint32_t norm_bound, lzx_ajtai_key **out, lzx_ajtai_key **key)
+{
+    if (!out) return LZX_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (k == 0 || m == 0) return LZX_INVALID_ARGUMENT;
+    lzx_ntt_plan *plan = nullptr;
+    lzx_status st = lzx_ntt_plan_create(q, adic, g, log_n, &plan);
+    if (st != LZX_OK) return st;
+    lzx_ajtai_key *kk = new lzx_ajtai_key();
+    kk->plan = plan;
+    kk->k = k;
+    kk->m = m;
+    kk->n = plan->n;
+    kk->norm_bound = norm_bound;
+    kk->A_dev = kk->Ahat_dev = nullptr;
+    kk->limbs_dev = nullptr;
+    kk->limbs_bytes = 0;
+    kk->device = plan->device;
+    *key = kk;
+    return LZX_OK;
+}
+
+} // namespace lzx
+
+using namespace lzx;
+
+extern "C" void lzx_ajtai_key_destroy(lzx_ajtai_key *key)
+{
+    if (!key) return;
+    if (key->A_dev) (void)hipFree(key->A_dev);
+    if (key->Ahat_dev) (void)hipFree(key->Ahat_dev);
+    if (key->limbs_dev) (void)hipFree(key->limbs_dev);
+    lzx_ntt_plan_destroy(key->plan);
+    delete key;
+}
+
+extern "C" lzx_status lzx_ajtai_key_from_matrix(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n,
+                                                size_t k, size_t m, uint32_t norm_bound, const uint32_t *A_host,
+                                                lzx_ajtai_key **out)
+{
+    if (!A_host) return LZX_INVALID_ARGUMENT;
+    lzx_ajtai_key *key = nullptr;
+    lzx_status st = key_new(q, two_adicity, generator, log_n, k, m, norm_bound, out, &key);
+    if (st != LZX_OK) return st;
+    st = key_finish(key, A_host);
+    if (st != LZX_OK) {
+        lzx_ajtai_key_destroy(key);
+        return st;
+    }
+    *out = key;
+    return LZX_OK;
+}
+
+extern "C" lzx_status lzx_ajtai_key_from_seed(uint32_t q, uint32_t two_adicity, uint32_t generator, uint32_t log_n,
+                                              size_t k, size_t m, uint32_t norm_bound, const uint8_t seed[32],
+                                              lzx_ajtai_key **out)
+{
+    if (!seed) return LZX_INVALID_ARGUMENT;
+    lzx_ajtai_key *key = nullptr;
+    lzx_status st = key_new(q, two_adicity, generator, log_n, k, m, norm_bound, out, &key);
+    if (st != LZX_OK) return st;
+    const size_t total = k * m, n = key->n;
+    std::vector<uint32_t> A(total * n);
+    static const uint8_t DOM[] = {'a', 'j', 't', 'a', 'i', '-', 'A'};
+    // ajtai.rs:91-96, element idx = i*m + j; independent elements derived in parallel
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned th = 0; th < nthreads; th++)
+        pool.emplace_back([&, th]() {
+            for (size_t idx = th; idx < total; idx += nthreads)
+                uniform_from_seed(q, log_n, DOM, 7, seed, 32, (uint64_t)idx, A.data() + idx * n);
+        });
+    for (auto &t : pool) t.join();
+    st = key_finish(key, A.data());
+    if (st != LZX_OK) {
+        lzx_ajtai_key_destroy(key);
+        return st;
+    }
+    *out = key;
+    return LZX_OK;
+}
+
+extern "C" lzx_status lzx_ajtai_key_matrix(const lzx_ajtai_key *key, uint32_t *A_host)
+{
+    if (!key || !A_host) return LZX_INVALID_ARGUMENT;
+    LZX_HIP(hipMemcpy(A_host, key->A_dev, key->k * key->m * key->n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return LZX_OK;
+}
+
+extern "C" lzx_status lzx_ajtai_key_matrix_ntt(const lzx_ajtai_key *key, uint32_t *Ahat_host)
+{
+    if (!key || !Ahat_host) return LZX_INVALID_ARGUMENT;
+    LZX_HIP(hipMemcpy(Ahat_host, key->Ahat_dev, key->k * key->m * key->n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return LZX_OK;
+}
+
+extern "C" lzx_status lzx_ajtai_commit_batch(const lzx_ajtai_key *key, const uint32_t *s_dev, size_t s_len, size_t W,
+                                             uint32_t *out_dev, void *stream)
+{
+    if (!key) return LZX_INVALID_ARGUMENT;
+    if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH; // ajtai.rs:119-124
+    if (W && (!s_dev || !out_dev)) return LZX_INVALID_ARGUMENT;
+    return ajtai_commit_ntt(key, s_dev, nullptr, W, out_dev, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_ajtai_commit_digits_batch(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t s_len,
+                                                    size_t W, uint32_t *out_dev, int32_t path, void *stream)
+{
+    if (!key) return LZX_INVALID_ARGUMENT;
+    if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH;
+    if (W && (!digits_dev || !out_dev)) return LZX_INVALID_ARGUMENT;
+    if (path < LZX_COMMIT_AUTO || path > LZX_COMMIT_INT8) return LZX_INVALID_ARGUMENT;
+    const bool int8_ok = ajtai_int8_supported(key);
+    if (path == LZX_COMMIT_INT8 && !int8_ok) return LZX_UNSUPPORTED;
+    if (path == LZX_COMMIT_INT8 || (path == LZX_COMMIT_AUTO && int8_ok))
+        return ajtai_commit_int8(key, digits_dev, W, out_dev, (hipStream_t)stream);
+    return ajtai_commit_ntt(key, nullptr, digits_dev, W, out_dev, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_ajtai_commit_host(const lzx_ajtai_key *key, const uint32_t *s_host, size_t s_len,
+                                            uint32_t *out_host)
+{
+    if (!key || !s_host || !out_host) return LZX_INVALID_ARGUMENT;
+    if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH;
+    LZX_HIP(hipSetDevice(key->device));
+    const size_t sw = key->m * key->n, ow = key->k * key->n;
+    uint32_t *s = nullptr, *o = nullptr;
+    LZX_HIP(hipMalloc(&s, sw * sizeof(uint32_t)));
+    hipError_t e = hipMalloc(&o, ow * sizeof(uint32_t));
+    lzx_status st = LZX_OK;
+    if (e == hipSuccess) e = hipMemcpy(s, s_host, sw * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = device_fail(e, "ajtai host commit staging");
+    if (st == LZX_OK) st = ajtai_commit_ntt(key, s, nullptr, 1, o, nullptr);
+    if (st == LZX_OK) {
+        e = hipMemcpy(out_host, o, ow * sizeof(uint32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = device_fail(e, "ajtai host commit readback");
+    }
+    (void)hipFree(s);
+    if (o) (void)hipFree(o);
+    return st;
+}
+
+extern "C" lzx_status lzx_ajtai_verify_opening_host(const lzx_ajtai_key *key, const uint32_t *t_host,
+                                                    const uint32_t *s_host, size_t s_len)
+{
+    if (!key || !t_host || !s_host) return LZX_INVALID_ARGUMENT;
+    if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH;
+    // ajtai.rs:216-224: infinity norm of the balanced representative
+    const uint32_t q = key->plan->q, half = q / 2;
+    for (size_t i = 0; i < key->m * key->n; i++) {
+        const uint32_t c = s_host[i];
+        const uint32_t norm = c <= half ? c : q - c;
+        if (norm > key->norm_bound) return LZX_AJTAI_NORM_EXCEEDED;
+    }
+    std::vector<uint32_t> t(key->k * key->n);
+    lzx_status st = lzx_ajtai_commit_host(key, s_host, s_len, t.data());
+    if (st != LZX_OK) return st;
+    for (size_t i = 0; i < t.size(); i++)
+        if (t[i] != t_host[i]) return LZX_AJTAI_COMMITMENT_MISMATCH;
+    return LZX_OK;
+}

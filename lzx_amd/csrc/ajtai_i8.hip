@@ -1,0 +1,263 @@
+// Ajtai commitment on int8 MFMA in the coefficient domain (d = 64).
+//
+// No counterpart exists in the reference: it computes the same t = A*s
+// (crates/lattice-commitment/src/ajtai.rs:118-153) exactly, as a dense integer
+// contraction.  For witness w, row i, output coefficient r:
+//   t_{w,i}[r] = sum_j sum_c S_{w,j}[r][c] * a_{i,j}[c]   (mod q)
+// where S_{w,j} is the 64x64 negacyclic (signed Toeplitz) matrix of the
+// witness element s_{w,j}: S[r][c] = s[r-c] for c <= r, -s[r-c+64] otherwise.
+// a_{i,j} is the centred coefficient vector of A_{i,j} split into four signed
+// 8-bit limbs, a = sum_L 256^L a^L with a^L in [-128, 127] (exact for
+// q < 2^32 - 2^25).  So the commitment is one int8 GEMM
+//   Out[(w, r)][(i, L)] = sum_(j, c) S_{w,j}[r][c] * a^L_{i,j}[c]
+// with M = 64 W, N = 4 k, K = 64 m, followed by the exact recombination
+// sum_L 256^L Out[.][(i, L)] mod q.
+//
+// Kernel mapping (v_mfma_i32_16x16x64_i8: one ring element = one K step):
+//   - a block of 4 waves owns ITW * 4 witnesses and a chunk of j; each wave
+//     owns ITW witnesses, all 64 output coefficients (4 row tiles) and 16*NT
+//     limb columns (NT column tiles), i.e. 4*NT accumulators per witness;
+//   - B fragments (limb planes, layout [j][col][c], 16 contiguous bytes per
+//     lane) come straight from HBM/L2 as 16-byte loads, reused across the
+//     wave's witnesses and row tiles;
+//   - A fragments are never materialised in HBM: per (w, j) the wave writes
+//     R = (s reversed, then -s reversed) -- 128 bytes -- to LDS, and lane
+//     (rho, g) reads one 64-byte window of R; because S is Toeplitz, the four
+//     row-tile fragments are consecutive 16-byte slices of that window, each
+//     realigned with v_alignbyte;
+//   - int32 accumulation is exact for chunks of <= 2048 ring elements with
+//     |digit| <= 127 (|product| <= 127*128, 64 products per ring element);
+//     chunk partials are summed in 64-bit and folded mod q per limb by
+//     ajtai_i8_reduce_kernel;
+//   - blocks that share a j-chunk are numbered congruent mod 8 so they land on
+//     one XCD (round-robin dispatch) and share that XCD's L2 copy of B.
+#include <algorithm>
+
+#include "ajtai.hpp"
+
+namespace lzx {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int I8_WAVES = 4;
+constexpr int I8_ITW = 2;          // witnesses per wave
+constexpr uint32_t I8_MAX_JCHUNK = 2048;
+constexpr uint64_t I8_Q_LIMIT = (1ULL << 32) - (1ULL << 25);
+
+__device__ __forceinline__ uint32_t neg_bytes(uint32_t x)
+{
+    const uint32_t n = ~x;
+    return ((n & 0x7f7f7f7fu) + 0x01010101u) ^ (n & 0x80808080u);
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void ajtai_i8_kernel(const int8_t *__restrict__ digits,
+                                                       const int8_t *__restrict__ limbs,
+                                                       int32_t *__restrict__ partial, uint32_t W, uint32_t m,
+                                                       uint32_t npad, uint32_t col0, uint32_t jchunk,
+                                                       uint32_t nchunks, uint32_t groups)
+{
+    __shared__ uint32_t Rl[I8_WAVES][I8_ITW][36];
+    const uint32_t b = blockIdx.x;
+    const uint32_t chunk = (b / (8 * groups)) * 8 + (b % 8);
+    const uint32_t grp = (b / 8) % groups;
+    if (chunk >= nchunks) return;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t rho = lane & 15, g = lane >> 4;
+    const uint32_t wbase = (grp * I8_WAVES + wave) * I8_ITW;
+    const uint32_t j0 = chunk * jchunk, j1 = min(m, j0 + jchunk);
+    const uint32_t omin = 15 - rho + 16 * g; // window start in R
+    const uint32_t d0 = omin >> 2, sh = omin & 3;
+
+    v4i acc[I8_ITW][4][NT];
+#pragma unroll
+    for (int x = 0; x < I8_ITW; x++)
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int u = 0; u < NT; u++) acc[x][t][u] = v4i{0, 0, 0, 0};
+
+    for (uint32_t j = j0; j < j1; j++) {
+        v4i bf[NT];
+#pragma unroll
+        for (int u = 0; u < NT; u++)
+            bf[u] = *reinterpret_cast<const v4i *>(limbs + ((size_t)j * npad + col0 + 16 * u + rho) * 64 + 16 * g);
+#pragma unroll
+        for (int x = 0; x < I8_ITW; x++) {
+            const uint32_t w = wbase + x;
+            uint32_t *R = Rl[wave][x];
+            if (lane < 16) {
+                const uint32_t s4 =
+                    w < W ? *reinterpret_cast<const uint32_t *>(digits + ((size_t)w * m + j) * 64 + 4 * lane) : 0u;
+                const uint32_t rv = __builtin_bswap32(s4);
+                R[15 - lane] = rv;            // R[y] = s[63 - y], y < 64
+                R[31 - lane] = neg_bytes(rv); // R[y] = -s[127 - y], 64 <= y < 127
+            } else if (lane < 20) {
+                R[lane + 16] = 0; // padding read by the last window dword
+            }
+            __syncthreads();
+            uint32_t win[17];
+#pragma unroll
+            for (int e = 0; e < 17; e++) win[e] = R[d0 + e];
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                v4i af;
+#pragma unroll
+                for (int q4 = 0; q4 < 4; q4++)
+                    af[q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
+#pragma unroll
+                for (int u = 0; u < NT; u++)
+                    acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[u], acc[x][t][u], 0, 0, 0);
+            }
+        }
+    }
+    // D layout: lane (rho, g), register i -> row 4g + i of the tile, column rho
+#pragma unroll
+    for (int x = 0; x < I8_ITW; x++) {
+        const uint32_t w = wbase + x;
+        if (w >= W) continue;
+        int32_t *base = partial + ((size_t)chunk * W + w) * 64 * npad;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int u = 0; u < NT; u++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    base[(size_t)(16 * t + 4 * g + i) * npad + col0 + 16 * u + rho] = acc[x][t][u][i];
+    }
+}
+
+// out[w][i][r] = sum_L 256^L * (sum_chunks partial[chunk][w][r][4i + L]) mod q
+__global__ void ajtai_i8_reduce_kernel(const int32_t *__restrict__ partial, uint32_t *__restrict__ out, uint32_t W,
+                                       uint32_t k, uint32_t npad, uint32_t nchunks, uint32_t q, uint64_t barrett,
+                                       uint4 pw)
+{
+    const size_t total = (size_t)W * k * 64;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t r = e & 63;
+        const size_t wi = e >> 6;
+        const uint32_t i = wi % k;
+        const size_t w = wi / k;
+        const uint32_t pws[4] = {pw.x, pw.y, pw.z, pw.w};
+        uint64_t acc = 0;
+#pragma unroll
+        for (int L = 0; L < 4; L++) {
+            int64_t s = 0;
+            for (uint32_t c = 0; c < nchunks; c++)
+                s += partial[(((size_t)c * W + w) * 64 + r) * npad + 4 * i + L];
+            const uint64_t mag = (uint64_t)(s < 0 ? -s : s);
+            uint32_t v = reduce_u64(mag, q, barrett);
+            if (s < 0 && v) v = q - v;
+            acc += (uint64_t)reduce_u64((uint64_t)v * pws[L], q, barrett);
+        }
+        out[(w * k + i) * 64 + r] = reduce_u64(acc, q, barrett);
+    }
+}
+
+// limbs[j][col][c], col = 4i + L (columns >= 4k are zero): balanced base-256
+// digits of the centred coefficient of A[i][j][c].
+__global__ void ajtai_limbs_kernel(const uint32_t *__restrict__ A, int8_t *__restrict__ limbs, uint32_t k,
+                                   uint32_t m, uint32_t npad, uint32_t q)
+{
+    const size_t total = (size_t)m * npad * 64;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t c = e & 63;
+        const size_t jc = e >> 6;
+        const uint32_t col = jc % npad;
+        const size_t j = jc / npad;
+        const uint32_t i = col >> 2, L = col & 3;
+        int8_t out = 0;
+        if (i < k) {
+            const uint32_t a = A[((size_t)i * m + j) * 64 + c];
+            int64_t x = a <= (q - 1) / 2 ? (int64_t)a : (int64_t)a - (int64_t)q;
+            int64_t d = 0;
+            for (uint32_t t = 0; t <= L; t++) {
+                d = ((x + 128) & 255) - 128;
+                x = (x - d) >> 8;
+            }
+            out = (int8_t)d;
+        }
+        limbs[e] = out;
+    }
+}
+
+bool ajtai_int8_supported(const lzx_ajtai_key *key) { return key->n == 64 && key->plan->q < I8_Q_LIMIT; }
+
+static uint32_t npad_of(size_t k) { return (uint32_t)(16 * ((4 * k + 15) / 16)); }
+
+lzx_status ajtai_build_limbs(lzx_ajtai_key *key)
+{
+    const uint32_t npad = npad_of(key->k);
+    key->limbs_bytes = key->m * npad * 64;
+    LZX_HIP(hipMalloc(&key->limbs_dev, key->limbs_bytes));
+    const uint32_t blocks = (uint32_t)std::min<size_t>((key->limbs_bytes + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(ajtai_limbs_kernel, dim3(blocks), dim3(256), 0, nullptr, key->A_dev, key->limbs_dev,
+                       (uint32_t)key->k, (uint32_t)key->m, npad, key->plan->q);
+    LZX_LAUNCH_CHECK("ajtai_limbs_kernel");
+    return LZX_OK;
+}
+
+lzx_status ajtai_commit_int8(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t W, uint32_t *out_dev,
+                             hipStream_t stream)
+{
+    if (W == 0) return LZX_OK;
+    if (!ajtai_int8_supported(key) || !key->limbs_dev) return LZX_UNSUPPORTED;
+    if ((uintptr_t)digits_dev % 4) return LZX_INVALID_ARGUMENT;
+    const uint32_t m = (uint32_t)key->m, k = (uint32_t)key->k, npad = npad_of(k);
+    const uint32_t q = key->plan->q;
+    const uint64_t barrett = key->plan->mod.barrett;
+    const uint32_t per_block = I8_WAVES * I8_ITW;
+    const uint32_t groups = (uint32_t)((W + per_block - 1) / per_block);
+    uint32_t nchunks = std::max<uint32_t>((m + I8_MAX_JCHUNK - 1) / I8_MAX_JCHUNK, (2048 + groups - 1) / groups);
+    nchunks = std::min<uint32_t>(nchunks, m);
+    const uint32_t jchunk = (m + nchunks - 1) / nchunks;
+    nchunks = (m + jchunk - 1) / jchunk;
+    const uint32_t padded = (nchunks + 7) / 8 * 8;
+    if ((uint64_t)padded * groups > 0x7FFFFFFFull) return LZX_UNSUPPORTED;
+
+    int32_t *partial = nullptr;
+    LZX_HIP(hipMallocAsync((void **)&partial, (size_t)nchunks * W * 64 * npad * sizeof(int32_t), stream));
+    lzx_status st = LZX_OK;
+    for (uint32_t col0 = 0; col0 < npad && st == LZX_OK; col0 += 64) {
+        const uint32_t nt = std::min<uint32_t>(4, (npad - col0) / 16);
+        const dim3 grid(padded * groups);
+        switch (nt) {
+        case 1:
+            hipLaunchKernelGGL(ajtai_i8_kernel<1>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
+                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+            break;
+        case 2:
+            hipLaunchKernelGGL(ajtai_i8_kernel<2>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
+                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+            break;
+        case 3:
+            hipLaunchKernelGGL(ajtai_i8_kernel<3>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
+                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+            break;
+        default:
+            hipLaunchKernelGGL(ajtai_i8_kernel<4>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
+                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+            break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = device_fail(e, "ajtai_i8_kernel");
+    }
+    if (st == LZX_OK) {
+        uint4 pw;
+        pw.x = 1 % q;
+        pw.y = (uint32_t)(256ULL % q);
+        pw.z = (uint32_t)(65536ULL % q);
+        pw.w = (uint32_t)(16777216ULL % q);
+        const size_t total = W * k * 64;
+        const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, 1 << 16);
+        hipLaunchKernelGGL(ajtai_i8_reduce_kernel, dim3(blocks), dim3(256), 0, stream, partial, out_dev, (uint32_t)W,
+                           k, npad, nchunks, q, barrett, pw);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = device_fail(e, "ajtai_i8_reduce_kernel");
+    }
+    (void)hipFreeAsync(partial, stream);
+    return st;
+}
+
+} // namespace lzx

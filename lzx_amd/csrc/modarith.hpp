@@ -122,6 +122,51 @@ LZX_HD uint32_t mulc(uint32_t a, uint32_t w, uint32_t wp, uint32_t q, uint32_t q
     }
 }
 
+LZX_HD uint32_t mul24(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul24(a, b);
+#else
+    return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+#endif
+}
+
+// Arithmetic classes of a u32 prime, chosen once per plan:
+//   ARITH_SHOUP        q < 2^31, generic Shoup (3 quarter-rate multiplies);
+//   ARITH_SHOUP_PRIME  q < 2^31 and q = c*2^k + 1 with k >= 8: the low word of
+//                      qhat*q is (mul24(qhat, c) << k) + qhat, because only
+//                      qhat*c mod 2^(32-k) <= 2^24 matters -- a full-rate
+//                      24-bit multiply replaces a quarter-rate one;
+//   ARITH_MONT         q > 2^31: Montgomery (4 multiplies).
+enum Arith { ARITH_SHOUP = 0, ARITH_SHOUP_PRIME = 1, ARITH_MONT = 2 };
+
+struct QConst {
+    uint32_t q, qinv, c, k;
+};
+
+template <int AR>
+LZX_HD uint32_t mulq(uint32_t a, uint32_t w, uint32_t wp, const QConst &Q)
+{
+    if (AR == ARITH_MONT) return mont_mul(a, w, Q.q, Q.qinv);
+    const uint32_t qh = mulhi32(a, wp);
+    const uint32_t qq = AR == ARITH_SHOUP_PRIME ? (mul24(qh, Q.c) << Q.k) + qh : qh * Q.q;
+    const uint32_t r = a * w - qq; // in [0, 2q)
+    const uint32_t t = r - Q.q;
+    return t < r ? t : r;
+}
+
+template <int AR>
+LZX_HD uint32_t addq(uint32_t a, uint32_t b, const QConst &Q)
+{
+    return AR == ARITH_MONT ? add_big(a, b, Q.q) : add_small(a, b, Q.q);
+}
+
+template <int AR>
+LZX_HD uint32_t subq(uint32_t a, uint32_t b, const QConst &Q)
+{
+    return AR == ARITH_MONT ? sub_big(a, b, Q.q) : sub_small(a, b, Q.q);
+}
+
 // Barrett reduction of any u64 (modulus.rs:65-76 restated with a 64-bit high
 // product): qhat in {floor(x/q)-1, floor(x/q)}, so <= 2 corrections.
 LZX_HD uint32_t reduce_u64(uint64_t x, uint32_t q, uint64_t barrett)

@@ -19,6 +19,7 @@
 //   ntt_lds_kernel  any n up to 2^13 and the incomplete transforms: a tile of
 //       polynomials in LDS, one butterfly per thread per level.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "ntt.hpp"
@@ -40,13 +41,13 @@ __device__ __forceinline__ uint32_t brv_dev(uint32_t x, uint32_t bits)
     return bits == 0 ? 0 : (__brev(x) >> (32 - bits));
 }
 
-template <int LOG_N, bool BIGQ, int MODE>
+template <int LOG_N, int AR, int MODE, bool DIRECT>
 __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
                                                      uint32_t blocks_per_plan)
 {
     constexpr uint32_t N = 1u << LOG_N;
     constexpr uint32_t S = N + 1;
-    __shared__ uint32_t tile[64 * S];
+    __shared__ uint32_t tile[DIRECT ? 1 : 64 * S];
 
     const uint32_t plan = blockIdx.x / blocks_per_plan;
     const uint32_t blk = blockIdx.x - plan * blocks_per_plan;
@@ -56,8 +57,29 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
     const uint32_t tid = threadIdx.x;
     uint32_t *__restrict__ g = data + ((size_t)plan * polys + p0) * N;
     const uint32_t nvec = np * N / 4;
+    uint32_t *row = tile + (DIRECT ? 0 : tid * S);
 
-    {
+    uint32_t a[N];
+    // coefficient (or evaluation) c of this lane's polynomial lands in slot
+    // a[c] (forward) or a[brv(c)] (inverse, bit-reversed working order)
+    if (DIRECT) {
+        if (tid < np) {
+            const uint4 *__restrict__ src = reinterpret_cast<const uint4 *>(g + (size_t)tid * N);
+#pragma unroll
+            for (uint32_t c4 = 0; c4 < N / 4; c4++) {
+                const uint4 v = src[c4];
+                const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (uint32_t e = 0; e < 4; e++) {
+                    const uint32_t c = 4 * c4 + e;
+                    a[MODE == NTT_FWD ? c : brv_bits(c, LOG_N)] = vv[e];
+                }
+            }
+        } else {
+#pragma unroll
+            for (uint32_t c = 0; c < N; c++) a[c] = 0;
+        }
+    } else {
         const uint4 *__restrict__ g4 = reinterpret_cast<const uint4 *>(g);
         for (uint32_t i = tid; i < nvec; i += 64) {
             const uint4 v = g4[i];
@@ -68,15 +90,13 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
             t[2] = v.z;
             t[3] = v.w;
         }
-    }
-    __syncthreads();
-
-    const uint32_t q = tab[0], qi = tab[1];
-    uint32_t a[N];
-    uint32_t *row = tile + tid * S;
-    if (MODE == NTT_FWD) {
+        __syncthreads();
 #pragma unroll
-        for (uint32_t c = 0; c < N; c++) a[c] = row[c];
+        for (uint32_t c = 0; c < N; c++) a[MODE == NTT_FWD ? c : brv_bits(c, LOG_N)] = row[c];
+    }
+
+    const QConst Q{tab[0], tab[1], tab[5], tab[6]};
+    if (MODE == NTT_FWD) {
 #pragma unroll
         for (uint32_t len = N / 2; len >= 1; len >>= 1) {
 #pragma unroll
@@ -86,18 +106,14 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
                 const uint32_t wp = tab[TabLayout::fwd_wp(N) + k];
 #pragma unroll
                 for (uint32_t j = s; j < s + len; j++) {
-                    const uint32_t t = mulc<BIGQ>(a[j + len], w, wp, q, qi);
+                    const uint32_t t = mulq<AR>(a[j + len], w, wp, Q);
                     const uint32_t u = a[j];
-                    a[j] = addm<BIGQ>(u, t, q);
-                    a[j + len] = subm<BIGQ>(u, t, q);
+                    a[j] = addq<AR>(u, t, Q);
+                    a[j + len] = subq<AR>(u, t, Q);
                 }
             }
         }
-#pragma unroll
-        for (uint32_t k = 0; k < N; k++) row[k] = a[brv_bits(k, LOG_N)];
     } else {
-#pragma unroll
-        for (uint32_t c = 0; c < N; c++) a[brv_bits(c, LOG_N)] = row[c];
 #pragma unroll
         for (uint32_t len = 1; len <= N / 2; len <<= 1) {
 #pragma unroll
@@ -108,8 +124,8 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
 #pragma unroll
                     for (uint32_t j = s; j < s + len; j++) {
                         const uint32_t u = a[j], v = a[j + len];
-                        a[j] = mulc<BIGQ>(addm<BIGQ>(u, v, q), nw, nwp, q, qi);
-                        a[j + len] = mulc<BIGQ>(subm<BIGQ>(u, v, q), lw, lwp, q, qi);
+                        a[j] = mulq<AR>(addq<AR>(u, v, Q), nw, nwp, Q);
+                        a[j + len] = mulq<AR>(subq<AR>(u, v, Q), lw, lwp, Q);
                     }
                 } else {
                     const uint32_t w = tab[TabLayout::inv_w(N) + k];
@@ -117,18 +133,34 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
 #pragma unroll
                     for (uint32_t j = s; j < s + len; j++) {
                         const uint32_t u = a[j], v = a[j + len];
-                        a[j] = addm<BIGQ>(u, v, q);
-                        a[j + len] = mulc<BIGQ>(subm<BIGQ>(u, v, q), w, wp, q, qi);
+                        a[j] = addq<AR>(u, v, Q);
+                        a[j + len] = mulq<AR>(subq<AR>(u, v, Q), w, wp, Q);
                     }
                 }
             }
         }
-#pragma unroll
-        for (uint32_t c = 0; c < N; c++) row[c] = a[c];
     }
-    __syncthreads();
 
-    {
+    // output slot k holds a[brv(k)] (forward: natural-order evaluations) or
+    // a[k] (inverse: natural-order coefficients)
+    if (DIRECT) {
+        if (tid < np) {
+            uint4 *__restrict__ dst = reinterpret_cast<uint4 *>(g + (size_t)tid * N);
+#pragma unroll
+            for (uint32_t c4 = 0; c4 < N / 4; c4++) {
+                uint32_t vv[4];
+#pragma unroll
+                for (uint32_t e = 0; e < 4; e++) {
+                    const uint32_t c = 4 * c4 + e;
+                    vv[e] = a[MODE == NTT_FWD ? brv_bits(c, LOG_N) : c];
+                }
+                dst[c4] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t c = 0; c < N; c++) row[c] = a[MODE == NTT_FWD ? brv_bits(c, LOG_N) : c];
+        __syncthreads();
         uint4 *__restrict__ g4 = reinterpret_cast<uint4 *>(g);
         for (uint32_t i = tid; i < nvec; i += 64) {
             const uint32_t w = 4 * i;
@@ -138,7 +170,7 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
     }
 }
 
-template <bool BIGQ, int MODE>
+template <int AR, int MODE>
 __global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
                                                       uint32_t blocks_per_plan, uint32_t log_n, uint32_t ppb,
                                                       uint32_t levels)
@@ -154,7 +186,7 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ dat
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     uint32_t *__restrict__ g = data + ((size_t)plan * polys + p0) * n;
     const uint32_t nwords = np * n;
-    const uint32_t q = tab[0], qi = tab[1];
+    const QConst Q{tab[0], tab[1], tab[5], tab[6]};
 
     for (uint32_t i = tid; i < nwords; i += nt) {
         const uint32_t p = i >> log_n, c = i & (n - 1);
@@ -164,7 +196,7 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ dat
         } else if (MODE == NTT_PARTIAL) {
             // ntt.rs:262-265: pre-scale by psi^c, then bit-reverse
             tile[p * n + brv_dev(c, log_n)] =
-                mulc<BIGQ>(x, tab[TabLayout::psi_w(n) + c], tab[TabLayout::psi_wp(n) + c], q, qi);
+                mulq<AR>(x, tab[TabLayout::psi_w(n) + c], tab[TabLayout::psi_wp(n) + c], Q);
         } else {
             tile[p * n + c] = x;
         }
@@ -179,11 +211,11 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ dat
                 const uint32_t grp = bi / len, j = bi % len;
                 const uint32_t k = n / (2 * len) + grp;
                 const uint32_t i0 = p * n + grp * 2 * len + j, i1 = i0 + len;
-                const uint32_t t = mulc<BIGQ>(tile[i1], tab[TabLayout::fwd_w(n) + k],
-                                              tab[TabLayout::fwd_wp(n) + k], q, qi);
+                const uint32_t t = mulq<AR>(tile[i1], tab[TabLayout::fwd_w(n) + k],
+                                              tab[TabLayout::fwd_wp(n) + k], Q);
                 const uint32_t u = tile[i0];
-                tile[i0] = addm<BIGQ>(u, t, q);
-                tile[i1] = subm<BIGQ>(u, t, q);
+                tile[i0] = addq<AR>(u, t, Q);
+                tile[i1] = subq<AR>(u, t, Q);
             }
             __syncthreads();
         }
@@ -196,12 +228,12 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ dat
                 const uint32_t i0 = p * n + grp * 2 * len + j, i1 = i0 + len;
                 const uint32_t u = tile[i0], v = tile[i1];
                 if (len == half) {
-                    tile[i0] = mulc<BIGQ>(addm<BIGQ>(u, v, q), tab[8], tab[9], q, qi);
-                    tile[i1] = mulc<BIGQ>(subm<BIGQ>(u, v, q), tab[10], tab[11], q, qi);
+                    tile[i0] = mulq<AR>(addq<AR>(u, v, Q), tab[8], tab[9], Q);
+                    tile[i1] = mulq<AR>(subq<AR>(u, v, Q), tab[10], tab[11], Q);
                 } else {
-                    tile[i0] = addm<BIGQ>(u, v, q);
-                    tile[i1] = mulc<BIGQ>(subm<BIGQ>(u, v, q), tab[TabLayout::inv_w(n) + k],
-                                          tab[TabLayout::inv_wp(n) + k], q, qi);
+                    tile[i0] = addq<AR>(u, v, Q);
+                    tile[i1] = mulq<AR>(subq<AR>(u, v, Q), tab[TabLayout::inv_w(n) + k],
+                                          tab[TabLayout::inv_wp(n) + k], Q);
                 }
             }
             __syncthreads();
@@ -219,10 +251,9 @@ __global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ dat
                 const uint32_t i0 = p * n + grp * len + j, i1 = i0 + hl;
                 const uint32_t e = stride * j;
                 const uint32_t u = tile[i0];
-                const uint32_t v = mulc<BIGQ>(tile[i1], tab[TabLayout::omg_w(n) + e], tab[TabLayout::omg_wp(n) + e],
-                                              q, qi);
-                tile[i0] = addm<BIGQ>(u, v, q);
-                tile[i1] = subm<BIGQ>(u, v, q);
+                const uint32_t v = mulq<AR>(tile[i1], tab[TabLayout::omg_w(n) + e], tab[TabLayout::omg_wp(n) + e], Q);
+                tile[i0] = addq<AR>(u, v, Q);
+                tile[i1] = subq<AR>(u, v, Q);
             }
             __syncthreads();
         }
@@ -247,51 +278,86 @@ __global__ void pointwise_mul_kernel(uint32_t *__restrict__ out, const uint32_t 
 // host side
 // ---------------------------------------------------------------------------
 
-template <int LOG_N, bool BIGQ>
+template <int LOG_N, int AR, bool DIRECT>
 static void launch_reg(NttMode mode, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys, uint32_t bpp,
                        hipStream_t stream)
 {
     if (mode == NTT_FWD)
-        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, BIGQ, NTT_FWD>), grid, dim3(64), 0, stream, data, ts, polys, bpp);
+        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, AR, NTT_FWD, DIRECT>), grid, dim3(64), 0, stream, data, ts, polys,
+                           bpp);
     else
-        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, BIGQ, NTT_INV>), grid, dim3(64), 0, stream, data, ts, polys, bpp);
+        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, AR, NTT_INV, DIRECT>), grid, dim3(64), 0, stream, data, ts, polys,
+                           bpp);
 }
 
-template <bool BIGQ>
-static void dispatch_reg(uint32_t log_n, NttMode mode, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys,
-                         uint32_t bpp, hipStream_t stream)
+template <int AR, bool DIRECT>
+static void dispatch_reg_n(uint32_t log_n, NttMode mode, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys,
+                           uint32_t bpp, hipStream_t stream)
 {
     switch (log_n) {
-    case 2: launch_reg<2, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
-    case 3: launch_reg<3, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
-    case 4: launch_reg<4, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
-    case 5: launch_reg<5, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
-    default: launch_reg<6, BIGQ>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 2: launch_reg<2, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 3: launch_reg<3, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 4: launch_reg<4, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 5: launch_reg<5, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
+    default: launch_reg<6, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
     }
 }
 
-template <bool BIGQ>
-static void dispatch_lds(NttMode mode, dim3 grid, size_t shmem, uint32_t *data, const TabSet &ts, uint32_t polys,
-                         uint32_t bpp, uint32_t log_n, uint32_t ppb, uint32_t levels, hipStream_t stream)
+static void dispatch_reg(int arith, bool direct, uint32_t log_n, NttMode mode, dim3 grid, uint32_t *data,
+                         const TabSet &ts, uint32_t polys, uint32_t bpp, hipStream_t stream)
+{
+    if (direct) {
+        if (arith == ARITH_MONT)
+            dispatch_reg_n<ARITH_MONT, true>(log_n, mode, grid, data, ts, polys, bpp, stream);
+        else if (arith == ARITH_SHOUP_PRIME)
+            dispatch_reg_n<ARITH_SHOUP_PRIME, true>(log_n, mode, grid, data, ts, polys, bpp, stream);
+        else
+            dispatch_reg_n<ARITH_SHOUP, true>(log_n, mode, grid, data, ts, polys, bpp, stream);
+    } else {
+        if (arith == ARITH_MONT)
+            dispatch_reg_n<ARITH_MONT, false>(log_n, mode, grid, data, ts, polys, bpp, stream);
+        else if (arith == ARITH_SHOUP_PRIME)
+            dispatch_reg_n<ARITH_SHOUP_PRIME, false>(log_n, mode, grid, data, ts, polys, bpp, stream);
+        else
+            dispatch_reg_n<ARITH_SHOUP, false>(log_n, mode, grid, data, ts, polys, bpp, stream);
+    }
+}
+
+template <int AR>
+static void dispatch_lds_mode(NttMode mode, dim3 grid, size_t shmem, uint32_t *data, const TabSet &ts,
+                              uint32_t polys, uint32_t bpp, uint32_t log_n, uint32_t ppb, uint32_t levels,
+                              hipStream_t stream)
 {
     switch (mode) {
     case NTT_FWD:
-        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_FWD>), grid, dim3(256), shmem, stream, data, ts, polys, bpp,
-                           log_n, ppb, levels);
+        hipLaunchKernelGGL((ntt_lds_kernel<AR, NTT_FWD>), grid, dim3(256), shmem, stream, data, ts, polys, bpp, log_n,
+                           ppb, levels);
         break;
     case NTT_INV:
-        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_INV>), grid, dim3(256), shmem, stream, data, ts, polys, bpp,
-                           log_n, ppb, levels);
+        hipLaunchKernelGGL((ntt_lds_kernel<AR, NTT_INV>), grid, dim3(256), shmem, stream, data, ts, polys, bpp, log_n,
+                           ppb, levels);
         break;
     case NTT_PARTIAL:
-        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_PARTIAL>), grid, dim3(256), shmem, stream, data, ts, polys,
-                           bpp, log_n, ppb, levels);
+        hipLaunchKernelGGL((ntt_lds_kernel<AR, NTT_PARTIAL>), grid, dim3(256), shmem, stream, data, ts, polys, bpp,
+                           log_n, ppb, levels);
         break;
     default:
-        hipLaunchKernelGGL((ntt_lds_kernel<BIGQ, NTT_COMPLETE>), grid, dim3(256), shmem, stream, data, ts, polys,
-                           bpp, log_n, ppb, levels);
+        hipLaunchKernelGGL((ntt_lds_kernel<AR, NTT_COMPLETE>), grid, dim3(256), shmem, stream, data, ts, polys, bpp,
+                           log_n, ppb, levels);
         break;
     }
+}
+
+static void dispatch_lds(int arith, NttMode mode, dim3 grid, size_t shmem, uint32_t *data, const TabSet &ts,
+                         uint32_t polys, uint32_t bpp, uint32_t log_n, uint32_t ppb, uint32_t levels,
+                         hipStream_t stream)
+{
+    if (arith == ARITH_MONT)
+        dispatch_lds_mode<ARITH_MONT>(mode, grid, shmem, data, ts, polys, bpp, log_n, ppb, levels, stream);
+    else if (arith == ARITH_SHOUP_PRIME)
+        dispatch_lds_mode<ARITH_SHOUP_PRIME>(mode, grid, shmem, data, ts, polys, bpp, log_n, ppb, levels, stream);
+    else
+        dispatch_lds_mode<ARITH_SHOUP>(mode, grid, shmem, data, ts, polys, bpp, log_n, ppb, levels, stream);
 }
 
 lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data, size_t polys, NttMode mode,
@@ -305,10 +371,10 @@ lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t 
     size_t i = 0;
     while (i < nplans) {
         // a launch covers consecutive plans of one arithmetic class
-        const bool bigq = plans[i]->bigq;
+        const int arith = plans[i]->arith;
         size_t cnt = 0;
         TabSet ts{};
-        while (i + cnt < nplans && cnt < (size_t)MAX_PLANS_PER_LAUNCH && plans[i + cnt]->bigq == bigq) {
+        while (i + cnt < nplans && cnt < (size_t)MAX_PLANS_PER_LAUNCH && plans[i + cnt]->arith == arith) {
             if (plans[i + cnt]->n != n) return LZX_INVALID_ARGUMENT;
             ts.p[cnt] = plans[i + cnt]->dev_tab;
             cnt++;
@@ -318,19 +384,13 @@ lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t 
         if (use_reg) {
             const uint32_t bpp = (uint32_t)((polys + 63) / 64);
             const dim3 grid((uint32_t)(bpp * cnt));
-            if (bigq)
-                dispatch_reg<true>(log_n, mode, grid, chunk, ts, (uint32_t)polys, bpp, stream);
-            else
-                dispatch_reg<false>(log_n, mode, grid, chunk, ts, (uint32_t)polys, bpp, stream);
+            dispatch_reg(arith, plans[i]->stage_direct, log_n, mode, grid, chunk, ts, (uint32_t)polys, bpp, stream);
         } else {
             const uint32_t ppb = std::max<uint32_t>(1u, 2048u >> log_n);
             const uint32_t bpp = (uint32_t)((polys + ppb - 1) / ppb);
             const dim3 grid((uint32_t)(bpp * cnt));
             const size_t shmem = (size_t)ppb * n * sizeof(uint32_t);
-            if (bigq)
-                dispatch_lds<true>(mode, grid, shmem, chunk, ts, (uint32_t)polys, bpp, log_n, ppb, levels, stream);
-            else
-                dispatch_lds<false>(mode, grid, shmem, chunk, ts, (uint32_t)polys, bpp, log_n, ppb, levels, stream);
+            dispatch_lds(arith, mode, grid, shmem, chunk, ts, (uint32_t)polys, bpp, log_n, ppb, levels, stream);
         }
         LZX_LAUNCH_CHECK("ntt kernel launch");
         i += cnt;
@@ -358,6 +418,9 @@ static void build_tables(lzx_ntt_plan *p)
     t[2] = n;
     t[3] = log_n;
     t[4] = p->bigq ? 1 : 0;
+    t[5] = p->prime_c;
+    t[6] = p->prime_k;
+    t[7] = (uint32_t)p->arith;
     put(8, 9, 0, p->n_inv);
     const uint32_t z1inv = host_pow(p->psi_inv, brv_bits(1, log_n), q);
     put(10, 11, 0, (uint32_t)((uint64_t)z1inv * p->n_inv % q));
@@ -395,7 +458,7 @@ static lzx_status run_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t le
         e = hipMemcpy(a_host, d, len * sizeof(uint32_t), hipMemcpyDeviceToHost);
         if (e != hipSuccess) st = device_fail(e, "hipMemcpy D2H");
     }
-    hipFree(d);
+    (void)hipFree(d);
     return st;
 }
 
@@ -418,6 +481,14 @@ extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint
     p->log_n = log_n;
     p->n = 1u << log_n;
     p->bigq = q >= (1u << 31);
+    p->prime_k = (uint32_t)__builtin_ctz(q - 1);
+    p->prime_c = (q - 1) >> p->prime_k;
+    p->arith = p->bigq ? ARITH_MONT : (p->prime_k >= 8 ? ARITH_SHOUP_PRIME : ARITH_SHOUP);
+    {
+        // staging variant of the register kernel; LZX_NTT_STAGE=lds|direct overrides
+        const char *stage = getenv("LZX_NTT_STAGE");
+        p->stage_direct = !(stage && strcmp(stage, "lds") == 0);
+    }
     p->mod.q = q;
     p->mod.qinv = host_inv32(q);
     p->mod.barrett = UINT64_MAX / q;
@@ -433,7 +504,7 @@ extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint
         e = hipMemcpy(p->dev_tab, p->host_tab.data(), p->host_tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         lzx_status st = device_fail(e, "ntt plan upload");
-        if (p->dev_tab) hipFree(p->dev_tab);
+        if (p->dev_tab) (void)hipFree(p->dev_tab);
         delete p;
         return st;
     }
@@ -444,7 +515,7 @@ extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint
 extern "C" void lzx_ntt_plan_destroy(lzx_ntt_plan *plan)
 {
     if (!plan) return;
-    if (plan->dev_tab) hipFree(plan->dev_tab);
+    if (plan->dev_tab) (void)hipFree(plan->dev_tab);
     delete plan;
 }
 
@@ -544,6 +615,6 @@ extern "C" lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_
         if (le != hipSuccess) st = device_fail(le, "pointwise_mul_kernel");
     }
     if (st == LZX_OK) st = ntt_launch(&plan, 1, out_dev, polys, NTT_INV, 0, s);
-    hipFreeAsync(tmp, s);
+    (void)hipFreeAsync(tmp, s);
     return st;
 }

@@ -50,6 +50,10 @@ struct lzx_ntt_plan {
     uint32_t q, two_adicity, generator, log_n, n;
     uint32_t psi, psi_inv, n_inv;
     bool bigq;
+    int arith;         // lzx::Arith class of q
+    uint32_t prime_c;  // q = prime_c * 2^prime_k + 1
+    uint32_t prime_k;
+    bool stage_direct; // register kernel loads/stores per lane (no LDS tile)
     lzx::Mod32 mod;
     std::vector<uint32_t> host_tab; // same words as dev_tab
     uint32_t *dev_tab;

@@ -1,0 +1,177 @@
+"""Host-side mirror of lattice-sumcheck / lattice-core's prover surface.
+
+  Transcript            crates/lattice-core/src/transcript.rs:36-238
+  DenseMle              crates/lattice-core/src/mle.rs:12-260 (device tables)
+  VirtualPolynomial     crates/lattice-sumcheck/src/virtual_poly.rs:48-160
+  prove / verify        crates/lattice-sumcheck/src/sumcheck.rs:62-371
+Factor tables live in HBM (torch int64 tensors holding canonical u64).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import device as dev
+from ._lib import check, lib
+
+GL_P = 0xFFFFFFFF00000001
+
+
+class Transcript:
+    """Transcript::new(protocol, budget) (transcript.rs:46)."""
+
+    def __init__(self, protocol: bytes = b"", query_budget: int = 1 << 20, _h=None):
+        if _h is not None:
+            self._h = _h
+            return
+        self._h = C.c_void_p()
+        check(lib().lzx_transcript_new(protocol, len(protocol), query_budget, C.byref(self._h)), "Transcript::new")
+
+    @classmethod
+    def new_default(cls, protocol: bytes) -> "Transcript":
+        return cls(protocol, 1 << 20)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().lzx_transcript_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def clone(self) -> "Transcript":
+        h = C.c_void_p()
+        check(lib().lzx_transcript_clone(self._h, C.byref(h)))
+        return Transcript(_h=h)
+
+    def append_message(self, label: bytes, message: bytes) -> None:
+        check(lib().lzx_transcript_append_message(self._h, label, len(label), message, len(message)),
+              "Transcript::append_message")
+
+    def append_field_slice(self, label: bytes, values) -> None:
+        v = np.ascontiguousarray(values, dtype=np.uint64)
+        check(lib().lzx_transcript_append_field_slice(self._h, label, len(label), v.ctypes.data, v.size),
+              "Transcript::append_field_slice")
+
+    def challenge_fields(self, label: bytes, n: int) -> np.ndarray:
+        out = np.zeros(max(n, 1), dtype=np.uint64)
+        check(lib().lzx_transcript_challenge_fields(self._h, label, len(label), n, out.ctypes.data),
+              "Transcript::challenge_fields")
+        return out[:n]
+
+    def challenge_field(self, label: bytes) -> int:
+        return int(self.challenge_fields(label, 1)[0])
+
+    def query_count(self) -> int:
+        return int(lib().lzx_transcript_query_count(self._h))
+
+
+@dataclass
+class VirtualPolynomial:
+    """P = sum_t c_t prod_{f in ids_t} F_f over device factor tables."""
+
+    num_vars: int
+    factors: list = field(default_factory=list)  # torch int64 tensors, 2^num_vars entries
+    terms: list = field(default_factory=list)    # (coeff, [factor ids])
+
+    def add_factor(self, table) -> int:
+        if table.numel() != 1 << self.num_vars:
+            raise ValueError("factor has the wrong number of evaluations")
+        self.factors.append(table)
+        return len(self.factors) - 1
+
+    def add_term(self, coeff: int, ids) -> None:
+        self.terms.append((int(coeff) % GL_P, list(ids)))
+
+    def max_degree(self) -> int:
+        return max((len(ids) for _, ids in self.terms), default=0)
+
+    def _ffi(self):
+        ptrs = (C.c_void_p * max(1, len(self.factors)))(*[t.data_ptr() for t in self.factors])
+        coeffs = np.array([c for c, _ in self.terms] or [0], dtype=np.uint64)
+        off = [0]
+        ids = []
+        for _, fs in self.terms:
+            ids.extend(fs)
+            off.append(len(ids))
+        off = np.array(off, dtype=np.uint32)
+        ids = np.array(ids or [0], dtype=np.uint32)
+        return ptrs, coeffs, off, ids
+
+    def sum_over_hypercube(self, stream=None) -> int:
+        ptrs, coeffs, off, ids = self._ffi()
+        out = C.c_uint64()
+        check(lib().lzx_vp_sum_over_hypercube(self.num_vars, len(self.factors), ptrs, len(self.terms),
+                                              coeffs.ctypes.data, off.ctypes.data, ids.ctypes.data, C.byref(out),
+                                              dev.stream_handle(stream)), "sum_over_hypercube")
+        return out.value
+
+
+@dataclass
+class SumcheckOutput:
+    rounds: np.ndarray        # [num_vars][degree+1]
+    challenges: np.ndarray    # [num_vars]
+    final_claim: int
+    factor_claims: np.ndarray  # [nfactors]
+
+
+def _prove(vp: VirtualPolynomial, claim: int, transcript, challenges, stream):
+    ptrs, coeffs, off, ids = vp._ffi()
+    d = max(vp.max_degree(), 1)
+    rounds = np.zeros(max(1, vp.num_vars * (d + 1)), dtype=np.uint64)
+    chals = np.zeros(max(1, vp.num_vars), dtype=np.uint64)
+    fclaims = np.zeros(max(1, len(vp.factors)), dtype=np.uint64)
+    fin = C.c_uint64()
+    deg = C.c_size_t()
+    if transcript is not None:
+        st = lib().lzx_sumcheck_prove(vp.num_vars, len(vp.factors), ptrs, len(vp.terms), coeffs.ctypes.data,
+                                      off.ctypes.data, ids.ctypes.data, claim, transcript.handle,
+                                      rounds.ctypes.data, chals.ctypes.data, C.byref(fin), fclaims.ctypes.data,
+                                      C.byref(deg), dev.stream_handle(stream))
+    else:
+        given = np.ascontiguousarray(challenges, dtype=np.uint64)
+        chals[: given.size] = given
+        st = lib().lzx_sumcheck_prove_with_challenges(vp.num_vars, len(vp.factors), ptrs, len(vp.terms),
+                                                      coeffs.ctypes.data, off.ctypes.data, ids.ctypes.data, claim,
+                                                      given.ctypes.data, rounds.ctypes.data, C.byref(fin),
+                                                      fclaims.ctypes.data, C.byref(deg), dev.stream_handle(stream))
+    check(st, "sumcheck::prove")
+    w = deg.value + 1
+    return SumcheckOutput(rounds[: vp.num_vars * w].reshape(vp.num_vars, w), chals[: vp.num_vars], fin.value,
+                          fclaims[: len(vp.factors)])
+
+
+def prove(vp: VirtualPolynomial, claim: int, transcript: Transcript, stream=None) -> SumcheckOutput:
+    """sumcheck::prove(&vp, claim, &mut transcript) (sumcheck.rs:62-162)."""
+    return _prove(vp, claim, transcript, None, stream)
+
+
+def prove_with_challenges(vp: VirtualPolynomial, claim: int, challenges, stream=None) -> SumcheckOutput:
+    """The same prover with caller-supplied challenges (no Fiat-Shamir)."""
+    return _prove(vp, claim, None, challenges, stream)
+
+
+def verify(rounds, num_vars: int, max_degree: int, claim: int, transcript: Transcript, expected_final=None):
+    """SumcheckProof::verify (sumcheck.rs:324-371): returns (point, final_claim)."""
+    r = np.ascontiguousarray(rounds, dtype=np.uint64)
+    nr = r.shape[0] if r.ndim == 2 else 0
+    rl = r.shape[1] if r.ndim == 2 else 0
+    point = np.zeros(max(1, nr), dtype=np.uint64)
+    fin = C.c_uint64()
+    check(lib().lzx_sumcheck_verify(r.ctypes.data, nr, rl, num_vars, max_degree, claim, transcript.handle,
+                                    1 if expected_final is not None else 0, expected_final or 0, point.ctypes.data,
+                                    C.byref(fin)), "SumcheckProof::verify")
+    return point[:nr], fin.value
+
+
+def eq_table(point, out=None, stream=None):
+    """field_simd.rs:391-407 eq table on device (variable 0 = MSB)."""
+    p = np.ascontiguousarray(point, dtype=np.uint64)
+    if out is None:
+        out = dev.empty(1 << p.size, np.uint64)
+    check(lib().lzx_eq_table(p.ctypes.data, p.size, dev.ptr(out), dev.stream_handle(stream)), "eq_table")
+    return out

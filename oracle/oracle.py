@@ -1,0 +1,449 @@
+"""ctypes view of the CPU oracle (oracle/liblzx_oracle.so) plus closed forms.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product package (lzx_amd) never imports it.
+
+Functions mirror the reference (idrees2516/lzx) items cited in
+oracle/lzx_oracle.c; the closed forms at the bottom are independent
+big-integer restatements of the mathematics used to pin the C code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_LIB_PATH = HERE / "liblzx_oracle.so"
+_lib = None
+
+GL_P = 0xFFFFFFFF00000001
+
+# Moduli used by the workloads: (q, two_adicity, generator).
+Q_32 = (3221225473, 30, 5)            # modulus.rs:48
+Q_12289 = (12289, 12, 11)             # modulus.rs:52
+Q_2013265921 = (2013265921, 27, 31)   # modulus.rs:55
+CRT_PRIMES = [(2013265921, 27, 31), (1811939329, 26, 13), (2113929217, 25, 5), (998244353, 23, 3)]
+Q_50_BUILD = (1125899437080577, 26, 5)  # ~50-bit prime chosen by this build (not in the reference)
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u8p = C.c_char_p
+
+
+class OrMod32(C.Structure):
+    _fields_ = [("q", C.c_uint32), ("two_adicity", C.c_uint32), ("generator", C.c_uint32), ("barrett", C.c_uint64)]
+
+
+def lib():
+    """Load (building first if absent) the oracle shared library."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        from oracle import build as _b  # noqa: WPS433
+        _b.build()
+    L = C.CDLL(str(_LIB_PATH))
+    vp = C.c_void_p
+    sz = C.c_size_t
+    u32, u64, i64 = C.c_uint32, C.c_uint64, C.c_int64
+    sig = {
+        "or_mod32_new": (OrMod32, [u32, u32, u32]),
+        "or_reduce_u64": (u32, [C.POINTER(OrMod32), u64]),
+        "or_reduce_i64": (u32, [C.POINTER(OrMod32), i64]),
+        "or_mul": (u32, [C.POINTER(OrMod32), u32, u32]),
+        "or_add": (u32, [C.POINTER(OrMod32), u32, u32]),
+        "or_sub": (u32, [C.POINTER(OrMod32), u32, u32]),
+        "or_neg": (u32, [C.POINTER(OrMod32), u32]),
+        "or_pow": (u32, [C.POINTER(OrMod32), u32, u64]),
+        "or_inv": (u32, [C.POINTER(OrMod32), u32]),
+        "or_root_of_unity": (u32, [C.POINTER(OrMod32), u32]),
+        "or_ntt_new": (C.c_int, [u32, u32, u32, u32, C.POINTER(vp)]),
+        "or_ntt_free": (None, [vp]),
+        "or_ntt_psi": (u32, [vp]),
+        "or_ntt_n_inv": (u32, [vp]),
+        "or_ntt_forward": (C.c_int, [vp, _u32p, sz]),
+        "or_ntt_inverse": (C.c_int, [vp, _u32p, sz]),
+        "or_ntt_forward_partial": (C.c_int, [vp, _u32p, sz, u32]),
+        "or_ntt_complete_partial": (C.c_int, [vp, _u32p, sz, u32]),
+        "or_ntt_forward_batch": (C.c_int, [vp, _u32p, sz]),
+        "or_ntt_inverse_batch": (C.c_int, [vp, _u32p, sz]),
+        "or_ring_mul": (C.c_int, [vp, _u32p, _u32p, _u32p]),
+        "or_keccak_f1600": (None, [_u64p]),
+        "or_shake256": (None, [_u8p, sz, C.c_void_p, sz]),
+        "or_sha3_256": (None, [_u8p, sz, C.c_void_p]),
+        "or_xof": (None, [_u8p, sz, _u8p, sz, C.c_void_p, sz]),
+        "or_uniform_from_seed": (None, [u32, u32, _u8p, sz, _u8p, sz, u64, _u32p]),
+        "or_sample_small_secret": (None, [u32, u32, sz, u32, _u8p, sz, _u32p]),
+        "or_ajtai_from_seed": (C.c_int, [u32, u32, u32, u32, sz, sz, _u8p, vp, vp]),
+        "or_ajtai_commit": (C.c_int, [vp, sz, sz, _u32p, _u32p, sz, _u32p]),
+        "or_ajtai_commit_naive": (C.c_int, [vp, sz, sz, _u32p, _u32p, sz, _u32p]),
+        "or_negacyclic_schoolbook": (None, [u32, u32, _u32p, _u32p, _u32p]),
+        "or_gl_from_u64": (u64, [u64]),
+        "or_gl_from_u128": (u64, [u64, u64]),
+        "or_gl_add": (u64, [u64, u64]),
+        "or_gl_sub": (u64, [u64, u64]),
+        "or_gl_neg": (u64, [u64]),
+        "or_gl_mul": (u64, [u64, u64]),
+        "or_gl_pow": (u64, [u64, u64]),
+        "or_gl_inverse": (u64, [u64]),
+        "or_transcript_new": (vp, [_u8p, sz, u64]),
+        "or_transcript_free": (None, [vp]),
+        "or_transcript_clone": (vp, [vp]),
+        "or_transcript_append_message": (C.c_int, [vp, _u8p, sz, _u8p, sz]),
+        "or_transcript_append_field_slice": (C.c_int, [vp, _u8p, sz, _u64p, sz]),
+        "or_transcript_challenge_fields": (C.c_int, [vp, _u8p, sz, sz, _u64p]),
+        "or_eq_table": (None, [_u64p, sz, _u64p]),
+        "or_bind_first_half": (None, [_u64p, sz, u64]),
+        "or_mle_evaluate": (u64, [_u64p, sz, _u64p]),
+        "or_interpolate_at": (u64, [_u64p, sz, u64]),
+        "or_vp_max_degree": (sz, [sz, _u32p]),
+        "or_vp_sum_over_hypercube": (u64, [sz, sz, C.POINTER(C.c_void_p), sz, _u64p, _u32p, _u32p]),
+        "or_sumcheck_prove": (C.c_int, [sz, sz, C.POINTER(C.c_void_p), sz, _u64p, _u32p, _u32p, u64, vp,
+                                        _u64p, _u64p, C.POINTER(u64), _u64p, C.POINTER(sz)]),
+        "or_sumcheck_verify": (C.c_int, [_u64p, sz, sz, sz, sz, u64, vp, C.c_int, u64, _u64p, C.POINTER(u64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+# ---------------------------------------------------------------------------
+# Modulus32 / NTT
+# ---------------------------------------------------------------------------
+
+class Mod32:
+    """Modulus32 (modulus.rs:22-162)."""
+
+    def __init__(self, q, two_adicity=1, generator=1):
+        self.m = lib().or_mod32_new(q, two_adicity, generator)
+        self.q = q
+
+    def reduce_u64(self, x):
+        return lib().or_reduce_u64(C.byref(self.m), x)
+
+    def reduce_i64(self, x):
+        return lib().or_reduce_i64(C.byref(self.m), x)
+
+    def mul(self, a, b):
+        return lib().or_mul(C.byref(self.m), a, b)
+
+    def add(self, a, b):
+        return lib().or_add(C.byref(self.m), a, b)
+
+    def sub(self, a, b):
+        return lib().or_sub(C.byref(self.m), a, b)
+
+    def inv(self, a):
+        return lib().or_inv(C.byref(self.m), a)
+
+    def root_of_unity(self, k):
+        return lib().or_root_of_unity(C.byref(self.m), k)
+
+
+class Ntt:
+    """NttTables (ntt.rs:16-342) restated; operates on uint32 numpy arrays."""
+
+    def __init__(self, q, two_adicity, generator, log_n):
+        h = C.c_void_p()
+        st = lib().or_ntt_new(q, two_adicity, generator, log_n, C.byref(h))
+        self.status = st
+        self.h = h if st == 0 else None
+        self.q, self.log_n, self.n = q, log_n, 1 << log_n
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.or_ntt_free(self.h)
+            self.h = None
+
+    @property
+    def psi(self):
+        return lib().or_ntt_psi(self.h)
+
+    @property
+    def n_inv(self):
+        return lib().or_ntt_n_inv(self.h)
+
+    def forward(self, a):
+        a = np.ascontiguousarray(a, dtype=np.uint32).copy()
+        st = lib().or_ntt_forward_batch(self.h, a, a.size // self.n) if a.size % self.n == 0 else 1
+        return st, a
+
+    def inverse(self, a):
+        a = np.ascontiguousarray(a, dtype=np.uint32).copy()
+        st = lib().or_ntt_inverse_batch(self.h, a, a.size // self.n) if a.size % self.n == 0 else 1
+        return st, a
+
+    def forward_partial(self, a, levels):
+        a = np.ascontiguousarray(a, dtype=np.uint32).copy()
+        flat = a.reshape(-1, self.n) if a.size % self.n == 0 else None
+        if flat is None:
+            return 1, a
+        for row in flat:
+            st = lib().or_ntt_forward_partial(self.h, row, self.n, levels)
+            if st:
+                return st, a
+        return 0, a
+
+    def complete_partial(self, a, already):
+        a = np.ascontiguousarray(a, dtype=np.uint32).copy()
+        flat = a.reshape(-1, self.n) if a.size % self.n == 0 else None
+        if flat is None:
+            return 1, a
+        for row in flat:
+            st = lib().or_ntt_complete_partial(self.h, row, self.n, already)
+            if st:
+                return st, a
+        return 0, a
+
+    def ring_mul(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        out = np.zeros_like(a)
+        for i in range(a.size // self.n):
+            o = np.zeros(self.n, dtype=np.uint32)
+            lib().or_ring_mul(self.h, np.ascontiguousarray(a.reshape(-1, self.n)[i]),
+                              np.ascontiguousarray(b.reshape(-1, self.n)[i]), o)
+            out.reshape(-1, self.n)[i] = o
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Keccak / XOF / sampling / Ajtai
+# ---------------------------------------------------------------------------
+
+def shake256(data: bytes, n: int) -> bytes:
+    buf = C.create_string_buffer(max(n, 1))
+    lib().or_shake256(data, len(data), buf, n)
+    return buf.raw[:n]
+
+
+def sha3_256(data: bytes) -> bytes:
+    buf = C.create_string_buffer(32)
+    lib().or_sha3_256(data, len(data), buf)
+    return buf.raw
+
+
+def keccak_f1600(state):
+    st = np.ascontiguousarray(state, dtype=np.uint64).copy()
+    lib().or_keccak_f1600(st)
+    return st
+
+
+def xof(domain: bytes, msg: bytes, n: int) -> bytes:
+    buf = C.create_string_buffer(max(n, 1))
+    lib().or_xof(domain, len(domain), msg, len(msg), buf, n)
+    return buf.raw[:n]
+
+
+def uniform_from_seed(q, log_n, domain: bytes, seed: bytes, index: int):
+    out = np.zeros(1 << log_n, dtype=np.uint32)
+    lib().or_uniform_from_seed(q, log_n, domain, len(domain), seed, len(seed), index, out)
+    return out
+
+
+def sample_small_secret(q, log_n, m, bound, seed: bytes):
+    out = np.zeros(m << log_n, dtype=np.uint32)
+    lib().or_sample_small_secret(q, log_n, m, bound, seed, len(seed), out)
+    return out.reshape(m, 1 << log_n)
+
+
+def ajtai_from_seed(mod, log_n, k, m, seed: bytes, want_A=True, want_Ahat=True):
+    """AjtaiPublicKey::from_seed (ajtai.rs:87-103): returns (A, Ahat), each (k*m, n)."""
+    q, adic, g = mod
+    n = 1 << log_n
+    A = np.zeros((k * m, n), dtype=np.uint32) if want_A else None
+    Ah = np.zeros((k * m, n), dtype=np.uint32) if want_Ahat else None
+    st = lib().or_ajtai_from_seed(q, adic, g, log_n, k, m, seed,
+                                  A.ctypes.data if want_A else None, Ah.ctypes.data if want_Ahat else None)
+    assert st == 0, st
+    return A, Ah
+
+
+def ajtai_commit(ntt: Ntt, k, m, Ahat, s):
+    """AjtaiPublicKey::commit (ajtai.rs:118-153); s is (m, n) canonical."""
+    s = np.ascontiguousarray(s, dtype=np.uint32)
+    out = np.zeros((k, ntt.n), dtype=np.uint32)
+    st = lib().or_ajtai_commit(ntt.h, k, m, np.ascontiguousarray(Ahat, dtype=np.uint32), s.reshape(-1),
+                               s.shape[0] if s.ndim > 1 else s.size // ntt.n, out.reshape(-1))
+    return st, out
+
+
+def ajtai_commit_naive(ntt: Ntt, k, m, A, s):
+    """commit_naive_reference (ajtai.rs:158-180)."""
+    s = np.ascontiguousarray(s, dtype=np.uint32)
+    out = np.zeros((k, ntt.n), dtype=np.uint32)
+    st = lib().or_ajtai_commit_naive(ntt.h, k, m, np.ascontiguousarray(A, dtype=np.uint32), s.reshape(-1),
+                                     s.shape[0], out.reshape(-1))
+    return st, out
+
+
+def negacyclic_schoolbook(q, a, b):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    out = np.zeros_like(a)
+    lib().or_negacyclic_schoolbook(q, a.size, a, b, out)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Goldilocks / transcript / sum-check
+# ---------------------------------------------------------------------------
+
+def gl_add(a, b): return lib().or_gl_add(a, b)
+def gl_sub(a, b): return lib().or_gl_sub(a, b)
+def gl_mul(a, b): return lib().or_gl_mul(a, b)
+def gl_neg(a): return lib().or_gl_neg(a)
+def gl_inverse(a): return lib().or_gl_inverse(a)
+def gl_from_u128(x): return lib().or_gl_from_u128(x >> 64, x & (2**64 - 1))
+
+
+class Transcript:
+    """Transcript (transcript.rs:36-238)."""
+
+    def __init__(self, protocol: bytes = b"", budget: int = 1 << 20, _h=None):
+        self.h = _h if _h is not None else lib().or_transcript_new(protocol, len(protocol), budget)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.or_transcript_free(self.h)
+            self.h = None
+
+    def clone(self):
+        return Transcript(_h=lib().or_transcript_clone(self.h))
+
+    def append_message(self, label: bytes, msg: bytes):
+        return lib().or_transcript_append_message(self.h, label, len(label), msg, len(msg))
+
+    def append_field_slice(self, label: bytes, vals):
+        v = np.ascontiguousarray(vals, dtype=np.uint64)
+        return lib().or_transcript_append_field_slice(self.h, label, len(label), v, v.size)
+
+    def challenge_fields(self, label: bytes, n: int):
+        out = np.zeros(max(n, 1), dtype=np.uint64)
+        st = lib().or_transcript_challenge_fields(self.h, label, len(label), n, out)
+        return st, out[:n]
+
+    def challenge_field(self, label: bytes):
+        st, out = self.challenge_fields(label, 1)
+        return st, int(out[0])
+
+
+def eq_table(point):
+    p = np.ascontiguousarray(point, dtype=np.uint64)
+    out = np.zeros(1 << p.size, dtype=np.uint64)
+    lib().or_eq_table(p, p.size, out)
+    return out
+
+
+def mle_evaluate(evals, point):
+    e = np.ascontiguousarray(evals, dtype=np.uint64)
+    p = np.ascontiguousarray(point, dtype=np.uint64)
+    return lib().or_mle_evaluate(e, p.size, p if p.size else np.zeros(1, np.uint64))
+
+
+def bind_first_half(evals, r):
+    e = np.ascontiguousarray(evals, dtype=np.uint64).copy()
+    lib().or_bind_first_half(e, e.size, r)
+    return e[: e.size // 2]
+
+
+def interpolate_at(evals, r):
+    e = np.ascontiguousarray(evals, dtype=np.uint64)
+    return lib().or_interpolate_at(e, e.size, r)
+
+
+def _terms_csr(terms):
+    coeffs = np.array([c for c, _ in terms] or [0], dtype=np.uint64)
+    off = [0]
+    ids = []
+    for _, fs in terms:
+        ids.extend(fs)
+        off.append(len(ids))
+    return coeffs, np.array(off, dtype=np.uint32), np.array(ids or [0], dtype=np.uint32)
+
+
+def vp_sum_over_hypercube(num_vars, factors, terms):
+    coeffs, off, ids = _terms_csr(terms)
+    facs = [np.ascontiguousarray(f, dtype=np.uint64) for f in factors]
+    ptrs = (C.c_void_p * max(len(facs), 1))(*[f.ctypes.data for f in facs])
+    return lib().or_vp_sum_over_hypercube(num_vars, len(facs), ptrs, len(terms), coeffs, off, ids)
+
+
+def sumcheck_prove(num_vars, factors, terms, claim, transcript: Transcript):
+    """sumcheck::prove (sumcheck.rs:62-162).  Returns (status, dict)."""
+    coeffs, off, ids = _terms_csr(terms)
+    facs = [np.ascontiguousarray(f, dtype=np.uint64) for f in factors]
+    ptrs = (C.c_void_p * max(len(facs), 1))(*[f.ctypes.data for f in facs])
+    d = max((len(fs) for _, fs in terms), default=1)
+    rounds = np.zeros(max(num_vars * (d + 1), 1), dtype=np.uint64)
+    chals = np.zeros(max(num_vars, 1), dtype=np.uint64)
+    fclaims = np.zeros(max(len(facs), 1), dtype=np.uint64)
+    fin = C.c_uint64(0)
+    dout = C.c_size_t(0)
+    st = lib().or_sumcheck_prove(num_vars, len(facs), ptrs, len(terms), coeffs, off, ids, claim, transcript.h,
+                                 rounds, chals, C.byref(fin), fclaims, C.byref(dout))
+    w = dout.value + 1
+    return st, {
+        "rounds": rounds[: num_vars * w].reshape(num_vars, w),
+        "challenges": chals[:num_vars],
+        "final_claim": fin.value,
+        "factor_claims": fclaims[: len(facs)],
+        "degree": dout.value,
+    }
+
+
+def sumcheck_verify(rounds, num_vars, max_degree, claim, transcript: Transcript, expected=None):
+    r = np.ascontiguousarray(rounds, dtype=np.uint64)
+    nr = r.shape[0] if r.ndim == 2 else 0
+    rl = r.shape[1] if r.ndim == 2 else 0
+    point = np.zeros(max(nr, 1), dtype=np.uint64)
+    fin = C.c_uint64(0)
+    st = lib().or_sumcheck_verify(r.reshape(-1) if r.size else np.zeros(1, np.uint64), nr, rl, num_vars,
+                                  max_degree, claim, transcript.h, 1 if expected is not None else 0,
+                                  expected or 0, point, C.byref(fin))
+    return st, point[:nr], fin.value
+
+
+# ---------------------------------------------------------------------------
+# Closed forms (pure Python big integers; small sizes only)
+# ---------------------------------------------------------------------------
+
+def closed_psi(q, g, log_n):
+    """psi = g^((q-1) >> (log_n+1)) (modulus.rs:153-155 with ntt.rs:52-54)."""
+    return pow(g, (q - 1) >> (log_n + 1), q)
+
+
+def closed_ntt(a, q, g, log_n):
+    """Direct evaluation out[k] = sum_i a_i psi^(i(2k+1)) mod q."""
+    n = 1 << log_n
+    psi = closed_psi(q, g, log_n)
+    return [sum(int(a[i]) * pow(psi, i * (2 * k + 1), q) for i in range(n)) % q for k in range(n)]
+
+
+def closed_negacyclic(a, b, q):
+    n = len(a)
+    out = [0] * n
+    for i in range(n):
+        for j in range(n):
+            v = int(a[i]) * int(b[j])
+            if i + j < n:
+                out[i + j] += v
+            else:
+                out[i + j - n] -= v
+    return [x % q for x in out]
+
+
+def splitmix64(seed):
+    """splitmix64 stream (the seeded generator the workloads use for inputs)."""
+    state = seed & (2**64 - 1)
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & (2**64 - 1)
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+        yield z ^ (z >> 31)

@@ -1,0 +1,177 @@
+"""GPU parity of the Ajtai commitment (NTT-domain and int8-MFMA paths).
+
+Mirrors crates/lattice-commitment/src/ajtai.rs:368-570: commit/open/verify,
+tamper rejection, norm bound, deterministic derivation, cached == naive
+across the reference's four shapes, linearity, zero elements; plus the
+batched device layouts and the full d=64, k=16, m=2^16, W=64 workload.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+VEC = json.loads((Path(__file__).resolve().parent / "golden" / "vectors.json").read_text())
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from lzx_amd import commitment as CM
+    from lzx_amd import device as D
+    from lzx_amd import ring as R
+    D.require_gpu()
+    return CM, D, R
+
+
+def _params(mods, log_n, k, m, bound, mod=O.Q_32):
+    CM, D, R = mods
+    return CM.AjtaiParams(R.RingConfig(R.Modulus32(*mod), log_n), k, m, bound)
+
+
+def test_key_derivation_matches_oracle(mods):
+    CM, D, R = mods
+    for log_n, k, m in [(3, 2, 2), (6, 3, 12)]:
+        pk = CM.AjtaiPublicKey.from_seed(_params(mods, log_n, k, m, 16), bytes([7]) * 32)
+        A, Ah = O.ajtai_from_seed(O.Q_32, log_n, k, m, bytes([7]) * 32)
+        assert np.array_equal(pk.matrix(), A)
+        assert np.array_equal(pk.matrix_ntt(), Ah)
+        pk2 = CM.AjtaiPublicKey.from_seed(_params(mods, log_n, k, m, 16), bytes([8]) * 32)
+        assert not np.array_equal(pk2.matrix()[0], A[0])
+
+
+def test_reference_shapes_match_golden_commitments(mods):
+    """ajtai.rs:490-506 shapes; expected values from schoolbook (gen_golden.py)."""
+    CM, D, R = mods
+    for v in VEC["ajtai"]:
+        pk = CM.AjtaiPublicKey.from_seed(_params(mods, v["log_n"], v["k"], v["m"], v["norm_bound"]),
+                                         bytes([v["key_seed_byte"]]) * 32)
+        s = O.sample_small_secret(v["q"], v["log_n"], v["m"], v["density"], v["secret_seed"].encode())
+        t = pk.commit(s)
+        assert t.rows_array.tolist() == v["commitment"], (v["log_n"], v["k"], v["m"])
+        pk.verify_opening(t, s)
+
+
+def test_open_verify_tamper_and_norm(mods):
+    """ajtai.rs:368-412."""
+    CM, D, R = mods
+    from lzx_amd._lib import LzxError
+    q = O.Q_32[0]
+    pk = CM.AjtaiPublicKey.from_seed(_params(mods, 4, 2, 3, 64), bytes([2]) * 32)
+    s = O.sample_small_secret(q, 4, 3, 8, b"secret")
+    t = pk.commit(s)
+    pk.verify_opening(t, s)
+    s2 = O.sample_small_secret(q, 4, 3, 8, b"other")
+    with pytest.raises(LzxError) as e:
+        pk.verify_opening(t, s2)
+    assert e.value.code == 6
+    bad = CM.AjtaiCommitment(t.rows_array.copy(), pk.params.ring)
+    bad.rows_array[0, 0] = (bad.rows_array[0, 0] + 1) % q
+    with pytest.raises(LzxError) as e:
+        pk.verify_opening(bad, s)
+    assert e.value.code == 6
+    pk8 = CM.AjtaiPublicKey.from_seed(_params(mods, 4, 2, 3, 8), bytes([3]) * 32)
+    big = np.zeros((3, 16), np.uint32)
+    big[:, 0] = 1000
+    with pytest.raises(LzxError) as e:
+        pk8.verify_opening(pk8.commit(big), big)
+    assert e.value.code == 5
+    with pytest.raises(LzxError) as e:
+        pk.commit(s[:2])
+    assert e.value.code == 4
+
+
+@pytest.mark.parametrize("shape", [(6, 3, 37, 11), (6, 16, 200, 9), (6, 5, 64, 3), (6, 1, 9, 1), (5, 4, 20, 6),
+                                   (7, 2, 10, 3)])
+def test_batched_commit_paths_match_oracle(mods, shape):
+    """Canonical-residue batch, digits via NTT path and digits via int8 MFMA
+    (d = 64 only) all equal the oracle's cached-NTT commit per witness."""
+    CM, D, R = mods
+    log_n, k, m, W = shape
+    n = 1 << log_n
+    q = O.Q_32[0]
+    pk = CM.AjtaiPublicKey.from_seed(_params(mods, log_n, k, m, 8), bytes([log_n + k]) * 32)
+    A, Ah = O.ajtai_from_seed(O.Q_32, log_n, k, m, bytes([log_n + k]) * 32)
+    rng = np.random.default_rng(k * 1000 + m)
+    digits = rng.integers(-8, 8, size=(W, m, n)).astype(np.int8)
+    digits[:, ::5] = 0  # whole zero ring elements (zero-skip)
+    resid = np.mod(digits.astype(np.int64), q).astype(np.uint32)
+    ot = O.Ntt(*O.Q_32, log_n)
+    want = np.stack([O.ajtai_commit(ot, k, m, Ah, resid[w])[1] for w in range(W)])
+    out = D.empty(W * k * n, np.uint32)
+    pk.commit_batch(D.to_device(resid), W, out)
+    assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, n), want), "residue batch"
+    dd = D.to_device(digits)
+    paths = [CM.COMMIT_NTT] + ([CM.COMMIT_INT8, CM.COMMIT_AUTO] if n == 64 else [])
+    for path in paths:
+        out = D.empty(W * k * n, np.uint32)
+        pk.commit_digits_batch(dd, W, out, path=path)
+        assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, n), want), f"digits path {path}"
+
+
+def test_int8_extreme_digits_and_limbs(mods):
+    """Digits at +-127 and A entries at the centred extremes (0, (q-1)/2,
+    (q+1)/2, q-1) exercise the limb split and int32 fold-back bound."""
+    CM, D, R = mods
+    q = O.Q_32[0]
+    k, m, W = 4, 2100, 2  # m above one 2048-element fold-back chunk
+    rng = np.random.default_rng(77)
+    ext = np.array([0, 1, (q - 1) // 2, (q + 1) // 2, q - 1, q - 2], dtype=np.uint64)
+    A = ext[rng.integers(0, ext.size, size=(k * m, 64))].astype(np.uint32)
+    pk = CM.AjtaiPublicKey.from_matrix(_params(mods, 6, k, m, 127), A)
+    digits = rng.choice(np.array([-127, 127, -1, 0], dtype=np.int8), size=(W, m, 64))
+    ot = O.Ntt(*O.Q_32, 6)
+    _, Ah = None, np.stack([ot.forward(A[i])[1] for i in range(k * m)])
+    resid = np.mod(digits.astype(np.int64), q).astype(np.uint32)
+    want = np.stack([O.ajtai_commit(ot, k, m, Ah, resid[w])[1] for w in range(W)])
+    for path in (CM.COMMIT_NTT, CM.COMMIT_INT8):
+        out = D.empty(W * k * 64, np.uint32)
+        pk.commit_digits_batch(D.to_device(digits), W, out, path=path)
+        assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, 64), want), f"path {path}"
+
+
+def test_full_workload_int8_equals_ntt_path(mods):
+    """d=64, k=16, m=2^16, W=64, A from seed [0x43;32], digits in [-8,8) with
+    P(0)=17/32: the two device paths agree bit for bit; witness 0 is checked
+    against the oracle; linearity holds across witnesses."""
+    CM, D, R = mods
+    from lzx_amd import workloads as Wk
+    q = O.Q_32[0]
+    k, m, W = 16, 1 << 16, 64
+    pk = CM.AjtaiPublicKey.from_seed(_params(mods, 6, k, m, 8), bytes([0x43]) * 32)
+    digits = Wk.small_digits(0xC3, W * m * 64).reshape(W, m, 64)
+    dd = D.to_device(digits)
+    o_ntt = D.empty(W * k * 64, np.uint32)
+    o_i8 = D.empty(W * k * 64, np.uint32)
+    pk.commit_digits_batch(dd, W, o_ntt, path=CM.COMMIT_NTT)
+    pk.commit_digits_batch(dd, W, o_i8, path=CM.COMMIT_INT8)
+    a = D.to_host(o_ntt, np.uint32).reshape(W, k, 64)
+    b = D.to_host(o_i8, np.uint32).reshape(W, k, 64)
+    assert np.array_equal(a, b)
+    # oracle spot check of witness 0
+    Ah = pk.matrix_ntt()
+    A = pk.matrix()
+    ot = O.Ntt(*O.Q_32, 6)
+    for idx in (0, 12345, k * m - 1):
+        assert np.array_equal(ot.forward(A[idx])[1], Ah[idx])
+    s0 = np.mod(digits[0].astype(np.int64), q).astype(np.uint32)
+    _, want0 = O.ajtai_commit(ot, k, m, Ah, s0)
+    assert np.array_equal(a[0], want0)
+    # linearity: commit(s1 + s2) == commit(s1) + commit(s2) (digits stay small)
+    s12 = (digits[1].astype(np.int16) + digits[2]).astype(np.int8)[None]
+    o12 = D.empty(k * 64, np.uint32)
+    pk.commit_digits_batch(D.to_device(np.ascontiguousarray(s12)), 1, o12, path=CM.COMMIT_INT8)
+    got = D.to_host(o12, np.uint32).reshape(k, 64)
+    assert np.array_equal(got, ((a[1].astype(np.uint64) + a[2]) % q).astype(np.uint32))
+
+
+def test_dimension_mismatch_batch(mods):
+    CM, D, R = mods
+    from lzx_amd._lib import LzxError
+    pk = CM.AjtaiPublicKey.from_seed(_params(mods, 6, 2, 8, 8), bytes([1]) * 32)
+    out = D.empty(2 * 64, np.uint32)
+    with pytest.raises(LzxError) as e:
+        pk.commit_digits_batch(D.to_device(np.zeros((1, 7, 64), np.int8)), 1, out, s_len=7)
+    assert e.value.code == 4

@@ -1,0 +1,297 @@
+"""CPU tests: pin the oracle against the reference's known answers, the
+first-principles vectors in tests/golden/ and closed-form invariants.
+
+A plausible slip in the oracle (a dropped term, wrong sign or index, a
+transposed operand, a wrong generator/psi) makes at least one of these fail:
+the NTT is checked against the direct odd-power DFT, Ajtai against schoolbook
+convolution of independently derived keys, sum-check rounds against hypercube
+enumeration of the multilinear extensions.
+"""
+import hashlib
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+G = Path(__file__).resolve().parent / "golden"
+KATS = json.loads((G / "reference_kats.json").read_text())
+VEC = json.loads((G / "vectors.json").read_text())
+
+
+# ---- reference known answers -------------------------------------------------
+
+@pytest.mark.parametrize("op", ["goldilocks_add", "goldilocks_sub", "goldilocks_mul"])
+def test_goldilocks_kats(op):
+    f = {"goldilocks_add": O.gl_add, "goldilocks_sub": O.gl_sub, "goldilocks_mul": O.gl_mul}[op]
+    for a, b, e in KATS[op]["cases"]:
+        assert f(a, b) == e
+
+
+def test_keccak_kats():
+    for msg, want in KATS["sha3_256"]["cases"]:
+        assert O.sha3_256(bytes.fromhex(msg)).hex() == want
+    for msg, want in KATS["shake256_32"]["cases"]:
+        assert O.shake256(bytes.fromhex(msg), 32).hex() == want
+
+
+def test_keccak_matches_hashlib_across_rate_boundaries():
+    rng = np.random.default_rng(5)
+    for ln in [0, 1, 135, 136, 137, 271, 272, 273, 1000]:
+        msg = rng.integers(0, 256, ln, dtype=np.uint8).tobytes()
+        assert O.shake256(msg, 500) == hashlib.shake_256(msg).digest(500)
+        assert O.sha3_256(msg) == hashlib.sha3_256(msg).digest()
+
+
+def test_reduce_i64_kats():
+    m = O.Mod32(KATS["reduce_i64_q32"]["q"])
+    for x, e in KATS["reduce_i64_q32"]["cases"]:
+        assert m.reduce_i64(x) == e
+
+
+def test_lagrange_kat():
+    k = KATS["lagrange"]
+    assert O.interpolate_at(k["evals"], k["r"]) == k["value"]
+
+
+def test_ntt_rejects_bad_geometry():
+    k = KATS["ntt_rejects"]
+    assert O.Ntt(k["q"], k["two_adicity"], 11, k["ok_log_n"]).status == 0
+    assert O.Ntt(k["q"], k["two_adicity"], 11, k["bad_log_n"]).status == 2
+    t = O.Ntt(k["q"], k["two_adicity"], 11, 8)
+    st, _ = t.forward(np.zeros(128, np.uint32) + 0)  # 128 is not a multiple of 256
+    assert st == 1
+
+
+# ---- Barrett vs % (modulus.rs:180-251 shape) ---------------------------------
+
+@pytest.mark.parametrize("mod", [O.Q_32, O.Q_12289, O.Q_2013265921])
+def test_barrett_matches_division(mod):
+    q = mod[0]
+    m = O.Mod32(q)
+    rng = np.random.default_rng(q)
+    cases = list(rng.integers(0, 2**63, 20000, dtype=np.uint64)) + [2**64 - 1, 2**64 - 2, (q - 1) ** 2]
+    cases += [(1 << k) + d for k in range(1, 64) for d in (-1, 0, 1)]
+    cases += [(k * q + d) % 2**64 for k in range(1, 300) for d in (-1, 0, 1)]
+    for x in cases:
+        x = int(x)
+        assert m.reduce_u64(x) == x % q
+
+
+# ---- first-principles vectors -----------------------------------------------
+
+def test_ntt_vectors_closed_form():
+    for v in VEC["ntt"]:
+        t = O.Ntt(v["q"], v["two_adicity"], v["generator"], v["log_n"])
+        assert t.psi == v["psi"]
+        st, f = t.forward(np.array(v["input"], dtype=np.uint32))
+        assert st == 0 and list(map(int, f)) == v["forward"], (v["q"], v["log_n"])
+        st, back = t.inverse(f)
+        assert list(map(int, back)) == v["input"]
+
+
+@pytest.mark.parametrize("mod", [O.Q_32, *O.CRT_PRIMES, O.Q_12289], ids=lambda m: str(m[0]))
+def test_ntt_direct_dft_random(mod):
+    q, adic, g = mod
+    rng = np.random.default_rng(1)
+    for log_n in range(0, 8):
+        if log_n + 1 > adic:
+            continue
+        a = rng.integers(0, q, 1 << log_n, dtype=np.uint64).astype(np.uint32)
+        st, f = O.Ntt(q, adic, g, log_n).forward(a)
+        assert list(map(int, f)) == O.closed_ntt(a, q, g, log_n)
+
+
+def test_ntt_product_is_negacyclic_convolution():
+    """ntt.rs:383-415: forward both, pointwise, inverse == schoolbook."""
+    q, adic, g = O.Q_32
+    rng = np.random.default_rng(2)
+    for log_n in (1, 4, 6):
+        n = 1 << log_n
+        a = rng.integers(0, q, n, dtype=np.uint64).astype(np.uint32)
+        b = rng.integers(0, q, n, dtype=np.uint64).astype(np.uint32)
+        t = O.Ntt(q, adic, g, log_n)
+        assert list(map(int, t.ring_mul(a, b))) == O.closed_negacyclic(a, b, q)
+        assert list(map(int, O.negacyclic_schoolbook(q, a, b))) == O.closed_negacyclic(a, b, q)
+
+
+def test_partial_then_complete_equals_full():
+    """ntt.rs:418-437, plus the partial state after the first level is the
+    closed form of one CT level on the psi-scaled, bit-reversed input."""
+    q, adic, g = O.Q_32
+    rng = np.random.default_rng(3)
+    log_n = 5
+    a = rng.integers(0, q, 1 << log_n, dtype=np.uint64).astype(np.uint32)
+    t = O.Ntt(q, adic, g, log_n)
+    _, full = t.forward(a)
+    for levels in range(log_n + 1):
+        _, p = t.forward_partial(a, levels)
+        _, c = t.complete_partial(p, levels)
+        assert np.array_equal(c, full)
+    # levels = 0 is exactly prescale + bit reversal
+    psi = O.closed_psi(q, g, log_n)
+    n = 1 << log_n
+    brv = [int(format(i, f"0{log_n}b")[::-1], 2) for i in range(n)]
+    _, p0 = t.forward_partial(a, 0)
+    assert [int(x) for x in p0] == [int(a[brv[i]]) * pow(psi, brv[i], q) % q for i in range(n)]
+
+
+def test_uniform_from_seed_vectors():
+    for v in VEC["uniform"]:
+        got = O.uniform_from_seed(v["q"], v["log_n"], v["domain"].encode(), bytes([v["seed_byte"]]) * 32, v["index"])
+        assert list(map(int, got)) == v["coeffs"], (v["log_n"], v["index"], v["seed_byte"])
+
+
+def test_small_secret_vectors():
+    for v in VEC["small_secret"]:
+        got = O.sample_small_secret(v["q"], v["log_n"], v["m"], v["bound"], v["seed"].encode())
+        assert got.tolist() == v["coeffs"]
+
+
+def test_ajtai_cached_and_naive_match_schoolbook_vectors():
+    """ajtai.rs:490-506 shapes: cached-NTT commit == naive == schoolbook."""
+    for v in VEC["ajtai"]:
+        q, log_n, k, m = v["q"], v["log_n"], v["k"], v["m"]
+        A, Ah = O.ajtai_from_seed(O.Q_32, log_n, k, m, bytes([v["key_seed_byte"]]) * 32)
+        s = O.sample_small_secret(q, log_n, m, v["density"], v["secret_seed"].encode())
+        t = O.Ntt(*O.Q_32, log_n)
+        st, fast = O.ajtai_commit(t, k, m, Ah, s)
+        assert st == 0 and fast.tolist() == v["commitment"]
+        st, naive = O.ajtai_commit_naive(t, k, m, A, s)
+        assert st == 0 and naive.tolist() == v["commitment"]
+
+
+def test_ajtai_linearity_and_zero_skip():
+    """ajtai.rs:467-487 (linearity) and :547-570 (zero elements)."""
+    q = O.Q_32[0]
+    log_n, k, m = 4, 2, 5
+    A, Ah = O.ajtai_from_seed(O.Q_32, log_n, k, m, bytes([6]) * 32)
+    t = O.Ntt(*O.Q_32, log_n)
+    s1 = O.sample_small_secret(q, log_n, m, 8, b"a")
+    s2 = O.sample_small_secret(q, log_n, m, 8, b"b")
+    ssum = ((s1.astype(np.uint64) + s2) % q).astype(np.uint32)
+    _, t1 = O.ajtai_commit(t, k, m, Ah, s1)
+    _, t2 = O.ajtai_commit(t, k, m, Ah, s2)
+    _, ts = O.ajtai_commit(t, k, m, Ah, ssum)
+    assert np.array_equal(ts, ((t1.astype(np.uint64) + t2) % q).astype(np.uint32))
+    z = np.zeros((m, 1 << log_n), np.uint32)
+    _, tz = O.ajtai_commit(t, k, m, Ah, z)
+    assert not tz.any()
+    st, _ = O.ajtai_commit(t, k, m, Ah, z[:-1])
+    assert st == 4  # DimensionMismatch
+
+
+def test_sumcheck_round_polys_match_hypercube_enumeration():
+    """Oracle round messages with fixed challenges == brute force g_j(t)."""
+    for v in VEC["sumcheck"]:
+        nv = v["num_vars"]
+        terms = [(c, fs) for c, fs in v["terms"]]
+        # replay the oracle's per-round evaluation with the given challenges
+        tables = [np.array(f, dtype=np.uint64) for f in v["factors"]]
+        assert O.vp_sum_over_hypercube(nv, tables, terms) == v["claim"]
+        d = max(len(fs) for _, fs in terms)
+        for j in range(nv):
+            # evaluate g_j(t) on the folded tables through the oracle's eval
+            rounds = _round_evals(nv - j, tables, terms, d)
+            assert rounds == v["rounds"][j], (nv, j)
+            tables = [O.bind_first_half(t, v["challenges"][j]) for t in tables]
+        assert [int(t[0]) for t in tables] == v["factor_claims"]
+
+
+def _round_evals(rem_vars, tables, terms, d):
+    import ctypes as C
+    coeffs, off, ids = O._terms_csr(terms)
+    L = O.lib()
+    L.or_sumcheck_round_evals.restype = None
+    L.or_sumcheck_round_evals.argtypes = [C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_size_t, C.c_void_p]
+    tabs = [np.ascontiguousarray(t) for t in tables]
+    ptrs = (C.c_void_p * len(tabs))(*[t.ctypes.data for t in tabs])
+    ev = np.zeros(d + 1, dtype=np.uint64)
+    L.or_sumcheck_round_evals(rem_vars, len(tabs), ptrs, len(terms), coeffs.ctypes.data, off.ctypes.data,
+                              ids.ctypes.data, d, ev.ctypes.data)
+    return [int(x) for x in ev]
+
+
+def test_sumcheck_prove_verify_roundtrip_and_rejections():
+    """sumcheck.rs:402-500: happy path, wrong claim, tampered round/final,
+    zero polynomial; the final claim equals P(r) by direct evaluation."""
+    nv = 5
+    rng = np.random.default_rng(9)
+    facs = [rng.integers(0, O.GL_P, 1 << nv, dtype=np.uint64) for _ in range(3)]
+    terms = [(3, [0, 1]), (5, [1, 2, 0]), (11, [2])]
+    claim = O.vp_sum_over_hypercube(nv, facs, terms)
+    st, out = O.sumcheck_prove(nv, facs, terms, claim, O.Transcript(b"lzx-sumcheck-test"))
+    assert st == 0
+    r = [int(x) for x in out["challenges"]]
+    vals = [O.mle_evaluate(f, r) for f in facs]
+    assert [int(x) for x in out["factor_claims"]] == vals
+    expect = 0
+    for c, fs in terms:
+        p = c
+        for f in fs:
+            p = O.gl_mul(p, vals[f])
+        expect = O.gl_add(expect, p)
+    assert out["final_claim"] == expect
+    st, point, fin = O.sumcheck_verify(out["rounds"], nv, 3, claim, O.Transcript(b"lzx-sumcheck-test"), expect)
+    assert st == 0 and [int(x) for x in point] == r and fin == expect
+    st, _, _ = O.sumcheck_verify(out["rounds"], nv, 3, O.gl_add(claim, 1), O.Transcript(b"lzx-sumcheck-test"))
+    assert st == 11
+    st, _, _ = O.sumcheck_verify(out["rounds"], nv, 3, claim, O.Transcript(b"lzx-sumcheck-test"),
+                                 O.gl_add(expect, 1))
+    assert st == 12
+    st, _ = O.sumcheck_prove(nv, facs, terms, O.gl_add(claim, 1), O.Transcript(b"lzx-sumcheck-test"))
+    assert st == 13
+    st, z = O.sumcheck_prove(3, [], [], 0, O.Transcript(b"lzx-sumcheck-test"))
+    assert st == 0 and z["rounds"].shape == (3, 2) and not z["rounds"].any()
+
+
+def test_transcript_determinism_and_budget():
+    """transcript.rs:244-306."""
+    def chal(proto, label, msg=b"hello"):
+        t = O.Transcript(proto)
+        t.append_message(b"a", msg)
+        return t.challenge_field(label)[1]
+    assert chal(b"test-proto", b"r") == chal(b"test-proto", b"r")
+    assert chal(b"test-proto", b"r") != chal(b"test-proto", b"other")
+    assert chal(b"test-proto", b"r") != chal(b"other-proto", b"r")
+    t = O.Transcript(b"budget", 3)
+    for lab in (b"a", b"b", b"c"):
+        assert t.challenge_field(lab)[0] == 0
+    assert t.challenge_field(b"d")[0] == 20
+    st, many = O.Transcript(b"many").challenge_fields(b"vec", 1000)
+    assert st == 0 and len(set(many.tolist())) == 1000 and (many < np.uint64(O.GL_P)).all()
+
+
+def test_transcript_squeeze_structure():
+    """The squeeze reads a permutation of the raw absorbed state (no padding):
+    restate it with the oracle's bare Keccak-f and compare."""
+    t = O.Transcript(b"proto")
+    t.append_message(b"x", b"data")
+    st, c = t.challenge_field(b"r")
+    # raw state: absorb the same bytes without padding
+    data = b"LZX1" + (5).to_bytes(4, "little") + b"proto"
+    data += (1).to_bytes(4, "little") + b"x" + (4).to_bytes(4, "little") + b"data"
+    data += b"CHAL" + (1).to_bytes(4, "little") + b"r" + (1).to_bytes(4, "little")
+    state = np.zeros(25, dtype=np.uint64)
+    for i, byte in enumerate(data):
+        assert i < 136
+        state[i // 8] ^= np.uint64(byte) << np.uint64(8 * (i % 8))
+    state = O.keccak_f1600(state)
+    cand = int(state[0])
+    assert cand < O.GL_P and c == cand
+
+
+def test_eq_table_closed_form():
+    rng = np.random.default_rng(4)
+    for m in range(0, 6):
+        pt = rng.integers(0, O.GL_P, m, dtype=np.uint64)
+        tab = O.eq_table(pt)
+        for b in range(1 << m):
+            w = 1
+            for i in range(m):
+                bit = (b >> (m - 1 - i)) & 1
+                w = w * (int(pt[i]) if bit else (1 - int(pt[i])) % O.GL_P) % O.GL_P
+            assert int(tab[b]) == w
