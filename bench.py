@@ -112,12 +112,123 @@ def cpu_baseline_ntt(polys: int = POLYS, reps: int = 5):
                       f"({med:.3f} s each)"}
 
 
+def _median_gpu_ms(torch, fn, reps, warmup=1):
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b))
+    return statistics.median(times)
+
+
+def bench_ajtai(reps: int = 5):
+    """Ajtai commit, d=64, k=16, m=2^16, W=64 small-digit witnesses (digits in
+    [-8, 8), P(0) = 17/32), A from seed [0x43; 32]: NTT-domain path vs int8
+    MFMA path.  2^26 ring-MACs per commit; the int8 path performs 2^40 int8
+    MACs (= 2.2e12 int8 ops)."""
+    import torch
+
+    from lzx_amd import commitment as CM
+    from lzx_amd import device as D
+    from lzx_amd import ring as R
+    from lzx_amd import workloads as W
+
+    k, m, nw = 16, 1 << 16, 64
+    t0 = time.perf_counter()
+    pk = CM.AjtaiPublicKey.from_seed(CM.AjtaiParams(R.RingConfig(R.Modulus32.Q_32, 6), k, m, 8), bytes([0x43]) * 32)
+    key_s = time.perf_counter() - t0
+    digits = D.to_device(W.small_digits(0xC3, nw * m * 64).reshape(nw, m, 64))
+    o1 = D.empty(nw * k * 64, np.uint32)
+    o2 = D.empty(nw * k * 64, np.uint32)
+    ms_ntt = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o1, path=CM.COMMIT_NTT), reps)
+    ms_i8 = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o2, path=CM.COMMIT_INT8), reps)
+    same = bool(torch.equal(o1, o2))
+    rmacs = k * m * nw
+    i8_ops = 2.0 * rmacs * 64 * 64 * 4
+    return {"workload": "Ajtai commit d=64 k=16 m=2^16 W=64, q=3221225473, digits in [-8,8)",
+            "ring_macs_per_commit": rmacs, "key_from_seed_s": round(key_s, 3),
+            "ntt_path": {"ms": round(ms_ntt, 4), "ring_macs_per_s": rmacs / (ms_ntt * 1e-3)},
+            "int8_path": {"ms": round(ms_i8, 4), "ring_macs_per_s": rmacs / (ms_i8 * 1e-3),
+                          "int8_tops": round(i8_ops / (ms_i8 * 1e-3) / 1e12, 1),
+                          "int8_mfma_frac_of_5000_tops": round(i8_ops / (ms_i8 * 1e-3) / 5e15, 4)},
+            "int8_over_ntt": round(ms_ntt / ms_i8, 3), "paths_bit_identical": same}
+
+
+def bench_sumcheck(reps: int = 3):
+    """Sum-check prove of eq(r,x)*(a(x)*b(x) - c(x)) over 2^24-entry Goldilocks
+    tables with the Fiat-Shamir transcript (24 rounds of host round trips).
+    Algorithmic bytes with 4 materialised tables: sum_j 4*8*(2^(24-j) + 2^(23-j))."""
+    import torch
+
+    from lzx_amd import device as D
+    from lzx_amd import sumcheck as S
+    from lzx_amd import workloads as W
+
+    nv = 24
+    N = 1 << nv
+    P = S.GL_P
+    vp = S.VirtualPolynomial(nv)
+    vp.add_factor(S.eq_table(W.uniform_gl(0xC4E, nv)))
+    for seed in (0xC4A, 0xC4B, 0xC4C):
+        vp.add_factor(D.to_device(W.uniform_gl(seed, N)))
+    vp.add_term(1, [0, 1, 2])
+    vp.add_term(P - 1, [0, 3])
+    claim = vp.sum_over_hypercube()
+    S.prove(vp, claim, S.Transcript(b"lzx-mi355x-sumcheck"))
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = S.prove(vp, claim, S.Transcript(b"lzx-mi355x-sumcheck"))
+        times.append(time.perf_counter() - t0)
+    sec = statistics.median(times)
+    alg_bytes = sum(4 * 8 * ((1 << (nv - j)) + (1 << (nv - 1 - j))) for j in range(nv))
+    return {"workload": "sum-check prove, 2^24-entry tables, eq*(a*b-c), degree 3, transcript challenges",
+            "ms_per_prove": round(sec * 1e3, 3), "alg_bytes": alg_bytes, "gbs": round(alg_bytes / sec / 1e9, 1),
+            "final_claim": int(out.final_claim)}
+
+
+def cpu_baseline_secondary():
+    """Bounded one-core oracle samples for the secondary workloads."""
+    from lzx_amd import workloads as W
+    from oracle import oracle as O
+
+    res = {}
+    k, m = 16, 1 << 12
+    _, Ah = O.ajtai_from_seed(O.Q_32, 6, k, m, bytes([0x43]) * 32, want_A=False)
+    s = np.mod(W.small_digits(0xC3, m * 64).reshape(m, 64).astype(np.int64), O.Q_32[0]).astype(np.uint32)
+    ot = O.Ntt(*O.Q_32, 6)
+    t0 = time.perf_counter()
+    O.ajtai_commit(ot, k, m, Ah, s)
+    dt = time.perf_counter() - t0
+    res["ajtai"] = {"ring_macs_per_s": k * m / dt, "cores": 1, "kind": "port",
+                    "sample": f"oracle cached-NTT commit (ajtai.rs:118-153), k=16, m=2^12, W=1: {dt:.3f} s"}
+    nv = 18
+    facs = [W.uniform_gl(s_, 1 << nv) for s_ in (0xC4E, 0xC4A, 0xC4B, 0xC4C)]
+    terms = [(1, [0, 1, 2]), (O.GL_P - 1, [0, 3])]
+    claim = O.vp_sum_over_hypercube(nv, facs, terms)
+    t0 = time.perf_counter()
+    O.sumcheck_prove(nv, facs, terms, claim, O.Transcript(b"lzx-mi355x-sumcheck"))
+    dt = time.perf_counter() - t0
+    res["sumcheck"] = {"ms_per_prove_2^18": round(dt * 1e3, 2), "cores": 1, "kind": "port",
+                       "sample": "oracle scalar sumcheck::prove (sumcheck.rs:62-162), 2^18 tables, same polynomial"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only-ntt", action="store_true", help="skip the secondary workloads (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -151,6 +262,10 @@ def main():
                      "inv_gbs": round(r["inv_gbs"], 2)},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_ntt(),
     }
+    if not args.only_ntt:
+        line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck()}
+        if not args.no_cpu:
+            line["secondary"]["cpu_baseline"] = cpu_baseline_secondary()
     print(json.dumps(line), flush=True)
 
 

@@ -75,18 +75,6 @@ LZX_HD uint32_t sub_big(uint32_t a, uint32_t b, uint32_t q)
     return a >= b ? a - b : a + (q - b);
 }
 
-template <bool BIGQ>
-LZX_HD uint32_t addm(uint32_t a, uint32_t b, uint32_t q)
-{
-    return BIGQ ? add_big(a, b, q) : add_small(a, b, q);
-}
-
-template <bool BIGQ>
-LZX_HD uint32_t subm(uint32_t a, uint32_t b, uint32_t q)
-{
-    return BIGQ ? sub_big(a, b, q) : sub_small(a, b, q);
-}
-
 // ---- multiplication by a precomputed constant -----------------------------
 
 LZX_HD uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t wp, uint32_t q)
@@ -108,51 +96,23 @@ LZX_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t q, uint32_t qinv)
     return th < mh ? r + q : r;
 }
 
-// a * w mod q with (w, wp) from the plan's constant tables: Shoup pair for
-// q < 2^31, (w*R mod q, unused) for q > 2^31.
-template <bool BIGQ>
-LZX_HD uint32_t mulc(uint32_t a, uint32_t w, uint32_t wp, uint32_t q, uint32_t qinv)
-{
-    if (BIGQ) {
-        (void)wp;
-        return mont_mul(a, w, q, qinv);
-    } else {
-        (void)qinv;
-        return shoup_mul(a, w, wp, q);
-    }
-}
-
-LZX_HD uint32_t mul24(uint32_t a, uint32_t b)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __umul24(a, b);
-#else
-    return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
-#endif
-}
-
 // Arithmetic classes of a u32 prime, chosen once per plan:
-//   ARITH_SHOUP        q < 2^31, generic Shoup (3 quarter-rate multiplies);
-//   ARITH_SHOUP_PRIME  q < 2^31 and q = c*2^k + 1 with k >= 8: the low word of
-//                      qhat*q is (mul24(qhat, c) << k) + qhat, because only
-//                      qhat*c mod 2^(32-k) <= 2^24 matters -- a full-rate
-//                      24-bit multiply replaces a quarter-rate one;
-//   ARITH_MONT         q > 2^31: Montgomery (4 multiplies).
-enum Arith { ARITH_SHOUP = 0, ARITH_SHOUP_PRIME = 1, ARITH_MONT = 2 };
+//   ARITH_SHOUP  q < 2^31: Shoup constant multiplication (3 multiplies) and
+//                canonical add/sub in 32 bits;
+//   ARITH_MONT   q > 2^31: Montgomery (4 multiplies), 33-bit add/sub.
+enum Arith { ARITH_SHOUP = 0, ARITH_MONT = 2 };
 
 struct QConst {
     uint32_t q, qinv, c, k;
 };
 
+// a * w mod q with (w, wp) from the plan's constant tables: the Shoup pair
+// for q < 2^31, (w * 2^32 mod q, unused) for q > 2^31.
 template <int AR>
 LZX_HD uint32_t mulq(uint32_t a, uint32_t w, uint32_t wp, const QConst &Q)
 {
     if (AR == ARITH_MONT) return mont_mul(a, w, Q.q, Q.qinv);
-    const uint32_t qh = mulhi32(a, wp);
-    const uint32_t qq = AR == ARITH_SHOUP_PRIME ? (mul24(qh, Q.c) << Q.k) + qh : qh * Q.q;
-    const uint32_t r = a * w - qq; // in [0, 2q)
-    const uint32_t t = r - Q.q;
-    return t < r ? t : r;
+    return shoup_mul(a, w, wp, Q.q);
 }
 
 template <int AR>

@@ -41,60 +41,15 @@ __device__ __forceinline__ uint32_t brv_dev(uint32_t x, uint32_t bits)
     return bits == 0 ? 0 : (__brev(x) >> (32 - bits));
 }
 
-template <int LOG_N, int AR, int MODE, bool DIRECT>
-__global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
-                                                     uint32_t blocks_per_plan)
+// Forward (merged psi twiddles, CT from natural order, output permuted to
+// natural-order evaluations) and inverse (GS, n^{-1} folded into the last
+// level) of one polynomial held in VGPRs.  a[] is the working order: the
+// forward reads coefficients into a[c]; the inverse reads evaluation k into
+// a[brv(k)].
+template <int LOG_N, int AR, int MODE>
+__device__ __forceinline__ void ntt_regs(uint32_t (&a)[1u << LOG_N], const uint32_t *__restrict__ tab)
 {
     constexpr uint32_t N = 1u << LOG_N;
-    constexpr uint32_t S = N + 1;
-    __shared__ uint32_t tile[DIRECT ? 1 : 64 * S];
-
-    const uint32_t plan = blockIdx.x / blocks_per_plan;
-    const uint32_t blk = blockIdx.x - plan * blocks_per_plan;
-    const uint32_t *__restrict__ tab = tabs.p[plan];
-    const uint32_t p0 = blk * 64;
-    const uint32_t np = min(64u, polys - p0);
-    const uint32_t tid = threadIdx.x;
-    uint32_t *__restrict__ g = data + ((size_t)plan * polys + p0) * N;
-    const uint32_t nvec = np * N / 4;
-    uint32_t *row = tile + (DIRECT ? 0 : tid * S);
-
-    uint32_t a[N];
-    // coefficient (or evaluation) c of this lane's polynomial lands in slot
-    // a[c] (forward) or a[brv(c)] (inverse, bit-reversed working order)
-    if (DIRECT) {
-        if (tid < np) {
-            const uint4 *__restrict__ src = reinterpret_cast<const uint4 *>(g + (size_t)tid * N);
-#pragma unroll
-            for (uint32_t c4 = 0; c4 < N / 4; c4++) {
-                const uint4 v = src[c4];
-                const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (uint32_t e = 0; e < 4; e++) {
-                    const uint32_t c = 4 * c4 + e;
-                    a[MODE == NTT_FWD ? c : brv_bits(c, LOG_N)] = vv[e];
-                }
-            }
-        } else {
-#pragma unroll
-            for (uint32_t c = 0; c < N; c++) a[c] = 0;
-        }
-    } else {
-        const uint4 *__restrict__ g4 = reinterpret_cast<const uint4 *>(g);
-        for (uint32_t i = tid; i < nvec; i += 64) {
-            const uint4 v = g4[i];
-            const uint32_t w = 4 * i;
-            uint32_t *t = tile + (w / N) * S + (w % N);
-            t[0] = v.x;
-            t[1] = v.y;
-            t[2] = v.z;
-            t[3] = v.w;
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t c = 0; c < N; c++) a[MODE == NTT_FWD ? c : brv_bits(c, LOG_N)] = row[c];
-    }
-
     const QConst Q{tab[0], tab[1], tab[5], tab[6]};
     if (MODE == NTT_FWD) {
 #pragma unroll
@@ -140,33 +95,121 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
             }
         }
     }
+}
 
-    // output slot k holds a[brv(k)] (forward: natural-order evaluations) or
-    // a[k] (inverse: natural-order coefficients)
-    if (DIRECT) {
-        if (tid < np) {
-            uint4 *__restrict__ dst = reinterpret_cast<uint4 *>(g + (size_t)tid * N);
+// LDS-staged register kernel (fallback for buffers that are not 16-byte
+// aligned): one 64-polynomial tile per block, padded LDS rows.
+template <int LOG_N, int AR, int MODE>
+__global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
+                                                     uint32_t blocks_per_plan)
+{
+    constexpr uint32_t N = 1u << LOG_N;
+    constexpr uint32_t S = N + 1;
+    __shared__ uint32_t tile[64 * S];
+    const uint32_t plan = blockIdx.x / blocks_per_plan;
+    const uint32_t blk = blockIdx.x - plan * blocks_per_plan;
+    const uint32_t *__restrict__ tab = tabs.p[plan];
+    const uint32_t p0 = blk * 64;
+    const uint32_t np = min(64u, polys - p0);
+    const uint32_t tid = threadIdx.x;
+    uint32_t *__restrict__ g = data + ((size_t)plan * polys + p0) * N;
+    const uint32_t nw = np * N;
+    for (uint32_t i = tid; i < nw; i += 64) tile[(i / N) * S + (i % N)] = g[i];
+    __syncthreads();
+    uint32_t a[N];
+    uint32_t *row = tile + tid * S;
 #pragma unroll
-            for (uint32_t c4 = 0; c4 < N / 4; c4++) {
-                uint32_t vv[4];
+    for (uint32_t c = 0; c < N; c++) a[MODE == NTT_FWD ? c : brv_bits(c, LOG_N)] = row[c];
+    ntt_regs<LOG_N, AR, MODE>(a, tab);
 #pragma unroll
-                for (uint32_t e = 0; e < 4; e++) {
-                    const uint32_t c = 4 * c4 + e;
-                    vv[e] = a[MODE == NTT_FWD ? brv_bits(c, LOG_N) : c];
-                }
-                dst[c4] = make_uint4(vv[0], vv[1], vv[2], vv[3]);
-            }
-        }
-    } else {
+    for (uint32_t c = 0; c < N; c++) row[c] = a[MODE == NTT_FWD ? brv_bits(c, LOG_N) : c];
+    __syncthreads();
+    for (uint32_t i = tid; i < nw; i += 64) g[i] = tile[(i / N) * S + (i % N)];
+}
+
+// Persistent, double-buffered register kernel (default for n in [4, 64]).
+// A tile is 64 polynomials of C = n/4 16-byte chunks.  Tiles stream from HBM
+// straight into LDS with global_load_lds_dwordx4 (fully coalesced: each
+// instruction moves 1 KiB of consecutive polynomials) while the wave runs the
+// butterflies of the previous tile in VGPRs.  Chunk c of polynomial p lives at
+// LDS slot C*p + ((c + p) mod C), so the transposed per-lane reads (lane =
+// polynomial) and writes are bank-conflict free.  Results go back through the
+// same slots and leave as coalesced 16-byte stores.  One wave per SIMD
+// (grid = 4 x CUs) keeps 32 KiB of LDS per wave.
+template <uint32_t C>
+__device__ __forceinline__ uint32_t tile_slot(uint32_t p, uint32_t c)
+{
+    return C * p + ((c + p) & (C - 1));
+}
+
+template <uint32_t C>
+__device__ __forceinline__ void tile_dma(const uint32_t *__restrict__ g, uint32_t np, uint32_t *T, uint32_t lane)
+{
 #pragma unroll
-        for (uint32_t c = 0; c < N; c++) row[c] = a[MODE == NTT_FWD ? brv_bits(c, LOG_N) : c];
+    for (uint32_t i = 0; i < C; i++) {
+        const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+        if (p < np) __builtin_amdgcn_global_load_lds(g + (p * C + c) * 4, T + 256 * i, 16, 0, 0);
+    }
+}
+
+template <int LOG_N, int AR, int MODE>
+__global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
+                                                     uint32_t blocks_per_plan, uint32_t total)
+{
+    constexpr uint32_t N = 1u << LOG_N, C = N / 4;
+    __shared__ __attribute__((aligned(16))) uint32_t lds[2][64 * N];
+    const uint32_t lane = threadIdx.x;
+    uint32_t b = blockIdx.x;
+    if (b >= total) return;
+    auto locate = [&](uint32_t bb, uint32_t &np, uint32_t &plan) {
+        plan = bb / blocks_per_plan;
+        const uint32_t p0 = (bb - plan * blocks_per_plan) * 64;
+        np = min(64u, polys - p0);
+        return data + ((size_t)plan * polys + p0) * N;
+    };
+    uint32_t np, plan;
+    uint32_t *g = locate(b, np, plan);
+    tile_dma<C>(g, np, lds[0], lane);
+    uint32_t cur = 0;
+    for (; b < total; b += gridDim.x) {
+        // vmcnt(0): this tile has landed in LDS.  The asm clobber makes the
+        // compiler treat LDS as rewritten (it does not model the DMA writes).
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        asm volatile("" ::: "memory");
         __syncthreads();
-        uint4 *__restrict__ g4 = reinterpret_cast<uint4 *>(g);
-        for (uint32_t i = tid; i < nvec; i += 64) {
-            const uint32_t w = 4 * i;
-            const uint32_t *t = tile + (w / N) * S + (w % N);
-            g4[i] = make_uint4(t[0], t[1], t[2], t[3]);
+        g = locate(b, np, plan);
+        const uint32_t *__restrict__ tab = tabs.p[plan];
+        uint32_t *T = lds[cur];
+        uint32_t a[N];
+#pragma unroll
+        for (uint32_t c4 = 0; c4 < C; c4++) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(T + 4 * tile_slot<C>(lane, c4));
+            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (uint32_t e = 0; e < 4; e++) a[MODE == NTT_FWD ? 4 * c4 + e : brv_bits(4 * c4 + e, LOG_N)] = vv[e];
         }
+        const uint32_t bn = b + gridDim.x;
+        if (bn < total) {
+            uint32_t npn, pn;
+            const uint32_t *gn = locate(bn, npn, pn);
+            tile_dma<C>(gn, npn, lds[cur ^ 1], lane);
+        }
+        ntt_regs<LOG_N, AR, MODE>(a, tab);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t c4 = 0; c4 < C; c4++) {
+            uint32_t vv[4];
+#pragma unroll
+            for (uint32_t e = 0; e < 4; e++) vv[e] = a[MODE == NTT_FWD ? brv_bits(4 * c4 + e, LOG_N) : 4 * c4 + e];
+            *reinterpret_cast<uint4 *>(T + 4 * tile_slot<C>(lane, c4)) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) {
+            const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+            if (p < np) *reinterpret_cast<uint4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const uint4 *>(T + 4 * s);
+        }
+        cur ^= 1;
     }
 }
 
@@ -278,49 +321,45 @@ __global__ void pointwise_mul_kernel(uint32_t *__restrict__ out, const uint32_t 
 // host side
 // ---------------------------------------------------------------------------
 
-template <int LOG_N, int AR, bool DIRECT>
-static void launch_reg(NttMode mode, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys, uint32_t bpp,
-                       hipStream_t stream)
+template <int LOG_N, int AR>
+static void launch_reg(NttMode mode, bool dma, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys,
+                       uint32_t bpp, uint32_t total, hipStream_t stream)
 {
-    if (mode == NTT_FWD)
-        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, AR, NTT_FWD, DIRECT>), grid, dim3(64), 0, stream, data, ts, polys,
-                           bpp);
-    else
-        hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, AR, NTT_INV, DIRECT>), grid, dim3(64), 0, stream, data, ts, polys,
-                           bpp);
+    if (dma) {
+        if (mode == NTT_FWD)
+            hipLaunchKernelGGL((ntt_dma_kernel<LOG_N, AR, NTT_FWD>), grid, dim3(64), 0, stream, data, ts, polys, bpp,
+                               total);
+        else
+            hipLaunchKernelGGL((ntt_dma_kernel<LOG_N, AR, NTT_INV>), grid, dim3(64), 0, stream, data, ts, polys, bpp,
+                               total);
+    } else {
+        if (mode == NTT_FWD)
+            hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, AR, NTT_FWD>), grid, dim3(64), 0, stream, data, ts, polys, bpp);
+        else
+            hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, AR, NTT_INV>), grid, dim3(64), 0, stream, data, ts, polys, bpp);
+    }
 }
 
-template <int AR, bool DIRECT>
-static void dispatch_reg_n(uint32_t log_n, NttMode mode, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys,
-                           uint32_t bpp, hipStream_t stream)
+template <int AR>
+static void dispatch_reg_n(uint32_t log_n, NttMode mode, bool dma, dim3 grid, uint32_t *data, const TabSet &ts,
+                           uint32_t polys, uint32_t bpp, uint32_t total, hipStream_t stream)
 {
     switch (log_n) {
-    case 2: launch_reg<2, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
-    case 3: launch_reg<3, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
-    case 4: launch_reg<4, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
-    case 5: launch_reg<5, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
-    default: launch_reg<6, AR, DIRECT>(mode, grid, data, ts, polys, bpp, stream); break;
+    case 2: launch_reg<2, AR>(mode, dma, grid, data, ts, polys, bpp, total, stream); break;
+    case 3: launch_reg<3, AR>(mode, dma, grid, data, ts, polys, bpp, total, stream); break;
+    case 4: launch_reg<4, AR>(mode, dma, grid, data, ts, polys, bpp, total, stream); break;
+    case 5: launch_reg<5, AR>(mode, dma, grid, data, ts, polys, bpp, total, stream); break;
+    default: launch_reg<6, AR>(mode, dma, grid, data, ts, polys, bpp, total, stream); break;
     }
 }
 
-static void dispatch_reg(int arith, bool direct, uint32_t log_n, NttMode mode, dim3 grid, uint32_t *data,
-                         const TabSet &ts, uint32_t polys, uint32_t bpp, hipStream_t stream)
+static void dispatch_reg(int arith, uint32_t log_n, NttMode mode, bool dma, dim3 grid, uint32_t *data,
+                         const TabSet &ts, uint32_t polys, uint32_t bpp, uint32_t total, hipStream_t stream)
 {
-    if (direct) {
-        if (arith == ARITH_MONT)
-            dispatch_reg_n<ARITH_MONT, true>(log_n, mode, grid, data, ts, polys, bpp, stream);
-        else if (arith == ARITH_SHOUP_PRIME)
-            dispatch_reg_n<ARITH_SHOUP_PRIME, true>(log_n, mode, grid, data, ts, polys, bpp, stream);
-        else
-            dispatch_reg_n<ARITH_SHOUP, true>(log_n, mode, grid, data, ts, polys, bpp, stream);
-    } else {
-        if (arith == ARITH_MONT)
-            dispatch_reg_n<ARITH_MONT, false>(log_n, mode, grid, data, ts, polys, bpp, stream);
-        else if (arith == ARITH_SHOUP_PRIME)
-            dispatch_reg_n<ARITH_SHOUP_PRIME, false>(log_n, mode, grid, data, ts, polys, bpp, stream);
-        else
-            dispatch_reg_n<ARITH_SHOUP, false>(log_n, mode, grid, data, ts, polys, bpp, stream);
-    }
+    if (arith == ARITH_MONT)
+        dispatch_reg_n<ARITH_MONT>(log_n, mode, dma, grid, data, ts, polys, bpp, total, stream);
+    else
+        dispatch_reg_n<ARITH_SHOUP>(log_n, mode, dma, grid, data, ts, polys, bpp, total, stream);
 }
 
 template <int AR>
@@ -354,8 +393,6 @@ static void dispatch_lds(int arith, NttMode mode, dim3 grid, size_t shmem, uint3
 {
     if (arith == ARITH_MONT)
         dispatch_lds_mode<ARITH_MONT>(mode, grid, shmem, data, ts, polys, bpp, log_n, ppb, levels, stream);
-    else if (arith == ARITH_SHOUP_PRIME)
-        dispatch_lds_mode<ARITH_SHOUP_PRIME>(mode, grid, shmem, data, ts, polys, bpp, log_n, ppb, levels, stream);
     else
         dispatch_lds_mode<ARITH_SHOUP>(mode, grid, shmem, data, ts, polys, bpp, log_n, ppb, levels, stream);
 }
@@ -383,8 +420,10 @@ lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t 
         const bool use_reg = (mode == NTT_FWD || mode == NTT_INV) && log_n >= 2 && log_n <= 6 && aligned;
         if (use_reg) {
             const uint32_t bpp = (uint32_t)((polys + 63) / 64);
-            const dim3 grid((uint32_t)(bpp * cnt));
-            dispatch_reg(arith, plans[i]->stage_direct, log_n, mode, grid, chunk, ts, (uint32_t)polys, bpp, stream);
+            const uint32_t total = (uint32_t)(bpp * cnt);
+            const bool dma = plans[i]->stage_dma;
+            const dim3 grid(dma ? std::min<uint32_t>(total, (uint32_t)plans[i]->persistent_waves) : total);
+            dispatch_reg(arith, log_n, mode, dma, grid, chunk, ts, (uint32_t)polys, bpp, total, stream);
         } else {
             const uint32_t ppb = std::max<uint32_t>(1u, 2048u >> log_n);
             const uint32_t bpp = (uint32_t)((polys + ppb - 1) / ppb);
@@ -483,12 +522,7 @@ extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint
     p->bigq = q >= (1u << 31);
     p->prime_k = (uint32_t)__builtin_ctz(q - 1);
     p->prime_c = (q - 1) >> p->prime_k;
-    p->arith = p->bigq ? ARITH_MONT : (p->prime_k >= 8 ? ARITH_SHOUP_PRIME : ARITH_SHOUP);
-    {
-        // staging variant of the register kernel; LZX_NTT_STAGE=lds|direct overrides
-        const char *stage = getenv("LZX_NTT_STAGE");
-        p->stage_direct = !(stage && strcmp(stage, "lds") == 0);
-    }
+    p->arith = p->bigq ? ARITH_MONT : ARITH_SHOUP;
     p->mod.q = q;
     p->mod.qinv = host_inv32(q);
     p->mod.barrett = UINT64_MAX / q;
@@ -496,9 +530,17 @@ extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint
     p->psi = host_pow(generator, ((uint64_t)q - 1) >> (log_n + 1), q);
     p->psi_inv = host_pow(p->psi, (uint64_t)q - 2, q);
     p->n_inv = host_pow(p->n, (uint64_t)q - 2, q);
+    {
+        // LZX_NTT_STAGE=lds selects the LDS-staged fallback kernel everywhere
+        const char *stage = getenv("LZX_NTT_STAGE");
+        p->stage_dma = !(stage && strcmp(stage, "lds") == 0);
+    }
     build_tables(p);
     p->dev_tab = nullptr;
     hipError_t e = hipGetDevice(&p->device);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    p->persistent_waves = 4 * std::max(cus, 1); // one wave per SIMD
     if (e == hipSuccess) e = hipMalloc(&p->dev_tab, p->host_tab.size() * sizeof(uint32_t));
     if (e == hipSuccess)
         e = hipMemcpy(p->dev_tab, p->host_tab.data(), p->host_tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);

@@ -53,7 +53,8 @@ struct lzx_ntt_plan {
     int arith;         // lzx::Arith class of q
     uint32_t prime_c;  // q = prime_c * 2^prime_k + 1
     uint32_t prime_k;
-    bool stage_direct; // register kernel loads/stores per lane (no LDS tile)
+    bool stage_dma;       // persistent DMA-staged register kernel (else LDS-staged blocks)
+    int persistent_waves; // grid of the persistent kernel: one wave per SIMD
     lzx::Mod32 mod;
     std::vector<uint32_t> host_tab; // same words as dev_tab
     uint32_t *dev_tab;
