@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Ajtai ring-MACs/s (int8-MFMA vs NTT path, bit-exact) + batched NTT GB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per forward launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (each scaled by the reading for a 64 MiB copy kernel): 65,834 KiB
+# read + 65,536 KiB written; the rocprofv3 summary is kept under profiles/.
+NTT_FWD_TRAFFIC_BYTES = (65834 + 65536) * 1024
 POLYS = 1 << 16
 LOG_N = 6
 N = 1 << LOG_N
@@ -229,6 +233,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-ntt", action="store_true", help="skip the secondary workloads (profiling runs)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 64 MiB device copy (counter calibration for --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -239,6 +245,12 @@ def main():
     from lzx_amd import device as D
     D.require_gpu()
     r = bench_ntt(args.steps, args.warmup)
+    if args.copy_calib:
+        src = torch.zeros(1 << 24, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)  # 64 MiB read + 64 MiB write per dispatch
+        torch.cuda.synchronize()
     line = {
         "metric": METRIC,
         "value": round(r["gbs"], 2),
@@ -255,9 +267,10 @@ def main():
         "config": {"workload": "batched negacyclic NTT forward+inverse, 4 CRT primes x 2^16 polys x d=64, u32",
                    "polys_per_prime": POLYS, "degree": N, "primes": [m.q for m in __import__(
                        "lzx_amd.ring", fromlist=["CRT_PRIMES"]).CRT_PRIMES], "roundtrip_exact": r["roundtrip_exact"]},
-        "roofline": {"bound": "hbm", "kernel": "ntt_reg_kernel<6,false,FWD>",
+        "roofline": {"bound": "hbm", "kernel": "ntt_dma_kernel<6,ARITH_SHOUP,NTT_FWD>",
                      "achieved": round(r["fwd_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["fwd_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(r["fwd_gbs"] / HBM_PEAK_GBS, 4), "traffic": NTT_FWD_TRAFFIC_BYTES,
+                     "alg_bytes": r["bytes_per_pass"],
                      "fwd_us": round(r["fwd_ms"] * 1e3, 3), "inv_us": round(r["inv_ms"] * 1e3, 3),
                      "inv_gbs": round(r["inv_gbs"], 2)},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_ntt(),

@@ -9,13 +9,14 @@
 //            omega = psi^2), whose intermediate state is part of the contract.
 //
 // Kernels
-//   ntt_reg_kernel  n in [4, 64]: one polynomial per lane, all log n levels in
-//       VGPRs (no cross-lane traffic), twiddles wave-uniform (scalar loads).
-//       A 64-polynomial tile is staged through LDS (padded row stride n+1, so
-//       the per-lane row reads are bank-conflict free) so HBM sees 16-byte
-//       coalesced loads/stores.  psi is folded into the twiddles; the forward
-//       permutes to natural order on the way back to LDS and the inverse folds
-//       n^{-1} into its last level.
+//   ntt_dma_kernel  n in [4, 64] (default): one polynomial per lane, all
+//       log n levels in VGPRs (no cross-lane traffic), twiddles wave-uniform
+//       (scalar loads); persistent waves stream 64-polynomial tiles through a
+//       double-buffered LDS tile fed by global_load_lds (see below).  psi is
+//       folded into the twiddles; the forward permutes to natural order on the
+//       way back to LDS and the inverse folds n^{-1} into its last level.
+//   ntt_reg_kernel  the same butterflies with a plain LDS-staged tile per
+//       block, for buffers that are not 16-byte aligned.
 //   ntt_lds_kernel  any n up to 2^13 and the incomplete transforms: a tile of
 //       polynomials in LDS, one butterfly per thread per level.
 #include <algorithm>
@@ -142,14 +143,24 @@ __device__ __forceinline__ uint32_t tile_slot(uint32_t p, uint32_t c)
     return C * p + ((c + p) & (C - 1));
 }
 
-template <uint32_t C>
-__device__ __forceinline__ void tile_dma(const uint32_t *__restrict__ g, uint32_t np, uint32_t *T, uint32_t lane)
+template <uint32_t C, bool PARTIAL>
+__device__ __forceinline__ void tile_dma_n(const uint32_t *__restrict__ g, uint32_t np, uint32_t *T, uint32_t lane)
 {
 #pragma unroll
     for (uint32_t i = 0; i < C; i++) {
         const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
-        if (p < np) __builtin_amdgcn_global_load_lds(g + (p * C + c) * 4, T + 256 * i, 16, 0, 0);
+        if (!PARTIAL || p < np) __builtin_amdgcn_global_load_lds(g + (p * C + c) * 4, T + 256 * i, 16, 0, 0);
     }
+}
+
+// full tiles skip the per-lane bound (an exec-masked branch per instruction)
+template <uint32_t C>
+__device__ __forceinline__ void tile_dma(const uint32_t *__restrict__ g, uint32_t np, uint32_t *T, uint32_t lane)
+{
+    if (np == 64)
+        tile_dma_n<C, false>(g, np, T, lane);
+    else
+        tile_dma_n<C, true>(g, np, T, lane);
 }
 
 template <int LOG_N, int AR, int MODE>
@@ -159,8 +170,12 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
     constexpr uint32_t N = 1u << LOG_N, C = N / 4;
     __shared__ __attribute__((aligned(16))) uint32_t lds[2][64 * N];
     const uint32_t lane = threadIdx.x;
-    uint32_t b = blockIdx.x;
-    if (b >= total) return;
+    // each wave owns a contiguous range of tiles, so it stays on one modulus
+    // (one twiddle table in the scalar cache) for most of its tiles
+    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+    uint32_t b = blockIdx.x * per;
+    const uint32_t b_end = min(total, b + per);
+    if (b >= b_end) return;
     auto locate = [&](uint32_t bb, uint32_t &np, uint32_t &plan) {
         plan = bb / blocks_per_plan;
         const uint32_t p0 = (bb - plan * blocks_per_plan) * 64;
@@ -171,7 +186,7 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
     uint32_t *g = locate(b, np, plan);
     tile_dma<C>(g, np, lds[0], lane);
     uint32_t cur = 0;
-    for (; b < total; b += gridDim.x) {
+    for (; b < b_end; b++) {
         // vmcnt(0): this tile has landed in LDS.  The asm clobber makes the
         // compiler treat LDS as rewritten (it does not model the DMA writes).
         __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -188,8 +203,8 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
 #pragma unroll
             for (uint32_t e = 0; e < 4; e++) a[MODE == NTT_FWD ? 4 * c4 + e : brv_bits(4 * c4 + e, LOG_N)] = vv[e];
         }
-        const uint32_t bn = b + gridDim.x;
-        if (bn < total) {
+        const uint32_t bn = b + 1;
+        if (bn < b_end) {
             uint32_t npn, pn;
             const uint32_t *gn = locate(bn, npn, pn);
             tile_dma<C>(gn, npn, lds[cur ^ 1], lane);
@@ -204,10 +219,19 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
             *reinterpret_cast<uint4 *>(T + 4 * tile_slot<C>(lane, c4)) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
         }
         __syncthreads();
+        if (np == 64) {
 #pragma unroll
-        for (uint32_t i = 0; i < C; i++) {
-            const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
-            if (p < np) *reinterpret_cast<uint4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const uint4 *>(T + 4 * s);
+            for (uint32_t i = 0; i < C; i++) {
+                const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+                *reinterpret_cast<uint4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const uint4 *>(T + 4 * s);
+            }
+        } else {
+#pragma unroll
+            for (uint32_t i = 0; i < C; i++) {
+                const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+                if (p < np)
+                    *reinterpret_cast<uint4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const uint4 *>(T + 4 * s);
+            }
         }
         cur ^= 1;
     }
