@@ -50,14 +50,26 @@ __device__ __forceinline__ uint32_t neg_bytes(uint32_t x)
     return ((n & 0x7f7f7f7fu) + 0x01010101u) ^ (n & 0x80808080u);
 }
 
+// Combines the four limb accumulators of one output held by a lane quad
+// (lane L of the quad has limb L) into sum_L 256^L acc_L on lane L = 0.
+__device__ __forceinline__ int64_t combine_limbs(int32_t acc)
+{
+    const int64_t x = acc;
+    const int64_t y = x + ((int64_t)__shfl_down(acc, 1) << 8);          // L = 0, 2
+    const int64_t hi = __shfl_down(y, 2);
+    return y + (hi << 16);                                                // L = 0
+}
+
 template <int NT>
 __global__ __launch_bounds__(256) void ajtai_i8_kernel(const int8_t *__restrict__ digits,
                                                        const int8_t *__restrict__ limbs,
-                                                       int32_t *__restrict__ partial, uint32_t W, uint32_t m,
-                                                       uint32_t npad, uint32_t col0, uint32_t jchunk,
-                                                       uint32_t nchunks, uint32_t groups)
+                                                       uint32_t *__restrict__ partial, uint32_t W, uint32_t m,
+                                                       uint32_t k, uint32_t npad, uint32_t col0, uint32_t jchunk,
+                                                       uint32_t nchunks, uint32_t groups, uint32_t q,
+                                                       uint64_t barrett)
 {
-    __shared__ uint32_t Rl[I8_WAVES][I8_ITW][36];
+    constexpr int JB = 16; // ring elements per digit batch (one 1 KiB load per witness)
+    __shared__ __attribute__((aligned(16))) uint32_t Rs[I8_WAVES][I8_ITW][JB][36];
     const uint32_t b = blockIdx.x;
     const uint32_t chunk = (b / (8 * groups)) * 8 + (b % 8);
     const uint32_t grp = (b / 8) % groups;
@@ -68,6 +80,11 @@ __global__ __launch_bounds__(256) void ajtai_i8_kernel(const int8_t *__restrict_
     const uint32_t j0 = chunk * jchunk, j1 = min(m, j0 + jchunk);
     const uint32_t omin = 15 - rho + 16 * g; // window start in R
     const uint32_t d0 = omin >> 2, sh = omin & 3;
+    const uint32_t e = lane >> 2, kq = lane & 3; // digit batch: ring element e, 16-byte chunk kq
+
+    // padding dwords 32..35 of every R slot stay zero
+    for (uint32_t idx = lane; idx < I8_ITW * JB; idx += 64)
+        *reinterpret_cast<uint4 *>(&Rs[wave][idx / JB][idx % JB][32]) = make_uint4(0, 0, 0, 0);
 
     v4i acc[I8_ITW][4][NT];
 #pragma unroll
@@ -77,81 +94,95 @@ __global__ __launch_bounds__(256) void ajtai_i8_kernel(const int8_t *__restrict_
 #pragma unroll
             for (int u = 0; u < NT; u++) acc[x][t][u] = v4i{0, 0, 0, 0};
 
-    for (uint32_t j = j0; j < j1; j++) {
-        v4i bf[NT];
+    auto load_digits = [&](uint32_t jb, uint4 (&dst)[I8_ITW]) {
+#pragma unroll
+        for (int x = 0; x < I8_ITW; x++) {
+            const uint32_t w = wbase + x, j = jb + e;
+            dst[x] = (w < W && j < j1)
+                         ? *reinterpret_cast<const uint4 *>(digits + ((size_t)w * m + j) * 64 + 16 * kq)
+                         : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto load_b = [&](uint32_t j, v4i (&bf)[NT]) {
 #pragma unroll
         for (int u = 0; u < NT; u++)
             bf[u] = *reinterpret_cast<const v4i *>(limbs + ((size_t)j * npad + col0 + 16 * u + rho) * 64 + 16 * g);
+    };
+
+    uint4 dnext[I8_ITW];
+    load_digits(j0, dnext);
+    v4i bnext[NT];
+    if (j0 < j1) load_b(j0, bnext);
+    for (uint32_t jb = j0; jb < j1; jb += JB) {
+        // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i])
 #pragma unroll
         for (int x = 0; x < I8_ITW; x++) {
-            const uint32_t w = wbase + x;
-            uint32_t *R = Rl[wave][x];
-            if (lane < 16) {
-                const uint32_t s4 =
-                    w < W ? *reinterpret_cast<const uint32_t *>(digits + ((size_t)w * m + j) * 64 + 4 * lane) : 0u;
-                const uint32_t rv = __builtin_bswap32(s4);
-                R[15 - lane] = rv;            // R[y] = s[63 - y], y < 64
-                R[31 - lane] = neg_bytes(rv); // R[y] = -s[127 - y], 64 <= y < 127
-            } else if (lane < 20) {
-                R[lane + 16] = 0; // padding read by the last window dword
-            }
-            __syncthreads();
-            uint32_t win[17];
+            const uint4 d = dnext[x];
+            uint32_t *R = Rs[wave][x][e];
+            const uint32_t r0 = __builtin_bswap32(d.x), r1 = __builtin_bswap32(d.y), r2 = __builtin_bswap32(d.z),
+                           r3 = __builtin_bswap32(d.w);
+            *reinterpret_cast<uint4 *>(R + 12 - 4 * kq) = make_uint4(r3, r2, r1, r0);
+            *reinterpret_cast<uint4 *>(R + 28 - 4 * kq) =
+                make_uint4(neg_bytes(r3), neg_bytes(r2), neg_bytes(r1), neg_bytes(r0));
+        }
+        if (jb + JB < j1) load_digits(jb + JB, dnext); // prefetch the next batch
+        const uint32_t jn = min((uint32_t)JB, j1 - jb);
+        for (uint32_t jj = 0; jj < jn; jj++) {
+            v4i bf[NT];
 #pragma unroll
-            for (int e = 0; e < 17; e++) win[e] = R[d0 + e];
-            __syncthreads();
+            for (int u = 0; u < NT; u++) bf[u] = bnext[u];
+            if (jb + jj + 1 < j1) load_b(jb + jj + 1, bnext);
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                v4i af;
+            for (int x = 0; x < I8_ITW; x++) {
+                const uint32_t *R = Rs[wave][x][jj];
+                uint32_t win[17];
 #pragma unroll
-                for (int q4 = 0; q4 < 4; q4++)
-                    af[q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
+                for (int q4 = 0; q4 < 17; q4++) win[q4] = R[d0 + q4];
 #pragma unroll
-                for (int u = 0; u < NT; u++)
-                    acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[u], acc[x][t][u], 0, 0, 0);
+                for (int t = 0; t < 4; t++) {
+                    v4i af;
+#pragma unroll
+                    for (int q4 = 0; q4 < 4; q4++)
+                        af[q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
+#pragma unroll
+                    for (int u = 0; u < NT; u++)
+                        acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[u], acc[x][t][u], 0, 0, 0);
+                }
             }
         }
     }
-    // D layout: lane (rho, g), register i -> row 4g + i of the tile, column rho
+    // D layout: lane (rho, g), register i -> row 4g + i of tile t, column 16u + rho = 4 * row_i + limb.
+    // Combine limbs across the lane quad, fold mod q, one u32 per (chunk, w, i, r).
 #pragma unroll
     for (int x = 0; x < I8_ITW; x++) {
         const uint32_t w = wbase + x;
-        if (w >= W) continue;
-        int32_t *base = partial + ((size_t)chunk * W + w) * 64 * npad;
 #pragma unroll
         for (int t = 0; t < 4; t++)
 #pragma unroll
             for (int u = 0; u < NT; u++)
 #pragma unroll
-                for (int i = 0; i < 4; i++)
-                    base[(size_t)(16 * t + 4 * g + i) * npad + col0 + 16 * u + rho] = acc[x][t][u][i];
+                for (int ireg = 0; ireg < 4; ireg++) {
+                    const int64_t v = combine_limbs(acc[x][t][u][ireg]);
+                    const uint32_t irow = (col0 + 16 * u + rho) >> 2;
+                    if ((lane & 3) == 0 && w < W && irow < k) {
+                        const uint64_t mag = (uint64_t)(v < 0 ? -v : v);
+                        uint32_t res = reduce_u64(mag, q, barrett);
+                        if (v < 0 && res) res = q - res;
+                        const uint32_t r = 16 * t + 4 * g + ireg;
+                        partial[(((size_t)chunk * W + w) * k + irow) * 64 + r] = res;
+                    }
+                }
     }
 }
 
-// out[w][i][r] = sum_L 256^L * (sum_chunks partial[chunk][w][r][4i + L]) mod q
-__global__ void ajtai_i8_reduce_kernel(const int32_t *__restrict__ partial, uint32_t *__restrict__ out, uint32_t W,
-                                       uint32_t k, uint32_t npad, uint32_t nchunks, uint32_t q, uint64_t barrett,
-                                       uint4 pw)
+// out[w][i][r] = sum over chunks of the per-chunk residues, mod q
+__global__ void ajtai_i8_reduce_kernel(const uint32_t *__restrict__ partial, uint32_t *__restrict__ out, size_t plane,
+                                       uint32_t nchunks, uint32_t q, uint64_t barrett)
 {
-    const size_t total = (size_t)W * k * 64;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t r = e & 63;
-        const size_t wi = e >> 6;
-        const uint32_t i = wi % k;
-        const size_t w = wi / k;
-        const uint32_t pws[4] = {pw.x, pw.y, pw.z, pw.w};
-        uint64_t acc = 0;
-#pragma unroll
-        for (int L = 0; L < 4; L++) {
-            int64_t s = 0;
-            for (uint32_t c = 0; c < nchunks; c++)
-                s += partial[(((size_t)c * W + w) * 64 + r) * npad + 4 * i + L];
-            const uint64_t mag = (uint64_t)(s < 0 ? -s : s);
-            uint32_t v = reduce_u64(mag, q, barrett);
-            if (s < 0 && v) v = q - v;
-            acc += (uint64_t)reduce_u64((uint64_t)v * pws[L], q, barrett);
-        }
-        out[(w * k + i) * 64 + r] = reduce_u64(acc, q, barrett);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < plane; e += (size_t)gridDim.x * blockDim.x) {
+        uint64_t s = 0;
+        for (uint32_t c = 0; c < nchunks; c++) s += partial[(size_t)c * plane + e];
+        out[e] = reduce_u64(s, q, barrett);
     }
 }
 
@@ -203,21 +234,24 @@ lzx_status ajtai_commit_int8(const lzx_ajtai_key *key, const int8_t *digits_dev,
 {
     if (W == 0) return LZX_OK;
     if (!ajtai_int8_supported(key) || !key->limbs_dev) return LZX_UNSUPPORTED;
-    if ((uintptr_t)digits_dev % 4) return LZX_INVALID_ARGUMENT;
+    if ((uintptr_t)digits_dev % 16) return LZX_INVALID_ARGUMENT;
     const uint32_t m = (uint32_t)key->m, k = (uint32_t)key->k, npad = npad_of(k);
     const uint32_t q = key->plan->q;
     const uint64_t barrett = key->plan->mod.barrett;
     const uint32_t per_block = I8_WAVES * I8_ITW;
     const uint32_t groups = (uint32_t)((W + per_block - 1) / per_block);
-    uint32_t nchunks = std::max<uint32_t>((m + I8_MAX_JCHUNK - 1) / I8_MAX_JCHUNK, (2048 + groups - 1) / groups);
-    nchunks = std::min<uint32_t>(nchunks, m);
+    // ~2 blocks per CU in flight; every chunk within the int32 fold-back bound
+    uint32_t nchunks = std::max<uint32_t>((m + I8_MAX_JCHUNK - 1) / I8_MAX_JCHUNK, (1024 + groups - 1) / groups);
+    nchunks = std::min<uint32_t>(nchunks, (m + 15) / 16);
+    nchunks = std::max<uint32_t>(nchunks, 1);
     const uint32_t jchunk = (m + nchunks - 1) / nchunks;
     nchunks = (m + jchunk - 1) / jchunk;
     const uint32_t padded = (nchunks + 7) / 8 * 8;
     if ((uint64_t)padded * groups > 0x7FFFFFFFull) return LZX_UNSUPPORTED;
+    const size_t plane = W * (size_t)k * 64;
 
-    int32_t *partial = nullptr;
-    LZX_HIP(hipMallocAsync((void **)&partial, (size_t)nchunks * W * 64 * npad * sizeof(int32_t), stream));
+    uint32_t *partial = nullptr;
+    LZX_HIP(hipMallocAsync((void **)&partial, (size_t)nchunks * plane * sizeof(uint32_t), stream));
     lzx_status st = LZX_OK;
     for (uint32_t col0 = 0; col0 < npad && st == LZX_OK; col0 += 64) {
         const uint32_t nt = std::min<uint32_t>(4, (npad - col0) / 16);
@@ -225,34 +259,28 @@ lzx_status ajtai_commit_int8(const lzx_ajtai_key *key, const int8_t *digits_dev,
         switch (nt) {
         case 1:
             hipLaunchKernelGGL(ajtai_i8_kernel<1>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
             break;
         case 2:
             hipLaunchKernelGGL(ajtai_i8_kernel<2>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
             break;
         case 3:
             hipLaunchKernelGGL(ajtai_i8_kernel<3>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
             break;
         default:
             hipLaunchKernelGGL(ajtai_i8_kernel<4>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, npad, col0, jchunk, nchunks, groups);
+                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
             break;
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) st = device_fail(e, "ajtai_i8_kernel");
     }
     if (st == LZX_OK) {
-        uint4 pw;
-        pw.x = 1 % q;
-        pw.y = (uint32_t)(256ULL % q);
-        pw.z = (uint32_t)(65536ULL % q);
-        pw.w = (uint32_t)(16777216ULL % q);
-        const size_t total = W * k * 64;
-        const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, 1 << 16);
-        hipLaunchKernelGGL(ajtai_i8_reduce_kernel, dim3(blocks), dim3(256), 0, stream, partial, out_dev, (uint32_t)W,
-                           k, npad, nchunks, q, barrett, pw);
+        const uint32_t blocks = (uint32_t)std::min<size_t>((plane + 255) / 256, 1 << 16);
+        hipLaunchKernelGGL(ajtai_i8_reduce_kernel, dim3(blocks), dim3(256), 0, stream, partial, out_dev, plane,
+                           nchunks, q, barrett);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) st = device_fail(e, "ajtai_i8_reduce_kernel");
     }
