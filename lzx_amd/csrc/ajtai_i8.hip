@@ -60,8 +60,10 @@ __device__ __forceinline__ int64_t combine_limbs(int32_t acc)
     return y + (hi << 16);                                                // L = 0
 }
 
+// Two waves per SIMD (accumulators live in VGPRs, <= 256 registers): the
+// second wave issues its VALU and LDS work while the first one's MFMAs run.
 template <int NT>
-__global__ __launch_bounds__(256) void ajtai_i8_kernel(const int8_t *__restrict__ digits,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void ajtai_i8_kernel(const int8_t *__restrict__ digits,
                                                        const int8_t *__restrict__ limbs,
                                                        uint32_t *__restrict__ partial, uint32_t W, uint32_t m,
                                                        uint32_t k, uint32_t npad, uint32_t col0, uint32_t jchunk,
@@ -74,7 +76,8 @@ __global__ __launch_bounds__(256) void ajtai_i8_kernel(const int8_t *__restrict_
     const uint32_t chunk = (b / (8 * groups)) * 8 + (b % 8);
     const uint32_t grp = (b / 8) % groups;
     if (chunk >= nchunks) return;
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // wave index as a scalar, so witness rows and their digit pointers stay in SGPRs
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const uint32_t rho = lane & 15, g = lane >> 4;
     const uint32_t wbase = (grp * I8_WAVES + wave) * I8_ITW;
     const uint32_t j0 = chunk * jchunk, j1 = min(m, j0 + jchunk);
@@ -94,61 +97,72 @@ __global__ __launch_bounds__(256) void ajtai_i8_kernel(const int8_t *__restrict_
 #pragma unroll
             for (int u = 0; u < NT; u++) acc[x][t][u] = v4i{0, 0, 0, 0};
 
+    // uniform base pointers + 32-bit lane offsets (saddr-form global loads)
+    const uint32_t boff = (col0 + rho) * 64 + 16 * g;
+    const uint32_t doff = e * 64 + 16 * kq;
     auto load_digits = [&](uint32_t jb, uint4 (&dst)[I8_ITW]) {
 #pragma unroll
         for (int x = 0; x < I8_ITW; x++) {
-            const uint32_t w = wbase + x, j = jb + e;
-            dst[x] = (w < W && j < j1)
-                         ? *reinterpret_cast<const uint4 *>(digits + ((size_t)w * m + j) * 64 + 16 * kq)
-                         : make_uint4(0, 0, 0, 0);
+            const uint32_t w = min(wbase + x, W - 1);
+            const int8_t *base = digits + ((size_t)w * m + jb) * 64;
+            dst[x] = (wbase + x < W && jb + e < j1) ? *reinterpret_cast<const uint4 *>(base + doff)
+                                                     : make_uint4(0, 0, 0, 0);
         }
     };
     auto load_b = [&](uint32_t j, v4i (&bf)[NT]) {
+        const int8_t *base = limbs + (size_t)j * npad * 64;
 #pragma unroll
-        for (int u = 0; u < NT; u++)
-            bf[u] = *reinterpret_cast<const v4i *>(limbs + ((size_t)j * npad + col0 + 16 * u + rho) * 64 + 16 * g);
+        for (int u = 0; u < NT; u++) bf[u] = *reinterpret_cast<const v4i *>(base + boff + 16 * 64 * u);
     };
 
-    uint4 dnext[I8_ITW];
-    load_digits(j0, dnext);
-    v4i bnext[NT];
-    if (j0 < j1) load_b(j0, bnext);
-    for (uint32_t jb = j0; jb < j1; jb += JB) {
-        // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i])
+    // One window of R per (witness, ring element): the four row-tile fragments.
+    auto compute = [&](uint32_t jj, const v4i (&bf)[NT]) {
 #pragma unroll
         for (int x = 0; x < I8_ITW; x++) {
-            const uint4 d = dnext[x];
-            uint32_t *R = Rs[wave][x][e];
-            const uint32_t r0 = __builtin_bswap32(d.x), r1 = __builtin_bswap32(d.y), r2 = __builtin_bswap32(d.z),
-                           r3 = __builtin_bswap32(d.w);
-            *reinterpret_cast<uint4 *>(R + 12 - 4 * kq) = make_uint4(r3, r2, r1, r0);
-            *reinterpret_cast<uint4 *>(R + 28 - 4 * kq) =
-                make_uint4(neg_bytes(r3), neg_bytes(r2), neg_bytes(r1), neg_bytes(r0));
-        }
-        if (jb + JB < j1) load_digits(jb + JB, dnext); // prefetch the next batch
-        const uint32_t jn = min((uint32_t)JB, j1 - jb);
-        for (uint32_t jj = 0; jj < jn; jj++) {
-            v4i bf[NT];
+            const uint32_t *R = Rs[wave][x][jj];
+            uint32_t win[17];
 #pragma unroll
-            for (int u = 0; u < NT; u++) bf[u] = bnext[u];
-            if (jb + jj + 1 < j1) load_b(jb + jj + 1, bnext);
+            for (int q4 = 0; q4 < 17; q4++) win[q4] = R[d0 + q4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                v4i af;
+#pragma unroll
+                for (int q4 = 0; q4 < 4; q4++)
+                    af[q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
+#pragma unroll
+                for (int u = 0; u < NT; u++)
+                    acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[u], acc[x][t][u], 0, 0, 0);
+            }
+        }
+    };
+
+    // B of ring element j lives in register set (j - j0) & 1 and is loaded one
+    // ring element ahead; digits are loaded one batch ahead.
+    uint4 dnext[I8_ITW];
+    load_digits(j0, dnext);
+    v4i b0[NT], b1[NT];
+    load_b(j0, b0);
+    for (uint32_t j = j0; j < j1; j += 2) {
+        const uint32_t jj = (j - j0) % JB;
+        if (jj == 0) {
+            // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i])
 #pragma unroll
             for (int x = 0; x < I8_ITW; x++) {
-                const uint32_t *R = Rs[wave][x][jj];
-                uint32_t win[17];
-#pragma unroll
-                for (int q4 = 0; q4 < 17; q4++) win[q4] = R[d0 + q4];
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    v4i af;
-#pragma unroll
-                    for (int q4 = 0; q4 < 4; q4++)
-                        af[q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
-#pragma unroll
-                    for (int u = 0; u < NT; u++)
-                        acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[u], acc[x][t][u], 0, 0, 0);
-                }
+                const uint4 d = dnext[x];
+                uint32_t *R = Rs[wave][x][e];
+                const uint32_t r0 = __builtin_bswap32(d.x), r1 = __builtin_bswap32(d.y), r2 = __builtin_bswap32(d.z),
+                               r3 = __builtin_bswap32(d.w);
+                *reinterpret_cast<uint4 *>(R + 12 - 4 * kq) = make_uint4(r3, r2, r1, r0);
+                *reinterpret_cast<uint4 *>(R + 28 - 4 * kq) =
+                    make_uint4(neg_bytes(r3), neg_bytes(r2), neg_bytes(r1), neg_bytes(r0));
             }
+            if (j + JB < j1) load_digits(j + JB, dnext);
+        }
+        if (j + 1 < j1) load_b(j + 1, b1);
+        compute(jj, b0);
+        if (j + 1 < j1) {
+            if (j + 2 < j1) load_b(j + 2, b0);
+            compute(jj + 1, b1);
         }
     }
     // D layout: lane (rho, g), register i -> row 4g + i of tile t, column 16u + rho = 4 * row_i + limb.
