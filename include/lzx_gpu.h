@@ -159,6 +159,62 @@ lzx_status lzx_ajtai_verify_opening_host(const lzx_ajtai_key *key, const uint32_
                                          size_t s_len);
 
 /* ---------------------------------------------------------------------------
+ * Uneven commitment batch: many independent commitments t = A s, each on its
+ * own ring Z_q[X]/(X^n + 1) with its own length m, in ONE persistent launch.
+ * Every job has the semantics of AjtaiPublicKey::commit (ajtai.rs:118-153);
+ * the reference has no batch form (its callers loop over commit).  Beyond the
+ * reference's Modulus32 (modulus.rs:22-162), moduli up to 2^50 are accepted,
+ * with u64 residues.  Results are the exact ring products, independent of
+ * how the library computes them.
+ * ------------------------------------------------------------------------- */
+typedef struct lzx_wide_key lzx_wide_key;
+typedef struct lzx_commit_batch lzx_commit_batch;
+
+/* Commitment key over Z_q[X]/(X^n + 1), n = 2^log_n with 6 <= log_n <= 8.
+ * q is an odd prime with 2n | q - 1 and 256 < q < 2^50; generator generates
+ * Z_q^* (psi = generator^((q-1)/2n), as root_of_unity, modulus.rs:146-161).
+ * A_dev is [k][m][n] residues (any representative): u32 elements when
+ * q < 2^32, u64 otherwise.  The key keeps an NTT-domain copy built on the
+ * device (stream-ordered); A_dev may be freed once `stream` has passed it.
+ * Errors: NttAdicityTooSmall when 2n does not divide q - 1, InvalidArgument
+ * when psi^n != -1 or a size is out of range. */
+lzx_status lzx_wide_key_create(uint64_t q, uint64_t generator, uint32_t log_n, size_t k, size_t m,
+                               const void *A_dev, void *stream, lzx_wide_key **out);
+void lzx_wide_key_destroy(lzx_wide_key *key);
+/* Element width in bytes (4 or 8) of A and of commitments under this key. */
+uint32_t lzx_wide_key_elem_bytes(const lzx_wide_key *key);
+
+typedef struct lzx_commit_job {
+    const lzx_wide_key *key;
+    const int8_t *digits_dev; /* [m][n] balanced digits (16-byte aligned); coefficient = digit mod q */
+    size_t m;                 /* witness length; the key's first m columns are used (m <= key m) */
+    void *out_dev;            /* [k][n] canonical residues of the key's element width */
+} lzx_commit_job;
+
+/* Plans a batch: device work queue (largest jobs first), per-class chunk
+ * sizes (the autotune cache, else LZX_BATCH_CHUNK, else a default), job
+ * descriptors and scratch.  The job array is copied; keys must outlive the
+ * batch. */
+lzx_status lzx_commit_batch_create(const lzx_commit_job *jobs, size_t njobs, lzx_commit_batch **out);
+/* Runs the whole batch as one persistent kernel launch, replayed from a
+ * hipGraph captured on first use.  Stream-ordered; calls on one batch must
+ * not overlap. */
+lzx_status lzx_commit_batch_run(lzx_commit_batch *b, void *stream);
+/* Synchronizes `stream`, then reports a device-side failure of the last run:
+ * a dependency wait past its bound gives LZX_DEVICE_ERROR. */
+lzx_status lzx_commit_batch_check(lzx_commit_batch *b, void *stream);
+/* Times candidate chunk sizes for each class present (runs that write the
+ * jobs' outputs), keeps the fastest in the process-wide cache and in the
+ * file named by LZX_AUTOTUNE_CACHE when set, and replans the batch. */
+lzx_status lzx_commit_batch_autotune(lzx_commit_batch *b, void *stream);
+/* Plan facts: work items, scratch bytes and the chunk (ring elements per work
+ * item) of each class, class = 3 * kind + log_n - 6 with kind 0 for
+ * q < 2^31, 1 for q < 2^32, 2 for q < 2^50 (0 when the class is absent). */
+lzx_status lzx_commit_batch_info(const lzx_commit_batch *b, size_t *items, size_t *scratch_bytes,
+                                 uint32_t chunks[9]);
+void lzx_commit_batch_destroy(lzx_commit_batch *b);
+
+/* ---------------------------------------------------------------------------
  * Fiat-Shamir transcript (lattice-core/src/transcript.rs), host side.
  * ------------------------------------------------------------------------- */
 typedef struct lzx_transcript lzx_transcript;

@@ -46,6 +46,16 @@ PROTOTYPES = {
     "lzx_ajtai_commit_digits_batch": (st_t, [vp, vp, sz, sz, vp, i32, vp]),
     "lzx_ajtai_commit_host": (st_t, [vp, vp, sz, vp]),
     "lzx_ajtai_verify_opening_host": (st_t, [vp, vp, vp, sz]),
+    # Uneven commitment batch
+    "lzx_wide_key_create": (st_t, [u64, u64, u32, sz, sz, vp, vp, C.POINTER(vp)]),
+    "lzx_wide_key_destroy": (None, [vp]),
+    "lzx_wide_key_elem_bytes": (u32, [vp]),
+    "lzx_commit_batch_create": (st_t, [vp, sz, C.POINTER(vp)]),
+    "lzx_commit_batch_run": (st_t, [vp, vp]),
+    "lzx_commit_batch_check": (st_t, [vp, vp]),
+    "lzx_commit_batch_autotune": (st_t, [vp, vp]),
+    "lzx_commit_batch_info": (st_t, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(u32)]),
+    "lzx_commit_batch_destroy": (None, [vp]),
     # Transcript
     "lzx_transcript_new": (st_t, [u8p, sz, u64, C.POINTER(vp)]),
     "lzx_transcript_clone": (st_t, [vp, C.POINTER(vp)]),

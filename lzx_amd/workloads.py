@@ -70,3 +70,39 @@ def ntt_batch_input(moduli, polys: int, n: int = 64, seed_base: int = 0xC2) -> n
     """Batched NTT input, [prime][poly][n] u32, prime i from seed seed_base + i."""
     return np.stack([uniform_u32_mod(seed_base + i, polys * n, m.q).reshape(polys, n)
                      for i, m in enumerate(moduli)])
+
+
+def uneven_batch_jobs(seed: int = 0xC5, njobs: int = 512, moduli=None):
+    """The uneven commitment batch: job i takes outputs 3i, 3i+1, 3i+2 of the
+    splitmix64 stream `seed`: u = (x0 >> 11) / 2^53, m = floor(2^(8 + 8u));
+    log_n = 6 + x1 mod 3 (n in {64, 128, 256}); modulus = moduli[x2 mod 3].
+    Returns a list of (m, log_n, (q, generator))."""
+    if moduli is None:
+        from .commit_batch import WIDE_MODULI as moduli  # noqa: WPS433
+    x = splitmix64(seed, 3 * njobs).reshape(njobs, 3)
+    u = (x[:, 0] >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    m = np.floor(np.exp2(8.0 + 8.0 * u)).astype(np.int64)
+    log_n = 6 + (x[:, 1] % np.uint64(3)).astype(np.int64)
+    mi = (x[:, 2] % np.uint64(3)).astype(np.int64)
+    return [(int(m[i]), int(log_n[i]), moduli[int(mi[i])]) for i in range(njobs)]
+
+
+def small_digits_torch(seed: int, count: int, device="cuda", start: int = 0):
+    """small_digits(seed, count) computed on the device (same splitmix64 rule;
+    int64 arithmetic wraps, logical shifts emulated with masks)."""
+    import torch  # noqa: WPS433
+    g = torch.tensor(np.array([int(GAMMA)], dtype=np.uint64).view(np.int64)[0], device=device)
+    m1 = torch.tensor(np.array([int(M1)], dtype=np.uint64).view(np.int64)[0], device=device)
+    m2 = torch.tensor(np.array([int(M2)], dtype=np.uint64).view(np.int64)[0], device=device)
+    s0 = torch.tensor(np.array([seed & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64).view(np.int64)[0], device=device)
+
+    def lsr(v, k):
+        return (v >> k) & ((1 << (64 - k)) - 1)
+
+    idx = torch.arange(start + 1, start + count + 1, dtype=torch.int64, device=device)
+    z = s0 + idx * g
+    z = (z ^ lsr(z, 30)) * m1
+    z = (z ^ lsr(z, 27)) * m2
+    z = z ^ lsr(z, 31)
+    d = (lsr(z, 60) - 8).to(torch.int8)
+    return torch.where((z & 1) == 0, torch.zeros_like(d), d)

@@ -1010,3 +1010,114 @@ int or_sumcheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_l
     *final_claim_out = current;
     return OR_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Wide-modulus commitments (checker for the uneven commitment batch)        */
+/* ------------------------------------------------------------------------ */
+/*
+ * The batch commits t_i = sum_j A_ij * s_j in Z_q[X]/(X^n+1) for moduli up
+ * to 2^50, beyond the reference's Modulus32.  The semantics are those of
+ * AjtaiPublicKey::commit (lattice-commitment/src/ajtai.rs:118-153) with the
+ * ring product defined by negacyclic convolution (the schoolbook shape of
+ * lattice-ring/src/ntt.rs:390-403), so the checker is that definition,
+ * computed exactly in 128-bit integers.  A is [k][lda][n] (u32 or u64
+ * residues, any representative), the witness is m elements of n balanced
+ * int8 digits (ring.rs:232-243 from_signed), the output [k][n] canonical u64.
+ */
+
+static uint64_t or_mulmod64(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((u128)a * b % q); }
+
+uint64_t or_powmod64(uint64_t b, uint64_t e, uint64_t q)
+{
+    uint64_t acc = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) acc = or_mulmod64(acc, b, q);
+        b = or_mulmod64(b, b, q);
+        e >>= 1;
+    }
+    return acc;
+}
+
+static uint64_t or_elem(const void *A, int elem_bytes, size_t idx)
+{
+    return elem_bytes == 4 ? ((const uint32_t *)A)[idx] : ((const uint64_t *)A)[idx];
+}
+
+int or_commit_wide_schoolbook(uint64_t q, uint32_t n, size_t k, size_t m, size_t lda, const void *A,
+                              int elem_bytes, const int8_t *digits, uint64_t *out)
+{
+    if (q < 2 || (q >> 62) || n == 0 || lda < m || (elem_bytes != 4 && elem_bytes != 8)) return OR_BAD_ARGUMENT;
+    /* |A mod q| < 2^62, |digit| <= 128: each product < 2^69, n*m terms fit i128 */
+    __int128 *acc = (__int128 *)malloc(sizeof(__int128) * n);
+    uint64_t *a = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    for (size_t i = 0; i < k; i++) {
+        memset(acc, 0, sizeof(__int128) * n);
+        for (size_t j = 0; j < m; j++) {
+            const int8_t *s = digits + j * n;
+            int nz = 0;
+            for (uint32_t c = 0; c < n; c++) nz |= s[c] != 0;
+            if (!nz) continue;
+            for (uint32_t c = 0; c < n; c++) a[c] = or_elem(A, elem_bytes, (i * lda + j) * n + c) % q;
+            for (uint32_t c = 0; c < n; c++) {
+                if (!s[c]) continue;
+                const __int128 sc = s[c];
+                /* a(X) * s_c X^c: coefficient r gets a[r-c], wrapping with a sign flip */
+                for (uint32_t r = 0; r < n; r++) {
+                    if (r >= c)
+                        acc[r] += (__int128)a[r - c] * sc;
+                    else
+                        acc[r] -= (__int128)a[r + n - c] * sc;
+                }
+            }
+        }
+        for (uint32_t r = 0; r < n; r++) {
+            __int128 v = acc[r] % (__int128)q;
+            if (v < 0) v += q;
+            out[i * n + r] = (uint64_t)v;
+        }
+    }
+    free(acc);
+    free(a);
+    return OR_OK;
+}
+
+/* Horner evaluation of an n-coefficient polynomial at x, mod q. */
+static uint64_t or_eval_poly(uint64_t q, uint32_t n, const void *p, int elem_bytes, size_t off, uint64_t x)
+{
+    uint64_t v = 0;
+    for (uint32_t c = n; c-- > 0;) v = (uint64_t)(((u128)v * x + or_elem(p, elem_bytes, off + c) % q) % q);
+    return v;
+}
+
+/* Evaluation check of t = A s at a root z of X^n + 1 (z^n = -1 mod q, so
+ * evaluation is a ring homomorphism Z_q[X]/(X^n+1) -> Z_q): counts the rows
+ * i with t_i(z) != sum_j A_ij(z) s_j(z).  A wrong row passes with
+ * probability <= (n-1)/q per random root (Schwartz-Zippel).  t is [k][n]
+ * with elem_bytes-wide elements. */
+size_t or_commit_wide_check_at(uint64_t q, uint32_t n, size_t k, size_t m, size_t lda, const void *A,
+                               int elem_bytes, const int8_t *digits, const void *t, int t_bytes, uint64_t z)
+{
+    uint64_t *sz = (uint64_t *)malloc(sizeof(uint64_t) * (m ? m : 1));
+    for (size_t j = 0; j < m; j++) {
+        uint64_t v = 0;
+        for (uint32_t c = n; c-- > 0;) {
+            int64_t d = digits[j * n + c];
+            uint64_t dr = d < 0 ? q - (uint64_t)(-d) % q : (uint64_t)d % q;
+            v = (uint64_t)(((u128)v * z + dr) % q);
+        }
+        sz[j] = v;
+    }
+    size_t bad = 0;
+    for (size_t i = 0; i < k; i++) {
+        uint64_t lhs = or_eval_poly(q, n, t, t_bytes, i * n, z);
+        uint64_t rhs = 0;
+        for (size_t j = 0; j < m; j++) {
+            if (!sz[j]) continue;
+            rhs = (rhs + or_mulmod64(or_eval_poly(q, n, A, elem_bytes, (i * lda + j) * n, z), sz[j], q)) % q;
+        }
+        bad += lhs != rhs;
+    }
+    free(sz);
+    return bad;
+}

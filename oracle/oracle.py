@@ -103,6 +103,9 @@ def lib():
         "or_sumcheck_prove": (C.c_int, [sz, sz, C.POINTER(C.c_void_p), sz, _u64p, _u32p, _u32p, u64, vp,
                                         _u64p, _u64p, C.POINTER(u64), _u64p, C.POINTER(sz)]),
         "or_sumcheck_verify": (C.c_int, [_u64p, sz, sz, sz, sz, u64, vp, C.c_int, u64, _u64p, C.POINTER(u64)]),
+        "or_powmod64": (u64, [u64, u64, u64]),
+        "or_commit_wide_schoolbook": (C.c_int, [u64, u32, sz, sz, sz, vp, C.c_int, vp, _u64p]),
+        "or_commit_wide_check_at": (sz, [u64, u32, sz, sz, sz, vp, C.c_int, vp, vp, C.c_int, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -288,6 +291,36 @@ def negacyclic_schoolbook(q, a, b):
     out = np.zeros_like(a)
     lib().or_negacyclic_schoolbook(q, a.size, a, b, out)
     return out
+
+
+def powmod64(b, e, q):
+    return lib().or_powmod64(b, e, q)
+
+
+def commit_wide_schoolbook(q, A, digits, m=None):
+    """t = A s over Z_q[X]/(X^n+1), exact schoolbook (ajtai.rs:118-153 semantics).
+    A: (k, lda, n) u32/u64, digits: (>= m, n) int8.  Returns (k, n) uint64."""
+    A = np.ascontiguousarray(A)
+    assert A.dtype in (np.uint32, np.uint64) and A.ndim == 3
+    k, lda, n = A.shape
+    digits = np.ascontiguousarray(digits, dtype=np.int8)
+    m = digits.shape[0] if m is None else m
+    out = np.zeros((k, n), dtype=np.uint64)
+    st = lib().or_commit_wide_schoolbook(q, n, k, m, lda, A.ctypes.data, A.itemsize, digits.ctypes.data, out)
+    assert st == 0, st
+    return out
+
+
+def commit_wide_check_at(q, A, digits, t, z, m=None):
+    """Number of rows i with t_i(z) != sum_j A_ij(z) s_j(z) mod q (z a root of X^n + 1)."""
+    A = np.ascontiguousarray(A)
+    k, lda, n = A.shape
+    digits = np.ascontiguousarray(digits, dtype=np.int8)
+    m = digits.shape[0] if m is None else m
+    t = np.ascontiguousarray(t)
+    assert t.shape == (k, n) and t.dtype in (np.uint32, np.uint64)
+    return lib().or_commit_wide_check_at(q, n, k, m, lda, A.ctypes.data, A.itemsize, digits.ctypes.data,
+                                         t.ctypes.data, t.itemsize, z)
 
 
 # ---------------------------------------------------------------------------

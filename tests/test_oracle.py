@@ -295,3 +295,114 @@ def test_eq_table_closed_form():
                 bit = (b >> (m - 1 - i)) & 1
                 w = w * (int(pt[i]) if bit else (1 - int(pt[i])) % O.GL_P) % O.GL_P
             assert int(tab[b]) == w
+
+
+# ---- wide-modulus commitments (checker of the uneven commitment batch) ----
+
+Q_50, G_50 = 1125899437080577, 5
+
+
+def _is_prime(n):
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d, s = d // 2, s + 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):  # deterministic below 3.3e24
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def _factor(n):
+    fs, p = [], 2
+    while p * p <= n:
+        while n % p == 0:
+            fs.append(p)
+            n //= p
+        p += 1
+    if n > 1:
+        fs.append(n)
+    return sorted(set(fs))
+
+
+def test_q50_is_an_ntt_prime_with_generator_5():
+    assert Q_50 < 2 ** 50 and (Q_50 - 1) % 2 ** 26 == 0 and _is_prime(Q_50)
+    for p in _factor(Q_50 - 1):
+        assert pow(G_50, (Q_50 - 1) // p, Q_50) != 1, p
+    for n in (64, 128, 256):
+        psi = pow(G_50, (Q_50 - 1) // (2 * n), Q_50)
+        assert pow(psi, n, Q_50) == Q_50 - 1
+        assert O.powmod64(G_50, (Q_50 - 1) // (2 * n), Q_50) == psi
+
+
+def test_wide_schoolbook_equals_reference_commit_at_u32_moduli():
+    """At a Modulus32 prime the wide checker equals commit_naive_reference
+    (ajtai.rs:158-180 restated, itself pinned to closed forms above)."""
+    rng = np.random.default_rng(21)
+    for (q, adic, g), log_n in [((2013265921, 27, 31), 6), ((3221225473, 30, 5), 7)]:
+        n, k, m = 1 << log_n, 3, 5
+        A = rng.integers(0, q, size=(k, m, n), dtype=np.uint64).astype(np.uint32)
+        d = rng.integers(-8, 8, size=(m, n)).astype(np.int8)
+        ntt = O.Ntt(q, adic, g, log_n)
+        st, want = O.ajtai_commit_naive(ntt, k, m, A.reshape(k * m, n), np.mod(d.astype(np.int64), q).astype(np.uint32))
+        assert st == 0
+        assert np.array_equal(O.commit_wide_schoolbook(q, A, d).astype(np.uint32), want)
+
+
+def test_wide_schoolbook_closed_form_at_50_bits():
+    rng = np.random.default_rng(22)
+    n, k, m = 8, 2, 3
+    A = rng.integers(0, Q_50, size=(k, m, n), dtype=np.uint64)
+    d = rng.integers(-128, 128, size=(m, n)).astype(np.int8)
+    got = O.commit_wide_schoolbook(Q_50, A, d)
+    for i in range(k):
+        acc = [0] * n
+        for j in range(m):
+            p = O.closed_negacyclic([int(x) for x in A[i, j]], [int(x) % Q_50 for x in d[j]], Q_50)
+            acc = [(a + b) % Q_50 for a, b in zip(acc, p)]
+        assert [int(x) for x in got[i]] == acc
+    # X^a * X^b = -X^(a+b-n) past the wrap
+    A1 = np.zeros((1, 1, n), np.uint64)
+    A1[0, 0, 5] = 1
+    d1 = np.zeros((1, n), np.int8)
+    d1[0, 6] = 1
+    t = O.commit_wide_schoolbook(Q_50, A1, d1)
+    assert int(t[0, 3]) == Q_50 - 1 and int(t.sum()) == Q_50 - 1
+
+
+def test_wide_evaluation_check_accepts_and_rejects():
+    rng = np.random.default_rng(23)
+    n, k, m = 64, 4, 6
+    A = rng.integers(0, Q_50, size=(k, m + 2, n), dtype=np.uint64)
+    d = rng.integers(-8, 8, size=(m, n)).astype(np.int8)
+    t = O.commit_wide_schoolbook(Q_50, A, d, m=m)
+    psi = pow(G_50, (Q_50 - 1) // (2 * n), Q_50)
+    for z in (psi, pow(psi, 3, Q_50)):
+        assert O.commit_wide_check_at(Q_50, A, d, t, z, m=m) == 0
+        bad = t.copy()
+        bad[2, 17] = (int(bad[2, 17]) + 1) % Q_50
+        assert O.commit_wide_check_at(Q_50, A, d, bad, z, m=m) == 1
+    t32 = O.commit_wide_schoolbook(2013265921, A.astype(np.uint32) % np.uint32(2013265921), d, m=m)
+    assert t32.max() < 2013265921
+
+
+def test_uneven_batch_job_rule():
+    from lzx_amd import workloads as W
+    jobs = W.uneven_batch_jobs()
+    assert len(jobs) == 512 and jobs == W.uneven_batch_jobs()
+    assert all(256 <= m < 65536 and ln in (6, 7, 8) for m, ln, _ in jobs)
+    assert {q for _, _, (q, _) in jobs} == {3221225473, 2013265921, Q_50}
+    x = O.splitmix64(0xC5)
+    u = (next(x) >> 11) / 2.0 ** 53
+    assert jobs[0][0] == int(2 ** (8 + 8 * u)) and jobs[0][1] == 6 + next(x) % 3
