@@ -199,6 +199,49 @@ def bench_sumcheck(reps: int = 3):
             "final_claim": int(out.final_claim)}
 
 
+def bench_uneven_batch(reps: int = 5, autotune: bool = True, only=False):
+    """The uneven commitment batch: 512 jobs (workloads.uneven_batch_jobs(0xC5)),
+    n in {64, 128, 256}, q in {3221225473, 2013265921, 1125899437080577},
+    k = 16, W = 1, m = floor(2^(8+8u)); one persistent launch per batch,
+    replayed from a hipGraph.  Bytes = sum over jobs of key + digits + output;
+    the key stream (77 GB) dominates, so the bound is HBM."""
+    import torch
+
+    from lzx_amd import commit_batch as CB
+
+    t0 = time.perf_counter()
+    wl = CB.uneven_batch_workload()
+    setup_s = time.perf_counter() - t0
+    b = wl["batch"]
+    if autotune:
+        b.autotune()
+    info = b.info()
+    b.run()
+    b.check()
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(reps):
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        b.run()
+        e.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(e))
+    b.check()
+    ms = statistics.median(times)
+    total = wl["key_bytes"] + wl["digit_bytes"] + wl["out_bytes"]
+    res = {"workload": "uneven commitment batch: 512 jobs, n in {64,128,256}, q in {Q_32, 2013265921, "
+                       "1125899437080577}, k=16, W=1, m=floor(2^(8+8u)); one persistent launch in a hipGraph",
+           "ms_per_batch": round(ms, 3), "ring_macs": wl["ring_macs"],
+           "ring_macs_per_s": wl["ring_macs"] / (ms * 1e-3), "bytes": total,
+           "gbs": round(total / (ms * 1e-3) / 1e9, 1), "hbm_frac": round(total / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "work_items": info["items"], "scratch_bytes": info["scratch_bytes"],
+           "chunks_by_class": info["chunks"], "autotuned": autotune, "setup_s": round(setup_s, 2)}
+    del wl, b
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline_secondary():
     """Bounded one-core oracle samples for the secondary workloads."""
     from lzx_amd import workloads as W
@@ -233,6 +276,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-ntt", action="store_true", help="skip the secondary workloads (profiling runs)")
+    ap.add_argument("--only-batch", action="store_true", help="run only the uneven commitment batch leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 64 MiB device copy (counter calibration for --pmc runs)")
     args = ap.parse_args()
@@ -244,6 +288,9 @@ def main():
 
     from lzx_amd import device as D
     D.require_gpu()
+    if args.only_batch:
+        print(json.dumps({"uneven_batch": bench_uneven_batch()}), flush=True)
+        return
     r = bench_ntt(args.steps, args.warmup)
     if args.copy_calib:
         src = torch.zeros(1 << 24, dtype=torch.int32, device="cuda")
@@ -276,7 +323,8 @@ def main():
         "cpu_baseline": None if args.no_cpu else cpu_baseline_ntt(),
     }
     if not args.only_ntt:
-        line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck()}
+        line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
+                             "uneven_batch": bench_uneven_batch()}
         if not args.no_cpu:
             line["secondary"]["cpu_baseline"] = cpu_baseline_secondary()
     print(json.dumps(line), flush=True)

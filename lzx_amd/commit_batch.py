@@ -113,3 +113,51 @@ class CommitBatch:
             self.close()
         except Exception:
             pass
+
+
+def uneven_batch_workload(seed: int = 0xC5, njobs: int = 512, k: int = 16, stream=None):
+    """The 512-job uneven batch on the device (synthetic, seeded).
+
+    Jobs follow workloads.uneven_batch_jobs(seed).  One key per (modulus, n)
+    class with k = 16 rows and as many columns as the class's longest job;
+    A uniform in [0, q) from torch's generator seeded with seed * 16 + class.
+    Job i's witness is workloads.small_digits(seed * 65536 + i, m * n) (digits
+    in [-8, 8), P(0) = 17/32), computed on the device; W = 1 per job.
+    Returns a dict with the batch, keys, per-job tensors and byte counts."""
+    import torch  # noqa: WPS433
+
+    from . import workloads as W
+
+    spec = W.uneven_batch_jobs(seed, njobs)
+    classes = {}
+    for m, log_n, (q, g) in spec:
+        c = class_of(q, log_n)
+        classes.setdefault(c, [q, g, log_n, 0])
+        classes[c][3] = max(classes[c][3], m)
+    keys, A_dev = {}, {}
+    for c, (q, g, log_n, mmax) in sorted(classes.items()):
+        n = 1 << log_n
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed * 16 + c)
+        A = torch.randint(0, q, (k, mmax, n), dtype=torch.int64, device="cuda", generator=gen)
+        if q < (1 << 32):
+            A = A.to(torch.int32)  # bit pattern of the u32 residue
+        keys[c] = WideKey(q, g, log_n, k, mmax, A, stream)
+        A_dev[c] = A
+    jobs, digits, outs = [], [], []
+    key_bytes = digit_bytes = out_bytes = 0
+    for i, (m, log_n, (q, g)) in enumerate(spec):
+        c = class_of(q, log_n)
+        key, n = keys[c], 1 << log_n
+        d = W.small_digits_torch(seed * 65536 + i, m * n).reshape(m, n)
+        out = D.empty(k * n, key.dtype)
+        jobs.append((key, d, m, out))
+        digits.append(d)
+        outs.append(out)
+        key_bytes += k * m * n * key.elem_bytes
+        digit_bytes += m * n
+        out_bytes += k * n * key.elem_bytes
+    D.synchronize()
+    return {"spec": spec, "keys": keys, "A": A_dev, "jobs": jobs, "batch": CommitBatch(jobs),
+            "ring_macs": sum(k * m for m, _, _ in spec), "key_bytes": key_bytes, "digit_bytes": digit_bytes,
+            "out_bytes": out_bytes}

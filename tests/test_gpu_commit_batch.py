@@ -203,3 +203,33 @@ def test_error_paths(env):
     with pytest.raises(LzxError) as e:  # digits pointer not 16-byte aligned
         CB.CommitBatch([(key, d[0][1:], 1, out)])
     assert e.value.code == 30
+
+
+def test_full_uneven_batch_sampled(env):
+    """The 512-job batch at full size; sampled jobs checked by evaluation at two
+    roots of X^n + 1 (and the smallest exactly)."""
+    CB, D, W = env
+    wl = CB.uneven_batch_workload()
+    b = wl["batch"]
+    b.run()
+    b.check()
+    spec = wl["spec"]
+    picks = set()
+    for kind in range(3):
+        idx = [i for i, (m, ln, (q, g)) in enumerate(spec) if CB.class_of(q, ln) // 3 == kind]
+        idx.sort(key=lambda i: spec[i][0] << spec[i][1])
+        picks.update(idx[:2])
+        picks.add(idx[len(idx) // 3])
+    smallest = min(picks, key=lambda i: spec[i][0] << spec[i][1])
+    for i in sorted(picks):
+        m, log_n, (q, g) = spec[i]
+        n = 1 << log_n
+        key, d, _, out = wl["jobs"][i]
+        A = wl["A"][CB.class_of(q, log_n)][:, :m, :].contiguous()
+        A = D.to_host(A, key.dtype)
+        dig = d.cpu().numpy()
+        t = D.to_host(out, key.dtype).reshape(16, n)
+        for z in _roots(q, g, n):
+            assert O.commit_wide_check_at(q, A, dig, t, z) == 0, (i, m, n, q)
+        if i == smallest:
+            assert np.array_equal(t.astype(np.uint64), O.commit_wide_schoolbook(q, A, dig)), i

@@ -406,3 +406,13 @@ def test_uneven_batch_job_rule():
     x = O.splitmix64(0xC5)
     u = (next(x) >> 11) / 2.0 ** 53
     assert jobs[0][0] == int(2 ** (8 + 8 * u)) and jobs[0][1] == 6 + next(x) % 3
+
+
+def test_device_digit_generator_matches_numpy_rule():
+    import torch
+
+    from lzx_amd import workloads as W
+    for seed in (0xC3, 0xC5 * 65536 + 7):
+        got = W.small_digits_torch(seed, 4099, device="cpu").numpy()
+        assert np.array_equal(got, W.small_digits(seed, 4099)), seed
+    assert torch is not None
