@@ -57,7 +57,8 @@ constexpr uint32_t CB_MIN_LOG_N = 6, CB_MAX_LOG_N = 8;
 constexpr uint32_t CB_CLASSES = 9;
 constexpr uint32_t CB_MAX_CHUNK = 4096;   // 25-bit limb sums exact up to 4096 products
 constexpr uint32_t CB_MAX_ITEMS = 8192;   // per job: phase B sums stay below 2^63
-constexpr uint32_t CB_ACC = 16;           // accumulated outputs per thread and row pass
+constexpr uint32_t CB_ROWS = 16;          // key rows per pass of a work item
+constexpr uint32_t CB_INFLIGHT = 8;       // 16-byte key loads a thread issues before consuming them
 constexpr uint32_t CB_SPIN_LIMIT = 1u << 26;
 
 struct KeyDev {
@@ -78,6 +79,34 @@ struct JobDev {
     void *out;
     uint64_t part; // byte offset of [nitems][k][n] partials
 };
+
+// ---- global-memory access ----------------------------------------------------
+// Key, witness, partial and output pointers reach the kernels through
+// descriptors in memory, so the compiler cannot infer their address space and
+// would emit flat accesses, which also count against the LDS wait counter and
+// so serialize with the LDS traffic of the same loop.  Every global access
+// goes through an explicit address_space(1) pointer instead.
+#define CB_GLOBAL __attribute__((address_space(1)))
+typedef unsigned cb_v4u __attribute__((ext_vector_type(4)));
+
+template <typename T> __device__ __forceinline__ CB_GLOBAL T *gbl(T *p) { return (CB_GLOBAL T *)p; }
+template <typename T> __device__ __forceinline__ const CB_GLOBAL T *gbl(const T *p)
+{
+    return (const CB_GLOBAL T *)p;
+}
+
+// 16-byte load of streamed-once data (the key): nontemporal, kept out of the caches' reuse set.
+__device__ __forceinline__ uint4 load16_stream(const void *p)
+{
+    const cb_v4u x = __builtin_nontemporal_load(gbl(reinterpret_cast<const cb_v4u *>(p)));
+    return make_uint4(x[0], x[1], x[2], x[3]);
+}
+
+__device__ __forceinline__ uint4 load16(const void *p)
+{
+    const cb_v4u x = *gbl(reinterpret_cast<const cb_v4u *>(p));
+    return make_uint4(x[0], x[1], x[2], x[3]);
+}
 
 // ---- arithmetic per kind ----------------------------------------------------
 
@@ -176,7 +205,7 @@ template <int LOG_N>
 __device__ __forceinline__ void load_twiddles(const KeyDev &K, uint64_t *tws)
 {
     constexpr uint32_t N = 1u << LOG_N;
-    for (uint32_t i = threadIdx.x; i < 4 * N; i += CB_THREADS) tws[i] = K.tw[i];
+    for (uint32_t i = threadIdx.x; i < 4 * N; i += CB_THREADS) tws[i] = gbl(K.tw)[i];
 }
 
 // Merged-psi Cooley-Tukey, natural coefficients -> bit-reversed evaluations,
@@ -236,7 +265,7 @@ __device__ __forceinline__ void load_witness(const KeyDev &K, const int8_t *digi
     constexpr uint32_t N = 1u << LOG_N;
     const uint32_t vecs = nb * N / 16;
     for (uint32_t v = threadIdx.x; v < vecs; v += CB_THREADS) {
-        const uint4 d = reinterpret_cast<const uint4 *>(digits)[v];
+        const uint4 d = load16(digits + 16 * (size_t)v);
         const uint32_t w[4] = {d.x, d.y, d.z, d.w};
 #pragma unroll
         for (int i = 0; i < 16; i++) {
@@ -248,16 +277,14 @@ __device__ __forceinline__ void load_witness(const KeyDev &K, const int8_t *digi
 
 template <typename T> struct Vec16;
 template <> struct Vec16<uint32_t> {
-    static __device__ __forceinline__ void load(const uint32_t *p, uint32_t (&v)[4])
+    static __device__ __forceinline__ void unpack(const uint4 &x, uint32_t (&v)[4])
     {
-        const uint4 x = *reinterpret_cast<const uint4 *>(p);
         v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w;
     }
 };
 template <> struct Vec16<uint64_t> {
-    static __device__ __forceinline__ void load(const uint64_t *p, uint64_t (&v)[2])
+    static __device__ __forceinline__ void unpack(const uint4 &x, uint64_t (&v)[2])
     {
-        const uint4 x = *reinterpret_cast<const uint4 *>(p);
         v[0] = x.x | ((uint64_t)x.y << 32), v[1] = x.z | ((uint64_t)x.w << 32);
     }
 };
@@ -276,7 +303,8 @@ __device__ void mac_item(const KeyDev &K, const JobDev &J, uint32_t item, unsign
     constexpr uint32_t VEC = 16 / sizeof(T);        // coefficients per thread (one 16-byte load)
     constexpr uint32_t TPR = N / VEC;               // threads per key row
     constexpr uint32_t RS = CB_THREADS / TPR;       // row stride between a thread's rows
-    constexpr uint32_t RP = CB_ACC / VEC;           // rows per thread per pass
+    constexpr uint32_t RP = RS < CB_ROWS ? CB_ROWS / RS : 1; // rows per thread per pass
+    constexpr uint32_t U = CB_INFLIGHT / RP;        // ring elements whose key loads are issued together
     constexpr uint32_t SB = CB_DATA_BYTES / (N * sizeof(T)); // witness elements per LDS sub-batch
     T *xs = reinterpret_cast<T *>(smem);
     const uint32_t tid = threadIdx.x;
@@ -300,18 +328,29 @@ __device__ void mac_item(const KeyDev &K, const JobDev &J, uint32_t item, unsign
             forward_ntt<KIND, LOG_N>(K, xs, nb, tws);
             const T *arow = ahat + ((size_t)(rbase + r0) * K.m + j0) * N + cv;
             const size_t rstep = (size_t)RS * K.m * N;
-#pragma unroll 2
-            for (uint32_t jj = 0; jj < nb; jj++) {
-                T b[VEC];
+            // U ring elements x RP rows of key loads in flight, then their MACs
+            for (uint32_t jj = 0; jj < nb; jj += U) {
+                uint4 raw[U][RP];
 #pragma unroll
-                for (uint32_t v = 0; v < VEC; v++) b[v] = xs[jj * N + cv + v];
+                for (uint32_t u = 0; u < U; u++)
 #pragma unroll
-                for (uint32_t r = 0; r < RP; r++) {
-                    if (rbase + r0 + RS * r < K.k) {
-                        T a[VEC];
-                        Vec16<T>::load(arow + r * rstep + (size_t)jj * N, a);
+                    for (uint32_t r = 0; r < RP; r++)
+                        raw[u][r] = (jj + u < nb && rbase + r0 + RS * r < K.k)
+                                        ? load16_stream(arow + r * rstep + (size_t)(jj + u) * N)
+                                        : make_uint4(0, 0, 0, 0);
 #pragma unroll
-                        for (uint32_t v = 0; v < VEC; v++) A::mac(acc[r][v], a[v], b[v]);
+                for (uint32_t u = 0; u < U; u++) {
+                    if (jj + u < nb) {
+                        T b[VEC];
+#pragma unroll
+                        for (uint32_t v = 0; v < VEC; v++) b[v] = xs[(jj + u) * N + cv + v];
+#pragma unroll
+                        for (uint32_t r = 0; r < RP; r++) {
+                            T a[VEC];
+                            Vec16<T>::unpack(raw[u][r], a);
+#pragma unroll
+                            for (uint32_t v = 0; v < VEC; v++) A::mac(acc[r][v], a[v], b[v]);
+                        }
                     }
                 }
             }
@@ -321,7 +360,7 @@ __device__ void mac_item(const KeyDev &K, const JobDev &J, uint32_t item, unsign
             const uint32_t row = rbase + r0 + RS * r;
             if (row < K.k) {
 #pragma unroll
-                for (uint32_t v = 0; v < VEC; v++) part[(size_t)row * N + cv + v] = A::finish(acc[r][v], K);
+                for (uint32_t v = 0; v < VEC; v++) gbl(part)[(size_t)row * N + cv + v] = A::finish(acc[r][v], K);
             }
         }
     }
@@ -341,7 +380,7 @@ __device__ void finish_item(const KeyDev &K, const JobDev &J, uint32_t row, unsi
     const T *part = reinterpret_cast<const T *>(part_base + J.part);
     load_twiddles<LOG_N>(K, tws);
     uint64_t s = 0; // < items * q <= 2^13 * 2^50
-    for (uint32_t it = g; it < J.nitems; it += G) s += part[((size_t)it * K.k + row) * N + c];
+    for (uint32_t it = g; it < J.nitems; it += G) s += gbl(part)[((size_t)it * K.k + row) * N + c];
     red[threadIdx.x] = s;
     __syncthreads();
     if (g == 0) {
@@ -353,7 +392,7 @@ __device__ void finish_item(const KeyDev &K, const JobDev &J, uint32_t row, unsi
     __syncthreads();
     inverse_ntt<KIND, LOG_N>(K, x, tws);
     T *out = reinterpret_cast<T *>(J.out) + (size_t)row * N;
-    for (uint32_t i = threadIdx.x; i < N; i += CB_THREADS) out[i] = x[i];
+    for (uint32_t i = threadIdx.x; i < N; i += CB_THREADS) gbl(out)[i] = x[i];
 }
 
 template <int KIND, int LOG_N>
@@ -372,10 +411,10 @@ __global__ __launch_bounds__(CB_THREADS) void wide_key_ntt_kernel(KeyDev K, cons
     for (size_t base = (size_t)blockIdx.x * SB; base < npolys; base += (size_t)gridDim.x * SB) {
         const uint32_t nb = (uint32_t)min<size_t>(SB, npolys - base);
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nb * N; i += CB_THREADS) xs[i] = A::reduce_any(src[base * N + i], K);
+        for (uint32_t i = threadIdx.x; i < nb * N; i += CB_THREADS) xs[i] = A::reduce_any(gbl(src)[base * N + i], K);
         __syncthreads();
         forward_ntt<KIND, LOG_N>(K, xs, nb, tws);
-        for (uint32_t i = threadIdx.x; i < nb * N; i += CB_THREADS) dst[base * N + i] = xs[i];
+        for (uint32_t i = threadIdx.x; i < nb * N; i += CB_THREADS) gbl(dst)[base * N + i] = xs[i];
     }
 }
 
@@ -398,7 +437,7 @@ __global__ void batch_reset_kernel(uint32_t *state, uint32_t words)
     default: FN<WK_W64, 8>(__VA_ARGS__); break;                                                                       \
     }
 
-__global__ __launch_bounds__(CB_THREADS) void commit_batch_kernel(const KeyDev *__restrict__ keys,
+__global__ __launch_bounds__(CB_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void commit_batch_kernel(const KeyDev *__restrict__ keys,
                                                                  const JobDev *__restrict__ jobs,
                                                                  const uint2 *__restrict__ items, uint32_t n_a,
                                                                  uint32_t n_total, uint32_t *state,
@@ -418,10 +457,16 @@ __global__ __launch_bounds__(CB_THREADS) void commit_batch_kernel(const KeyDev *
         const KeyDev K = keys[J.key];
         if (it < n_a) {
             CB_DISPATCH(mac_item, K, J, w.y, smem, tws, part_base);
-            // every thread publishes its partial stores before the count moves
-            __threadfence();
+            // Publish the partial: each wave waits until its own stores have
+            // completed (vmcnt(0) also counts stores on gfx9), then one agent-scope
+            // release fence (L2 write-back) covers the whole block before the
+            // count moves.  A fence per wave costs a write-back each.
+            __builtin_amdgcn_s_waitcnt(0x0F70);
             __syncthreads();
-            if (threadIdx.x == 0) atomicAdd(&done[w.x], 1u);
+            if (threadIdx.x == 0) {
+                __threadfence();
+                atomicAdd(&done[w.x], 1u);
+            }
         } else {
             if (threadIdx.x == 0) {
                 uint32_t spins = 0;
@@ -543,14 +588,9 @@ uint32_t env_chunk()
     return (v >= 1 && v <= CB_MAX_CHUNK) ? (uint32_t)v : 0;
 }
 
-uint32_t default_chunk(size_t k, uint32_t n, uint32_t elem)
-{
-    // about 256 KiB of key per item, a power of two in [16, 1024]
-    const size_t per = std::max<size_t>(1, k * n * elem);
-    uint32_t c = 16;
-    while (c < 1024 && (size_t)c * 2 * per <= (256u << 10)) c *= 2;
-    return c;
-}
+// Untuned default: 512 ring elements per work item was the best common
+// choice for every class on the 512-job mixed batch (16 key rows).
+uint32_t default_chunk(size_t k, uint32_t n, uint32_t elem) { return 512; }
 
 } // namespace
 } // namespace lzx
@@ -852,70 +892,68 @@ extern "C" lzx_status lzx_commit_batch_check(lzx_commit_batch *b, void *stream)
     return LZX_OK;
 }
 
+namespace {
+
+// Mean of `reps` timed runs of the whole batch after one warm-up run.
+lzx_status time_batch(lzx_commit_batch *b, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int reps, float &ms_out)
+{
+    float total = 0, ms = 0;
+    for (int rep = 0; rep <= reps; rep++) {
+        LZX_HIP(hipEventRecord(e0, s));
+        lzx_status st = launch_direct(b, s);
+        if (st != LZX_OK) return st;
+        LZX_HIP(hipEventRecord(e1, s));
+        LZX_HIP(hipEventSynchronize(e1));
+        LZX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (rep > 0) total += ms;
+    }
+    uint32_t err = 0;
+    LZX_HIP(hipMemcpy(&err, b->d_state + 1, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return device_fail(hipErrorLaunchTimeOut, "commit batch autotune: a dependency wait hit its bound");
+    ms_out = total / reps;
+    return LZX_OK;
+}
+
+} // namespace
+
+// One pass of coordinate descent over the classes present: each class's
+// candidate chunks are timed on the WHOLE batch with the other classes at their
+// current choice (a class alone favours smaller items than the mixed queue).
 extern "C" lzx_status lzx_commit_batch_autotune(lzx_commit_batch *b, void *stream)
 {
     if (!b) return LZX_INVALID_ARGUMENT;
     hipStream_t s = (hipStream_t)stream;
-    static const uint32_t CANDS[] = {16, 32, 64, 128, 256, 512, 1024};
+    static const uint32_t CANDS[] = {64, 128, 256, 512, 1024, 2048};
     hipEvent_t e0, e1;
     LZX_HIP(hipEventCreate(&e0));
     LZX_HIP(hipEventCreate(&e1));
     lzx_status st = LZX_OK;
     for (uint32_t c = 0; c < CB_CLASSES && st == LZX_OK; c++) {
         if (!b->present[c]) continue;
-        // the jobs of this class alone, one plan per candidate chunk
-        std::vector<lzx_commit_job> sub;
-        for (const lzx_commit_job &J : b->jobs)
-            if (job_class(J.key) == c) sub.push_back(J);
+        const uint32_t start = b->chunk[c];
+        uint32_t best_chunk = start;
         float best = 0;
-        uint32_t best_chunk = b->chunk[c];
         for (uint32_t cand : CANDS) {
-            lzx_commit_batch t;
-            t.device = b->device;
-            t.jobs = sub;
-            std::map<const lzx_wide_key *, uint32_t> kidx;
-            for (const lzx_commit_job &J : sub) {
-                auto it = kidx.find(J.key);
-                if (it == kidx.end()) {
-                    it = kidx.emplace(J.key, (uint32_t)t.keys.size()).first;
-                    t.keys.push_back(J.key);
-                }
-                t.job_key.push_back(it->second);
-            }
-            t.chunk[c] = cand;
-            st = plan_batch(&t);
-            float ms = 0, total = 0;
-            for (int rep = 0; rep < 4 && st == LZX_OK; rep++) {
-                if (hipEventRecord(e0, s) != hipSuccess) st = LZX_DEVICE_ERROR;
-                if (st == LZX_OK) st = launch_direct(&t, s);
-                if (st == LZX_OK && hipEventRecord(e1, s) != hipSuccess) st = LZX_DEVICE_ERROR;
-                if (st == LZX_OK && hipEventSynchronize(e1) != hipSuccess) st = LZX_DEVICE_ERROR;
-                if (st == LZX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = LZX_DEVICE_ERROR;
-                if (rep > 0) total += ms; // first run warms up
-            }
-            if (st == LZX_OK) {
-                uint32_t err = 0;
-                if (hipMemcpy(&err, t.d_state + 1, sizeof err, hipMemcpyDeviceToHost) != hipSuccess || err)
-                    st = LZX_DEVICE_ERROR;
-            }
-            free_plan(&t);
-            if (st == LZX_UNSUPPORTED) { // candidate not usable for these sizes
+            b->chunk[c] = cand;
+            st = plan_batch(b);
+            if (st == LZX_UNSUPPORTED) { // item limit: this candidate is too small for the class
                 st = LZX_OK;
                 continue;
             }
+            float ms = 0;
+            if (st == LZX_OK) st = time_batch(b, s, e0, e1, 3, ms);
             if (st != LZX_OK) break;
-            if (best == 0 || total < best) best = total, best_chunk = cand;
+            if (best == 0 || ms < best) best = ms, best_chunk = cand;
         }
-        if (st == LZX_OK) {
-            b->chunk[c] = best_chunk;
+        b->chunk[c] = st == LZX_OK ? best_chunk : start;
+        if (st == LZX_OK)
             for (const lzx_wide_key *K : b->keys)
                 if (job_class(K) == c) cache_put(cache_key(K->kind, K->log_n, K->k), best_chunk);
-        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (st != LZX_OK) return st;
-    return plan_batch(b);
+    lzx_status st2 = plan_batch(b);
+    return st != LZX_OK ? st : st2;
 }
 
 extern "C" lzx_status lzx_commit_batch_info(const lzx_commit_batch *b, size_t *items, size_t *scratch_bytes,
