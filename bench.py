@@ -199,7 +199,7 @@ def bench_sumcheck(reps: int = 3):
             "final_claim": int(out.final_claim)}
 
 
-def bench_uneven_batch(reps: int = 5, autotune: bool = True, only=False):
+def bench_uneven_batch(reps: int = 5, autotune: bool = True):
     """The uneven commitment batch: 512 jobs (workloads.uneven_batch_jobs(0xC5)),
     n in {64, 128, 256}, q in {3221225473, 2013265921, 1125899437080577},
     k = 16, W = 1, m = floor(2^(8+8u)); one persistent launch per batch,
@@ -269,6 +269,15 @@ def cpu_baseline_secondary():
     return res
 
 
+def _copy_calibration(torch):
+    """A plain 64 MiB device copy of known size, for scaling the HBM counters."""
+    src = torch.zeros(1 << 24, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)  # 64 MiB read + 64 MiB write per dispatch
+    torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +286,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-ntt", action="store_true", help="skip the secondary workloads (profiling runs)")
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven commitment batch leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batch leg: use cached (LZX_AUTOTUNE_CACHE) or default chunks instead of tuning")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 64 MiB device copy (counter calibration for --pmc runs)")
     args = ap.parse_args()
@@ -289,15 +300,14 @@ def main():
     from lzx_amd import device as D
     D.require_gpu()
     if args.only_batch:
-        print(json.dumps({"uneven_batch": bench_uneven_batch()}), flush=True)
+        res = bench_uneven_batch(autotune=not args.no_autotune)
+        if args.copy_calib:
+            _copy_calibration(torch)
+        print(json.dumps({"uneven_batch": res}), flush=True)
         return
     r = bench_ntt(args.steps, args.warmup)
     if args.copy_calib:
-        src = torch.zeros(1 << 24, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)  # 64 MiB read + 64 MiB write per dispatch
-        torch.cuda.synchronize()
+        _copy_calibration(torch)
     line = {
         "metric": METRIC,
         "value": round(r["gbs"], 2),
