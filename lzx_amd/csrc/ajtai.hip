@@ -145,15 +145,25 @@ lzx_status ajtai_commit_ntt(const lzx_ajtai_key *key, const uint32_t *s_dev, con
         return device_fail(e, "ajtai partial alloc");
     }
     lzx_status st = LZX_OK;
+    // digit witnesses: conversion fused into the forward NTT where the shape allows
+    bool transformed = false;
     if (digits_dev) {
-        hipLaunchKernelGGL(digits_to_residues_kernel, dim3(grid_for(slen)), dim3(256), 0, stream, digits_dev, shat,
-                           slen, q);
-        e = hipGetLastError();
-    } else {
-        e = hipMemcpyAsync(shat, s_dev, slen * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+        st = ntt_forward_digits(key->plan, digits_dev, shat, W * m, stream);
+        transformed = st == LZX_OK;
+        if (st == LZX_UNSUPPORTED) st = LZX_OK;
     }
-    if (e != hipSuccess) st = device_fail(e, "ajtai witness staging");
-    if (st == LZX_OK) st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, shat, W * m, NTT_FWD, 0, stream);
+    if (st == LZX_OK && !transformed) {
+        if (digits_dev) {
+            hipLaunchKernelGGL(digits_to_residues_kernel, dim3(grid_for(slen)), dim3(256), 0, stream, digits_dev,
+                               shat, slen, q);
+            e = hipGetLastError();
+        } else {
+            e = hipMemcpyAsync(shat, s_dev, slen * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream);
+        }
+        if (e != hipSuccess) st = device_fail(e, "ajtai witness staging");
+        if (st == LZX_OK)
+            st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, shat, W * m, NTT_FWD, 0, stream);
+    }
     if (st == LZX_OK) {
         const uint32_t r64 = (uint32_t)((((unsigned __int128)1) << 64) % q);
         hipLaunchKernelGGL(ajtai_mac_kernel, dim3((uint32_t)(n / tb) * wblocks, iblocks, chunks), dim3(tb * wg * ig), 0, stream, key->Ahat_dev, shat,

@@ -237,6 +237,65 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
     }
 }
 
+// Forward NTT of small-digit polynomials (i8 in, u32 NTT-domain out): the
+// witness transform of the digit-input Ajtai commit with the digit -> residue
+// conversion fused into the load (ring.rs:232-243 from_signed).  One
+// 64-polynomial tile per block: each lane turns one coalesced 16-byte digit
+// chunk (16 coefficients) into four residue chunks in the conflict-free LDS
+// slots, then runs the register butterflies of one polynomial; results leave
+// through the same slots as coalesced 16-byte stores.  n >= 16.
+template <int LOG_N, int AR>
+__global__ __launch_bounds__(64) void ntt_digits_kernel(const int8_t *__restrict__ digits, uint32_t *__restrict__ out,
+                                                        const uint32_t *__restrict__ tab, uint32_t npolys)
+{
+    constexpr uint32_t N = 1u << LOG_N, C = N / 4, D = N / 16; // 16-byte output / digit chunks per polynomial
+    __shared__ __attribute__((aligned(16))) uint32_t T[64 * N];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t p0 = blockIdx.x * 64, np = min(64u, npolys - p0);
+    const uint32_t q = tab[0];
+    const int8_t *__restrict__ dg = digits + (size_t)p0 * N;
+    uint32_t *__restrict__ g = out + (size_t)p0 * N;
+#pragma unroll
+    for (uint32_t i = 0; i < D; i++) {
+        const uint32_t d = 64 * i + lane, p = d / D, c0 = 4 * (d % D);
+        if (p < np) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(dg + 16 * (size_t)d);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+                uint32_t r[4];
+#pragma unroll
+                for (uint32_t e = 0; e < 4; e++) {
+                    const int x = (int8_t)(w[j] >> (8 * e));
+                    r[e] = x < 0 ? q - (uint32_t)(-x) : (uint32_t)x;
+                }
+                *reinterpret_cast<uint4 *>(T + 4 * tile_slot<C>(p, c0 + j)) = make_uint4(r[0], r[1], r[2], r[3]);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t a[N];
+#pragma unroll
+    for (uint32_t c4 = 0; c4 < C; c4++) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(T + 4 * tile_slot<C>(lane, c4));
+        a[4 * c4] = v.x, a[4 * c4 + 1] = v.y, a[4 * c4 + 2] = v.z, a[4 * c4 + 3] = v.w;
+    }
+    ntt_regs<LOG_N, AR, NTT_FWD>(a, tab);
+#pragma unroll
+    for (uint32_t c4 = 0; c4 < C; c4++) {
+        uint32_t vv[4];
+#pragma unroll
+        for (uint32_t e = 0; e < 4; e++) vv[e] = a[brv_bits(4 * c4 + e, LOG_N)];
+        *reinterpret_cast<uint4 *>(T + 4 * tile_slot<C>(lane, c4)) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < C; i++) {
+        const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+        if (p < np) *reinterpret_cast<uint4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const uint4 *>(T + 4 * s);
+    }
+}
+
 template <int AR, int MODE>
 __global__ __launch_bounds__(256) void ntt_lds_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
                                                       uint32_t blocks_per_plan, uint32_t log_n, uint32_t ppb,
@@ -503,6 +562,28 @@ static void build_tables(lzx_ntt_plan *p)
         put(TabLayout::omg_w(n), TabLayout::omg_wp(n), e, (uint32_t)ow);
         ow = ow * omega % q;
     }
+}
+
+lzx_status ntt_forward_digits(const lzx_ntt_plan *plan, const int8_t *digits, uint32_t *out, size_t polys,
+                              hipStream_t stream)
+{
+    if (polys == 0) return LZX_OK;
+    if (plan->log_n < 4 || plan->log_n > 6 || ((uintptr_t)digits % 16) || ((uintptr_t)out % 16))
+        return LZX_UNSUPPORTED;
+    if (polys > 0xFFFFFFFFull - 63) return LZX_UNSUPPORTED;
+    const dim3 grid((uint32_t)((polys + 63) / 64));
+    const uint32_t np = (uint32_t)polys;
+#define LZX_DIGITS_LAUNCH(LN, AR)                                                                                   \
+    hipLaunchKernelGGL((ntt_digits_kernel<LN, AR>), grid, dim3(64), 0, stream, digits, out, plan->dev_tab, np)
+    const bool mont = plan->arith == ARITH_MONT;
+    switch (plan->log_n) {
+    case 4: if (mont) LZX_DIGITS_LAUNCH(4, ARITH_MONT); else LZX_DIGITS_LAUNCH(4, ARITH_SHOUP); break;
+    case 5: if (mont) LZX_DIGITS_LAUNCH(5, ARITH_MONT); else LZX_DIGITS_LAUNCH(5, ARITH_SHOUP); break;
+    default: if (mont) LZX_DIGITS_LAUNCH(6, ARITH_MONT); else LZX_DIGITS_LAUNCH(6, ARITH_SHOUP); break;
+    }
+#undef LZX_DIGITS_LAUNCH
+    LZX_LAUNCH_CHECK("ntt_digits_kernel");
+    return LZX_OK;
 }
 
 static lzx_status run_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t len, NttMode mode, uint32_t levels)
