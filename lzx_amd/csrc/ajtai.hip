@@ -17,10 +17,10 @@
 // All-zero witness elements contribute zero, which is exactly the
 // reference's zero-skip (:130-137).
 #include <algorithm>
-#include <thread>
 #include <vector>
 
 #include "ajtai.hpp"
+#include "keccak.hpp"
 #include "seeded.hpp"
 
 namespace lzx {
@@ -184,12 +184,11 @@ lzx_status ajtai_commit_ntt(const lzx_ajtai_key *key, const uint32_t *s_dev, con
     return st;
 }
 
-static lzx_status key_finish(lzx_ajtai_key *key, const uint32_t *A_host)
+// Ahat = NTT(A) and the int8 limb planes, from A already in key->A_dev.
+static lzx_status key_finish_device(lzx_ajtai_key *key)
 {
     const size_t words = key->k * key->m * key->n;
-    LZX_HIP(hipMalloc(&key->A_dev, words * sizeof(uint32_t)));
     LZX_HIP(hipMalloc(&key->Ahat_dev, words * sizeof(uint32_t)));
-    LZX_HIP(hipMemcpy(key->A_dev, A_host, words * sizeof(uint32_t), hipMemcpyHostToDevice));
     LZX_HIP(hipMemcpy(key->Ahat_dev, key->A_dev, words * sizeof(uint32_t), hipMemcpyDeviceToDevice));
     lzx_status st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, key->Ahat_dev, key->k * key->m, NTT_FWD,
                                0, nullptr);
@@ -200,6 +199,93 @@ static lzx_status key_finish(lzx_ajtai_key *key, const uint32_t *A_host)
     }
     LZX_HIP(hipDeviceSynchronize());
     return LZX_OK;
+}
+
+static lzx_status key_finish(lzx_ajtai_key *key, const uint32_t *A_host)
+{
+    const size_t words = key->k * key->m * key->n;
+    LZX_HIP(hipMalloc(&key->A_dev, words * sizeof(uint32_t)));
+    LZX_HIP(hipMemcpy(key->A_dev, A_host, words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return key_finish_device(key);
+}
+
+// ---- seeded derivation on the device ----------------------------------------
+//
+// RingConfig::uniform_from_seed (ring.rs:102-180) for element idx: one SHAKE-256
+// stream XOF("uniform", salt = domain || seed || u64le(idx)) (transcript.rs:
+// 229-237 framing), read as 4-byte little-endian words from byte 8 on; a word
+// v < 2^32 - (2^32 mod q) gives the next coefficient v mod q.  The reference
+// first reads n + max(n, 16) words and then re-squeezes a fresh XOF for each
+// further word, taking its last 4 bytes; SHAKE output is prefix-stable, so
+// those are exactly the next words of the same stream, and the kernel simply
+// keeps squeezing until n coefficients are accepted.
+//
+// The absorbed message ("LZXX" || u32le 7 || "uniform" || u32le |salt| ||
+// salt) fits one 136-byte block; the host builds that padded block with
+// idx = 0 and the kernel XORs each thread's idx in at byte offset `io`.
+struct XofBlock {
+    uint64_t lane[17];
+};
+
+__global__ __launch_bounds__(256) void derive_uniform_kernel(XofBlock base, uint32_t io, uint32_t q, uint32_t n,
+                                                             uint64_t total, uint32_t *__restrict__ A)
+{
+    const uint32_t limit = (uint32_t)((1ULL << 32) - ((1ULL << 32) % q));
+    const uint32_t L = io >> 3, sh = 8 * (io & 7);
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t st[25];
+#pragma unroll
+        for (int l = 0; l < 25; l++) st[l] = l < 17 ? base.lane[l] : 0;
+#pragma unroll
+        for (int l = 0; l < 16; l++) {
+            if (l == (int)L) st[l] ^= idx << sh;
+            if (l == (int)L + 1 && sh) st[l] ^= idx >> (64 - sh);
+        }
+        keccak_f1600(st);
+        uint32_t *out = A + idx * n;
+        uint32_t got = 0;
+        for (uint32_t blk = 0; got < n; blk++) {
+            if (blk) keccak_f1600(st);
+#pragma unroll
+            for (int w = 0; w < 34; w++) { // the block's 4-byte words; stream words 0 and 1 are skipped
+                const uint32_t v = (uint32_t)(st[w >> 1] >> (32 * (w & 1)));
+                if ((blk || w >= 2) && got < n && v < limit) out[got++] = v % q;
+            }
+        }
+    }
+}
+
+// Host side of the block: absorb the fixed framing + salt with idx = 0, pad.
+static void build_xof_block(const uint8_t *domain, size_t dlen, const uint8_t *seed, size_t slen, XofBlock &blk,
+                            uint32_t &io)
+{
+    uint8_t b[136] = {};
+    size_t o = 0;
+    auto put = [&](const void *p, size_t len) {
+        memcpy(b + o, p, len);
+        o += len;
+    };
+    auto put32 = [&](uint32_t v) {
+        const uint8_t x[4] = {(uint8_t)v, (uint8_t)(v >> 8), (uint8_t)(v >> 16), (uint8_t)(v >> 24)};
+        put(x, 4);
+    };
+    static const uint8_t TAG[] = {'L', 'Z', 'X', 'X'}, DOM[] = {'u', 'n', 'i', 'f', 'o', 'r', 'm'};
+    put(TAG, 4);
+    put32(7);
+    put(DOM, 7);
+    put32((uint32_t)(dlen + slen + 8));
+    put(domain, dlen);
+    put(seed, slen);
+    io = (uint32_t)o;
+    o += 8; // u64le(idx), XORed in per element (o + 1 <= 135: the callers' salts are 39 bytes)
+    b[o] ^= 0x1f;  // SHAKE-256 domain suffix
+    b[135] ^= 0x80; // pad10*1
+    for (int l = 0; l < 17; l++) {
+        uint64_t v = 0;
+        for (int i = 0; i < 8; i++) v |= (uint64_t)b[8 * l + i] << (8 * i);
+        blk.lane[l] = v;
+    }
 }
 
 static lzx_status key_new(uint32_t q, uint32_t adic, uint32_t g, uint32_t log_n, size_t k, size_t m,
@@ -265,18 +351,19 @@ extern "C" lzx_status lzx_ajtai_key_from_seed(uint32_t q, uint32_t two_adicity, 
     lzx_status st = key_new(q, two_adicity, generator, log_n, k, m, norm_bound, out, &key);
     if (st != LZX_OK) return st;
     const size_t total = k * m, n = key->n;
-    std::vector<uint32_t> A(total * n);
     static const uint8_t DOM[] = {'a', 'j', 't', 'a', 'i', '-', 'A'};
-    // ajtai.rs:91-96, element idx = i*m + j; independent elements derived in parallel
-    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> pool;
-    for (unsigned th = 0; th < nthreads; th++)
-        pool.emplace_back([&, th]() {
-            for (size_t idx = th; idx < total; idx += nthreads)
-                uniform_from_seed(q, log_n, DOM, 7, seed, 32, (uint64_t)idx, A.data() + idx * n);
-        });
-    for (auto &t : pool) t.join();
-    st = key_finish(key, A.data());
+    // ajtai.rs:91-96: element idx = i*m + j, derived on the device
+    XofBlock blk;
+    uint32_t io = 0;
+    build_xof_block(DOM, 7, seed, 32, blk, io);
+    hipError_t e = hipMalloc(&key->A_dev, total * n * sizeof(uint32_t));
+    if (e == hipSuccess) {
+        const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(derive_uniform_kernel, dim3(blocks), dim3(256), 0, nullptr, blk, io, q, (uint32_t)n,
+                           (uint64_t)total, key->A_dev);
+        e = hipGetLastError();
+    }
+    st = e == hipSuccess ? key_finish_device(key) : device_fail(e, "derive_uniform_kernel");
     if (st != LZX_OK) {
         lzx_ajtai_key_destroy(key);
         return st;

@@ -57,46 +57,6 @@ lzx_status transcript_challenge(lzx_transcript *t, const char *label, uint64_t *
     return lzx_transcript_challenge_fields(t, (const uint8_t *)label, strlen(label), 1, out);
 }
 
-// ring.rs:102-180 for one element: one XOF squeeze of 8 + 4(n + max(n,16))
-// bytes walked in 4-byte LE windows from offset 8, accepting w < limit;
-// the counter-indexed continuation covers the (negligible) shortfall.
-void uniform_from_seed(uint32_t q, uint32_t log_n, const uint8_t *domain, size_t dlen, const uint8_t *seed,
-                       size_t slen, uint64_t index, uint32_t *coeffs)
-{
-    const size_t n = (size_t)1 << log_n;
-    const uint64_t two32 = 1ULL << 32;
-    const uint64_t limit = two32 - (two32 % q);
-    std::vector<uint8_t> salt(dlen + slen + 8);
-    memcpy(salt.data(), domain, dlen);
-    memcpy(salt.data() + dlen, seed, slen);
-    for (int b = 0; b < 8; b++) salt[dlen + slen + b] = (uint8_t)(index >> (8 * b));
-    const size_t slack = n > 16 ? n : 16;
-    const size_t stream_len = 8 + 4 * (n + slack);
-    std::vector<uint8_t> stream(stream_len);
-    Sponge sp;
-    static const uint8_t DOM[] = {'u', 'n', 'i', 'f', 'o', 'r', 'm'};
-    xof_begin(sp, DOM, 7, salt.size());
-    sp.absorb(salt.data(), salt.size());
-    sp.finalize(stream.data(), stream_len);
-    size_t got = 0;
-    for (size_t w = 0; w < n + slack && got < n; w++) {
-        const uint8_t *p = stream.data() + 8 + 4 * w;
-        const uint64_t cand = (uint64_t)p[0] | ((uint64_t)p[1] << 8) | ((uint64_t)p[2] << 16) | ((uint64_t)p[3] << 24);
-        if (cand < limit) coeffs[got++] = (uint32_t)(cand % q);
-    }
-    size_t counter = (stream_len - 8) / 4;
-    while (got < n) { // ring.rs:155-175
-        const size_t blen = 8 + counter * 4 + 4;
-        std::vector<uint8_t> bytes(blen);
-        xof_begin(sp, DOM, 7, salt.size());
-        sp.absorb(salt.data(), salt.size());
-        sp.finalize(bytes.data(), blen);
-        const uint8_t *p = bytes.data() + blen - 4;
-        const uint64_t cand = (uint64_t)p[0] | ((uint64_t)p[1] << 8) | ((uint64_t)p[2] << 16) | ((uint64_t)p[3] << 24);
-        if (cand < limit) coeffs[got++] = (uint32_t)(cand % q);
-        counter++;
-    }
-}
 
 } // namespace lzx
 

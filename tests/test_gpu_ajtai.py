@@ -42,6 +42,20 @@ def test_key_derivation_matches_oracle(mods):
         assert not np.array_equal(pk2.matrix()[0], A[0])
 
 
+def test_device_derivation_through_the_rejection_fallback(mods):
+    """from_seed derives A on the device.  At q = 2^31 + 129 half of all words
+    are rejected, so about half of the elements run past the first
+    n + max(n, 16) words into ring.rs:155-175's re-squeeze loop; the oracle
+    restates that loop literally (a fresh XOF per further word)."""
+    CM, D, R = mods
+    q = (2147483777, 7, 3)
+    for log_n, k, m in [(6, 2, 40), (3, 3, 50), (4, 1, 7)]:
+        pk = CM.AjtaiPublicKey.from_seed(_params(mods, log_n, k, m, 16, mod=q), bytes([0x5A]) * 32)
+        A, Ah = O.ajtai_from_seed(q, log_n, k, m, bytes([0x5A]) * 32)
+        assert np.array_equal(pk.matrix(), A), (log_n, k, m)
+        assert np.array_equal(pk.matrix_ntt(), Ah), (log_n, k, m)
+
+
 def test_reference_shapes_match_golden_commitments(mods):
     """ajtai.rs:490-506 shapes; expected values from schoolbook (gen_golden.py)."""
     CM, D, R = mods
