@@ -167,7 +167,7 @@ def bench_ajtai(reps: int = 5):
 
 def bench_sumcheck(reps: int = 3):
     """Sum-check prove of eq(r,x)*(a(x)*b(x) - c(x)) over 2^24-entry Goldilocks
-    tables with the Fiat-Shamir transcript (24 rounds of host round trips).
+    tables with the Fiat-Shamir transcript (run on the device; one host sync).
     Algorithmic bytes with 4 materialised tables: sum_j 4*8*(2^(24-j) + 2^(23-j))."""
     import torch
 
@@ -286,6 +286,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-ntt", action="store_true", help="skip the secondary workloads (profiling runs)")
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven commitment batch leg")
+    ap.add_argument("--only-sumcheck", action="store_true", help="run only the 2^24 sum-check prove leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch leg: use cached (LZX_AUTOTUNE_CACHE) or default chunks instead of tuning")
     ap.add_argument("--copy-calib", action="store_true",
@@ -299,6 +300,9 @@ def main():
 
     from lzx_amd import device as D
     D.require_gpu()
+    if args.only_sumcheck:
+        print(json.dumps({"sumcheck_prove": bench_sumcheck(reps=5)}), flush=True)
+        return
     if args.only_batch:
         res = bench_uneven_batch(autotune=not args.no_autotune)
         if args.copy_calib:

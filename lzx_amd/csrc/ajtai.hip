@@ -138,10 +138,10 @@ lzx_status ajtai_commit_ntt(const lzx_ajtai_key *key, const uint32_t *s_dev, con
     const size_t plane = W * k * n;
 
     uint32_t *shat = nullptr, *partial = nullptr;
-    LZX_HIP(hipMallocAsync((void **)&shat, slen * sizeof(uint32_t), stream));
-    hipError_t e = hipMallocAsync((void **)&partial, (size_t)chunks * plane * sizeof(uint32_t), stream);
+    LZX_HIP(scratch_alloc((void **)&shat, slen * sizeof(uint32_t), stream));
+    hipError_t e = scratch_alloc((void **)&partial, (size_t)chunks * plane * sizeof(uint32_t), stream);
     if (e != hipSuccess) {
-        (void)hipFreeAsync(shat, stream);
+        (void)scratch_free(shat, stream);
         return device_fail(e, "ajtai partial alloc");
     }
     lzx_status st = LZX_OK;
@@ -179,8 +179,8 @@ lzx_status ajtai_commit_ntt(const lzx_ajtai_key *key, const uint32_t *s_dev, con
         if (e != hipSuccess) st = device_fail(e, "ajtai_reduce_kernel");
     }
     if (st == LZX_OK) st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, out_dev, W * k, NTT_INV, 0, stream);
-    (void)hipFreeAsync(partial, stream);
-    (void)hipFreeAsync(shat, stream);
+    (void)scratch_free(partial, stream);
+    (void)scratch_free(shat, stream);
     return st;
 }
 

@@ -265,7 +265,7 @@ lzx_status ajtai_commit_int8(const lzx_ajtai_key *key, const int8_t *digits_dev,
     const size_t plane = W * (size_t)k * 64;
 
     uint32_t *partial = nullptr;
-    LZX_HIP(hipMallocAsync((void **)&partial, (size_t)nchunks * plane * sizeof(uint32_t), stream));
+    LZX_HIP(scratch_alloc((void **)&partial, (size_t)nchunks * plane * sizeof(uint32_t), stream));
     lzx_status st = LZX_OK;
     for (uint32_t col0 = 0; col0 < npad && st == LZX_OK; col0 += 64) {
         const uint32_t nt = std::min<uint32_t>(4, (npad - col0) / 16);
@@ -298,7 +298,7 @@ lzx_status ajtai_commit_int8(const lzx_ajtai_key *key, const int8_t *digits_dev,
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) st = device_fail(e, "ajtai_i8_reduce_kernel");
     }
-    (void)hipFreeAsync(partial, stream);
+    (void)scratch_free(partial, stream);
     return st;
 }
 

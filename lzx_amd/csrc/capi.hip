@@ -1,11 +1,19 @@
-// Status strings and HIP error capture for the C ABI (include/lzx_gpu.h).
+// Status strings, HIP error capture and the library's scratch allocators for
+// the C ABI (include/lzx_gpu.h).
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "runtime.hpp"
 
 namespace {
 thread_local std::string g_last_error;
-}
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_alloc_mu;
+hipMemPool_t g_pools[MAX_DEVICES];            // one stream-ordered pool per device
+std::vector<lzx::HostStage> g_stages;         // idle pinned staging buffers
+} // namespace
 
 namespace lzx {
 
@@ -15,6 +23,71 @@ lzx_status device_fail(hipError_t e, const char *what)
     if (e == hipErrorOutOfMemory) return LZX_DEVICE_OUT_OF_MEMORY;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return LZX_NO_DEVICE;
     return LZX_DEVICE_ERROR;
+}
+
+// Stream-ordered scratch from a per-device pool that keeps freed memory for
+// reuse (release threshold = unlimited), so a call that needs the same
+// working set as the last one maps no new memory.  The pool holds the
+// largest working set seen until the process ends.
+hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t s)
+{
+    hipDevice_t dev = 0;
+    hipError_t e = hipStreamGetDevice(s, &dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipMallocAsync(p, bytes, s);
+    hipMemPool_t pool;
+    {
+        std::lock_guard<std::mutex> g(g_alloc_mu);
+        if (!g_pools[dev]) {
+            hipMemPoolProps props{};
+            props.allocType = hipMemAllocationTypePinned;
+            props.location.type = hipMemLocationTypeDevice;
+            props.location.id = dev;
+            hipMemPool_t np = nullptr;
+            e = hipMemPoolCreate(&np, &props);
+            if (e != hipSuccess) return e;
+            uint64_t keep = UINT64_MAX;
+            e = hipMemPoolSetAttribute(np, hipMemPoolAttrReleaseThreshold, &keep);
+            if (e != hipSuccess) {
+                (void)hipMemPoolDestroy(np);
+                return e;
+            }
+            g_pools[dev] = np;
+        }
+        pool = g_pools[dev];
+    }
+    return hipMallocFromPoolAsync(p, bytes, pool, s);
+}
+
+hipError_t scratch_free(void *p, hipStream_t s) { return p ? hipFreeAsync(p, s) : hipSuccess; }
+
+// Pinned host staging buffers, reused across calls and threads: a call takes
+// an idle buffer of sufficient size (or allocates one) and hands it back.
+hipError_t stage_acquire(HostStage &h, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> g(g_alloc_mu);
+        for (size_t i = 0; i < g_stages.size(); i++)
+            if (g_stages[i].cap >= bytes) {
+                h = g_stages[i];
+                g_stages[i] = g_stages.back();
+                g_stages.pop_back();
+                return hipSuccess;
+            }
+    }
+    const size_t cap = bytes < 65536 ? 65536 : bytes;
+    h.cap = 0;
+    hipError_t e = hipHostMalloc(&h.p, cap, 0);
+    if (e == hipSuccess) h.cap = cap;
+    return e;
+}
+
+void stage_release(HostStage &h)
+{
+    if (!h.p) return;
+    std::lock_guard<std::mutex> g(g_alloc_mu);
+    g_stages.push_back(h);
+    h = HostStage{};
 }
 
 } // namespace lzx

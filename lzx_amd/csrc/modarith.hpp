@@ -183,13 +183,53 @@ LZX_HD uint64_t gl_sub(uint64_t a, uint64_t b) // field.rs:82-89
     return gl_canon(d);
 }
 
-// Reduce hi*2^64 + lo with 2^64 = 2^32 - 1 and 2^96 = -1 (mod p).
+// a - b for canonical a, b: a - b, or a - b + 2^64 - eps = a - b + p when a < b,
+// both already in [0, p).
+LZX_HD uint64_t gl_sub_c(uint64_t a, uint64_t b)
+{
+    uint64_t d = a - b;
+    if (a < b) d -= GL_EPS;
+    return d;
+}
+
+// A representative in [0, 2^64) of a + b for any a and canonical b (not
+// reduced below p): a wrapped sum is below b < p, so adding eps cannot wrap.
+LZX_HD uint64_t gl_add_nc(uint64_t a, uint64_t b)
+{
+    uint64_t s = a + b;
+    if (s < b) s += GL_EPS;
+    return s;
+}
+
+// Full 128-bit product: four 32x32->64 multiply-adds on the device.
+LZX_HD void mul128(uint64_t a, uint64_t b, uint64_t &hi, uint64_t &lo)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0;
+    const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);      // < 2^64
+    const uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;      // < 2^64
+    lo = (uint32_t)p00 | (u << 32);
+    hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);          // <= 2^64 - 1
+#else
+    const unsigned __int128 pr = (unsigned __int128)a * b;
+    hi = (uint64_t)(pr >> 64);
+    lo = (uint64_t)pr;
+#endif
+}
+
+// Reduce hi*2^64 + lo (any hi, lo) to [0, p) with 2^64 = 2^32 - 1 and
+// 2^96 = -1 (mod p).
 LZX_HD uint64_t gl_reduce128(uint64_t hi, uint64_t lo)
 {
-    uint64_t hh = hi >> 32, hl = hi & GL_EPS;
+    const uint32_t hh = (uint32_t)(hi >> 32), hl = (uint32_t)hi;
     uint64_t t0 = lo - hh;
     if (lo < hh) t0 -= GL_EPS; // borrowed 2^64 = p + eps
-    uint64_t t1 = (hl << 32) - hl; // hl * (2^32 - 1)
+    uint64_t sh = (uint64_t)hl << 32;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(sh)); // keeps hl * (2^32 - 1) a subtraction, not a 64-bit multiply
+#endif
+    const uint64_t t1 = sh - hl; // hl * (2^32 - 1)
     uint64_t r = t0 + t1;
     if (r < t1) r += GL_EPS;
     return gl_canon(r);
@@ -197,7 +237,9 @@ LZX_HD uint64_t gl_reduce128(uint64_t hi, uint64_t lo)
 
 LZX_HD uint64_t gl_mul(uint64_t a, uint64_t b)
 {
-    return gl_reduce128(mulhi64(a, b), a * b);
+    uint64_t hi, lo;
+    mul128(a, b, hi, lo);
+    return gl_reduce128(hi, lo);
 }
 
 // Lazy sum: true value = acc + carries * 2^64.
@@ -213,13 +255,21 @@ LZX_HD void gl_acc_add(GlAcc &s, uint64_t v)
     s.acc = n;
 }
 
-LZX_HD uint64_t gl_acc_finish(const GlAcc &s)
+// A representative in [0, 2^64) of the lazy sum: carries * 2^64 + acc =
+// carries * (2^32 - 1) + acc (mod p).  c <= (2^32 - 1)^2, so a wrapped r is
+// below c and adding eps cannot wrap again.
+LZX_HD uint64_t gl_acc_value(const GlAcc &s)
 {
-    // carries * 2^64 + acc = carries*(2^32-1) + acc (mod p); fits 128 bits.
-    uint64_t c = (uint64_t)s.carries * GL_EPS;
+    uint64_t sh = (uint64_t)s.carries << 32;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(sh)); // a subtraction, not a 64-bit multiply
+#endif
+    const uint64_t c = sh - s.carries;
     uint64_t r = s.acc + c;
-    uint64_t hi = r < c ? 1 : 0;
-    return gl_reduce128(hi, r);
+    if (r < c) r += GL_EPS;
+    return r;
 }
+
+LZX_HD uint64_t gl_acc_finish(const GlAcc &s) { return gl_canon(gl_acc_value(s)); }
 
 } // namespace lzx

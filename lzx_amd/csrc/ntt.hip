@@ -745,7 +745,7 @@ extern "C" lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_
     if (len == 0) return LZX_OK;
     hipStream_t s = (hipStream_t)stream;
     uint32_t *tmp = nullptr;
-    LZX_HIP(hipMallocAsync((void **)&tmp, len * sizeof(uint32_t), s));
+    LZX_HIP(scratch_alloc((void **)&tmp, len * sizeof(uint32_t), s));
     lzx_status st = LZX_OK;
     hipError_t e = hipMemcpyAsync(tmp, b_dev, len * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess && out_dev != a_dev)
@@ -762,6 +762,6 @@ extern "C" lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_
         if (le != hipSuccess) st = device_fail(le, "pointwise_mul_kernel");
     }
     if (st == LZX_OK) st = ntt_launch(&plan, 1, out_dev, polys, NTT_INV, 0, s);
-    (void)hipFreeAsync(tmp, s);
+    (void)scratch_free(tmp, s);
     return st;
 }

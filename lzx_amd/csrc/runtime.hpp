@@ -13,6 +13,19 @@ namespace lzx {
 // status code.
 lzx_status device_fail(hipError_t e, const char *what);
 
+// Stream-ordered device scratch from the library's per-device pool (freed
+// memory stays cached in the pool for the next call).
+hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t s);
+hipError_t scratch_free(void *p, hipStream_t s);
+
+// A pinned host buffer from the library's reuse list.
+struct HostStage {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+hipError_t stage_acquire(HostStage &h, size_t bytes);
+void stage_release(HostStage &h);
+
 } // namespace lzx
 
 #define LZX_HIP(call)                                                                                        \

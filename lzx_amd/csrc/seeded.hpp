@@ -1,10 +1,19 @@
-// Host-side transcript helpers shared across modules.
+// The Fiat-Shamir transcript object and its host-side helpers, shared by the
+// transcript C ABI and the sum-check prover (which runs the same transcript
+// on the device between rounds and hands the state back).
 #pragma once
 
 #include <stddef.h>
 #include <stdint.h>
 
+#include "keccak.hpp"
 #include "runtime.hpp"
+
+struct lzx_transcript {
+    lzx::Sponge sponge;
+    uint64_t query_count;
+    uint64_t query_budget;
+};
 
 namespace lzx {
 

@@ -14,15 +14,18 @@
 // results are exactly its pair for the next round.  So each round streams the
 // tables once (read 2*half, write half per factor).  Round 0 reads the
 // caller's tables (left untouched) and writes the folded tables to scratch;
-// later rounds fold in place in scratch.  Per-block partial sums (canonical)
-// are reduced by sc_reduce_kernel; the d+1 values go to the host for the
-// transcript.
+// later rounds fold in place in scratch.  Each large kernel publishes
+// per-block totals and a one-block sc_close_kernel sums them and runs the
+// round's transcript step; once a fold's half drops to SC_TAIL_HALF, a single
+// sc_tail_kernel block runs every remaining round.  All rounds are enqueued
+// at once and the host synchronizes once.
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "modarith.hpp"
 #include "runtime.hpp"
+#include "keccak.hpp"
 #include "seeded.hpp"
 
 namespace lzx {
@@ -33,11 +36,14 @@ constexpr int SC_MAXD = 8;
 constexpr int SC_BLOCK = 256;
 constexpr uint32_t SC_MAX_BLOCKS = 2048;
 
+// Terms with the factor powers shared by every term pulled out:
+// sum_t c_t prod_f v_f^e_tf = prod_f v_f^common_f * sum_t c_t prod_f v_f^(e_tf - common_f).
+// For eq * (a*b - c) this is eq * (a*b - c): two products per point instead of three.
 struct ScTerms {
-    uint32_t nterms, nf, d;
+    uint32_t nterms, nf, d, has_common;
     uint64_t coeff[SC_MAXT];
-    uint8_t exps[SC_MAXT][SC_MAXF]; // multiplicity of factor f in term t
-    uint8_t first[SC_MAXT];         // a factor with exps > 0 (seeds the product)
+    uint8_t exps[SC_MAXT][SC_MAXF]; // multiplicity of factor f in term t beyond the common part
+    uint8_t common[SC_MAXF];        // multiplicity of factor f in every term
 };
 
 struct ScTables {
@@ -45,11 +51,11 @@ struct ScTables {
     uint64_t *dst[SC_MAXF];
 };
 
-// sum_terms c * prod_f v[f]^e over the current point values v.
+// sum_terms c * prod_f v[f]^e over the point values v (any representatives
+// below 2^64); the result is a representative below 2^64.
 template <int NF>
 __device__ __forceinline__ uint64_t eval_terms(const uint64_t (&v)[NF], const ScTerms &T)
 {
-    uint64_t total = 0;
     GlAcc acc{0, 0};
     for (uint32_t t = 0; t < T.nterms; t++) {
         uint64_t prod = 1;
@@ -63,14 +69,21 @@ __device__ __forceinline__ uint64_t eval_terms(const uint64_t (&v)[NF], const Sc
             }
         }
         const uint64_t c = T.coeff[t];
-        if (c == GL_P - 1)
+        if (c == GL_P - 1) {
+            prod = gl_canon(prod); // a lone factor may be an unreduced point value
             prod = prod == 0 ? 0 : GL_P - prod;
-        else if (c != 1)
+        } else if (c != 1) {
             prod = gl_mul(prod, c);
+        }
         gl_acc_add(acc, prod);
     }
-    (void)total;
-    return gl_acc_finish(acc);
+    uint64_t sum = gl_acc_value(acc); // exact mod p; reduced by the products below or by the caller's sum
+    if (T.has_common) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+            for (uint32_t x = 0; x < T.common[f]; x++) sum = gl_mul(sum, v[f]);
+    }
+    return sum;
 }
 
 // Accumulate g(t), t = 0..d, of one pair (lo, hi) into acc[].
@@ -81,7 +94,7 @@ __device__ __forceinline__ void eval_pair(const uint64_t (&lo)[NF], const uint64
     uint64_t diff[NF], cur[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-        diff[f] = gl_sub(hi[f], lo[f]);
+        diff[f] = gl_sub_c(hi[f], lo[f]);
         cur[f] = lo[f];
     }
     gl_acc_add(acc[0], eval_terms<NF>(cur, T));
@@ -92,32 +105,301 @@ __device__ __forceinline__ void eval_pair(const uint64_t (&lo)[NF], const uint64
     for (int t = 2; t <= D; t++) {
         if ((uint32_t)t > T.d) break;
 #pragma unroll
-        for (int f = 0; f < NF; f++) cur[f] = gl_add(cur[f], diff[f]); // lo + t*(hi-lo)
+        for (int f = 0; f < NF; f++) cur[f] = gl_add_nc(cur[f], diff[f]); // lo + t*(hi-lo), unreduced
         gl_acc_add(acc[t], eval_terms<NF>(cur, T));
     }
 }
 
-// Block-reduce the per-thread lazy sums and write canonical partials.
-template <int D>
-__device__ __forceinline__ void block_store(GlAcc (&acc)[D + 1], uint64_t *__restrict__ partial, uint32_t dd)
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
 {
-    __shared__ uint64_t red[SC_BLOCK];
-    for (uint32_t t = 0; t <= dd; t++) {
-        red[threadIdx.x] = gl_acc_finish(acc[t]);
-        __syncthreads();
-        for (uint32_t s = SC_BLOCK / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] = gl_add(red[threadIdx.x], red[threadIdx.x + s]);
-            __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = gl_add(v, __shfl_xor(v, o));
+    return v;
+}
+
+constexpr int SC_TAIL_BLOCK = 512;
+constexpr int SC_MAX_WAVES = SC_TAIL_BLOCK / 64;
+// The rounds whose fold has half <= SC_TAIL_HALF run in one single-block kernel.
+constexpr uint64_t SC_TAIL_HALF = 2048;
+
+struct ScShared {
+    uint64_t wsum[SC_MAX_WAVES][SC_MAXD + 1];
+    uint64_t ev[SC_MAXD + 1]; // the round polynomial's values at 0..d
+    uint64_t chal;            // the latest challenge (tail kernel)
+    uint64_t given[64];       // caller-supplied challenges (tail kernel)
+    uint8_t msg[128]; // 22 + 8 (d+1) + 30 <= 124 transcript bytes per round
+    uint32_t status;
+};
+
+// Canonical block totals of the per-thread lazy sums acc[0..dd] into out[].
+// The loop runs to the compile-time D so acc[] stays in registers.
+template <int D, int WAVES>
+__device__ __forceinline__ void block_totals(GlAcc (&acc)[D + 1], uint32_t dd, ScShared &sh, uint64_t *out)
+{
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int t = 0; t <= D; t++)
+        if ((uint32_t)t <= dd) {
+            const uint64_t v = wave_sum(gl_acc_finish(acc[t]));
+            if (lane == 0) sh.wsum[wave][t] = v;
         }
-        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * (dd + 1) + t] = red[0];
-        __syncthreads();
+    __syncthreads();
+    if (threadIdx.x <= dd) {
+        uint64_t s = sh.wsum[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < WAVES; w++) s = gl_add(s, sh.wsum[w][threadIdx.x]);
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// Thread 0 publishes the block's totals for the close kernel.
+template <int D>
+__device__ __forceinline__ void publish_totals(GlAcc (&acc)[D + 1], uint32_t dd, ScShared &sh,
+                                               uint64_t *__restrict__ partial)
+{
+    block_totals<D, SC_BLOCK / 64>(acc, dd, sh, sh.ev);
+    if (threadIdx.x <= dd) partial[(size_t)blockIdx.x * (dd + 1) + threadIdx.x] = sh.ev[threadIdx.x];
+}
+
+// ---- device-resident rounds -------------------------------------------------
+//
+// The prover's Fiat-Shamir transcript (transcript.rs:36-238) lives in HBM
+// during a prove, so all rounds are enqueued at once with no host round trip.
+// Closing round j sums the blocks' totals into g_j, absorbs it
+// (append_field_slice "sumcheck-round", sumcheck.rs:110-112), draws the
+// challenge (challenge_fields "sumcheck-challenge": "CHAL" framing, squeeze
+// from a copy of the raw state, 16-byte chunks whose first 8 bytes LE are
+// accepted below p), checks g_j(0) + g_j(1) against the running claim
+// (sumcheck.rs:120-123) and interpolates the next claim.  The first failure is
+// kept in `status`; later rounds then leave the transcript alone.  Large rounds
+// are closed by sc_close_kernel between fold kernels; the small rounds at the
+// end run in sc_tail_kernel, one block that folds, sums and runs the
+// transcript round after round with the sponge held in registers.
+struct ScDevTx {
+    uint64_t st[25];
+    uint32_t pos, pad;
+    uint64_t query_count, query_budget;
+};
+
+struct ScCtl {
+    ScDevTx tx;
+    uint64_t current; // running claim
+    uint32_t status;  // lzx_status of the first failing round (0: none)
+    uint32_t pad;
+};
+
+struct ScLagrange {
+    uint64_t w[SC_MAXD + 1]; // w_i = prod_{j != i} (i - j)^{-1}, nodes 0..d
+};
+
+struct ScRoundArgs {
+    ScCtl *ctl;
+    uint64_t *rounds;      // [num_vars][width]
+    uint64_t *chal;        // [num_vars]
+    const uint64_t *given; // caller-supplied challenges, or nullptr for the transcript
+    uint64_t *claims;      // [nf] the fully folded tables
+    uint32_t width;
+    ScLagrange L;
+};
+
+// The transcript and running claim as wave 0 holds them: lane l < 50 keeps
+// half l % 2 of sponge word l / 2 in the bit-interleaved form.
+struct TxRegs {
+    uint32_t a;
+    uint32_t pos, status;
+    uint64_t qc, budget, current;
+};
+
+__device__ __forceinline__ TxRegs tx_load(const ScCtl *ctl, uint32_t lane)
+{
+    TxRegs s;
+    const uint64_t word = lane < 50 ? ctl->tx.st[lane >> 1] : 0;
+    s.a = (uint32_t)(bit_unshuffle64(word) >> (32 * (lane & 1)));
+    s.pos = ctl->tx.pos;
+    s.status = ctl->status;
+    s.qc = ctl->tx.query_count;
+    s.budget = ctl->tx.query_budget;
+    s.current = ctl->current;
+    return s;
+}
+
+__device__ __forceinline__ void tx_store(ScCtl *ctl, const TxRegs &s, uint32_t lane)
+{
+    const uint32_t other = lane_get(s.a, lane ^ 1);
+    if (lane < 50 && (lane & 1) == 0) ctl->tx.st[lane >> 1] = bit_shuffle64((uint64_t)s.a | (uint64_t)other << 32);
+    if (lane == 0) {
+        ctl->tx.pos = s.pos;
+        ctl->tx.query_count = s.qc;
+        ctl->current = s.current;
+        ctl->status = s.status;
     }
 }
 
+// The round's transcript bytes in one buffer: append_field_slice framing
+// u32le|label| "sumcheck-round" u32le(8 (d+1)), the d+1 values (u64 LE, written
+// per round), then challenge_fields' framing "CHAL" u32le|label|
+// "sumcheck-challenge" u32le(1) (transcript.rs:70-120).
+__device__ __forceinline__ uint32_t tx_frame_byte(uint32_t i, uint32_t width)
+{
+    const uint32_t vend = 22 + 8 * width;
+    if (i < 4) return i == 0 ? 14 : 0;
+    if (i < 18) return (uint8_t)"sumcheck-round"[i - 4];
+    if (i < 22) return ((8 * width) >> (8 * (i - 18))) & 0xFF;
+    if (i < vend) return 0;
+    i -= vend;
+    if (i < 4) return (uint8_t)"CHAL"[i];
+    if (i < 8) return i == 4 ? 18 : 0;
+    if (i < 26) return (uint8_t)"sumcheck-challenge"[i - 8];
+    if (i < 30) return i == 26 ? 1 : 0;
+    return 0;
+}
+
+// Wave 0: the constant framing bytes of sh.msg.
+__device__ __forceinline__ void tx_frame_init(ScShared &sh, uint32_t width, uint32_t lane)
+{
+    for (uint32_t i = lane; i < sizeof(sh.msg); i += 64) sh.msg[i] = (uint8_t)tx_frame_byte(i, width);
+}
+
+// XOR len bytes of msg into the sponge at its position (rate 136 = 17 words),
+// permuting whenever the rate fills.
+__device__ __forceinline__ void tx_absorb(TxRegs &s, const uint8_t *msg, uint32_t len, uint32_t lane)
+{
+    for (uint32_t off = 0; off < len;) {
+        const uint32_t take = min(136u - s.pos, len - off);
+        if (lane < 34) {
+            uint64_t xw = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < 8; b++) {
+                const uint32_t g = 8 * (lane >> 1) + b;
+                if (g >= s.pos && g < s.pos + take) xw |= (uint64_t)msg[off + g - s.pos] << (8 * b);
+            }
+            s.a ^= (uint32_t)(bit_unshuffle64(xw) >> (32 * (lane & 1)));
+        }
+        s.pos += take;
+        off += take;
+        if (s.pos == 136) {
+            s.a = keccak_f1600_lanes(s.a, lane);
+            s.pos = 0;
+        }
+    }
+}
+
+// Wave 0: close round rnd from sh.ev (transcript step, claim check, next
+// claim).  sh.msg holds the framing bytes (tx_frame_init).
+template <int D>
+__device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t rnd, uint64_t given)
+{
+    const uint32_t lane = threadIdx.x & 63, width = A.width;
+    if (s.status) return; // an earlier round failed
+    if (lane < width) A.rounds[(size_t)rnd * width + lane] = sh.ev[lane];
+    // g(ch) = sum_i ev_i w_i prod_{j != i} (ch - j) (sumcheck.rs:286-306): node i on lane i;
+    // ev_i w_i does not depend on the challenge
+    const uint64_t evw = lane < width ? gl_mul(sh.ev[lane], A.L.w[lane]) : 0;
+    uint64_t ch = given;
+    if (!A.given) {
+        for (uint32_t i = lane; i < 8 * width; i += 64) sh.msg[22 + i] = (uint8_t)(sh.ev[i >> 3] >> (8 * (i & 7)));
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        // append_field_slice, then challenge_fields' budget check (the count moves
+        // and the append stays absorbed, transcript.rs:109-120) and framing
+        s.qc++;
+        const bool over = s.qc > s.budget;
+        tx_absorb(s, sh.msg, 22 + 8 * width + (over ? 0 : 30), lane);
+        if (over) {
+            s.status = LZX_TRANSCRIPT_REJECTION_BUDGET;
+            return;
+        }
+        // squeeze from a copy: candidate k is stream word 2k (chunk 16k), block (2k) / 17
+        uint32_t cp = s.a;
+        int32_t blk = -1;
+        bool found = false;
+        for (uint32_t k = 0; k < 4096 && !found; k++) { // each candidate is rejected with probability 2^-32
+            const uint32_t w = 2 * k;
+            while ((int32_t)(w / 17) > blk) {
+                cp = keccak_f1600_lanes(cp, lane);
+                blk++;
+            }
+            const uint32_t lo = lane_get(cp, 2 * (w % 17)), hi = lane_get(cp, 2 * (w % 17) + 1);
+            const uint64_t cand = bit_shuffle64((uint64_t)lo | (uint64_t)hi << 32);
+            if (cand < GL_P) {
+                ch = cand;
+                found = true;
+            }
+        }
+        if (!found) { // unreachable in practice (2^-131072); reported rather than looped on
+            s.status = LZX_DEVICE_ERROR;
+            return;
+        }
+    }
+    if (lane == 0) A.chal[rnd] = ch;
+    sh.chal = ch;
+    if (gl_add(sh.ev[0], sh.ev[1]) != s.current) {
+        s.status = LZX_SUMCHECK_CLAIM_MISMATCH;
+        return;
+    }
+    // the D+1 leaves ev_i w_i and (ch - j), j != i (1 beyond the width), as a product tree
+    constexpr int LEAVES = D + 1 < 2 ? 2 : (D + 1 <= 4 ? 4 : (D + 1 <= 8 ? 8 : 16));
+    uint64_t x[LEAVES];
+#pragma unroll
+    for (int k = 0; k < LEAVES; k++) {
+        const uint32_t j = (uint32_t)k < lane ? (uint32_t)k : (uint32_t)k + 1; // skip node i
+        x[k] = (uint32_t)k + 1 < width ? gl_sub(ch, j) : ((uint32_t)k + 1 == width ? evw : 1);
+    }
+#pragma unroll
+    for (int n = LEAVES / 2; n > 0; n >>= 1)
+#pragma unroll
+        for (int k = 0; k < n; k++) x[k] = gl_mul(x[k], x[k + n]);
+    uint64_t term = lane < width ? x[0] : 0;
+#pragma unroll
+    for (int o = 1; o < LEAVES; o <<= 1) term = gl_add(term, __shfl_xor(term, o)); // nodes sit on lanes < LEAVES
+    s.current = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(term >> 32)) << 32) |
+                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)term); // lane 0's group holds the sum
+}
+
+// One block: sum the eval / fold kernel's block totals and close round rnd.
+template <int D>
+__global__ __launch_bounds__(SC_BLOCK) void sc_close_kernel(const uint64_t *__restrict__ partial, uint32_t nblocks,
+                                                            ScRoundArgs A, uint32_t rnd)
+{
+    __shared__ ScShared sh;
+    const uint32_t width = A.width;
+    TxRegs s{};
+    uint64_t given = 0;
+    if (threadIdx.x < 64) { // issued before the partial loads: one round trip for both
+        s = tx_load(A.ctl, threadIdx.x);
+        if (A.given) given = A.given[rnd];
+        tx_frame_init(sh, width, threadIdx.x);
+    }
+    GlAcc acc[D + 1];
+#pragma unroll
+    for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
+    constexpr int U = SC_MAX_BLOCKS / SC_BLOCK; // every load issued before the first add
+    for (uint32_t b0 = threadIdx.x; b0 < nblocks; b0 += U * SC_BLOCK) {
+        uint64_t v[U][D + 1];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t b = b0 + u * SC_BLOCK;
+#pragma unroll
+            for (int t = 0; t <= D; t++) v[u][t] = b < nblocks && (uint32_t)t < width ? partial[(size_t)b * width + t] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int t = 0; t <= D; t++) gl_acc_add(acc[t], v[u][t]);
+    }
+    block_totals<D, SC_BLOCK / 64>(acc, width - 1, sh, sh.ev);
+    if (threadIdx.x >= 64) return;
+    tx_round<D>(s, sh, A, rnd, given);
+    tx_store(A.ctl, s, threadIdx.x);
+}
+
+// Round 0: evaluate g_0 over the caller's tables.
 template <int NF, int D>
 __global__ __launch_bounds__(SC_BLOCK) void sc_eval_kernel(ScTables tabs, ScTerms T, uint64_t half,
                                                            uint64_t *__restrict__ partial)
 {
+    __shared__ ScShared sh;
     GlAcc acc[D + 1];
 #pragma unroll
     for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
@@ -130,40 +412,25 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_eval_kernel(ScTables tabs, ScTerm
         }
         eval_pair<NF, D>(lo, hi, T, acc);
     }
-    block_store<D>(acc, partial, T.d);
+    publish_totals<D>(acc, T.d, sh, partial);
 }
 
-// Fold with r (tables of length 2*half -> half) and evaluate the next round
-// (pairs (p, p + half/2) of the folded tables) when half >= 2.
+// Fold pairs p < quarter of tables of length 2*half with challenge r (in place
+// from the second round on) and accumulate the next round's values.
 template <int NF, int D>
-__global__ __launch_bounds__(SC_BLOCK) void sc_fold_eval_kernel(ScTables tabs, ScTerms T, uint64_t half, uint64_t r,
-                                                                uint64_t *__restrict__ partial)
+__device__ __forceinline__ void fold_eval_range(const ScTables &tabs, const ScTerms &T, uint64_t half, uint64_t r,
+                                                uint64_t p0, uint64_t stride, GlAcc (&acc)[D + 1])
 {
-    GlAcc acc[D + 1];
-#pragma unroll
-    for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
     const uint64_t quarter = half >> 1;
-    if (quarter == 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-                if ((uint32_t)f < T.nf) {
-                    const uint64_t a = tabs.src[f][0], b = tabs.src[f][1];
-                    tabs.dst[f][0] = gl_add(a, gl_mul(gl_sub(b, a), r));
-                }
-        }
-        return;
-    }
-    for (uint64_t p = (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x; p < quarter;
-         p += (uint64_t)gridDim.x * SC_BLOCK) {
+    for (uint64_t p = p0; p < quarter; p += stride) {
         uint64_t n0[NF], n1[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
             if ((uint32_t)f < T.nf) {
                 const uint64_t *s = tabs.src[f];
                 const uint64_t a0 = s[p], b0 = s[p + half], a1 = s[p + quarter], b1 = s[p + half + quarter];
-                n0[f] = gl_add(a0, gl_mul(gl_sub(b0, a0), r));
-                n1[f] = gl_add(a1, gl_mul(gl_sub(b1, a1), r));
+                n0[f] = gl_add(a0, gl_mul(gl_sub_c(b0, a0), r));
+                n1[f] = gl_add(a1, gl_mul(gl_sub_c(b1, a1), r));
                 tabs.dst[f][p] = n0[f];
                 tabs.dst[f][p + quarter] = n1[f];
             } else {
@@ -172,9 +439,78 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_fold_eval_kernel(ScTables tabs, S
         }
         eval_pair<NF, D>(n0, n1, T, acc);
     }
-    block_store<D>(acc, partial, T.d);
 }
 
+// Fold with challenge rnd (half >= 2) and evaluate round rnd+1.
+template <int NF, int D>
+__global__ __launch_bounds__(SC_BLOCK) void sc_fold_eval_kernel(ScTables tabs, ScTerms T, uint64_t half,
+                                                                const uint64_t *__restrict__ rp,
+                                                                uint64_t *__restrict__ partial)
+{
+    __shared__ ScShared sh;
+    const uint64_t r = *rp; // written on the device by the close kernel
+    GlAcc acc[D + 1];
+#pragma unroll
+    for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
+    fold_eval_range<NF, D>(tabs, T, half, r, (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x,
+                           (uint64_t)gridDim.x * SC_BLOCK, acc);
+    publish_totals<D>(acc, T.d, sh, partial);
+}
+
+// Rounds r0 .. num_vars-1 in one block: fold with the challenge of round r,
+// sum, close round r+1; the last fold leaves one entry per table (the factor
+// claims, sumcheck.rs:139-143).  tabs.src is the input of round r0's fold.
+template <int NF, int D>
+__global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, ScTerms T, uint64_t half0, uint32_t r0,
+                                                                uint32_t num_vars, ScRoundArgs A)
+{
+    __shared__ ScShared sh;
+    const uint32_t lane = threadIdx.x & 63;
+    TxRegs s{};
+    if (threadIdx.x < 64) {
+        s = tx_load(A.ctl, lane);
+        tx_frame_init(sh, A.width, lane);
+        if (lane == 0) {
+            sh.chal = A.chal[r0];
+            sh.status = s.status;
+        }
+        if (A.given)
+            for (uint32_t i = lane; r0 + 1 + i < num_vars && i < 64; i += 64) sh.given[i] = A.given[r0 + 1 + i];
+    }
+    __syncthreads();
+    uint64_t half = half0;
+    for (uint32_t r = r0; r < num_vars && !sh.status; r++, half >>= 1) {
+        const uint64_t ch = sh.chal;
+        if (half == 1) { // the last fold
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    if ((uint32_t)f < T.nf) {
+                        const uint64_t a = tabs.src[f][0], b = tabs.src[f][1];
+                        const uint64_t v = gl_add(a, gl_mul(gl_sub(b, a), ch));
+                        tabs.dst[f][0] = v;
+                        A.claims[f] = v;
+                    }
+            }
+            break;
+        }
+        GlAcc acc[D + 1];
+#pragma unroll
+        for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
+        fold_eval_range<NF, D>(tabs, T, half, ch, threadIdx.x, SC_TAIL_BLOCK, acc);
+#pragma unroll
+        for (int f = 0; f < NF; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
+        block_totals<D, SC_MAX_WAVES>(acc, T.d, sh, sh.ev); // its barriers also order the folded tables
+        if (threadIdx.x < 64) {
+            tx_round<D>(s, sh, A, r + 1, A.given ? sh.given[r - r0] : 0);
+            if (lane == 0) sh.status = s.status;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 64) tx_store(A.ctl, s, lane);
+}
+
+// Sums the block partials into the d+1 values of a round (sum_over_hypercube).
 __global__ void sc_reduce_kernel(const uint64_t *__restrict__ partial, uint32_t nblocks, uint32_t width,
                                  uint64_t *__restrict__ out)
 {
@@ -231,8 +567,22 @@ static uint64_t h_interpolate(const uint64_t *ev, size_t n, uint64_t r) // sumch
 
 struct ScProblem {
     size_t num_vars, nf, nterms, d;
-    ScTerms T;
+    uint8_t exps[SC_MAXT][SC_MAXF]; // full multiplicities (host-side evaluation)
+    ScTerms T;                      // device form, common factor powers pulled out
 };
+
+// sum_t c_t prod_f v[f]^e_tf on the host (sumcheck.rs:144-154 final claim).
+static uint64_t host_eval_terms(const ScProblem &P, const uint64_t *v)
+{
+    uint64_t tot = 0;
+    for (size_t t = 0; t < P.nterms; t++) {
+        uint64_t prod = P.T.coeff[t];
+        for (size_t f = 0; f < P.nf; f++)
+            for (uint32_t x = 0; x < P.exps[t][f]; x++) prod = gl_mul(prod, v[f]);
+        tot = gl_add(tot, prod);
+    }
+    return tot;
+}
 
 static lzx_status build_problem(size_t num_vars, size_t nfactors, const uint64_t *const *factors, size_t nterms,
                                 const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids,
@@ -245,6 +595,7 @@ static lzx_status build_problem(size_t num_vars, size_t nfactors, const uint64_t
     P.num_vars = num_vars;
     P.nf = nfactors;
     P.nterms = nterms;
+    memset(P.exps, 0, sizeof(P.exps));
     memset(&P.T, 0, sizeof(P.T));
     P.T.nterms = (uint32_t)nterms;
     P.T.nf = (uint32_t)nfactors;
@@ -258,20 +609,30 @@ static lzx_status build_problem(size_t num_vars, size_t nfactors, const uint64_t
         P.T.coeff[t] = coeffs[t];
         for (uint32_t k = term_off[t]; k < term_off[t + 1]; k++) {
             if (term_ids[k] >= nfactors) return LZX_SUMCHECK_VIRTUAL_POLY; // add_term index check
-            P.T.exps[t][term_ids[k]]++;
+            P.exps[t][term_ids[k]]++;
         }
-        P.T.first[t] = (uint8_t)term_ids[term_off[t]];
     }
     if (d > (size_t)SC_MAXD) return LZX_UNSUPPORTED;
     for (size_t f = 0; f < nfactors; f++)
         if (!factors[f]) return LZX_INVALID_ARGUMENT;
     P.d = d;
     P.T.d = (uint32_t)d;
+    for (size_t f = 0; f < nfactors; f++) {
+        uint8_t c = nterms ? 255 : 0;
+        for (size_t t = 0; t < nterms; t++) c = std::min(c, P.exps[t][f]);
+        P.T.common[f] = c;
+        P.T.has_common |= c != 0;
+        for (size_t t = 0; t < nterms; t++) P.T.exps[t][f] = (uint8_t)(P.exps[t][f] - c);
+    }
     return LZX_OK;
 }
 
-using EvalFn = void (*)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint64_t *);
-using FoldFn = void (*)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint64_t, uint64_t *);
+struct ScKernels {
+    void (*eval)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint64_t *);
+    void (*fold)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const uint64_t *, uint64_t *);
+    void (*close)(hipStream_t, const uint64_t *, uint32_t, const ScRoundArgs &, uint32_t);
+    void (*tail)(hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint32_t, uint32_t, const ScRoundArgs &);
+};
 
 template <int NF, int D>
 static void launch_eval(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half, uint64_t *part)
@@ -280,133 +641,167 @@ static void launch_eval(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms
 }
 
 template <int NF, int D>
-static void launch_fold(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half, uint64_t r,
-                        uint64_t *part)
+static void launch_fold(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half,
+                        const uint64_t *rp, uint64_t *part)
 {
-    hipLaunchKernelGGL((sc_fold_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, r, part);
+    hipLaunchKernelGGL((sc_fold_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, rp, part);
 }
 
-static void pick_kernels(size_t nf, size_t d, EvalFn &ev, FoldFn &fo)
+template <int D>
+static void launch_close(hipStream_t s, const uint64_t *part, uint32_t nblocks, const ScRoundArgs &A, uint32_t rnd)
 {
-    if (nf <= 2 && d <= 2) {
-        ev = launch_eval<2, 2>;
-        fo = launch_fold<2, 2>;
-    } else if (nf <= 4 && d <= 3) {
-        ev = launch_eval<4, 3>;
-        fo = launch_fold<4, 3>;
-    } else {
-        ev = launch_eval<SC_MAXF, SC_MAXD>;
-        fo = launch_fold<SC_MAXF, SC_MAXD>;
-    }
+    hipLaunchKernelGGL((sc_close_kernel<D>), dim3(1), dim3(SC_BLOCK), 0, s, part, nblocks, A, rnd);
 }
 
-// The prover loop shared by the transcript and explicit-challenge forms.
+template <int NF, int D>
+static void launch_tail(hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half0, uint32_t r0,
+                        uint32_t num_vars, const ScRoundArgs &A)
+{
+    hipLaunchKernelGGL((sc_tail_kernel<NF, D>), dim3(1), dim3(SC_TAIL_BLOCK), 0, s, tb, T, half0, r0, num_vars, A);
+}
+
+template <int NF, int D>
+static ScKernels kernels_for()
+{
+    return ScKernels{launch_eval<NF, D>, launch_fold<NF, D>, launch_close<D>, launch_tail<NF, D>};
+}
+
+static ScKernels pick_kernels(size_t nf, size_t d)
+{
+    if (nf <= 2 && d <= 2) return kernels_for<2, 2>();
+    if (nf <= 4 && d <= 3) return kernels_for<4, 3>();
+    return kernels_for<SC_MAXF, SC_MAXD>();
+}
+
+static uint32_t sc_grid(uint64_t work)
+{
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + SC_BLOCK - 1) / SC_BLOCK, SC_MAX_BLOCKS));
+}
+
+// The prover loop shared by the transcript and explicit-challenge forms: every
+// round is enqueued up front (eval, then one fold per round, each closing its
+// round on the device), and the host synchronizes once at the end.
 static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors, uint64_t claim, lzx_transcript *tr,
                              const uint64_t *given, uint64_t *rounds_out, uint64_t *challenges_out,
                              uint64_t *final_claim, uint64_t *factor_claims_out, hipStream_t stream)
 {
     const size_t m = P.num_vars, d = P.d, width = d + 1, nf = P.nf;
     const uint64_t N = 1ULL << m;
-    EvalFn evf;
-    FoldFn fof;
-    pick_kernels(nf, d, evf, fof);
+    const ScKernels K = pick_kernels(nf, d);
 
-    uint64_t *scratch = nullptr, *partial = nullptr, *dev_out = nullptr, *host_out = nullptr;
+    ScRoundArgs A{};
+    A.width = (uint32_t)width;
+    for (size_t i = 0; i < width; i++) { // Lagrange weights for nodes 0..d
+        uint64_t den = 1;
+        for (size_t j = 0; j < width; j++)
+            if (j != i) den = gl_mul(den, gl_sub(i, j));
+        uint64_t inv = 1, base = den, e = GL_P - 2;
+        while (e) {
+            if (e & 1) inv = gl_mul(inv, base);
+            base = gl_mul(base, base);
+            e >>= 1;
+        }
+        A.L.w[i] = inv;
+    }
+    // one device allocation: folded tables | block totals | control area
+    // control area: ScCtl | rounds[m][width] | chal[m] | given[m] | claims[nf]
+    const size_t mm = std::max<size_t>(1, m), nff = std::max<size_t>(1, nf);
     const uint64_t halfN = m ? N / 2 : 1;
+    const size_t tab_bytes = nff * halfN * sizeof(uint64_t), part_bytes = SC_MAX_BLOCKS * width * sizeof(uint64_t);
+    const size_t o_rounds = sizeof(ScCtl), o_chal = o_rounds + mm * width * 8, o_given = o_chal + mm * 8,
+                 o_claims = o_given + mm * 8, area = o_claims + nff * 8;
+    uint8_t *dev = nullptr;
+    HostStage hs;
+    bool enqueued = false;
     lzx_status st = LZX_OK;
-    hipError_t e = hipMallocAsync((void **)&scratch, std::max<size_t>(1, nf) * halfN * sizeof(uint64_t), stream);
-    if (e == hipSuccess)
-        e = hipMallocAsync((void **)&partial, (size_t)SC_MAX_BLOCKS * width * sizeof(uint64_t), stream);
-    if (e == hipSuccess) e = hipMallocAsync((void **)&dev_out, std::max<size_t>(width, nf) * sizeof(uint64_t), stream);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&host_out, std::max<size_t>(width, nf) * sizeof(uint64_t), 0);
+    hipError_t e = scratch_alloc((void **)&dev, tab_bytes + part_bytes + area, stream);
+    if (e == hipSuccess) e = stage_acquire(hs, area);
     if (e != hipSuccess) st = device_fail(e, "sumcheck scratch");
-
-    auto grid_for = [](uint64_t work) {
-        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((work + SC_BLOCK - 1) / SC_BLOCK, SC_MAX_BLOCKS));
-    };
-    auto fetch_evals = [&](uint32_t nblocks, uint64_t *evals) -> lzx_status {
-        hipLaunchKernelGGL(sc_reduce_kernel, dim3(1), dim3(SC_BLOCK), 0, stream, partial, nblocks, (uint32_t)width,
-                           dev_out);
-        hipError_t le = hipGetLastError();
-        if (le != hipSuccess) return device_fail(le, "sc_reduce_kernel");
-        le = hipMemcpyAsync(host_out, dev_out, width * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
-        if (le == hipSuccess) le = hipStreamSynchronize(stream);
-        if (le != hipSuccess) return device_fail(le, "sumcheck round readback");
-        memcpy(evals, host_out, width * sizeof(uint64_t));
-        return LZX_OK;
-    };
+    uint8_t *h = (uint8_t *)hs.p, *dctl = dev + tab_bytes + part_bytes;
+    if (st == LZX_OK) {
+        memset(h, 0, area);
+        ScCtl &c = *(ScCtl *)h;
+        if (tr) {
+            for (int i = 0; i < 25; i++) c.tx.st[i] = tr->sponge.st[i];
+            c.tx.pos = tr->sponge.pos;
+            c.tx.query_count = tr->query_count;
+            c.tx.query_budget = tr->query_budget;
+        }
+        c.current = claim;
+        if (given) memcpy(h + o_given, given, m * sizeof(uint64_t));
+        e = hipMemcpyAsync(dctl, h, area, hipMemcpyHostToDevice, stream);
+        enqueued = true;
+        if (e != hipSuccess) st = device_fail(e, "sumcheck setup");
+    }
+    uint64_t *partial = (uint64_t *)(dev + tab_bytes);
+    A.ctl = (ScCtl *)dctl;
+    A.rounds = (uint64_t *)(dctl + o_rounds);
+    A.chal = (uint64_t *)(dctl + o_chal);
+    A.given = given ? (const uint64_t *)(dctl + o_given) : nullptr;
+    A.claims = (uint64_t *)(dctl + o_claims);
 
     ScTables tabs{};
     for (size_t f = 0; f < nf; f++) {
         tabs.src[f] = factors[f];
-        tabs.dst[f] = scratch + f * halfN;
+        tabs.dst[f] = (uint64_t *)dev + f * halfN;
     }
-    uint64_t current = claim;
+    auto check = [&](const char *what) {
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess && st == LZX_OK) st = device_fail(le, what);
+    };
     if (st == LZX_OK && m > 0) {
-        const uint32_t g0 = grid_for(N / 2);
-        evf(dim3(g0), stream, tabs, P.T, N / 2, partial);
-        e = hipGetLastError();
-        if (e != hipSuccess) st = device_fail(e, "sc_eval_kernel");
-        if (st == LZX_OK) st = fetch_evals(g0, rounds_out);
+        const uint32_t g0 = sc_grid(N / 2);
+        K.eval(dim3(g0), stream, tabs, P.T, N / 2, partial);
+        check("sc_eval_kernel");
+        K.close(stream, partial, g0, A, 0);
+        check("sc_close_kernel");
+        size_t r = 0;
+        for (; st == LZX_OK && (N >> (r + 1)) > SC_TAIL_HALF; r++) { // large rounds: a grid per fold
+            const uint64_t half = N >> (r + 1);
+            const uint32_t g = sc_grid(half / 2);
+            K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, partial);
+            check("sc_fold_eval_kernel");
+            K.close(stream, partial, g, A, (uint32_t)r + 1);
+            check("sc_close_kernel");
+            for (size_t f = 0; f < nf; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
+        }
+        if (st == LZX_OK) {
+            K.tail(stream, tabs, P.T, N >> (r + 1), (uint32_t)r, (uint32_t)m, A);
+            check("sc_tail_kernel");
+        }
     }
-    for (size_t r = 0; st == LZX_OK && r < m; r++) {
-        uint64_t *ev = rounds_out + r * width;
-        uint64_t ch;
-        if (tr) {
-            st = transcript_append_fields(tr, "sumcheck-round", ev, width); // sumcheck.rs:110-112
-            if (st == LZX_OK) st = transcript_challenge(tr, "sumcheck-challenge", &ch);
-            if (st != LZX_OK) break;
-        } else {
-            ch = given[r];
-        }
-        challenges_out[r] = ch;
-        if (gl_add(ev[0], ev[1]) != current) { // sumcheck.rs:120-123
-            st = LZX_SUMCHECK_CLAIM_MISMATCH;
-            break;
-        }
-        current = h_interpolate(ev, width, ch);
-        const uint64_t half = N >> (r + 1);
-        const uint32_t g = grid_for(std::max<uint64_t>(1, half / 2));
-        fof(dim3(g), stream, tabs, P.T, half, ch, partial);
-        e = hipGetLastError();
-        if (e != hipSuccess) {
-            st = device_fail(e, "sc_fold_eval_kernel");
-            break;
-        }
-        for (size_t f = 0; f < nf; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
-        if (r + 1 < m) st = fetch_evals(g, rounds_out + (r + 1) * width);
+    // zero rounds: the claims are the tables' single entries (sumcheck.rs:139-143)
+    for (size_t f = 0; st == LZX_OK && m == 0 && f < nf; f++) {
+        e = hipMemcpyAsync(dctl + o_claims + 8 * f, factors[f], sizeof(uint64_t), hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess) st = device_fail(e, "factor claim copy");
+    }
+    if (st == LZX_OK) { // one readback of the control area
+        e = hipMemcpyAsync(h, dctl, area, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) st = device_fail(e, "sumcheck readback");
+        enqueued = false;
     }
     if (st == LZX_OK) {
-        // factor claims: the single remaining entry of every table (sumcheck.rs:139-143)
-        for (size_t f = 0; f < nf && st == LZX_OK; f++) {
-            e = hipMemcpyAsync(host_out + f, m ? tabs.dst[f] : factors[f], sizeof(uint64_t), hipMemcpyDeviceToHost,
-                               stream);
-            if (e != hipSuccess) st = device_fail(e, "factor claim readback");
+        const ScCtl &c = *(const ScCtl *)h;
+        if (tr) { // the caller's transcript advances exactly as the reference's would
+            for (int i = 0; i < 25; i++) tr->sponge.st[i] = c.tx.st[i];
+            tr->sponge.pos = c.tx.pos;
+            tr->query_count = c.tx.query_count;
         }
-        if (st == LZX_OK) {
-            e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) st = device_fail(e, "factor claim sync");
-        }
-        if (st == LZX_OK) {
-            for (size_t f = 0; f < nf; f++) factor_claims_out[f] = host_out[f];
-            uint64_t fin = 0; // sumcheck.rs:144-154
-            for (size_t t = 0; t < P.nterms; t++) {
-                uint64_t prod = P.T.coeff[t];
-                for (size_t f = 0; f < nf; f++)
-                    for (uint32_t x = 0; x < P.T.exps[t][f]; x++) prod = gl_mul(prod, factor_claims_out[f]);
-                fin = gl_add(fin, prod);
-            }
+        memcpy(rounds_out, h + o_rounds, m * width * sizeof(uint64_t));
+        memcpy(challenges_out, h + o_chal, m * sizeof(uint64_t));
+        if (c.status) {
+            st = (lzx_status)c.status;
+        } else {
+            memcpy(factor_claims_out, h + o_claims, nf * sizeof(uint64_t));
+            const uint64_t fin = host_eval_terms(P, factor_claims_out); // sumcheck.rs:144-154
             *final_claim = fin;
-            if (fin != current) st = LZX_SUMCHECK_FINAL_CHECK_FAILED;
+            if (fin != c.current) st = LZX_SUMCHECK_FINAL_CHECK_FAILED;
         }
     }
-    if (scratch) (void)hipFreeAsync(scratch, stream);
-    if (partial) (void)hipFreeAsync(partial, stream);
-    if (dev_out) (void)hipFreeAsync(dev_out, stream);
-    if (host_out) {
-        (void)hipStreamSynchronize(stream);
-        (void)hipHostFree(host_out);
-    }
+    if (enqueued) (void)hipStreamSynchronize(stream); // the staging buffer may still be read
+    if (dev) (void)scratch_free(dev, stream);
+    stage_release(hs);
     return st;
 }
 
@@ -524,16 +919,16 @@ extern "C" lzx_status lzx_eq_table(const uint64_t *point_host, size_t num_vars, 
     std::vector<uint64_t> ehi = table(0, hbits), elo = table(hbits, lbits);
     hipStream_t s = (hipStream_t)stream;
     uint64_t *dh = nullptr, *dl = nullptr;
-    LZX_HIP(hipMallocAsync((void **)&dh, ehi.size() * sizeof(uint64_t), s));
-    LZX_HIP(hipMallocAsync((void **)&dl, elo.size() * sizeof(uint64_t), s));
+    LZX_HIP(scratch_alloc((void **)&dh, ehi.size() * sizeof(uint64_t), s));
+    LZX_HIP(scratch_alloc((void **)&dl, elo.size() * sizeof(uint64_t), s));
     LZX_HIP(hipMemcpyAsync(dh, ehi.data(), ehi.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     LZX_HIP(hipMemcpyAsync(dl, elo.data(), elo.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     const uint64_t total = 1ULL << num_vars;
     const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 1 << 16));
     hipLaunchKernelGGL(eq_combine_kernel, dim3(blocks), dim3(256), 0, s, dh, dl, lbits, total, out_dev);
     LZX_LAUNCH_CHECK("eq_combine_kernel");
-    LZX_HIP(hipFreeAsync(dh, s));
-    LZX_HIP(hipFreeAsync(dl, s));
+    LZX_HIP(scratch_free(dh, s));
+    LZX_HIP(scratch_free(dl, s));
     // the host vectors must outlive the async copies
     LZX_HIP(hipStreamSynchronize(s));
     return LZX_OK;
@@ -552,41 +947,30 @@ extern "C" lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors
     lzx_status st = build_problem(num_vars, nfactors, factors_dev, nterms, coeffs, term_off, term_ids, P);
     if (st != LZX_OK) return st;
     hipStream_t s = (hipStream_t)stream;
-    EvalFn evf;
-    FoldFn fof;
-    pick_kernels(P.nf, P.d, evf, fof);
+    const ScKernels K = pick_kernels(P.nf, P.d);
     ScTables tabs{};
     for (size_t f = 0; f < P.nf; f++) tabs.src[f] = factors_dev[f];
     if (num_vars == 0) {
         std::vector<uint64_t> v(P.nf);
         for (size_t f = 0; f < P.nf; f++)
             LZX_HIP(hipMemcpy(&v[f], factors_dev[f], sizeof(uint64_t), hipMemcpyDeviceToHost));
-        uint64_t tot = 0;
-        for (size_t t = 0; t < P.nterms; t++) {
-            uint64_t prod = P.T.coeff[t];
-            for (size_t f = 0; f < P.nf; f++)
-                for (uint32_t x = 0; x < P.T.exps[t][f]; x++) prod = gl_mul(prod, v[f]);
-            tot = gl_add(tot, prod);
-        }
-        *sum_out = tot;
+        *sum_out = host_eval_terms(P, v.data());
         return LZX_OK;
     }
     const uint64_t half = 1ULL << (num_vars - 1);
-    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((half + SC_BLOCK - 1) / SC_BLOCK,
-                                                                           SC_MAX_BLOCKS));
+    const uint32_t g = sc_grid(half);
     const size_t width = P.d + 1;
     uint64_t *partial = nullptr, *dev_out = nullptr;
-    LZX_HIP(hipMallocAsync((void **)&partial, (size_t)g * width * sizeof(uint64_t), s));
-    LZX_HIP(hipMallocAsync((void **)&dev_out, width * sizeof(uint64_t), s));
-    evf(dim3(g), s, tabs, P.T, half, partial);
+    LZX_HIP(scratch_alloc((void **)&partial, ((size_t)g + 1) * width * sizeof(uint64_t), s));
+    dev_out = partial + (size_t)g * width;
+    K.eval(dim3(g), s, tabs, P.T, half, partial);
     LZX_LAUNCH_CHECK("sc_eval_kernel");
     hipLaunchKernelGGL(sc_reduce_kernel, dim3(1), dim3(SC_BLOCK), 0, s, partial, g, (uint32_t)width, dev_out);
     LZX_LAUNCH_CHECK("sc_reduce_kernel");
     std::vector<uint64_t> ev(width);
     LZX_HIP(hipMemcpyAsync(ev.data(), dev_out, width * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     LZX_HIP(hipStreamSynchronize(s));
-    LZX_HIP(hipFreeAsync(partial, s));
-    LZX_HIP(hipFreeAsync(dev_out, s));
+    LZX_HIP(scratch_free(partial, s));
     *sum_out = gl_add(ev[0], ev[1]); // sum over x0 = 0 and x0 = 1
     return LZX_OK;
 }

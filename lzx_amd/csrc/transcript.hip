@@ -16,12 +16,6 @@
 #include "runtime.hpp"
 #include "seeded.hpp"
 
-struct lzx_transcript {
-    lzx::Sponge sponge;
-    uint64_t query_count;
-    uint64_t query_budget;
-};
-
 namespace lzx {
 
 static void squeeze_copy(const lzx_transcript *t, size_t *offset, uint8_t *out, size_t n)
