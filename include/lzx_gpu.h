@@ -44,6 +44,7 @@ enum {
     LZX_SUMCHECK_FINAL_CHECK_FAILED = 12, /* SumcheckError::FinalCheckFailed sumcheck.rs:43 */
     LZX_SUMCHECK_CLAIM_MISMATCH = 13,   /* SumcheckError::ClaimMismatch    sumcheck.rs:45 */
     LZX_SUMCHECK_VIRTUAL_POLY = 14,     /* SumcheckError::VirtualPoly      sumcheck.rs:30 */
+    LZX_ZEROCHECK_FINAL_CLAIM_NONZERO = 15, /* ZerocheckError::FinalClaimNonZero zerocheck.rs:20 */
     LZX_TRANSCRIPT_REJECTION_BUDGET = 20, /* TranscriptError::RejectionBudgetExceeded transcript.rs:16 */
     LZX_TRANSCRIPT_MESSAGE_TOO_LARGE = 21, /* TranscriptError::MessageTooLarge transcript.rs:18 */
     LZX_INVALID_ARGUMENT = 30,          /* null handle/pointer or malformed shape */
@@ -268,6 +269,42 @@ lzx_status lzx_sumcheck_prove_with_challenges(size_t num_vars, size_t nfactors, 
 lzx_status lzx_sumcheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_len, size_t num_vars,
                                size_t max_degree, uint64_t claim, lzx_transcript *transcript, int32_t has_expected,
                                uint64_t expected_final, uint64_t *point_out, uint64_t *final_claim);
+
+/* fastprover::prove_fast_with_eq (fastprover.rs:212-224; prove_fast :201-208
+ * is this call with n_eq = 0): the same proof as lzx_sumcheck_prove, byte
+ * for byte (the reference's fast prover is transcript-identical to
+ * sumcheck::prove).  The factors named by eq_indices are eq(w, .) for the
+ * host point w (num_vars canonical values) and their tables are never read,
+ * so those factor pointers may be NULL.  When one eq factor appears exactly
+ * once in every term the device never materialises it: each round's eq values
+ * come from a prefix/suffix split of w, the tables stream one factor fewer and
+ * the other factors are evaluated at d points instead of d + 1.  Otherwise the
+ * eq tables are built from w on the device first.  The window and statistics
+ * options of the reference's FastProverOpts have no effect on the result and
+ * are not taken. */
+lzx_status lzx_sumcheck_prove_eq(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev, size_t nterms,
+                                 const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids,
+                                 uint64_t claim, lzx_transcript *transcript, const size_t *eq_indices, size_t n_eq,
+                                 const uint64_t *w, uint64_t *rounds_out, uint64_t *challenges_out,
+                                 uint64_t *final_claim, uint64_t *factor_claims_out, size_t *degree_out,
+                                 void *stream);
+
+/* zerocheck::prove (zerocheck.rs:26-54): r = challenge_fields("zerocheck-point",
+ * num_vars) from the transcript; refuses with LZX_ZEROCHECK_FINAL_CLAIM_NONZERO
+ * when P(r) != 0 (P does not vanish on the hypercube); otherwise runs the
+ * sum-check of P(x) eq(r, x) with claim 0 (factors [P, eq(r, .)], one term,
+ * degree 2).  point_out receives r (num_vars), rounds_out num_vars x 3 values,
+ * factor_claims_out 2 values. */
+lzx_status lzx_zerocheck_prove(size_t num_vars, const uint64_t *poly_dev, lzx_transcript *transcript,
+                               uint64_t *point_out, uint64_t *rounds_out, uint64_t *challenges_out,
+                               uint64_t *final_claim, uint64_t *factor_claims_out, void *stream);
+
+/* zerocheck::verify (zerocheck.rs:58-84): draws r as the prover did, verifies
+ * the degree-2 rounds against claim 0 (and the final claim against
+ * expected_poly_eval when has_expected) and returns r in point_out. */
+lzx_status lzx_zerocheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_len, size_t num_vars,
+                                lzx_transcript *transcript, int32_t has_expected, uint64_t expected_poly_eval,
+                                uint64_t *point_out);
 
 /* Multilinear helpers on device tables (field_simd.rs:391 eq_table,
  * virtual_poly.rs:107 sum_over_hypercube). */

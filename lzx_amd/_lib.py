@@ -71,6 +71,10 @@ PROTOTYPES = {
                                                   C.POINTER(u64), vp, C.POINTER(sz), vp]),
     "lzx_sumcheck_verify": (st_t, [vp, sz, sz, sz, sz, u64, vp, i32, u64, vp, C.POINTER(u64)]),
     "lzx_eq_table": (st_t, [vp, sz, vp, vp]),
+    "lzx_sumcheck_prove_eq": (st_t, [sz, sz, C.POINTER(vp), sz, vp, vp, vp, u64, vp, vp, sz, vp, vp, vp,
+                                     C.POINTER(u64), vp, C.POINTER(sz), vp]),
+    "lzx_zerocheck_prove": (st_t, [sz, vp, vp, vp, vp, vp, C.POINTER(u64), vp, vp]),
+    "lzx_zerocheck_verify": (st_t, [vp, sz, sz, sz, vp, i32, u64, vp]),
     "lzx_vp_sum_over_hypercube": (st_t, [sz, sz, C.POINTER(vp), sz, vp, vp, vp, C.POINTER(u64), vp]),
 }
 

@@ -109,6 +109,7 @@ extern "C" const char *lzx_status_string(lzx_status s)
     case LZX_SUMCHECK_FINAL_CHECK_FAILED: return "SumcheckError::FinalCheckFailed";
     case LZX_SUMCHECK_CLAIM_MISMATCH: return "SumcheckError::ClaimMismatch";
     case LZX_SUMCHECK_VIRTUAL_POLY: return "SumcheckError::VirtualPoly";
+    case LZX_ZEROCHECK_FINAL_CLAIM_NONZERO: return "ZerocheckError::FinalClaimNonZero";
     case LZX_TRANSCRIPT_REJECTION_BUDGET: return "TranscriptError::RejectionBudgetExceeded";
     case LZX_TRANSCRIPT_MESSAGE_TOO_LARGE: return "TranscriptError::MessageTooLarge";
     case LZX_INVALID_ARGUMENT: return "InvalidArgument";

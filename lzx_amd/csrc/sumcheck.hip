@@ -41,10 +41,25 @@ constexpr uint32_t SC_MAX_BLOCKS = 2048;
 // For eq * (a*b - c) this is eq * (a*b - c): two products per point instead of three.
 struct ScTerms {
     uint32_t nterms, nf, d, has_common;
+    uint32_t npts;                  // points evaluated per pair: d + 1, or d with the split eq factor
+    int32_t eq_f;                   // the split eq factor (never read), or -1
     uint64_t coeff[SC_MAXT];
     uint8_t exps[SC_MAXT][SC_MAXF]; // multiplicity of factor f in term t beyond the common part
     uint8_t common[SC_MAXF];        // multiplicity of factor f in every term
 };
+
+// eq(w, .) over a round's pair index p without a table: E(p) =
+// hi[p >> lb] * lo[p & (2^lb - 1)], the prefix/suffix factorisation of eq
+// over the variables after the one being bound.
+struct ScEqArgs {
+    const uint64_t *hi, *lo;
+    uint32_t lb, pad;
+};
+
+__device__ __forceinline__ uint64_t eq_split_at(const ScEqArgs &E, uint64_t p)
+{
+    return gl_mul(E.hi[p >> E.lb], E.lo[p & ((1ULL << E.lb) - 1)]);
+}
 
 struct ScTables {
     const uint64_t *src[SC_MAXF];
@@ -86,27 +101,30 @@ __device__ __forceinline__ uint64_t eval_terms(const uint64_t (&v)[NF], const Sc
     return sum;
 }
 
-// Accumulate g(t), t = 0..d, of one pair (lo, hi) into acc[].
+// Accumulate g(t) (t < T.npts) of one pair (lo, hi) into acc[]; with the
+// split eq factor every point's term sum is weighted by eq_w = E(p).
 template <int NF, int D>
 __device__ __forceinline__ void eval_pair(const uint64_t (&lo)[NF], const uint64_t (&hi)[NF], const ScTerms &T,
-                                          GlAcc (&acc)[D + 1])
+                                          uint64_t eq_w, GlAcc (&acc)[D + 1])
 {
+    const bool split = T.eq_f >= 0;
+    auto put = [&](GlAcc &a, uint64_t v) { gl_acc_add(a, split ? gl_mul(v, eq_w) : v); };
     uint64_t diff[NF], cur[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         diff[f] = gl_sub_c(hi[f], lo[f]);
         cur[f] = lo[f];
     }
-    gl_acc_add(acc[0], eval_terms<NF>(cur, T));
-    if (D >= 1) gl_acc_add(acc[1], eval_terms<NF>(hi, T));
+    put(acc[0], eval_terms<NF>(cur, T));
+    if (T.npts > 1) put(acc[1], eval_terms<NF>(hi, T));
 #pragma unroll
     for (int f = 0; f < NF; f++) cur[f] = hi[f];
 #pragma unroll
     for (int t = 2; t <= D; t++) {
-        if ((uint32_t)t > T.d) break;
+        if ((uint32_t)t >= T.npts) break;
 #pragma unroll
         for (int f = 0; f < NF; f++) cur[f] = gl_add_nc(cur[f], diff[f]); // lo + t*(hi-lo), unreduced
-        gl_acc_add(acc[t], eval_terms<NF>(cur, T));
+        put(acc[t], eval_terms<NF>(cur, T));
     }
 }
 
@@ -127,6 +145,8 @@ struct ScShared {
     uint64_t ev[SC_MAXD + 1]; // the round polynomial's values at 0..d
     uint64_t chal;            // the latest challenge (tail kernel)
     uint64_t given[64];       // caller-supplied challenges (tail kernel)
+    uint64_t eqc[64][2];      // split eq constants per round (tail kernel)
+    ScEqArgs eqr[64];         // split eq tables per round (tail kernel)
     uint8_t msg[128]; // 22 + 8 (d+1) + 30 <= 124 transcript bytes per round
     uint32_t status;
 };
@@ -184,13 +204,15 @@ struct ScDevTx {
 
 struct ScCtl {
     ScDevTx tx;
-    uint64_t current; // running claim
-    uint32_t status;  // lzx_status of the first failing round (0: none)
+    uint64_t current;  // running claim
+    uint64_t eq_scale; // split eq factor: prod_{i<j} eq(w_i, r_i) over the bound variables
+    uint32_t status;   // lzx_status of the first failing round (0: none)
     uint32_t pad;
 };
 
 struct ScLagrange {
     uint64_t w[SC_MAXD + 1]; // w_i = prod_{j != i} (i - j)^{-1}, nodes 0..d
+    uint64_t x[SC_MAXD];     // S(d) = sum_{i<d} x_i S(i) for S of degree < d (split eq)
 };
 
 struct ScRoundArgs {
@@ -199,7 +221,10 @@ struct ScRoundArgs {
     uint64_t *chal;        // [num_vars]
     const uint64_t *given; // caller-supplied challenges, or nullptr for the transcript
     uint64_t *claims;      // [nf] the fully folded tables
-    uint32_t width;
+    const uint64_t *eq_c;  // split eq: [num_vars][2] = (1 - w_j, 2 w_j - 1), else nullptr
+    const ScEqArgs *eq_rounds; // split eq: round j's E(p) tables
+    uint32_t width;        // d + 1 values per round message
+    uint32_t pw;           // values per block total: d + 1, or d with the split eq factor
     ScLagrange L;
 };
 
@@ -208,7 +233,7 @@ struct ScRoundArgs {
 struct TxRegs {
     uint32_t a;
     uint32_t pos, status;
-    uint64_t qc, budget, current;
+    uint64_t qc, budget, current, eq_scale;
 };
 
 __device__ __forceinline__ TxRegs tx_load(const ScCtl *ctl, uint32_t lane)
@@ -221,6 +246,7 @@ __device__ __forceinline__ TxRegs tx_load(const ScCtl *ctl, uint32_t lane)
     s.qc = ctl->tx.query_count;
     s.budget = ctl->tx.query_budget;
     s.current = ctl->current;
+    s.eq_scale = ctl->eq_scale;
     return s;
 }
 
@@ -232,6 +258,7 @@ __device__ __forceinline__ void tx_store(ScCtl *ctl, const TxRegs &s, uint32_t l
         ctl->tx.pos = s.pos;
         ctl->tx.query_count = s.qc;
         ctl->current = s.current;
+        ctl->eq_scale = s.eq_scale;
         ctl->status = s.status;
     }
 }
@@ -288,10 +315,31 @@ __device__ __forceinline__ void tx_absorb(TxRegs &s, const uint8_t *msg, uint32_
 // Wave 0: close round rnd from sh.ev (transcript step, claim check, next
 // claim).  sh.msg holds the framing bytes (tx_frame_init).
 template <int D>
-__device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t rnd, uint64_t given)
+__device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t rnd, uint64_t given, uint64_t omw,
+                         uint64_t tw)
 {
     const uint32_t lane = threadIdx.x & 63, width = A.width;
     if (s.status) return; // an earlier round failed
+    constexpr int LANES = D + 1 <= 4 ? 4 : (D + 1 <= 8 ? 8 : 16);
+    if (A.eq_c) {
+        // sh.ev[t] = S(t) = sum_p E(p) Q_t(p) for t < d, where eq's value at
+        // point t is scale (1 - w_j + t (2 w_j - 1)) E(p): extrapolate S(d)
+        // (degree < d) and scale, g(t) = c_t S(t)
+        const uint32_t d = width - 1;
+        uint64_t sv = lane < d ? sh.ev[lane] : 0;
+        uint64_t part = lane < d ? gl_mul(sv, A.L.x[lane]) : 0;
+#pragma unroll
+        for (int o = 1; o < LANES; o <<= 1) part = gl_add(part, __shfl_xor(part, o));
+        const uint64_t sd = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(part >> 32)) << 32) |
+                            (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)part);
+        if (lane == d) sv = sd;
+        const uint64_t g = gl_mul(gl_mul(s.eq_scale, gl_add(omw, gl_mul(tw, lane))), sv);
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+        if (lane < width) sh.ev[lane] = g;
+        __builtin_amdgcn_wave_barrier();
+        asm volatile("" ::: "memory");
+    }
     if (lane < width) A.rounds[(size_t)rnd * width + lane] = sh.ev[lane];
     // g(ch) = sum_i ev_i w_i prod_{j != i} (ch - j) (sumcheck.rs:286-306): node i on lane i;
     // ev_i w_i does not depend on the challenge
@@ -334,12 +382,13 @@ __device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t
     }
     if (lane == 0) A.chal[rnd] = ch;
     sh.chal = ch;
+    if (A.eq_c) s.eq_scale = gl_mul(s.eq_scale, gl_add(omw, gl_mul(tw, ch))); // eq(w_j, r_j)
     if (gl_add(sh.ev[0], sh.ev[1]) != s.current) {
         s.status = LZX_SUMCHECK_CLAIM_MISMATCH;
         return;
     }
     // the D+1 leaves ev_i w_i and (ch - j), j != i (1 beyond the width), as a product tree
-    constexpr int LEAVES = D + 1 < 2 ? 2 : (D + 1 <= 4 ? 4 : (D + 1 <= 8 ? 8 : 16));
+    constexpr int LEAVES = LANES;
     uint64_t x[LEAVES];
 #pragma unroll
     for (int k = 0; k < LEAVES; k++) {
@@ -363,13 +412,14 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_close_kernel(const uint64_t *__re
                                                             ScRoundArgs A, uint32_t rnd)
 {
     __shared__ ScShared sh;
-    const uint32_t width = A.width;
+    const uint32_t pw = A.pw;
     TxRegs s{};
-    uint64_t given = 0;
-    if (threadIdx.x < 64) { // issued before the partial loads: one round trip for both
+    uint64_t given = 0, omw = 0, tw = 0;
+    if (threadIdx.x < 64) { // issued before the partial loads: one round trip for all
         s = tx_load(A.ctl, threadIdx.x);
         if (A.given) given = A.given[rnd];
-        tx_frame_init(sh, width, threadIdx.x);
+        if (A.eq_c) omw = A.eq_c[2 * rnd], tw = A.eq_c[2 * rnd + 1];
+        tx_frame_init(sh, A.width, threadIdx.x);
     }
     GlAcc acc[D + 1];
 #pragma unroll
@@ -381,22 +431,22 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_close_kernel(const uint64_t *__re
         for (int u = 0; u < U; u++) {
             const uint32_t b = b0 + u * SC_BLOCK;
 #pragma unroll
-            for (int t = 0; t <= D; t++) v[u][t] = b < nblocks && (uint32_t)t < width ? partial[(size_t)b * width + t] : 0;
+            for (int t = 0; t <= D; t++) v[u][t] = b < nblocks && (uint32_t)t < pw ? partial[(size_t)b * pw + t] : 0;
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
             for (int t = 0; t <= D; t++) gl_acc_add(acc[t], v[u][t]);
     }
-    block_totals<D, SC_BLOCK / 64>(acc, width - 1, sh, sh.ev);
+    block_totals<D, SC_BLOCK / 64>(acc, pw - 1, sh, sh.ev);
     if (threadIdx.x >= 64) return;
-    tx_round<D>(s, sh, A, rnd, given);
+    tx_round<D>(s, sh, A, rnd, given, omw, tw);
     tx_store(A.ctl, s, threadIdx.x);
 }
 
-// Round 0: evaluate g_0 over the caller's tables.
+// Round 0: evaluate g_0 (or S_0 with the split eq factor) over the caller's tables.
 template <int NF, int D>
-__global__ __launch_bounds__(SC_BLOCK) void sc_eval_kernel(ScTables tabs, ScTerms T, uint64_t half,
+__global__ __launch_bounds__(SC_BLOCK) void sc_eval_kernel(ScTables tabs, ScTerms T, uint64_t half, ScEqArgs E,
                                                            uint64_t *__restrict__ partial)
 {
     __shared__ ScShared sh;
@@ -407,26 +457,29 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_eval_kernel(ScTables tabs, ScTerm
         uint64_t lo[NF], hi[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            lo[f] = (uint32_t)f < T.nf ? tabs.src[f][p] : 0;
-            hi[f] = (uint32_t)f < T.nf ? tabs.src[f][p + half] : 0;
+            const bool live = (uint32_t)f < T.nf && f != T.eq_f;
+            lo[f] = live ? tabs.src[f][p] : 0;
+            hi[f] = live ? tabs.src[f][p + half] : 0;
         }
-        eval_pair<NF, D>(lo, hi, T, acc);
+        eval_pair<NF, D>(lo, hi, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
     }
-    publish_totals<D>(acc, T.d, sh, partial);
+    publish_totals<D>(acc, T.npts - 1, sh, partial);
 }
 
 // Fold pairs p < quarter of tables of length 2*half with challenge r (in place
-// from the second round on) and accumulate the next round's values.
+// from the second round on) and accumulate the next round's values (E: the
+// next round's split eq tables).
 template <int NF, int D>
 __device__ __forceinline__ void fold_eval_range(const ScTables &tabs, const ScTerms &T, uint64_t half, uint64_t r,
-                                                uint64_t p0, uint64_t stride, GlAcc (&acc)[D + 1])
+                                                const ScEqArgs &E, uint64_t p0, uint64_t stride,
+                                                GlAcc (&acc)[D + 1])
 {
     const uint64_t quarter = half >> 1;
     for (uint64_t p = p0; p < quarter; p += stride) {
         uint64_t n0[NF], n1[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            if ((uint32_t)f < T.nf) {
+            if ((uint32_t)f < T.nf && f != T.eq_f) {
                 const uint64_t *s = tabs.src[f];
                 const uint64_t a0 = s[p], b0 = s[p + half], a1 = s[p + quarter], b1 = s[p + half + quarter];
                 n0[f] = gl_add(a0, gl_mul(gl_sub_c(b0, a0), r));
@@ -437,14 +490,14 @@ __device__ __forceinline__ void fold_eval_range(const ScTables &tabs, const ScTe
                 n0[f] = n1[f] = 0;
             }
         }
-        eval_pair<NF, D>(n0, n1, T, acc);
+        eval_pair<NF, D>(n0, n1, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
     }
 }
 
 // Fold with challenge rnd (half >= 2) and evaluate round rnd+1.
 template <int NF, int D>
 __global__ __launch_bounds__(SC_BLOCK) void sc_fold_eval_kernel(ScTables tabs, ScTerms T, uint64_t half,
-                                                                const uint64_t *__restrict__ rp,
+                                                                const uint64_t *__restrict__ rp, ScEqArgs E,
                                                                 uint64_t *__restrict__ partial)
 {
     __shared__ ScShared sh;
@@ -452,9 +505,9 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_fold_eval_kernel(ScTables tabs, S
     GlAcc acc[D + 1];
 #pragma unroll
     for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
-    fold_eval_range<NF, D>(tabs, T, half, r, (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x,
+    fold_eval_range<NF, D>(tabs, T, half, r, E, (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x,
                            (uint64_t)gridDim.x * SC_BLOCK, acc);
-    publish_totals<D>(acc, T.d, sh, partial);
+    publish_totals<D>(acc, T.npts - 1, sh, partial);
 }
 
 // Rounds r0 .. num_vars-1 in one block: fold with the challenge of round r,
@@ -474,8 +527,14 @@ __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, S
             sh.chal = A.chal[r0];
             sh.status = s.status;
         }
-        if (A.given)
-            for (uint32_t i = lane; r0 + 1 + i < num_vars && i < 64; i += 64) sh.given[i] = A.given[r0 + 1 + i];
+        for (uint32_t i = lane; r0 + 1 + i < num_vars && i < 64; i += 64) {
+            if (A.given) sh.given[i] = A.given[r0 + 1 + i];
+            if (A.eq_c) {
+                sh.eqc[i][0] = A.eq_c[2 * (r0 + 1 + i)];
+                sh.eqc[i][1] = A.eq_c[2 * (r0 + 1 + i) + 1];
+                sh.eqr[i] = A.eq_rounds[r0 + 1 + i];
+            }
+        }
     }
     __syncthreads();
     uint64_t half = half0;
@@ -485,7 +544,7 @@ __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, S
             if (threadIdx.x == 0) {
 #pragma unroll
                 for (int f = 0; f < NF; f++)
-                    if ((uint32_t)f < T.nf) {
+                    if ((uint32_t)f < T.nf && f != T.eq_f) {
                         const uint64_t a = tabs.src[f][0], b = tabs.src[f][1];
                         const uint64_t v = gl_add(a, gl_mul(gl_sub(b, a), ch));
                         tabs.dst[f][0] = v;
@@ -497,12 +556,16 @@ __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, S
         GlAcc acc[D + 1];
 #pragma unroll
         for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
-        fold_eval_range<NF, D>(tabs, T, half, ch, threadIdx.x, SC_TAIL_BLOCK, acc);
+        const uint32_t k = r - r0; // round r+1's slot in the preloaded tables
+        ScEqArgs E{};
+        if (A.eq_c) E = sh.eqr[k];
+        fold_eval_range<NF, D>(tabs, T, half, ch, E, threadIdx.x, SC_TAIL_BLOCK, acc);
 #pragma unroll
         for (int f = 0; f < NF; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
-        block_totals<D, SC_MAX_WAVES>(acc, T.d, sh, sh.ev); // its barriers also order the folded tables
+        block_totals<D, SC_MAX_WAVES>(acc, T.npts - 1, sh, sh.ev); // its barriers also order the folded tables
         if (threadIdx.x < 64) {
-            tx_round<D>(s, sh, A, r + 1, A.given ? sh.given[r - r0] : 0);
+            tx_round<D>(s, sh, A, r + 1, A.given ? sh.given[k] : 0, A.eq_c ? sh.eqc[k][0] : 0,
+                        A.eq_c ? sh.eqc[k][1] : 0);
             if (lane == 0) sh.status = s.status;
         }
         __syncthreads();
@@ -586,7 +649,7 @@ static uint64_t host_eval_terms(const ScProblem &P, const uint64_t *v)
 
 static lzx_status build_problem(size_t num_vars, size_t nfactors, const uint64_t *const *factors, size_t nterms,
                                 const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids,
-                                ScProblem &P)
+                                ScProblem &P, int no_table = -1)
 {
     if (nterms && (!coeffs || !term_off || !term_ids)) return LZX_INVALID_ARGUMENT;
     if (nfactors && !factors) return LZX_INVALID_ARGUMENT;
@@ -614,9 +677,11 @@ static lzx_status build_problem(size_t num_vars, size_t nfactors, const uint64_t
     }
     if (d > (size_t)SC_MAXD) return LZX_UNSUPPORTED;
     for (size_t f = 0; f < nfactors; f++)
-        if (!factors[f]) return LZX_INVALID_ARGUMENT;
+        if (!factors[f] && (int)f != no_table) return LZX_INVALID_ARGUMENT;
     P.d = d;
     P.T.d = (uint32_t)d;
+    P.T.npts = (uint32_t)d + 1;
+    P.T.eq_f = -1;
     for (size_t f = 0; f < nfactors; f++) {
         uint8_t c = nterms ? 255 : 0;
         for (size_t t = 0; t < nterms; t++) c = std::min(c, P.exps[t][f]);
@@ -628,23 +693,25 @@ static lzx_status build_problem(size_t num_vars, size_t nfactors, const uint64_t
 }
 
 struct ScKernels {
-    void (*eval)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint64_t *);
-    void (*fold)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const uint64_t *, uint64_t *);
+    void (*eval)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const ScEqArgs &, uint64_t *);
+    void (*fold)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const uint64_t *, const ScEqArgs &,
+                 uint64_t *);
     void (*close)(hipStream_t, const uint64_t *, uint32_t, const ScRoundArgs &, uint32_t);
     void (*tail)(hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint32_t, uint32_t, const ScRoundArgs &);
 };
 
 template <int NF, int D>
-static void launch_eval(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half, uint64_t *part)
+static void launch_eval(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half,
+                        const ScEqArgs &E, uint64_t *part)
 {
-    hipLaunchKernelGGL((sc_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, part);
+    hipLaunchKernelGGL((sc_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, E, part);
 }
 
 template <int NF, int D>
 static void launch_fold(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half,
-                        const uint64_t *rp, uint64_t *part)
+                        const uint64_t *rp, const ScEqArgs &E, uint64_t *part)
 {
-    hipLaunchKernelGGL((sc_fold_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, rp, part);
+    hipLaunchKernelGGL((sc_fold_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, rp, E, part);
 }
 
 template <int D>
@@ -681,9 +748,28 @@ static uint32_t sc_grid(uint64_t work)
 // The prover loop shared by the transcript and explicit-challenge forms: every
 // round is enqueued up front (eval, then one fold per round, each closing its
 // round on the device), and the host synchronizes once at the end.
+// eq(w_a .. w_{b-1}, x) for x in {0,1}^(b-a), the first variable the most
+// significant bit (field_simd.rs:391-407 doubling order).
+static void host_eq_append(const uint64_t *w, size_t a, size_t b, std::vector<uint64_t> &out)
+{
+    std::vector<uint64_t> t(1, 1);
+    for (size_t i = a; i < b; i++) {
+        std::vector<uint64_t> n(t.size() * 2);
+        const uint64_t om = gl_sub(1, w[i]);
+        for (size_t k = 0; k < t.size(); k++) {
+            n[2 * k] = gl_mul(t[k], om);
+            n[2 * k + 1] = gl_mul(t[k], w[i]);
+        }
+        t.swap(n);
+    }
+    out.insert(out.end(), t.begin(), t.end());
+}
+
+// eq_w: the split eq factor's point (P.T.eq_f >= 0), else nullptr.
 static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors, uint64_t claim, lzx_transcript *tr,
-                             const uint64_t *given, uint64_t *rounds_out, uint64_t *challenges_out,
-                             uint64_t *final_claim, uint64_t *factor_claims_out, hipStream_t stream)
+                             const uint64_t *given, const uint64_t *eq_w, uint64_t *rounds_out,
+                             uint64_t *challenges_out, uint64_t *final_claim, uint64_t *factor_claims_out,
+                             hipStream_t stream)
 {
     const size_t m = P.num_vars, d = P.d, width = d + 1, nf = P.nf;
     const uint64_t N = 1ULL << m;
@@ -703,13 +789,60 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
         }
         A.L.w[i] = inv;
     }
+    const bool split = P.T.eq_f >= 0;
+    A.pw = split ? (uint32_t)d : (uint32_t)width;
+    // split eq: per-round (1 - w_j, 2 w_j - 1), round j's E tables (pairs over
+    // variables j+1..m-1: hi over all but the last lb, lo over the last lb)
+    // and S(d) = sum_{i<d} x_i S(i) for the degree-(d-1) sum S
+    std::vector<uint64_t> eqc, arena;
+    std::vector<uint64_t> hi_off(m), lo_off(m), lbits(m);
+    if (split) {
+        for (size_t i = 0; i < d; i++) {
+            uint64_t num = 1, den = 1;
+            for (size_t k = 0; k < d; k++)
+                if (k != i) {
+                    num = gl_mul(num, gl_sub(d, k));
+                    den = gl_mul(den, gl_sub(i, k));
+                }
+            uint64_t inv = 1, base = den, e = GL_P - 2;
+            while (e) {
+                if (e & 1) inv = gl_mul(inv, base);
+                base = gl_mul(base, base);
+                e >>= 1;
+            }
+            A.L.x[i] = gl_mul(num, inv);
+        }
+        const size_t LB = m / 2;
+        arena.push_back(1); // the empty hi table
+        const size_t o_lfix = arena.size();
+        host_eq_append(eq_w, m - LB, m, arena);
+        for (size_t j = 0; j < m; j++) {
+            eqc.push_back(gl_sub(1, eq_w[j]));
+            eqc.push_back(gl_sub(gl_add(eq_w[j], eq_w[j]), 1));
+            const size_t R = m - j - 1;
+            if (R > LB) {
+                hi_off[j] = arena.size();
+                host_eq_append(eq_w, j + 1, m - LB, arena);
+                lo_off[j] = o_lfix;
+                lbits[j] = LB;
+            } else {
+                hi_off[j] = 0;
+                lo_off[j] = arena.size();
+                host_eq_append(eq_w, j + 1, m, arena);
+                lbits[j] = R;
+            }
+        }
+    }
     // one device allocation: folded tables | block totals | control area
-    // control area: ScCtl | rounds[m][width] | chal[m] | given[m] | claims[nf]
+    // control area: ScCtl | rounds[m][width] | chal[m] | given[m] | claims[nf] (read back), then
+    // eq_c[m][2] | eq_rounds[m] | eq arena (setup only)
     const size_t mm = std::max<size_t>(1, m), nff = std::max<size_t>(1, nf);
     const uint64_t halfN = m ? N / 2 : 1;
     const size_t tab_bytes = nff * halfN * sizeof(uint64_t), part_bytes = SC_MAX_BLOCKS * width * sizeof(uint64_t);
     const size_t o_rounds = sizeof(ScCtl), o_chal = o_rounds + mm * width * 8, o_given = o_chal + mm * 8,
-                 o_claims = o_given + mm * 8, area = o_claims + nff * 8;
+                 o_claims = o_given + mm * 8, area_rb = o_claims + nff * 8;
+    const size_t o_eqc = (area_rb + 15) & ~(size_t)15, o_eqr = o_eqc + eqc.size() * 8,
+                 o_arena = o_eqr + (split ? m * sizeof(ScEqArgs) : 0), area = o_arena + arena.size() * 8;
     uint8_t *dev = nullptr;
     HostStage hs;
     bool enqueued = false;
@@ -728,7 +861,15 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
             c.tx.query_budget = tr->query_budget;
         }
         c.current = claim;
+        c.eq_scale = 1;
         if (given) memcpy(h + o_given, given, m * sizeof(uint64_t));
+        if (split) {
+            memcpy(h + o_eqc, eqc.data(), eqc.size() * 8);
+            const uint64_t *ar = (const uint64_t *)(dctl + o_arena);
+            ScEqArgs *er = (ScEqArgs *)(h + o_eqr);
+            for (size_t j = 0; j < m; j++) er[j] = ScEqArgs{ar + hi_off[j], ar + lo_off[j], (uint32_t)lbits[j], 0};
+            memcpy(h + o_arena, arena.data(), arena.size() * 8);
+        }
         e = hipMemcpyAsync(dctl, h, area, hipMemcpyHostToDevice, stream);
         enqueued = true;
         if (e != hipSuccess) st = device_fail(e, "sumcheck setup");
@@ -739,6 +880,13 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     A.chal = (uint64_t *)(dctl + o_chal);
     A.given = given ? (const uint64_t *)(dctl + o_given) : nullptr;
     A.claims = (uint64_t *)(dctl + o_claims);
+    A.eq_c = split ? (const uint64_t *)(dctl + o_eqc) : nullptr;
+    A.eq_rounds = split ? (const ScEqArgs *)(dctl + o_eqr) : nullptr;
+    auto eq_args = [&](size_t j) {
+        if (!split) return ScEqArgs{};
+        const uint64_t *ar = (const uint64_t *)(dctl + o_arena);
+        return ScEqArgs{ar + hi_off[j], ar + lo_off[j], (uint32_t)lbits[j], 0};
+    };
 
     ScTables tabs{};
     for (size_t f = 0; f < nf; f++) {
@@ -751,7 +899,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     };
     if (st == LZX_OK && m > 0) {
         const uint32_t g0 = sc_grid(N / 2);
-        K.eval(dim3(g0), stream, tabs, P.T, N / 2, partial);
+        K.eval(dim3(g0), stream, tabs, P.T, N / 2, eq_args(0), partial);
         check("sc_eval_kernel");
         K.close(stream, partial, g0, A, 0);
         check("sc_close_kernel");
@@ -759,7 +907,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
         for (; st == LZX_OK && (N >> (r + 1)) > SC_TAIL_HALF; r++) { // large rounds: a grid per fold
             const uint64_t half = N >> (r + 1);
             const uint32_t g = sc_grid(half / 2);
-            K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, partial);
+            K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, eq_args(r + 1), partial);
             check("sc_fold_eval_kernel");
             K.close(stream, partial, g, A, (uint32_t)r + 1);
             check("sc_close_kernel");
@@ -772,11 +920,12 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     }
     // zero rounds: the claims are the tables' single entries (sumcheck.rs:139-143)
     for (size_t f = 0; st == LZX_OK && m == 0 && f < nf; f++) {
+        if ((int)f == P.T.eq_f) continue; // eq over no variables is 1 (the scale)
         e = hipMemcpyAsync(dctl + o_claims + 8 * f, factors[f], sizeof(uint64_t), hipMemcpyDeviceToDevice, stream);
         if (e != hipSuccess) st = device_fail(e, "factor claim copy");
     }
     if (st == LZX_OK) { // one readback of the control area
-        e = hipMemcpyAsync(h, dctl, area, hipMemcpyDeviceToHost, stream);
+        e = hipMemcpyAsync(h, dctl, area_rb, hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) st = device_fail(e, "sumcheck readback");
         enqueued = false;
@@ -794,6 +943,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
             st = (lzx_status)c.status;
         } else {
             memcpy(factor_claims_out, h + o_claims, nf * sizeof(uint64_t));
+            if (split) factor_claims_out[P.T.eq_f] = c.eq_scale; // eq(w, r): the bound eq factor
             const uint64_t fin = host_eval_terms(P, factor_claims_out); // sumcheck.rs:144-154
             *final_claim = fin;
             if (fin != c.current) st = LZX_SUMCHECK_FINAL_CHECK_FAILED;
@@ -805,11 +955,14 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     return st;
 }
 
+// eq_index >= 0: that factor is eq(eq_w, .) and its table is never read (may
+// be null); the split applies when it appears exactly once in every term.
 static lzx_status prove_entry(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev, size_t nterms,
                               const uint64_t *coeffs, const uint32_t *term_off, const uint32_t *term_ids,
                               uint64_t claim, lzx_transcript *tr, const uint64_t *given, uint64_t *rounds_out,
                               uint64_t *challenges_out, uint64_t *final_claim, uint64_t *factor_claims_out,
-                              size_t *degree_out, hipStream_t stream)
+                              size_t *degree_out, hipStream_t stream, int eq_index = -1,
+                              const uint64_t *eq_w = nullptr)
 {
     if (!final_claim || !degree_out || (num_vars && (!rounds_out || !challenges_out))) return LZX_INVALID_ARGUMENT;
     if (!tr && !given && num_vars) return LZX_INVALID_ARGUMENT;
@@ -831,12 +984,19 @@ static lzx_status prove_entry(size_t num_vars, size_t nfactors, const uint64_t *
         return LZX_OK;
     }
     ScProblem P;
-    lzx_status st = build_problem(num_vars, nfactors, factors_dev, nterms, coeffs, term_off, term_ids, P);
+    lzx_status st = build_problem(num_vars, nfactors, factors_dev, nterms, coeffs, term_off, term_ids, P, eq_index);
     if (st != LZX_OK) return st;
     if (!factor_claims_out) return LZX_INVALID_ARGUMENT;
     *degree_out = P.d;
-    return prove_impl(P, factors_dev, claim, tr, given, rounds_out, challenges_out, final_claim, factor_claims_out,
-                      stream);
+    if (eq_index >= 0) {
+        P.T.eq_f = eq_index;
+        P.T.common[eq_index] = 0;
+        P.T.has_common = 0;
+        for (size_t f = 0; f < nfactors; f++) P.T.has_common |= P.T.common[f] != 0;
+        P.T.npts = (uint32_t)P.d;
+    }
+    return prove_impl(P, factors_dev, claim, tr, given, eq_index >= 0 ? eq_w : nullptr, rounds_out, challenges_out,
+                      final_claim, factor_claims_out, stream);
 }
 
 } // namespace lzx
@@ -868,6 +1028,65 @@ extern "C" lzx_status lzx_sumcheck_prove_with_challenges(size_t num_vars, size_t
     return prove_entry(num_vars, nfactors, factors_dev, nterms, coeffs, term_off, term_ids, claim, nullptr,
                        challenges, rounds_out, chal_copy.data(), final_claim, factor_claims_out, degree_out,
                        (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_sumcheck_prove_eq(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev,
+                                            size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
+                                            const uint32_t *term_ids, uint64_t claim, lzx_transcript *transcript,
+                                            const size_t *eq_indices, size_t n_eq, const uint64_t *w,
+                                            uint64_t *rounds_out, uint64_t *challenges_out, uint64_t *final_claim,
+                                            uint64_t *factor_claims_out, size_t *degree_out, void *stream)
+{
+    if (!transcript || (n_eq && !eq_indices) || (n_eq && num_vars && !w)) return LZX_INVALID_ARGUMENT;
+    if (nfactors > (size_t)SC_MAXF || nterms > (size_t)SC_MAXT) return LZX_UNSUPPORTED;
+    if (num_vars > 40) return LZX_UNSUPPORTED;
+    if (nfactors && !factors_dev) return LZX_INVALID_ARGUMENT;
+    if (nterms && (!coeffs || !term_off || !term_ids)) return LZX_INVALID_ARGUMENT;
+    // fastprover.rs:245-282: an index names a factor to treat as eq(w, .); indices
+    // matching no factor select nothing
+    std::vector<int> eqf;
+    for (size_t i = 0; i < n_eq; i++)
+        if (eq_indices[i] < nfactors && std::find(eqf.begin(), eqf.end(), (int)eq_indices[i]) == eqf.end())
+            eqf.push_back((int)eq_indices[i]);
+    if (!eqf.empty())
+        for (size_t j = 0; j < num_vars; j++)
+            if (w[j] >= GL_P) return LZX_INVALID_ARGUMENT;
+    hipStream_t s = (hipStream_t)stream;
+    if (nterms && eqf.size() == 1) { // split when eq is in every term exactly once
+        bool once = true;
+        for (size_t t = 0; t < nterms && once; t++) {
+            if (term_off[t + 1] < term_off[t]) return LZX_INVALID_ARGUMENT;
+            uint32_t c = 0;
+            for (uint32_t k = term_off[t]; k < term_off[t + 1]; k++) c += term_ids[k] == (uint32_t)eqf[0];
+            once = c == 1;
+        }
+        if (once)
+            return prove_entry(num_vars, nfactors, factors_dev, nterms, coeffs, term_off, term_ids, claim, transcript,
+                               nullptr, rounds_out, challenges_out, final_claim, factor_claims_out, degree_out, s,
+                               eqf[0], w);
+    }
+    // otherwise the eq factors are materialised from w: the same proof
+    std::vector<const uint64_t *> fac(factors_dev, factors_dev + nfactors);
+    std::vector<uint64_t *> owned;
+    lzx_status st = LZX_OK;
+    for (int f : eqf) {
+        uint64_t *tab = nullptr;
+        const hipError_t e = scratch_alloc((void **)&tab, ((size_t)1 << num_vars) * sizeof(uint64_t), s);
+        if (e != hipSuccess) {
+            st = device_fail(e, "eq table");
+            break;
+        }
+        owned.push_back(tab);
+        st = lzx_eq_table(w, num_vars, tab, stream);
+        if (st != LZX_OK) break;
+        fac[f] = tab;
+    }
+    if (st == LZX_OK)
+        st = prove_entry(num_vars, nfactors, fac.data(), nterms, coeffs, term_off, term_ids, claim, transcript,
+                         nullptr, rounds_out, challenges_out, final_claim, factor_claims_out, degree_out, s);
+    for (uint64_t *tab : owned) (void)scratch_free(tab, s);
+    if (!owned.empty()) (void)hipStreamSynchronize(s);
+    return st;
 }
 
 extern "C" lzx_status lzx_sumcheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_len,
@@ -963,7 +1182,7 @@ extern "C" lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors
     uint64_t *partial = nullptr, *dev_out = nullptr;
     LZX_HIP(scratch_alloc((void **)&partial, ((size_t)g + 1) * width * sizeof(uint64_t), s));
     dev_out = partial + (size_t)g * width;
-    K.eval(dim3(g), s, tabs, P.T, half, partial);
+    K.eval(dim3(g), s, tabs, P.T, half, ScEqArgs{}, partial);
     LZX_LAUNCH_CHECK("sc_eval_kernel");
     hipLaunchKernelGGL(sc_reduce_kernel, dim3(1), dim3(SC_BLOCK), 0, s, partial, g, (uint32_t)width, dev_out);
     LZX_LAUNCH_CHECK("sc_reduce_kernel");
@@ -972,5 +1191,115 @@ extern "C" lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors
     LZX_HIP(hipStreamSynchronize(s));
     LZX_HIP(scratch_free(partial, s));
     *sum_out = gl_add(ev[0], ev[1]); // sum over x0 = 0 and x0 = 1
+    return LZX_OK;
+}
+
+// ---- zerocheck (zerocheck.rs:26-84) -------------------------------------------
+
+// sum_x P(x) eq(w, x) without an eq table: round 0 of the split prover gives
+// S(t) = sum_p E_0(p) P(t, p) (t = 0, 1) and the sum is (1 - w_0) S(0) + w_0 S(1).
+static lzx_status split_eq_sum(size_t num_vars, const uint64_t *poly_dev, const uint64_t *w, uint64_t *sum_out,
+                               hipStream_t s)
+{
+    const uint64_t *fac[2] = {poly_dev, nullptr};
+    const uint64_t coeff[1] = {1};
+    const uint32_t off[2] = {0, 2}, ids[2] = {0, 1};
+    ScProblem P;
+    lzx_status st = build_problem(num_vars, 2, fac, 1, coeff, off, ids, P, 1);
+    if (st != LZX_OK) return st;
+    P.T.eq_f = 1;
+    P.T.common[1] = 0;
+    P.T.has_common = P.T.common[0] != 0;
+    P.T.npts = 2;
+    if (num_vars == 0) { // P(w) is P's single entry
+        LZX_HIP(hipMemcpy(sum_out, poly_dev, sizeof(uint64_t), hipMemcpyDeviceToHost));
+        return LZX_OK;
+    }
+    const size_t m = num_vars, LB = m / 2, R = m - 1;
+    std::vector<uint64_t> arena(1, 1);
+    size_t hi_off = 0, lo_off, lb;
+    if (R > LB) {
+        hi_off = arena.size();
+        host_eq_append(w, 1, m - LB, arena);
+        lo_off = arena.size();
+        host_eq_append(w, m - LB, m, arena);
+        lb = LB;
+    } else {
+        lo_off = arena.size();
+        host_eq_append(w, 1, m, arena);
+        lb = R;
+    }
+    const uint64_t half = 1ULL << R;
+    const uint32_t g = sc_grid(half);
+    uint64_t *buf = nullptr;
+    LZX_HIP(scratch_alloc((void **)&buf, ((size_t)g * 2 + 2 + arena.size()) * sizeof(uint64_t), s));
+    uint64_t *partial = buf, *dev_out = buf + (size_t)g * 2, *dar = dev_out + 2;
+    hipError_t e = hipMemcpyAsync(dar, arena.data(), arena.size() * 8, hipMemcpyHostToDevice, s);
+    ScTables tabs{};
+    tabs.src[0] = poly_dev;
+    const ScKernels K = pick_kernels(2, 2);
+    if (e == hipSuccess) {
+        K.eval(dim3(g), s, tabs, P.T, half, ScEqArgs{dar + hi_off, dar + lo_off, (uint32_t)lb, 0}, partial);
+        hipLaunchKernelGGL(sc_reduce_kernel, dim3(1), dim3(SC_BLOCK), 0, s, partial, g, 2u, dev_out);
+        e = hipGetLastError();
+    }
+    uint64_t S2[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(S2, dev_out, sizeof(S2), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s); // the arena and S2 are host memory
+    (void)scratch_free(buf, s);
+    if (e != hipSuccess) return device_fail(e, "zerocheck evaluation");
+    *sum_out = gl_add(gl_mul(gl_sub(1, w[0]), S2[0]), gl_mul(w[0], S2[1]));
+    return LZX_OK;
+}
+
+extern "C" lzx_status lzx_zerocheck_prove(size_t num_vars, const uint64_t *poly_dev, lzx_transcript *transcript,
+                                          uint64_t *point_out, uint64_t *rounds_out, uint64_t *challenges_out,
+                                          uint64_t *final_claim, uint64_t *factor_claims_out, void *stream)
+{
+    if (!transcript || !poly_dev || !final_claim || !factor_claims_out ||
+        (num_vars && (!point_out || !rounds_out || !challenges_out)))
+        return LZX_INVALID_ARGUMENT;
+    if (num_vars > 40) return LZX_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    // commit to the random point r (zerocheck.rs:31-34)
+    std::vector<uint64_t> r(num_vars ? num_vars : 1);
+    static const char LBL[] = "zerocheck-point";
+    lzx_status st =
+        lzx_transcript_challenge_fields(transcript, (const uint8_t *)LBL, sizeof(LBL) - 1, num_vars, r.data());
+    if (st != LZX_OK) return st;
+    if (num_vars) memcpy(point_out, r.data(), num_vars * sizeof(uint64_t));
+    // the claim P(r) = sum_x P(x) eq(r, x) must be zero (zerocheck.rs:37-45)
+    uint64_t pr = 0;
+    st = split_eq_sum(num_vars, poly_dev, r.data(), &pr, s);
+    if (st != LZX_OK) return st;
+    if (pr != 0) return LZX_ZEROCHECK_FINAL_CLAIM_NONZERO;
+    // sum-check of P(x) eq(r, x), claim 0 (zerocheck.rs:46-54); eq is never materialised
+    const uint64_t *fac[2] = {poly_dev, nullptr};
+    const uint64_t coeff[1] = {1};
+    const uint32_t off[2] = {0, 2}, ids[2] = {0, 1};
+    const size_t eqi[1] = {1};
+    size_t deg = 0;
+    return lzx_sumcheck_prove_eq(num_vars, 2, fac, 1, coeff, off, ids, 0, transcript, eqi, 1, r.data(), rounds_out,
+                                 challenges_out, final_claim, factor_claims_out, &deg, stream);
+}
+
+extern "C" lzx_status lzx_zerocheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_len,
+                                           size_t num_vars, lzx_transcript *transcript, int32_t has_expected,
+                                           uint64_t expected_poly_eval, uint64_t *point_out)
+{
+    if (!transcript || (num_vars && !point_out)) return LZX_INVALID_ARGUMENT;
+    if (num_vars > 40) return LZX_UNSUPPORTED;
+    std::vector<uint64_t> r(num_vars ? num_vars : 1);
+    static const char LBL[] = "zerocheck-point";
+    lzx_status st =
+        lzx_transcript_challenge_fields(transcript, (const uint8_t *)LBL, sizeof(LBL) - 1, num_vars, r.data());
+    if (st != LZX_OK) return st;
+    // degree-2 rounds (a product of two multilinears), claim 0 (zerocheck.rs:68-77)
+    std::vector<uint64_t> point(num_rounds ? num_rounds : 1);
+    uint64_t fin = 0;
+    st = lzx_sumcheck_verify(rounds, num_rounds, round_len, num_vars, 2, 0, transcript, has_expected,
+                             expected_poly_eval, point.data(), &fin);
+    if (st != LZX_OK) return st;
+    if (num_vars) memcpy(point_out, r.data(), num_vars * sizeof(uint64_t));
     return LZX_OK;
 }

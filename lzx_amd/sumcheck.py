@@ -79,7 +79,9 @@ class VirtualPolynomial:
     terms: list = field(default_factory=list)    # (coeff, [factor ids])
 
     def add_factor(self, table) -> int:
-        if table.numel() != 1 << self.num_vars:
+        """Append a device table; None stands for an eq(w, .) factor that only
+        prove_fast_with_eq supplies (never materialised)."""
+        if table is not None and table.numel() != 1 << self.num_vars:
             raise ValueError("factor has the wrong number of evaluations")
         self.factors.append(table)
         return len(self.factors) - 1
@@ -91,7 +93,8 @@ class VirtualPolynomial:
         return max((len(ids) for _, ids in self.terms), default=0)
 
     def _ffi(self):
-        ptrs = (C.c_void_p * max(1, len(self.factors)))(*[t.data_ptr() for t in self.factors])
+        ptrs = (C.c_void_p * max(1, len(self.factors)))(*[t.data_ptr() if t is not None else None
+                                                           for t in self.factors])
         coeffs = np.array([c for c, _ in self.terms] or [0], dtype=np.uint64)
         off = [0]
         ids = []
@@ -150,6 +153,39 @@ def prove(vp: VirtualPolynomial, claim: int, transcript: Transcript, stream=None
     return _prove(vp, claim, transcript, None, stream)
 
 
+def prove_fast(vp: VirtualPolynomial, claim: int, transcript: Transcript, opts=None, stream=None) -> SumcheckOutput:
+    """fastprover::prove_fast / prove_fast_with_opts (fastprover.rs:201-237): the
+    reference's fast prover is transcript-identical to sumcheck::prove, so this
+    is the device prover; the window / statistics options do not change the proof."""
+    return _prove(vp, claim, transcript, None, stream)
+
+
+def prove_fast_with_eq(vp: VirtualPolynomial, claim: int, transcript: Transcript, opts, eq_indices, w,
+                       stream=None) -> SumcheckOutput:
+    """fastprover::prove_fast_with_eq (fastprover.rs:212-224): the factors at
+    eq_indices are eq(w, .) and are never read (they may be None)."""
+    ptrs, coeffs, off, ids = vp._ffi()
+    d = max(vp.max_degree(), 1)
+    rounds = np.zeros(max(1, vp.num_vars * (d + 1)), dtype=np.uint64)
+    chals = np.zeros(max(1, vp.num_vars), dtype=np.uint64)
+    fclaims = np.zeros(max(1, len(vp.factors)), dtype=np.uint64)
+    eqi = np.array(list(eq_indices) or [0], dtype=np.uintp)
+    wv = np.ascontiguousarray(np.asarray(w, dtype=np.uint64).reshape(-1)[: vp.num_vars]) if len(w) else \
+        np.zeros(1, dtype=np.uint64)
+    if len(eq_indices) and wv.size < vp.num_vars:
+        raise ValueError("w needs one coordinate per variable")
+    fin = C.c_uint64()
+    deg = C.c_size_t()
+    check(lib().lzx_sumcheck_prove_eq(vp.num_vars, len(vp.factors), ptrs, len(vp.terms), coeffs.ctypes.data,
+                                      off.ctypes.data, ids.ctypes.data, claim, transcript.handle, eqi.ctypes.data,
+                                      len(eq_indices), wv.ctypes.data, rounds.ctypes.data, chals.ctypes.data,
+                                      C.byref(fin), fclaims.ctypes.data, C.byref(deg), dev.stream_handle(stream)),
+          "fastprover::prove_fast_with_eq")
+    wd = deg.value + 1
+    return SumcheckOutput(rounds[: vp.num_vars * wd].reshape(vp.num_vars, wd), chals[: vp.num_vars], fin.value,
+                          fclaims[: len(vp.factors)])
+
+
 def prove_with_challenges(vp: VirtualPolynomial, claim: int, challenges, stream=None) -> SumcheckOutput:
     """The same prover with caller-supplied challenges (no Fiat-Shamir)."""
     return _prove(vp, claim, None, challenges, stream)
@@ -175,3 +211,39 @@ def eq_table(point, out=None, stream=None):
         out = dev.empty(1 << p.size, np.uint64)
     check(lib().lzx_eq_table(p.ctypes.data, p.size, dev.ptr(out), dev.stream_handle(stream)), "eq_table")
     return out
+
+
+@dataclass
+class ZerocheckOutput:
+    point: np.ndarray       # r, drawn as "zerocheck-point" (num_vars)
+    output: SumcheckOutput  # the sum-check of P(x) eq(r, x) with claim 0
+
+
+def zerocheck_prove(poly, num_vars: int, transcript: Transcript, stream=None) -> ZerocheckOutput:
+    """zerocheck::prove(&poly, &mut transcript) (zerocheck.rs:26-54) on a device
+    table of 2^num_vars canonical entries.  Raises LzxError with
+    ZerocheckError::FinalClaimNonZero when P does not vanish on the hypercube."""
+    if poly.numel() != 1 << num_vars:
+        raise ValueError("poly has the wrong number of evaluations")
+    point = np.zeros(max(1, num_vars), dtype=np.uint64)
+    rounds = np.zeros(max(1, num_vars * 3), dtype=np.uint64)
+    chals = np.zeros(max(1, num_vars), dtype=np.uint64)
+    fclaims = np.zeros(2, dtype=np.uint64)
+    fin = C.c_uint64()
+    check(lib().lzx_zerocheck_prove(num_vars, dev.ptr(poly), transcript.handle, point.ctypes.data,
+                                    rounds.ctypes.data, chals.ctypes.data, C.byref(fin), fclaims.ctypes.data,
+                                    dev.stream_handle(stream)), "zerocheck::prove")
+    out = SumcheckOutput(rounds[: num_vars * 3].reshape(num_vars, 3), chals[:num_vars], fin.value, fclaims)
+    return ZerocheckOutput(point[:num_vars], out)
+
+
+def zerocheck_verify(rounds, num_vars: int, transcript: Transcript, expected_poly_eval=None) -> np.ndarray:
+    """zerocheck::verify (zerocheck.rs:58-84): returns the point r."""
+    r = np.ascontiguousarray(rounds, dtype=np.uint64)
+    nr = r.shape[0] if r.ndim == 2 else 0
+    rl = r.shape[1] if r.ndim == 2 else 0
+    point = np.zeros(max(1, num_vars), dtype=np.uint64)
+    check(lib().lzx_zerocheck_verify(r.ctypes.data if r.size else None, nr, rl, num_vars, transcript.handle,
+                                     1 if expected_poly_eval is not None else 0, expected_poly_eval or 0,
+                                     point.ctypes.data), "zerocheck::verify")
+    return point[:num_vars]

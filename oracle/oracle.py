@@ -480,3 +480,33 @@ def splitmix64(seed):
         z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
         z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
         yield z ^ (z >> 31)
+
+
+# ---------------------------------------------------------------------------
+# zerocheck (crates/lattice-sumcheck/src/zerocheck.rs:26-84), composed from the
+# oracle's transcript, eq table, MLE evaluation and sum-check.
+# ---------------------------------------------------------------------------
+
+ZEROCHECK_FINAL_CLAIM_NONZERO = 15
+
+
+def zerocheck_prove(num_vars, poly, transcript: Transcript):
+    """zerocheck::prove (zerocheck.rs:26-54).  Returns (status, point, sumcheck dict)."""
+    st, r = transcript.challenge_fields(b"zerocheck-point", num_vars)  # :31-34
+    if st:
+        return st, None, None
+    r = np.asarray(r, dtype=np.uint64)[:num_vars]
+    eq = eq_table(r)                                   # :37
+    if mle_evaluate(poly, r) != 0:                     # :38-45
+        return ZEROCHECK_FINAL_CLAIM_NONZERO, r, None
+    st, out = sumcheck_prove(num_vars, [poly, eq], [(1, [0, 1])], 0, transcript)  # :46-54
+    return st, r, out
+
+
+def zerocheck_verify(rounds, num_vars, transcript: Transcript, expected_poly_eval=None):
+    """zerocheck::verify (zerocheck.rs:58-84).  Returns (status, r)."""
+    st, r = transcript.challenge_fields(b"zerocheck-point", num_vars)
+    if st:
+        return st, None
+    st, _point, _fin = sumcheck_verify(rounds, num_vars, 2, 0, transcript, expected_poly_eval)
+    return st, (np.asarray(r, dtype=np.uint64)[:num_vars] if st == 0 else None)

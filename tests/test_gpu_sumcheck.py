@@ -175,3 +175,149 @@ def test_full_workload_matches_oracle(sc):
     assert out.challenges.tolist() == want["challenges"].tolist()
     assert out.factor_claims.tolist() == want["factor_claims"].tolist()
     assert out.final_claim == want["final_claim"]
+
+
+@pytest.mark.parametrize("nv", [1, 4, 11, 14])
+def test_zerocheck_matches_oracle(sc, nv):
+    # zerocheck.rs:26-84: point, rounds, challenges, claims and the transcript
+    # state after the proof, across the close-kernel and tail paths
+    S, D = sc
+    z = np.zeros(1 << nv, dtype=np.uint64)
+    t_dev, t_or = S.Transcript(b"lzx-zerocheck-test"), O.Transcript(b"lzx-zerocheck-test")
+    t_dev.append_message(b"ctx", b"zc")
+    t_or.append_message(b"ctx", b"zc")
+    got = S.zerocheck_prove(D.to_device(z), nv, t_dev)
+    st, r, want = O.zerocheck_prove(nv, z, t_or)
+    assert st == 0
+    assert got.point.tolist() == r.tolist()
+    assert got.output.rounds.tolist() == want["rounds"].tolist()
+    assert got.output.challenges.tolist() == want["challenges"].tolist()
+    assert got.output.factor_claims.tolist() == want["factor_claims"].tolist()
+    assert got.output.final_claim == want["final_claim"]
+    assert t_dev.challenge_field(b"ident") == t_or.challenge_field(b"ident")[1]
+    # the device proof verifies (zerocheck::verify), expected P(r') = 0
+    tv = S.Transcript(b"lzx-zerocheck-test")
+    tv.append_message(b"ctx", b"zc")
+    assert S.zerocheck_verify(got.output.rounds, nv, tv, 0).tolist() == r.tolist()
+
+
+def test_zerocheck_refuses_nonzero_and_rejects_tampering(sc):
+    from lzx_amd._lib import LzxError
+    S, D = sc
+    nv = 6
+    p = np.zeros(1 << nv, dtype=np.uint64)
+    p[3] = 1
+    with pytest.raises(LzxError) as e:
+        S.zerocheck_prove(D.to_device(p), nv, S.Transcript(b"lzx-zerocheck-test"))
+    assert e.value.code == O.ZEROCHECK_FINAL_CLAIM_NONZERO
+    out = S.zerocheck_prove(D.to_device(np.zeros(1 << nv, dtype=np.uint64)), nv, S.Transcript(b"lzx-zerocheck-test"))
+    bad = out.output.rounds.copy()
+    bad[0, 0] = 1
+    with pytest.raises(LzxError):
+        S.zerocheck_verify(bad, nv, S.Transcript(b"lzx-zerocheck-test"), None)
+
+
+# ---- fastprover::prove_fast_with_eq (fastprover.rs:212-224) ----------------------
+# The reference's fast prover is transcript-identical to sumcheck::prove with
+# the eq factor materialised (tests/fastprover.rs:140-170), so the oracle is the
+# oracle prover over eq_table(w).
+
+EQ_SHAPES = [
+    # (num_vars, nfactors, eq index, terms); ids refer to factors, eq appears once per term -> split
+    (1, 2, 1, [(1, [0, 1])]),
+    (7, 2, 1, [(1, [0, 1])]),
+    (12, 2, 1, [(1, [0, 1])]),
+    (16, 2, 1, [(1, [0, 1])]),
+    (5, 4, 0, [(1, [0, 1, 2]), (P - 1, [0, 3])]),
+    (13, 4, 0, [(1, [0, 1, 2]), (P - 1, [0, 3])]),
+    (18, 4, 0, [(1, [0, 1, 2]), (P - 1, [0, 3])]),
+    (10, 3, 1, [(3, [0, 1, 2]), (5, [1])]),                 # eq alone in a term: degree-0 remainder
+    (9, 5, 2, [(7, [0, 2, 1, 1]), (P - 2, [3, 2, 4])]),     # a squared factor beside eq, NF = 8 kernels
+    # eq twice in a term or missing from a term: materialised from w
+    (8, 3, 1, [(1, [0, 1, 1])]),
+    (8, 3, 1, [(1, [0, 1]), (1, [2])]),
+]
+
+
+@pytest.mark.parametrize("shape", EQ_SHAPES, ids=lambda s: f"nv{s[0]}_f{s[1]}_eq{s[2]}_{len(s[3])}t")
+def test_prove_fast_with_eq_matches_oracle(sc, shape):
+    S, D = sc
+    nv, nf, e, terms = shape
+    rng = np.random.default_rng(1000 + nv * 7 + nf)
+    w = rng.integers(0, P, nv, dtype=np.uint64)
+    tabs = [rng.integers(0, P, 1 << nv, dtype=np.uint64) for _ in range(nf)]
+    tabs[e] = O.eq_table(w)
+    claim = O.vp_sum_over_hypercube(nv, tabs, terms)
+    t_dev, t_or = S.Transcript(b"sc-eq"), O.Transcript(b"sc-eq")
+    st, want = O.sumcheck_prove(nv, tabs, terms, claim, t_or)
+    assert st == 0
+    vp = S.VirtualPolynomial(nv)
+    for f, t in enumerate(tabs):
+        vp.add_factor(None if f == e else D.to_device(t))  # the eq table is never handed over
+    for c, ids in terms:
+        vp.add_term(c, ids)
+    got = S.prove_fast_with_eq(vp, claim, t_dev, None, [e], w)
+    assert got.rounds.tolist() == want["rounds"].tolist()
+    assert got.challenges.tolist() == want["challenges"].tolist()
+    assert got.factor_claims.tolist() == want["factor_claims"].tolist()
+    assert got.final_claim == want["final_claim"]
+    assert t_dev.challenge_field(b"ident") == t_or.challenge_field(b"ident")[1]
+
+
+def test_prove_fast_equals_prove_and_rejects_bad_claims(sc):
+    from lzx_amd._lib import LzxError
+    S, D = sc
+    nv = 9
+    rng = np.random.default_rng(77)
+    w = rng.integers(0, P, nv, dtype=np.uint64)
+    a = rng.integers(0, P, 1 << nv, dtype=np.uint64)
+    tabs = [a, O.eq_table(w)]
+    vp = _vp(S, D, nv, tabs, [(1, [0, 1])])
+    claim = O.vp_sum_over_hypercube(nv, tabs, [(1, [0, 1])])
+    x = S.prove(vp, claim, S.Transcript(b"pf"))
+    y = S.prove_fast(vp, claim, S.Transcript(b"pf"))
+    z = S.prove_fast_with_eq(vp, claim, S.Transcript(b"pf"), None, [1], w)
+    for o in (y, z):
+        assert o.rounds.tolist() == x.rounds.tolist() and o.factor_claims.tolist() == x.factor_claims.tolist()
+    with pytest.raises(LzxError) as e:  # a wrong claim fails the first round check (sumcheck.rs:120-123)
+        S.prove_fast_with_eq(vp, (claim + 1) % P, S.Transcript(b"pf"), None, [1], w)
+    assert e.value.code == 13
+    with pytest.raises(LzxError) as e:  # w must be canonical field elements
+        S.prove_fast_with_eq(vp, claim, S.Transcript(b"pf"), None, [1], np.full(nv, P, dtype=np.uint64))
+    assert e.value.code == 30
+
+
+@pytest.mark.parametrize("nv", [3, 12, 17])
+def test_zerocheck_nonzero_table_with_vanishing_point(sc, nv):
+    # r is fixed by the transcript before P is seen, so a P that is non-zero on
+    # the hypercube but has P(r) = 0 passes zerocheck::prove's refusal check
+    # (zerocheck.rs:37-45) and yields non-trivial rounds; one entry off and
+    # both provers refuse.  Pins the device's eq-free evaluation of P(r).
+    S, D = sc
+    probe = O.Transcript(b"zc-vanish")
+    st, r = probe.challenge_fields(b"zerocheck-point", nv)
+    assert st == 0
+    eq = O.eq_table(r)
+    rng = np.random.default_rng(nv)
+    p = rng.integers(0, P, 1 << nv, dtype=np.uint64)
+    k = int(rng.integers(0, 1 << nv))
+    p[k] = 0
+    rest = O.vp_sum_over_hypercube(nv, [p, eq], [(1, [0, 1])])
+    p[k] = (P - rest) * O.gl_inverse(int(eq[k])) % P
+    assert O.mle_evaluate(p, r) == 0
+    t_dev, t_or = S.Transcript(b"zc-vanish"), O.Transcript(b"zc-vanish")
+    got = S.zerocheck_prove(D.to_device(p), nv, t_dev)
+    st, r2, want = O.zerocheck_prove(nv, p, t_or)
+    assert st == 0 and got.point.tolist() == r2.tolist() == r.tolist()
+    assert want["rounds"].any()
+    assert got.output.rounds.tolist() == want["rounds"].tolist()
+    assert got.output.factor_claims.tolist() == want["factor_claims"].tolist()
+    assert got.output.final_claim == want["final_claim"]
+    assert t_dev.challenge_field(b"ident") == t_or.challenge_field(b"ident")[1]
+    from lzx_amd._lib import LzxError
+    p[k] = (int(p[k]) + 1) % P
+    st, _, _ = O.zerocheck_prove(nv, p, O.Transcript(b"zc-vanish"))
+    assert st == O.ZEROCHECK_FINAL_CLAIM_NONZERO
+    with pytest.raises(LzxError) as e:
+        S.zerocheck_prove(D.to_device(p), nv, S.Transcript(b"zc-vanish"))
+    assert e.value.code == O.ZEROCHECK_FINAL_CLAIM_NONZERO

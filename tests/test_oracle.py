@@ -416,3 +416,61 @@ def test_device_digit_generator_matches_numpy_rule():
         got = W.small_digits_torch(seed, 4099, device="cpu").numpy()
         assert np.array_equal(got, W.small_digits(seed, 4099)), seed
     assert torch is not None
+
+
+# ---- zerocheck (zerocheck.rs:26-84) -------------------------------------------
+
+def _mle_closed_form(evals, r):
+    """sum_x P(x) prod_i (x_i r_i + (1 - x_i)(1 - r_i)), variable 0 = MSB (mle.rs)."""
+    m, tot = len(r), 0
+    for b in range(1 << m):
+        w = 1
+        for i in range(m):
+            bit = (b >> (m - 1 - i)) & 1
+            w = w * (int(r[i]) if bit else (1 - int(r[i])) % O.GL_P) % O.GL_P
+        tot = (tot + int(evals[b]) * w) % O.GL_P
+    return tot
+
+
+def test_mle_evaluate_closed_form_and_eq_identity():
+    # P(r) = sum_x P(x) eq(r, x): the identity the device zerocheck uses for its refusal check
+    rng = np.random.default_rng(15)
+    for m in range(0, 6):
+        p = rng.integers(0, O.GL_P, 1 << m, dtype=np.uint64)
+        r = rng.integers(0, O.GL_P, m, dtype=np.uint64)
+        want = _mle_closed_form(p, r)
+        assert O.mle_evaluate(p, r) == want
+        assert O.vp_sum_over_hypercube(m, [p, O.eq_table(r)], [(1, [0, 1])]) == want
+
+
+def test_zerocheck_zero_polynomial_proves_and_verifies():
+    # zerocheck.rs:95-113 zero_poly_proves_and_verifies
+    nv = 5
+    p = np.zeros(1 << nv, dtype=np.uint64)
+    pt = O.Transcript(b"lzx-zerocheck-test")
+    st, r, out = O.zerocheck_prove(nv, p, pt)
+    assert st == 0 and r.size == nv
+    assert out["rounds"].shape == (nv, 3) and not out["rounds"].any()
+    assert int(out["factor_claims"][0]) == 0
+    # eq's claim is eq(r, r') at the sum-check point r'
+    assert int(out["factor_claims"][1]) == _mle_closed_form(O.eq_table(r), out["challenges"])
+    vt = O.Transcript(b"lzx-zerocheck-test")
+    st, rv = O.zerocheck_verify(out["rounds"], nv, vt, 0)
+    assert st == 0 and rv.tolist() == r.tolist()
+    assert pt.challenge_field(b"ident") == vt.challenge_field(b"ident")
+
+
+def test_zerocheck_refuses_nonzero_and_rejects_tampered_rounds():
+    # zerocheck.rs:115-141 nonzero_poly_refused, :143-161 zerocheck_tampered_round_rejected
+    nv = 4
+    p = np.zeros(1 << nv, dtype=np.uint64)
+    p[3] = 1
+    st, _r, out = O.zerocheck_prove(nv, p, O.Transcript(b"lzx-zerocheck-test"))
+    assert st == O.ZEROCHECK_FINAL_CLAIM_NONZERO and out is None
+    z = np.zeros(1 << nv, dtype=np.uint64)
+    st, _r, out = O.zerocheck_prove(nv, z, O.Transcript(b"lzx-zerocheck-test"))
+    assert st == 0
+    bad = out["rounds"].copy()
+    bad[0, 0] = 1
+    st, _ = O.zerocheck_verify(bad, nv, O.Transcript(b"lzx-zerocheck-test"), None)
+    assert st != 0
