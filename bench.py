@@ -220,6 +220,46 @@ def bench_sumcheck(reps: int = 3):
                                             and fast.final_claim == out.final_claim)}}
 
 
+def bench_ring50(reps: int = 10):
+    """The ~2^50 quadratic-slot ring (q = 2^50 - 2687): 2^16 polynomials of
+    n = 128, u64 residues.  Forward transform in place (read + write 64 MiB) and
+    the fused ring product (forward both, slotwise Karatsuba, inverse: read
+    2 x 64 MiB, write 64 MiB); HIP events on the launch stream."""
+    import torch
+
+    from lzx_amd import device as D
+    from lzx_amd import ring50 as R
+
+    polys, n = 1 << 16, 128
+    rng = np.random.default_rng(0x50)
+    a = D.to_device(rng.integers(0, R.Q_50, polys * n, dtype=np.uint64))
+    b = D.to_device(rng.integers(0, R.Q_50, polys * n, dtype=np.uint64))
+    out = D.empty(polys * n, np.uint64)
+    T = R.QuadSlotTables(R.Modulus50.Q_50, 7)
+    stream = torch.cuda.current_stream()
+    words = polys * n
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return statistics.median(ts)
+
+    t_fwd = timed(lambda: T.forward_batch(a))
+    t_mul = timed(lambda: T.mul_batch(a, b, out))
+    return {"workload": "quadratic-slot ring q=2^50-2687, 2^16 polys x n=128, u64",
+            "forward_us": round(t_fwd * 1e6, 2), "forward_gbs": round(2 * 8 * words / t_fwd / 1e9, 1),
+            "mul_us": round(t_mul * 1e6, 2), "mul_gbs": round(3 * 8 * words / t_mul / 1e9, 1),
+            "mul_polys_per_s": round(polys / t_mul, 1)}
+
+
 def bench_uneven_batch(reps: int = 5, autotune: bool = True):
     """The uneven commitment batch: 512 jobs (workloads.uneven_batch_jobs(0xC5)),
     n in {64, 128, 256}, q in {3221225473, 2013265921, 1125899437080577},
@@ -308,6 +348,7 @@ def main():
     ap.add_argument("--only-ntt", action="store_true", help="skip the secondary workloads (profiling runs)")
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven commitment batch leg")
     ap.add_argument("--only-sumcheck", action="store_true", help="run only the 2^24 sum-check prove leg")
+    ap.add_argument("--only-ring50", action="store_true", help="run only the 50-bit quadratic-slot ring leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch leg: use cached (LZX_AUTOTUNE_CACHE) or default chunks instead of tuning")
     ap.add_argument("--copy-calib", action="store_true",
@@ -321,6 +362,9 @@ def main():
 
     from lzx_amd import device as D
     D.require_gpu()
+    if args.only_ring50:
+        print(json.dumps({"ring50": bench_ring50()}), flush=True)
+        return
     if args.only_sumcheck:
         print(json.dumps({"sumcheck_prove": bench_sumcheck(reps=5)}), flush=True)
         return
@@ -359,7 +403,7 @@ def main():
     }
     if not args.only_ntt:
         line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
-                             "uneven_batch": bench_uneven_batch()}
+                             "uneven_batch": bench_uneven_batch(), "ring50": bench_ring50()}
         if not args.no_cpu:
             line["secondary"]["cpu_baseline"] = cpu_baseline_secondary()
     print(json.dumps(line), flush=True)

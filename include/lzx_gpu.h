@@ -45,6 +45,10 @@ enum {
     LZX_SUMCHECK_CLAIM_MISMATCH = 13,   /* SumcheckError::ClaimMismatch    sumcheck.rs:45 */
     LZX_SUMCHECK_VIRTUAL_POLY = 14,     /* SumcheckError::VirtualPoly      sumcheck.rs:30 */
     LZX_ZEROCHECK_FINAL_CLAIM_NONZERO = 15, /* ZerocheckError::FinalClaimNonZero zerocheck.rs:20 */
+    LZX_MOD50_LOG_N_OUT_OF_RANGE = 40,  /* Mod50Error::LogNOutOfRange    modulus50.rs:418 */
+    LZX_MOD50_ADICITY_TOO_SMALL = 41,   /* Mod50Error::AdicityTooSmall   modulus50.rs:419 */
+    LZX_MOD50_GENERATOR_INVALID = 42,   /* Mod50Error::GeneratorInvalid  modulus50.rs:420 */
+    LZX_MOD50_LENGTH_MISMATCH = 43,     /* Mod50Error::LengthMismatch    modulus50.rs:421 */
     LZX_TRANSCRIPT_REJECTION_BUDGET = 20, /* TranscriptError::RejectionBudgetExceeded transcript.rs:16 */
     LZX_TRANSCRIPT_MESSAGE_TOO_LARGE = 21, /* TranscriptError::MessageTooLarge transcript.rs:18 */
     LZX_INVALID_ARGUMENT = 30,          /* null handle/pointer or malformed shape */
@@ -312,6 +316,30 @@ lzx_status lzx_eq_table(const uint64_t *point_host, size_t num_vars, uint64_t *o
 lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev,
                                      size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
                                      const uint32_t *term_ids, uint64_t *sum_out, void *stream);
+
+/* ---- the ~2^50 quadratic-slot ring (modulus50.rs:44-499) -------------------
+ * QuadSlotTables / RingConfig50 over a prime q < 2^50 of two-adicity >= 7
+ * (Q_50 = 2^50 - 2687, generator 3), n = 2^log_n with log_n in [1, 7].
+ * Batched calls take len = (number of polynomials) x n canonical u64
+ * residues in HBM, [poly][coeff]; len not a multiple of n is
+ * LZX_MOD50_LENGTH_MISMATCH. */
+typedef struct lzx_quad_plan lzx_quad_plan;
+
+/* QuadSlotTables::new (modulus50.rs:241-300). */
+lzx_status lzx_quad_plan_create(uint64_t q, uint32_t two_adicity, uint64_t generator, uint32_t log_n,
+                                lzx_quad_plan **out);
+void lzx_quad_plan_destroy(lzx_quad_plan *plan);
+/* RingConfig50::slot_constants (modulus50.rs:446-449): the n/2 leaf gammas; and zeta. */
+lzx_status lzx_quad_slot_constants(const lzx_quad_plan *plan, uint64_t *gamma_out, uint64_t *zeta_out);
+/* QuadSlotTables::forward / inverse (modulus50.rs:317-375), in place. */
+lzx_status lzx_quad_forward_batch(const lzx_quad_plan *plan, uint64_t *data_dev, size_t len, void *stream);
+lzx_status lzx_quad_inverse_batch(const lzx_quad_plan *plan, uint64_t *data_dev, size_t len, void *stream);
+/* QuadSlotTables::slotwise_mul (modulus50.rs:380-402) on slot-form inputs; out may alias a or b. */
+lzx_status lzx_quad_slotwise_mul_batch(const lzx_quad_plan *plan, const uint64_t *a_dev, const uint64_t *b_dev,
+                                       uint64_t *out_dev, size_t len, void *stream);
+/* QuadSlotTables::mul / RingConfig50::mul (modulus50.rs:406-421, 451-454): ring products. */
+lzx_status lzx_quad_mul_batch(const lzx_quad_plan *plan, const uint64_t *a_dev, const uint64_t *b_dev,
+                              uint64_t *out_dev, size_t len, void *stream);
 
 #ifdef __cplusplus
 }

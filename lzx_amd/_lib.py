@@ -36,6 +36,14 @@ PROTOTYPES = {
     "lzx_ntt_forward_host": (st_t, [vp, vp, sz]),
     "lzx_ntt_inverse_host": (st_t, [vp, vp, sz]),
     "lzx_ring_mul_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    # Quadratic-slot ring over ~2^50 moduli
+    "lzx_quad_plan_create": (st_t, [u64, u32, u64, u32, C.POINTER(vp)]),
+    "lzx_quad_plan_destroy": (None, [vp]),
+    "lzx_quad_slot_constants": (st_t, [vp, vp, C.POINTER(u64)]),
+    "lzx_quad_forward_batch": (st_t, [vp, vp, sz, vp]),
+    "lzx_quad_inverse_batch": (st_t, [vp, vp, sz, vp]),
+    "lzx_quad_slotwise_mul_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    "lzx_quad_mul_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
     # Ajtai
     "lzx_ajtai_key_from_seed": (st_t, [u32, u32, u32, u32, sz, sz, u32, u8p, C.POINTER(vp)]),
     "lzx_ajtai_key_from_matrix": (st_t, [u32, u32, u32, u32, sz, sz, u32, vp, C.POINTER(vp)]),

@@ -110,6 +110,10 @@ extern "C" const char *lzx_status_string(lzx_status s)
     case LZX_SUMCHECK_CLAIM_MISMATCH: return "SumcheckError::ClaimMismatch";
     case LZX_SUMCHECK_VIRTUAL_POLY: return "SumcheckError::VirtualPoly";
     case LZX_ZEROCHECK_FINAL_CLAIM_NONZERO: return "ZerocheckError::FinalClaimNonZero";
+    case LZX_MOD50_LOG_N_OUT_OF_RANGE: return "Mod50Error::LogNOutOfRange";
+    case LZX_MOD50_ADICITY_TOO_SMALL: return "Mod50Error::AdicityTooSmall";
+    case LZX_MOD50_GENERATOR_INVALID: return "Mod50Error::GeneratorInvalid";
+    case LZX_MOD50_LENGTH_MISMATCH: return "Mod50Error::LengthMismatch";
     case LZX_TRANSCRIPT_REJECTION_BUDGET: return "TranscriptError::RejectionBudgetExceeded";
     case LZX_TRANSCRIPT_MESSAGE_TOO_LARGE: return "TranscriptError::MessageTooLarge";
     case LZX_INVALID_ARGUMENT: return "InvalidArgument";

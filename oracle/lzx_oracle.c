@@ -1121,3 +1121,131 @@ size_t or_commit_wide_check_at(uint64_t q, uint32_t n, size_t k, size_t m, size_
     free(sz);
     return bad;
 }
+
+/* ---- the ~2^50 quadratic-slot ring (crates/lattice-ring/src/modulus50.rs) ----
+ * Status codes follow Mod50Error (:417-422): 40 LogNOutOfRange, 41
+ * AdicityTooSmall, 42 GeneratorInvalid, 43 LengthMismatch. */
+
+static uint64_t q50_pow(uint64_t b, uint64_t e, uint64_t q) /* modulus50.rs:168-180 */
+{
+    uint64_t acc = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) acc = (uint64_t)((u128)acc * b % q);
+        b = (uint64_t)((u128)b * b % q);
+        e >>= 1;
+    }
+    return acc;
+}
+
+static size_t q50_rev(size_t x, uint32_t bits) /* modulus50.rs:229-236 */
+{
+    size_t r = 0;
+    for (uint32_t b = 0; b < bits; b++) r |= ((x >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
+/* QuadSlotTables::new (modulus50.rs:241-300): zeta, the per-level twiddles
+ * flattened (level d, block i at 2^d - 1 + i; 2^(log_n - 1) - 1 entries),
+ * their inverses, and the n/2 leaf constants. */
+int or_quad_tables(uint64_t q, uint32_t two_adicity, uint64_t generator, uint32_t log_n, uint64_t *zeta_out,
+                   uint64_t *tw, uint64_t *tw_inv, uint64_t *gamma)
+{
+    if (log_n == 0 || log_n > 7) return 40;
+    if (two_adicity < 7) return 41;
+    const uint64_t zeta = q50_pow(generator, (q - 1) >> 7, q); /* root_of_unity(7) */
+    if (q50_pow(zeta, 64, q) != q - 1) return 42;
+    if (zeta_out) *zeta_out = zeta;
+    for (uint32_t d = 0; d + 1 < log_n; d++)
+        for (size_t i = 0; i < ((size_t)1 << d); i++) {
+            const uint64_t e = (((uint64_t)(2 * q50_rev(i, d) + 1)) << (5 - d)) % 128;
+            const uint64_t w = q50_pow(zeta, e, q);
+            if (tw) tw[((size_t)1 << d) - 1 + i] = w;
+            if (tw_inv) tw_inv[((size_t)1 << d) - 1 + i] = q50_pow(w, q - 2, q);
+        }
+    for (size_t p = 0; p < ((size_t)1 << (log_n - 1)); p++) {
+        const uint64_t e = (((uint64_t)(2 * q50_rev(p, log_n - 1) + 1)) << (7 - log_n)) % 128;
+        if (gamma) gamma[p] = q50_pow(zeta, e, q);
+    }
+    return 0;
+}
+
+static uint64_t q50_add(uint64_t a, uint64_t b, uint64_t q) { return a + b >= q ? a + b - q : a + b; }
+static uint64_t q50_sub(uint64_t a, uint64_t b, uint64_t q) { return a >= b ? a - b : a + (q - b); }
+static uint64_t q50_mul(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((u128)a * b % q); }
+
+/* forward (modulus50.rs:317-342): incomplete DIF, lo' = lo + w hi, hi' = lo - w hi */
+int or_quad_forward(uint64_t q, uint32_t two_adicity, uint64_t generator, uint32_t log_n, uint64_t *a)
+{
+    uint64_t tw[64];
+    const int st = or_quad_tables(q, two_adicity, generator, log_n, NULL, tw, NULL, NULL);
+    if (st) return st;
+    const size_t n = (size_t)1 << log_n;
+    for (uint32_t d = 0; d + 1 < log_n; d++) {
+        const size_t span = n >> d, half = span / 2;
+        for (size_t i = 0; i < ((size_t)1 << d); i++) {
+            const uint64_t w = tw[((size_t)1 << d) - 1 + i];
+            for (size_t j = 0; j < half; j++) {
+                const uint64_t lo = a[i * span + j], t = q50_mul(w, a[i * span + j + half], q);
+                a[i * span + j] = q50_add(lo, t, q);
+                a[i * span + j + half] = q50_sub(lo, t, q);
+            }
+        }
+    }
+    return 0;
+}
+
+/* inverse (modulus50.rs:346-375): lo = (lo' + hi') / 2, hi = (lo' - hi') / (2 w), levels reversed */
+int or_quad_inverse(uint64_t q, uint32_t two_adicity, uint64_t generator, uint32_t log_n, uint64_t *a)
+{
+    uint64_t twi[64];
+    const int st = or_quad_tables(q, two_adicity, generator, log_n, NULL, NULL, twi, NULL);
+    if (st) return st;
+    const size_t n = (size_t)1 << log_n;
+    const uint64_t inv2 = q50_pow(2, q - 2, q);
+    for (uint32_t d = log_n - 1; d-- > 0;) {
+        const size_t span = n >> d, half = span / 2;
+        for (size_t i = 0; i < ((size_t)1 << d); i++) {
+            const uint64_t wi = twi[((size_t)1 << d) - 1 + i];
+            for (size_t j = 0; j < half; j++) {
+                const uint64_t lo = a[i * span + j], hi = a[i * span + j + half];
+                a[i * span + j] = q50_mul(q50_add(lo, hi, q), inv2, q);
+                a[i * span + j + half] = q50_mul(q50_mul(q50_sub(lo, hi, q), inv2, q), wi, q);
+            }
+        }
+    }
+    return 0;
+}
+
+/* slotwise_mul (modulus50.rs:380-402): (u0 + u1 X)(v0 + v1 X) mod (X^2 - gamma_p), Karatsuba */
+int or_quad_slotwise_mul(uint64_t q, uint32_t two_adicity, uint64_t generator, uint32_t log_n, const uint64_t *a,
+                         const uint64_t *b, uint64_t *out)
+{
+    uint64_t gamma[64];
+    const int st = or_quad_tables(q, two_adicity, generator, log_n, NULL, NULL, NULL, gamma);
+    if (st) return st;
+    for (size_t p = 0; p < ((size_t)1 << (log_n - 1)); p++) {
+        const uint64_t u0 = a[2 * p], u1 = a[2 * p + 1], v0 = b[2 * p], v1 = b[2 * p + 1];
+        const uint64_t p0 = q50_mul(u0, v0, q), p1 = q50_mul(u1, v1, q);
+        const uint64_t cross = q50_mul(q50_add(u0, u1, q), q50_add(v0, v1, q), q);
+        out[2 * p] = q50_add(p0, q50_mul(gamma[p], p1, q), q);
+        out[2 * p + 1] = q50_sub(cross, q50_add(p0, p1, q), q);
+    }
+    return 0;
+}
+
+/* mul (modulus50.rs:406-421): forward both, slotwise, inverse */
+int or_quad_mul(uint64_t q, uint32_t two_adicity, uint64_t generator, uint32_t log_n, const uint64_t *a,
+                const uint64_t *b, uint64_t *out)
+{
+    const size_t n = (size_t)1 << (log_n <= 7 ? log_n : 0);
+    uint64_t ta[128], tb[128];
+    int st = or_quad_tables(q, two_adicity, generator, log_n, NULL, NULL, NULL, NULL);
+    if (st) return st;
+    memcpy(ta, a, n * 8);
+    memcpy(tb, b, n * 8);
+    or_quad_forward(q, two_adicity, generator, log_n, ta);
+    or_quad_forward(q, two_adicity, generator, log_n, tb);
+    or_quad_slotwise_mul(q, two_adicity, generator, log_n, ta, tb, out);
+    return or_quad_inverse(q, two_adicity, generator, log_n, out);
+}

@@ -106,6 +106,11 @@ def lib():
         "or_powmod64": (u64, [u64, u64, u64]),
         "or_commit_wide_schoolbook": (C.c_int, [u64, u32, sz, sz, sz, vp, C.c_int, vp, _u64p]),
         "or_commit_wide_check_at": (sz, [u64, u32, sz, sz, sz, vp, C.c_int, vp, vp, C.c_int, u64]),
+        "or_quad_tables": (C.c_int, [u64, u32, u64, u32, C.POINTER(u64), vp, vp, vp]),
+        "or_quad_forward": (C.c_int, [u64, u32, u64, u32, _u64p]),
+        "or_quad_inverse": (C.c_int, [u64, u32, u64, u32, _u64p]),
+        "or_quad_slotwise_mul": (C.c_int, [u64, u32, u64, u32, _u64p, _u64p, _u64p]),
+        "or_quad_mul": (C.c_int, [u64, u32, u64, u32, _u64p, _u64p, _u64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -510,3 +515,56 @@ def zerocheck_verify(rounds, num_vars, transcript: Transcript, expected_poly_eva
         return st, None
     st, _point, _fin = sumcheck_verify(rounds, num_vars, 2, 0, transcript, expected_poly_eval)
     return st, (np.asarray(r, dtype=np.uint64)[:num_vars] if st == 0 else None)
+
+
+# ---------------------------------------------------------------------------
+# The ~2^50 quadratic-slot ring (crates/lattice-ring/src/modulus50.rs:44-499)
+# ---------------------------------------------------------------------------
+
+Q50 = (1125899906839937, 7, 3)  # 2^50 - 2687, two-adicity 7, generator 3 (modulus50.rs:44-53)
+
+
+def quad_tables(q, two_adicity, generator, log_n):
+    """QuadSlotTables::new: (status, zeta, twiddles, inverse twiddles, leaf gammas)."""
+    n = 1 << min(max(log_n, 1), 7)
+    zeta = C.c_uint64(0)
+    tw = np.zeros(n, dtype=np.uint64)
+    twi = np.zeros(n, dtype=np.uint64)
+    gam = np.zeros(n, dtype=np.uint64)
+    st = lib().or_quad_tables(q, two_adicity, generator, log_n, C.byref(zeta), tw.ctypes.data, twi.ctypes.data,
+                              gam.ctypes.data)
+    return st, zeta.value, tw[: n // 2 - 1], twi[: n // 2 - 1], gam[: n // 2]
+
+
+def _quad_each(fn, q, two_adicity, generator, log_n, *arrays):
+    n = 1 << log_n
+    ins = [np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, n) for a in arrays]
+    out = np.zeros_like(ins[0])
+    for p in range(out.shape[0]):
+        if len(ins) == 1:
+            row = ins[0][p].copy()
+            st = fn(q, two_adicity, generator, log_n, row)
+            out[p] = row
+        else:
+            row = np.zeros(n, dtype=np.uint64)
+            st = fn(q, two_adicity, generator, log_n, ins[0][p].copy(), ins[1][p].copy(), row)
+            out[p] = row
+        if st:
+            return st, None
+    return 0, out.reshape(-1)
+
+
+def quad_forward(q, two_adicity, generator, log_n, a):
+    return _quad_each(lib().or_quad_forward, q, two_adicity, generator, log_n, a)
+
+
+def quad_inverse(q, two_adicity, generator, log_n, a):
+    return _quad_each(lib().or_quad_inverse, q, two_adicity, generator, log_n, a)
+
+
+def quad_slotwise_mul(q, two_adicity, generator, log_n, a, b):
+    return _quad_each(lib().or_quad_slotwise_mul, q, two_adicity, generator, log_n, a, b)
+
+
+def quad_mul(q, two_adicity, generator, log_n, a, b):
+    return _quad_each(lib().or_quad_mul, q, two_adicity, generator, log_n, a, b)

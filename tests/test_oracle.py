@@ -474,3 +474,123 @@ def test_zerocheck_refuses_nonzero_and_rejects_tampered_rounds():
     bad[0, 0] = 1
     st, _ = O.zerocheck_verify(bad, nv, O.Transcript(b"lzx-zerocheck-test"), None)
     assert st != 0
+
+
+# ---- the ~2^50 quadratic-slot ring (modulus50.rs:44-499) ---------------------
+
+def _is_prime_u64(n):
+    """Deterministic Miller-Rabin for n < 2^64 (bases 2..37)."""
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d, s = d // 2, s + 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def _neg_schoolbook(q, a, b):
+    n = len(a)
+    acc = [0] * n
+    for i in range(n):
+        for j in range(n):
+            k = i + j
+            if k < n:
+                acc[k] += int(a[i]) * int(b[j])
+            else:
+                acc[k - n] -= int(a[i]) * int(b[j])
+    return [c % q for c in acc]
+
+
+def _rev(x, bits):
+    return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def test_q50_modulus_kats():
+    # modulus50.rs:536-553 modulus_is_the_paper_prime
+    q, adic, g = O.Q50
+    assert q == 2**50 - 2687 and q % 256 == 129 and ((q - 1) >> 7) & 1 == 1 and _is_prime_u64(q)
+    st, zeta, _tw, _twi, _g = O.quad_tables(q, adic, g, 7)
+    assert st == 0 and zeta == pow(g, (q - 1) >> 7, q)
+    assert pow(zeta, 128, q) == 1 and pow(zeta, 64, q) == q - 1
+
+
+def test_quad_forward_is_reduction_mod_slot_quadratics():
+    # slot p of forward(a) is a(X) mod (X^2 - gamma_p), gamma_p = zeta^(2^(7-k) (2 rev_{k-1}(p) + 1))
+    # (modulus50.rs:200-208, 289-296): pins the level twiddles and the slot order
+    q, adic, g = O.Q50
+    rng = np.random.default_rng(50)
+    zeta = pow(g, (q - 1) >> 7, q)
+    for log_n in range(1, 8):
+        n = 1 << log_n
+        gam = [pow(zeta, ((2 * _rev(p, log_n - 1) + 1) << (7 - log_n)) % 128, q) for p in range(n // 2)]
+        st, _z, _tw, _twi, got_gam = O.quad_tables(q, adic, g, log_n)
+        assert st == 0 and [int(x) for x in got_gam] == gam
+        a = rng.integers(0, q, n, dtype=np.uint64)
+        st, f = O.quad_forward(q, adic, g, log_n, a)
+        assert st == 0
+        for p in range(n // 2):
+            c0 = sum(int(a[i]) * pow(gam[p], i // 2, q) for i in range(0, n, 2)) % q
+            c1 = sum(int(a[i]) * pow(gam[p], i // 2, q) for i in range(1, n, 2)) % q
+            assert (int(f[2 * p]), int(f[2 * p + 1])) == (c0, c1), (log_n, p)
+
+
+def test_quad_roundtrip_and_mul_match_schoolbook():
+    # modulus50.rs:637-661 roundtrip_all_log_n, mul_matches_schoolbook_all_log_n
+    q, adic, g = O.Q50
+    rng = np.random.default_rng(51)
+    for log_n in range(1, 8):
+        n = 1 << log_n
+        a = rng.integers(0, q, 2 * n, dtype=np.uint64)
+        b = rng.integers(0, q, 2 * n, dtype=np.uint64)
+        _, f = O.quad_forward(q, adic, g, log_n, a)
+        _, back = O.quad_inverse(q, adic, g, log_n, f)
+        assert back.tolist() == a.tolist()
+        st, m = O.quad_mul(q, adic, g, log_n, a, b)
+        assert st == 0
+        for p in range(2):
+            assert [int(x) for x in m[p * n:(p + 1) * n]] == _neg_schoolbook(q, a[p * n:(p + 1) * n],
+                                                                             b[p * n:(p + 1) * n])
+
+
+def test_quad_ring_axioms_and_slot_constants():
+    # modulus50.rs:663-738
+    q, adic, g = O.Q50
+    n = 128
+    x = np.zeros(n, dtype=np.uint64)
+    x[1] = 1
+    xn1 = np.zeros(n, dtype=np.uint64)
+    xn1[n - 1] = 1
+    _, prod = O.quad_mul(q, adic, g, 7, x, xn1)
+    assert int(prod[0]) == q - 1 and not prod[1:].any()
+    rng = np.random.default_rng(52)
+    a = rng.integers(0, q, n, dtype=np.uint64)
+    one = np.zeros(n, dtype=np.uint64)
+    one[0] = 1
+    assert O.quad_mul(q, adic, g, 7, a, one)[1].tolist() == a.tolist()
+    for log_n, half in ((7, 64), (6, 32)):
+        gam = [int(v) for v in O.quad_tables(q, adic, g, log_n)[4]]
+        assert len(gam) == half and len(set(gam)) == half
+        assert all(v != 0 and pow(v, half, q) == q - 1 for v in gam)
+
+
+def test_quad_errors():
+    # modulus50.rs:740-762 errors_rejected (status = Mod50Error: 40 LogNOutOfRange,
+    # 41 AdicityTooSmall, 42 GeneratorInvalid)
+    q, adic, g = O.Q50
+    assert O.quad_tables(q, adic, g, 0)[0] == 40
+    assert O.quad_tables(q, adic, g, 8)[0] == 40
+    assert O.quad_tables(q, 5, g, 3)[0] == 41
+    assert O.quad_tables(q, adic, 1, 3)[0] == 42  # generator 1: zeta = 1 is not primitive
