@@ -49,6 +49,15 @@ enum {
     LZX_MOD50_ADICITY_TOO_SMALL = 41,   /* Mod50Error::AdicityTooSmall   modulus50.rs:419 */
     LZX_MOD50_GENERATOR_INVALID = 42,   /* Mod50Error::GeneratorInvalid  modulus50.rs:420 */
     LZX_MOD50_LENGTH_MISMATCH = 43,     /* Mod50Error::LengthMismatch    modulus50.rs:421 */
+    LZX_PACKING_LENGTH_MISMATCH = 50,   /* PackingError::LengthMismatch  packing.rs:210 */
+    LZX_PACKING_LIMB_OUT_OF_RANGE = 51, /* PackingError::LimbOutOfRange  packing.rs:211 */
+    LZX_PACKING_MODULUS_TOO_LARGE = 52, /* PackingError::ModulusTooLarge packing.rs:212 */
+    LZX_PACKING_VALUE_OUT_OF_RANGE = 53, /* PackingError::ValueOutOfRange packing.rs:213 */
+    LZX_PACKING_NOT_COPRIME = 54,       /* PackingError::NotCoprime      packing.rs:214 */
+    LZX_DECOMP_DIGIT_COUNT_MISMATCH = 55, /* DecompositionError::DigitCountMismatch decomposition.rs:110 */
+    LZX_DECOMP_DIGIT_OUT_OF_RANGE = 56, /* DecompositionError::DigitOutOfRange decomposition.rs:111 */
+    LZX_DECOMP_VALUE_OUT_OF_RANGE = 57, /* DecompositionError::ValueOutOfRange decomposition.rs:112 */
+    LZX_DECOMP_VALUE_TOO_LARGE = 58,    /* DecompositionError::ValueTooLarge   decomposition.rs:113 */
     LZX_TRANSCRIPT_REJECTION_BUDGET = 20, /* TranscriptError::RejectionBudgetExceeded transcript.rs:16 */
     LZX_TRANSCRIPT_MESSAGE_TOO_LARGE = 21, /* TranscriptError::MessageTooLarge transcript.rs:18 */
     LZX_INVALID_ARGUMENT = 30,          /* null handle/pointer or malformed shape */
@@ -316,6 +325,51 @@ lzx_status lzx_eq_table(const uint64_t *point_host, size_t num_vars, uint64_t *o
 lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev,
                                      size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
                                      const uint32_t *term_ids, uint64_t *sum_out, void *stream);
+
+/* ---- witness packing feeding the commitment ----------------------------------
+ * Device buffers throughout; ring elements are u32 [elems][n] residues. */
+
+/* pack_field_elements (packing.rs:30-52): each canonical Goldilocks value
+ * becomes three 22-bit limbs (low first) in consecutive coefficients, each
+ * reduced mod q; out_dev holds lzx_pack_field_elements_len(count, n) =
+ * max(1, ceil(3 count / n)) elements, zero-padded. */
+size_t lzx_pack_field_elements_len(size_t count, uint32_t n);
+lzx_status lzx_pack_field_elements(const uint64_t *values_dev, size_t count, uint32_t q, uint32_t n,
+                                   uint32_t *out_dev, void *stream);
+/* unpack_field_elements (packing.rs:57-91): out_dev receives floor(elems n / 3)
+ * values; LZX_PACKING_LIMB_OUT_OF_RANGE if any balanced coefficient is
+ * outside (-2^22, 2^22).  Synchronizes the stream. */
+lzx_status lzx_unpack_field_elements(const uint32_t *packed_dev, size_t elems, uint32_t q, uint32_t n,
+                                     uint64_t *out_dev, void *stream);
+/* crt_pack (packing.rs:95-155): groups of k values share coefficient 0 of one
+ * element (max(1, ceil(count / k)) elements); moduli on the host.
+ * Synchronizes the stream. */
+lzx_status lzx_crt_pack(const uint64_t *values_dev, size_t count, const uint64_t *moduli, size_t k, uint32_t q,
+                        uint32_t n, uint32_t *out_dev, void *stream);
+/* crt_unpack (packing.rs:158-185). */
+lzx_status lzx_crt_unpack(const uint32_t *packed_dev, size_t elems, uint32_t n, const uint64_t *moduli, size_t k,
+                          size_t expected, uint64_t *out_dev, void *stream);
+/* pack_bits / unpack_bits (sparse.rs:30-71): 31 bits per coefficient, bit b of
+ * coefficient g is bits[31 g + b]; bits are bytes (0 / non-zero).
+ * lzx_pack_bits_len = max(1, ceil(nbits / (31 n))) elements. */
+size_t lzx_pack_bits_len(size_t nbits, uint32_t n);
+lzx_status lzx_pack_bits(const uint8_t *bits_dev, size_t nbits, uint32_t n, uint32_t *out_dev, void *stream);
+lzx_status lzx_unpack_bits(const uint32_t *elems_dev, size_t elems, uint32_t n, uint8_t *bits_dev, size_t nbits,
+                           void *stream);
+/* commit_onehot_columns (sparse.rs:118-144): column c is the indicator of bit
+ * positions[c] among lens[c], bit-packed and padded to m elements; s_dev
+ * (u32 [m][n]) receives the sum of every column mod q and t_dev (u32 [k][n])
+ * the commitment of s. */
+lzx_status lzx_ajtai_commit_onehot(const lzx_ajtai_key *key, const uint64_t *positions, const uint64_t *lens,
+                                   size_t ncols, uint32_t *s_dev, uint32_t *t_dev, void *stream);
+/* GadgetDecomposition::decompose / recompose (decomposition.rs:47-93):
+ * balanced base-b digits (a digit r > b/2 becomes r - b), [count][num_digits],
+ * digit_bytes 1 (int8, b <= 255: feeds lzx_ajtai_commit_digits_batch) or 8
+ * (int64).  Synchronize the stream. */
+lzx_status lzx_gadget_decompose(const uint64_t *x_dev, size_t count, uint64_t base, uint32_t num_digits,
+                                void *digits_dev, uint32_t digit_bytes, void *stream);
+lzx_status lzx_gadget_recompose(const void *digits_dev, uint32_t digit_bytes, size_t count, uint64_t base,
+                                uint32_t num_digits, uint64_t *x_dev, void *stream);
 
 /* ---- the ~2^50 quadratic-slot ring (modulus50.rs:44-499) -------------------
  * QuadSlotTables / RingConfig50 over a prime q < 2^50 of two-adicity >= 7

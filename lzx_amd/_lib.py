@@ -54,6 +54,18 @@ PROTOTYPES = {
     "lzx_ajtai_commit_digits_batch": (st_t, [vp, vp, sz, sz, vp, i32, vp]),
     "lzx_ajtai_commit_host": (st_t, [vp, vp, sz, vp]),
     "lzx_ajtai_verify_opening_host": (st_t, [vp, vp, vp, sz]),
+    # Witness packing
+    "lzx_pack_field_elements_len": (sz, [sz, u32]),
+    "lzx_pack_field_elements": (st_t, [vp, sz, u32, u32, vp, vp]),
+    "lzx_unpack_field_elements": (st_t, [vp, sz, u32, u32, vp, vp]),
+    "lzx_crt_pack": (st_t, [vp, sz, vp, sz, u32, u32, vp, vp]),
+    "lzx_crt_unpack": (st_t, [vp, sz, u32, vp, sz, sz, vp, vp]),
+    "lzx_pack_bits_len": (sz, [sz, u32]),
+    "lzx_pack_bits": (st_t, [vp, sz, u32, vp, vp]),
+    "lzx_unpack_bits": (st_t, [vp, sz, u32, vp, sz, vp]),
+    "lzx_ajtai_commit_onehot": (st_t, [vp, vp, vp, sz, vp, vp, vp]),
+    "lzx_gadget_decompose": (st_t, [vp, sz, u64, u32, vp, u32, vp]),
+    "lzx_gadget_recompose": (st_t, [vp, u32, sz, u64, u32, vp, vp]),
     # Uneven commitment batch
     "lzx_wide_key_create": (st_t, [u64, u64, u32, sz, sz, vp, vp, C.POINTER(vp)]),
     "lzx_wide_key_destroy": (None, [vp]),

@@ -114,6 +114,15 @@ extern "C" const char *lzx_status_string(lzx_status s)
     case LZX_MOD50_ADICITY_TOO_SMALL: return "Mod50Error::AdicityTooSmall";
     case LZX_MOD50_GENERATOR_INVALID: return "Mod50Error::GeneratorInvalid";
     case LZX_MOD50_LENGTH_MISMATCH: return "Mod50Error::LengthMismatch";
+    case LZX_PACKING_LENGTH_MISMATCH: return "PackingError::LengthMismatch";
+    case LZX_PACKING_LIMB_OUT_OF_RANGE: return "PackingError::LimbOutOfRange";
+    case LZX_PACKING_MODULUS_TOO_LARGE: return "PackingError::ModulusTooLarge";
+    case LZX_PACKING_VALUE_OUT_OF_RANGE: return "PackingError::ValueOutOfRange";
+    case LZX_PACKING_NOT_COPRIME: return "PackingError::NotCoprime";
+    case LZX_DECOMP_DIGIT_COUNT_MISMATCH: return "DecompositionError::DigitCountMismatch";
+    case LZX_DECOMP_DIGIT_OUT_OF_RANGE: return "DecompositionError::DigitOutOfRange";
+    case LZX_DECOMP_VALUE_OUT_OF_RANGE: return "DecompositionError::ValueOutOfRange";
+    case LZX_DECOMP_VALUE_TOO_LARGE: return "DecompositionError::ValueTooLarge";
     case LZX_TRANSCRIPT_REJECTION_BUDGET: return "TranscriptError::RejectionBudgetExceeded";
     case LZX_TRANSCRIPT_MESSAGE_TOO_LARGE: return "TranscriptError::MessageTooLarge";
     case LZX_INVALID_ARGUMENT: return "InvalidArgument";

@@ -1249,3 +1249,153 @@ int or_quad_mul(uint64_t q, uint32_t two_adicity, uint64_t generator, uint32_t l
     or_quad_slotwise_mul(q, two_adicity, generator, log_n, ta, tb, out);
     return or_quad_inverse(q, two_adicity, generator, log_n, out);
 }
+
+/* ---- witness packing (packing.rs:30-200, sparse.rs:30-54/118-144,
+ * decomposition.rs:47-100) -------------------------------------------------
+ * Status codes: PackingError 50 LengthMismatch, 51 LimbOutOfRange, 52
+ * ModulusTooLarge, 53 ValueOutOfRange, 54 NotCoprime; DecompositionError 55
+ * DigitCountMismatch, 56 DigitOutOfRange, 57 ValueOutOfRange, 58 ValueTooLarge. */
+
+/* pack_field_elements (packing.rs:30-52): three 22-bit limbs per value, from_signed per element */
+size_t or_pack_field_elements(uint32_t q, uint32_t n, const uint64_t *values, size_t count, uint32_t *out)
+{
+    const size_t limbs = 3 * count, elems = limbs ? (limbs + n - 1) / n : 1;
+    for (size_t g = 0; g < elems * n; g++) {
+        uint64_t l = 0;
+        if (g < limbs) {
+            const uint64_t raw = values[g / 3];
+            l = g % 3 == 0 ? raw & 0x3FFFFF : g % 3 == 1 ? (raw >> 22) & 0x3FFFFF : raw >> 44;
+        }
+        out[g] = (uint32_t)(l % q);
+    }
+    return elems;
+}
+
+/* unpack_field_elements (packing.rs:57-91); returns the status, *got = values written */
+int or_unpack_field_elements(uint32_t q, uint32_t n, const uint32_t *packed, size_t elems, uint64_t *out,
+                             size_t *got)
+{
+    const int64_t half = q / 2, bound = (int64_t)1 << 22;
+    const size_t total = elems * n;
+    *got = 0;
+    for (size_t g = 0; g < total; g++) {
+        const int64_t c = packed[g], b = c <= half ? c : c - (int64_t)q;
+        if (b <= -bound || b >= bound) return 51;
+    }
+    for (size_t i = 0; 3 * i + 3 <= total; i++) {
+        int64_t b[3];
+        for (int j = 0; j < 3; j++) {
+            const int64_t c = packed[3 * i + j];
+            b[j] = c <= half ? c : c - (int64_t)q;
+        }
+        const uint64_t raw = ((uint64_t)b[0] & 0x3FFFFF) | (((uint64_t)b[1] & 0x3FFFFF) << 22) | ((uint64_t)b[2] << 44);
+        out[i] = raw >= GL_P ? raw - GL_P : raw; /* Goldilocks::from_u64 */
+        *got = i + 1;
+    }
+    return 0;
+}
+
+/* pack_bits (sparse.rs:30-54): 31 bits per coefficient */
+size_t or_pack_bits(uint32_t n, const uint8_t *bits, size_t nbits, uint32_t *out)
+{
+    size_t elems = (nbits + 31 * (size_t)n - 1) / (31 * (size_t)n);
+    const size_t real = elems;
+    if (elems == 0) elems = 1;
+    for (size_t g = 0; g < elems * n; g++) {
+        uint32_t v = 0;
+        for (size_t b = 0; b < 31; b++) {
+            const size_t idx = g * 31 + b;
+            if (real && idx < nbits && bits[idx]) v |= 1u << b;
+        }
+        out[g] = v;
+    }
+    return elems;
+}
+
+/* unpack_bits (sparse.rs:57-71) */
+void or_unpack_bits(uint32_t n, const uint32_t *elems, size_t nelems, uint8_t *bits, size_t nbits)
+{
+    for (size_t idx = 0; idx < nbits; idx++) {
+        const size_t e = idx / (31 * (size_t)n), within = idx % (31 * (size_t)n), c = within / 31, b = within % 31;
+        bits[idx] = e < nelems ? (elems[e * n + c] >> b) & 1 : 0;
+    }
+}
+
+/* GadgetDecomposition::decompose (decomposition.rs:47-70) */
+int or_gadget_decompose(uint64_t base, size_t num_digits, uint64_t x, int64_t *digits)
+{
+    u128 rem = x;
+    const int64_t half = (int64_t)(base / 2);
+    for (size_t i = 0; i < num_digits; i++) {
+        const int64_t digit = (int64_t)(rem % base);
+        const int64_t d = digit > half ? digit - (int64_t)base : digit;
+        rem = (u128)(((__int128)rem - d) / (__int128)base);
+        digits[i] = d;
+    }
+    return rem != 0 ? 58 : 0;
+}
+
+/* GadgetDecomposition::recompose (decomposition.rs:73-93) */
+int or_gadget_recompose(uint64_t base, size_t num_digits, const int64_t *digits, size_t ndigits, uint64_t *x)
+{
+    if (ndigits != num_digits) return 55;
+    __int128 acc = 0, power = 1;
+    const int64_t half = (int64_t)(base / 2);
+    for (size_t i = 0; i < ndigits; i++) {
+        if (digits[i] < -half || digits[i] > half) return 56;
+        acc += (__int128)digits[i] * power;
+        power *= (__int128)base;
+    }
+    if (acc < 0 || acc > (__int128)UINT64_MAX) return 57;
+    *x = (uint64_t)acc;
+    return 0;
+}
+
+/* crt_pack (packing.rs:95-155): one combined residue in coefficient 0 per group */
+int or_crt_pack(uint32_t q, uint32_t n, const uint64_t *values, size_t count, const uint64_t *moduli, size_t k,
+                uint32_t *out, size_t *elems_out)
+{
+    if (k == 0) return 50;
+    uint64_t product = 1;
+    for (size_t i = 0; i < k; i++) {
+        const uint64_t m = moduli[i];
+        if (m == 0 || m >= q) return 52;
+        if ((u128)product * m >= q) return 52;
+        product *= m;
+    }
+    const size_t elems = count ? (count + k - 1) / k : 1;
+    for (size_t e = 0; e < elems; e++) {
+        uint64_t combined = 0, prod = 1;
+        for (size_t slot = 0; slot < (k < n ? k : n); slot++) {
+            const size_t idx = e * k + slot;
+            if (idx >= count) break;
+            const uint64_t m = moduli[slot], v = values[idx];
+            if (v >= m) return 53;
+            /* inverse of prod mod m by the extended Euclid algorithm */
+            int64_t r0 = (int64_t)(prod % m), r1 = (int64_t)m, s0 = 1, s1 = 0;
+            while (r1) {
+                const int64_t qq = r0 / r1, t = r0 - qq * r1, ts = s0 - qq * s1;
+                r0 = r1, r1 = t, s0 = s1, s1 = ts;
+            }
+            if (m <= 1 || r0 != 1) return 54;
+            const uint64_t inv = (uint64_t)(((s0 % (int64_t)m) + (int64_t)m) % (int64_t)m);
+            const uint64_t diff = (v + m - combined % m) % m, t = (uint64_t)((u128)diff * inv % m);
+            combined += prod * t;
+            prod *= m;
+        }
+        for (uint32_t c = 0; c < n; c++) out[e * n + c] = 0;
+        out[e * n] = (uint32_t)(combined % q);
+    }
+    *elems_out = elems;
+    return 0;
+}
+
+/* crt_unpack (packing.rs:158-185) */
+int or_crt_unpack(uint32_t n, const uint32_t *packed, size_t elems, const uint64_t *moduli, size_t k, size_t expected,
+                  uint64_t *out)
+{
+    size_t got = 0;
+    for (size_t e = 0; e < elems && got < expected; e++)
+        for (size_t i = 0; i < k && got < expected; i++) out[got++] = packed[e * n] % moduli[i];
+    return got != expected ? 50 : 0;
+}

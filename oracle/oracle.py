@@ -111,6 +111,14 @@ def lib():
         "or_quad_inverse": (C.c_int, [u64, u32, u64, u32, _u64p]),
         "or_quad_slotwise_mul": (C.c_int, [u64, u32, u64, u32, _u64p, _u64p, _u64p]),
         "or_quad_mul": (C.c_int, [u64, u32, u64, u32, _u64p, _u64p, _u64p]),
+        "or_pack_field_elements": (sz, [u32, u32, _u64p, sz, _u32p]),
+        "or_unpack_field_elements": (C.c_int, [u32, u32, _u32p, sz, _u64p, C.POINTER(sz)]),
+        "or_pack_bits": (sz, [u32, C.c_void_p, sz, _u32p]),
+        "or_unpack_bits": (None, [u32, _u32p, sz, C.c_void_p, sz]),
+        "or_gadget_decompose": (C.c_int, [u64, sz, u64, C.c_void_p]),
+        "or_gadget_recompose": (C.c_int, [u64, sz, C.c_void_p, sz, C.POINTER(u64)]),
+        "or_crt_pack": (C.c_int, [u32, u32, _u64p, sz, _u64p, sz, _u32p, C.POINTER(sz)]),
+        "or_crt_unpack": (C.c_int, [u32, _u32p, sz, _u64p, sz, sz, _u64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -568,3 +576,70 @@ def quad_slotwise_mul(q, two_adicity, generator, log_n, a, b):
 
 def quad_mul(q, two_adicity, generator, log_n, a, b):
     return _quad_each(lib().or_quad_mul, q, two_adicity, generator, log_n, a, b)
+
+
+# ---------------------------------------------------------------------------
+# Witness packing (packing.rs:30-200, sparse.rs:30-71, decomposition.rs:47-93)
+# ---------------------------------------------------------------------------
+
+def pack_field_elements(q, n, values):
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    out = np.zeros(max(1, -(-3 * v.size // n)) * n, dtype=np.uint32)
+    e = lib().or_pack_field_elements(q, n, v if v.size else np.zeros(1, np.uint64), v.size, out)
+    return out[: e * n].reshape(e, n)
+
+
+def unpack_field_elements(q, n, packed):
+    pk = np.ascontiguousarray(packed, dtype=np.uint32).reshape(-1)
+    out = np.zeros(max(1, pk.size // 3), dtype=np.uint64)
+    got = C.c_size_t(0)
+    st = lib().or_unpack_field_elements(q, n, pk, pk.size // n, out, C.byref(got))
+    return st, out[: got.value]
+
+
+def pack_bits(n, bits):
+    b = np.ascontiguousarray(bits, dtype=np.uint8)
+    e = max(1, -(-b.size // (31 * n)))
+    out = np.zeros(e * n, dtype=np.uint32)
+    b1 = b if b.size else np.zeros(1, np.uint8)
+    lib().or_pack_bits(n, b1.ctypes.data, b.size, out)
+    return out.reshape(e, n)
+
+
+def unpack_bits(n, elems, nbits):
+    el = np.ascontiguousarray(elems, dtype=np.uint32).reshape(-1)
+    out = np.zeros(max(1, nbits), dtype=np.uint8)
+    lib().or_unpack_bits(n, el if el.size else np.zeros(1, np.uint32), el.size // n, out.ctypes.data, nbits)
+    return out[:nbits]
+
+
+def gadget_decompose(base, num_digits, x):
+    d = np.zeros(max(1, num_digits), dtype=np.int64)
+    st = lib().or_gadget_decompose(base, num_digits, x, d.ctypes.data)
+    return st, d[:num_digits]
+
+
+def gadget_recompose(base, num_digits, digits):
+    d = np.ascontiguousarray(digits, dtype=np.int64)
+    x = C.c_uint64(0)
+    st = lib().or_gadget_recompose(base, num_digits, d.ctypes.data if d.size else None, d.size, C.byref(x))
+    return st, x.value
+
+
+def crt_pack(q, n, values, moduli):
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    m = np.ascontiguousarray(moduli, dtype=np.uint64)
+    e_cap = max(1, -(-v.size // max(1, m.size)))
+    out = np.zeros(e_cap * n, dtype=np.uint32)
+    got = C.c_size_t(0)
+    st = lib().or_crt_pack(q, n, v if v.size else np.zeros(1, np.uint64), v.size,
+                           m if m.size else np.zeros(1, np.uint64), m.size, out, C.byref(got))
+    return st, out[: got.value * n].reshape(got.value, n)
+
+
+def crt_unpack(n, packed, moduli, expected):
+    pk = np.ascontiguousarray(packed, dtype=np.uint32).reshape(-1)
+    m = np.ascontiguousarray(moduli, dtype=np.uint64)
+    out = np.zeros(max(1, expected), dtype=np.uint64)
+    st = lib().or_crt_unpack(n, pk if pk.size else np.zeros(1, np.uint32), pk.size // n, m, m.size, expected, out)
+    return st, out[:expected]

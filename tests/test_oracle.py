@@ -594,3 +594,82 @@ def test_quad_errors():
     assert O.quad_tables(q, adic, g, 8)[0] == 40
     assert O.quad_tables(q, 5, g, 3)[0] == 41
     assert O.quad_tables(q, adic, 1, 3)[0] == 42  # generator 1: zeta = 1 is not primitive
+
+
+# ---- witness packing (packing.rs, sparse.rs, decomposition.rs) ----------------
+
+Q32 = 3221225473
+
+
+def test_split_pack_reference_cases():
+    # packing.rs:227-270: 40 values at n = 16 -> 8 elements; edge values survive; an oversized limb is rejected
+    vals = [(i * 0x9E3779B97F4A7C15 + i) % 2**64 for i in range(40)]
+    vals = [v - O.GL_P if v >= O.GL_P else v for v in vals]  # Goldilocks::from_u64
+    packed = O.pack_field_elements(Q32, 16, vals)
+    assert packed.shape == (8, 16)
+    st, back = O.unpack_field_elements(Q32, 16, packed)
+    assert st == 0 and back[:40].tolist() == vals
+    edge = [0, 1, O.GL_P - 1, O.GL_P // 2, 0x7FFFFFFF00000000]
+    st, back = O.unpack_field_elements(Q32, 8, O.pack_field_elements(Q32, 8, edge))
+    assert st == 0 and back[:5].tolist() == edge
+    bad = np.zeros(8, dtype=np.uint32)
+    bad[0] = 1 << 22
+    assert O.unpack_field_elements(Q32, 8, bad)[0] == 51
+    bad[0] = Q32 - (1 << 22)  # balanced -2^22 is out of range too
+    assert O.unpack_field_elements(Q32, 8, bad)[0] == 51
+    # the limb layout itself: raw = l0 + 2^22 l1 + 2^44 l2
+    rng = np.random.default_rng(60)
+    v = rng.integers(0, O.GL_P, 30, dtype=np.uint64)
+    pk = O.pack_field_elements(Q32, 64, v).reshape(-1)
+    for i in range(30):
+        assert int(pk[3 * i]) + (int(pk[3 * i + 1]) << 22) + (int(pk[3 * i + 2]) << 44) == int(v[i])
+
+
+def test_bit_packing_closed_form():
+    # sparse.rs:30-71: coefficient g holds bits 31 g .. 31 g + 30, little-endian
+    rng = np.random.default_rng(61)
+    for n, nbits in ((8, 0), (8, 1), (16, 31 * 16), (16, 31 * 16 + 5), (64, 10000)):
+        bits = rng.integers(0, 2, nbits, dtype=np.uint8)
+        pk = O.pack_bits(n, bits)
+        assert pk.shape[0] == max(1, -(-nbits // (31 * n)))
+        flat = pk.reshape(-1)
+        for g in range(flat.size):
+            want = sum(int(bits[31 * g + b]) << b for b in range(31) if 31 * g + b < nbits)
+            assert int(flat[g]) == want
+        assert O.unpack_bits(n, pk, nbits).tolist() == bits.tolist()
+
+
+def test_crt_pack_reference_cases():
+    # packing.rs:272-292
+    st, pk = O.crt_pack(Q32, 16, [3, 5, 9, 15, 0, 6, 10, 2, 1, 1], [7, 11, 13, 17])
+    assert st == 0
+    for e, grp in enumerate(([3, 5, 9, 15], [0, 6, 10, 2], [1, 1])):
+        for v, m in zip(grp, (7, 11, 13, 17)):
+            assert int(pk[e, 0]) % m == v
+    st, back = O.crt_unpack(16, pk, [7, 11, 13, 17], 10)
+    assert st == 0 and back.tolist() == [3, 5, 9, 15, 0, 6, 10, 2, 1, 1]
+    assert O.crt_pack(Q32, 16, [7, 0], [7, 11])[0] == 53
+    assert O.crt_pack(Q32, 16, [1, 1, 1, 1], [1000000, 1000003, 1000019, 999999])[0] != 0
+
+
+def test_gadget_decomposition_reference_cases():
+    # decomposition.rs:120-197
+    for x in (0, 1, 127, 128, 129, 255, 256, 65535, 2**32 - 1, 2**64 - 2, 2**64 - 1):
+        st, d = O.gadget_decompose(256, 9, x)  # power_of_two(64, 8): 9 digits
+        assert st == 0 and sum(int(v) * 256**i for i, v in enumerate(d)) == x
+        assert all(-127 <= v <= 128 for v in d)
+        assert O.gadget_recompose(256, 9, d) == (0, x)
+    for x in range(0, 100000, 977):  # power_of_two(64, 4): base 16, 17 digits in [-8, 8]
+        st, d = O.gadget_decompose(16, 17, x)
+        assert st == 0 and all(-8 <= v <= 8 for v in d)
+    digits = np.zeros(9, dtype=np.int64)
+    digits[0] = 200
+    assert O.gadget_recompose(256, 9, digits)[0] == 56
+    assert O.gadget_recompose(256, 9, np.zeros(7, dtype=np.int64))[0] == 55
+    assert O.gadget_decompose(256, 2, 70000)[0] == 58  # two base-256 digits cannot hold 70000
+    # a non-power-of-two base: the identity sum d_i b^i = x and the balanced range
+    rng = np.random.default_rng(62)
+    for x in rng.integers(0, 2**63, 50, dtype=np.uint64):
+        st, d = O.gadget_decompose(10, 21, int(x))
+        assert st == 0 and sum(int(v) * 10**i for i, v in enumerate(d)) == int(x)
+        assert all(-4 <= v <= 5 for v in d)
