@@ -45,6 +45,9 @@ enum {
     LZX_SUMCHECK_CLAIM_MISMATCH = 13,   /* SumcheckError::ClaimMismatch    sumcheck.rs:45 */
     LZX_SUMCHECK_VIRTUAL_POLY = 14,     /* SumcheckError::VirtualPoly      sumcheck.rs:30 */
     LZX_ZEROCHECK_FINAL_CLAIM_NONZERO = 15, /* ZerocheckError::FinalClaimNonZero zerocheck.rs:20 */
+    LZX_BATCH_VARIABLE_COUNT_MISMATCH = 16, /* BatchError::VariableCountMismatch batch.rs:19 */
+    LZX_BATCH_EMPTY = 17,               /* BatchError::EmptyBatch          batch.rs:20 */
+    LZX_BATCH_CHALLENGE_COUNT = 18,     /* BatchError::ChallengeCount      batch.rs:21 */
     LZX_MOD50_LOG_N_OUT_OF_RANGE = 40,  /* Mod50Error::LogNOutOfRange    modulus50.rs:418 */
     LZX_MOD50_ADICITY_TOO_SMALL = 41,   /* Mod50Error::AdicityTooSmall   modulus50.rs:419 */
     LZX_MOD50_GENERATOR_INVALID = 42,   /* Mod50Error::GeneratorInvalid  modulus50.rs:420 */
@@ -318,6 +321,60 @@ lzx_status lzx_zerocheck_prove(size_t num_vars, const uint64_t *poly_dev, lzx_tr
 lzx_status lzx_zerocheck_verify(const uint64_t *rounds, size_t num_rounds, size_t round_len, size_t num_vars,
                                 lzx_transcript *transcript, int32_t has_expected, uint64_t expected_poly_eval,
                                 uint64_t *point_out);
+
+/* One claim of a batch (batch.rs:24-28 BatchClaim): a virtual polynomial in
+ * the lzx_sumcheck_prove form (device tables of 2^num_vars canonical u64,
+ * terms in CSR form) and its asserted sum. */
+typedef struct {
+    size_t num_vars;
+    size_t nfactors;
+    const uint64_t *const *factors_dev;
+    size_t nterms;
+    const uint64_t *coeffs;
+    const uint32_t *term_off;
+    const uint32_t *term_ids;
+    uint64_t claimed_sum;
+} lzx_batch_claim;
+
+/* batch::prove_batch (batch.rs:33-80): draws rho_i = challenge_fields("batch-rho", k),
+ * proves sum_i rho_i P_i with claim sum_i rho_i c_i on the device prover.
+ * All claims share num_vars (else LZX_BATCH_VARIABLE_COUNT_MISMATCH).
+ * Outputs: rhos_out[k], rounds_out[num_vars][degree+1], challenges_out[num_vars],
+ * factor_claims_out[sum_i nfactors_i] in global factor order (claim 0's first).
+ * Factor tables passed more than once (the same device pointer) are folded
+ * once. At most 8 distinct tables and 16 terms in all (LZX_UNSUPPORTED). */
+lzx_status lzx_sumcheck_prove_batch(const lzx_batch_claim *claims, size_t nclaims, lzx_transcript *transcript,
+                                    uint64_t *rhos_out, uint64_t *rounds_out, uint64_t *challenges_out,
+                                    uint64_t *final_claim, uint64_t *factor_claims_out, size_t *degree_out,
+                                    void *stream);
+
+/* batch::prove_batch_star (batch.rs:161-225): claims of different num_vars;
+ * a claim over r < N = max num_vars variables is lifted by ignoring its last
+ * N - r variables (f'[j 2^(N-r) + s] = f[j]; its claim scales by 2^(N-r)),
+ * rho_i = challenge_fields("batch-star-rho", k). N rounds; *num_vars_out = N.
+ * The per-claim point is challenges_out[0 .. num_vars_i). Padded tables are
+ * built in device scratch and count towards the 8 distinct tables. */
+lzx_status lzx_sumcheck_prove_batch_star(const lzx_batch_claim *claims, size_t nclaims, lzx_transcript *transcript,
+                                         uint64_t *rhos_out, uint64_t *rounds_out, uint64_t *challenges_out,
+                                         uint64_t *final_claim, uint64_t *factor_claims_out, size_t *num_vars_out,
+                                         size_t *degree_out, void *stream);
+
+/* batch::verify_batch (batch.rs:300-336): rho_i from "batch-rho", the rounds
+ * verified against sum_i rho_i c_i over num_vars variables. */
+lzx_status lzx_sumcheck_verify_batch(const uint64_t *rounds, size_t num_rounds, size_t round_len, size_t num_vars,
+                                     size_t max_degree, const uint64_t *claimed_sums, size_t nclaims,
+                                     lzx_transcript *transcript, int32_t has_expected, uint64_t expected_final,
+                                     uint64_t *point_out);
+
+/* batch::verify_batch_star (batch.rs:236-262) when rhos is null, else
+ * verify_batch_star_with_rhos (batch.rs:268-296) with the caller's nrhos rhos:
+ * the rounds are verified against sum_i rho_i 2^(N - num_vars_i) c_i over
+ * N = max num_vars variables. */
+lzx_status lzx_sumcheck_verify_batch_star(const uint64_t *rounds, size_t num_rounds, size_t round_len,
+                                          const size_t *num_vars, size_t n_num_vars, size_t max_degree,
+                                          const uint64_t *claimed_sums, size_t nclaims, const uint64_t *rhos,
+                                          size_t nrhos, lzx_transcript *transcript, int32_t has_expected,
+                                          uint64_t expected_final, uint64_t *point_out);
 
 /* Multilinear helpers on device tables (field_simd.rs:391 eq_table,
  * virtual_poly.rs:107 sum_over_hypercube). */

@@ -95,6 +95,11 @@ PROTOTYPES = {
                                      C.POINTER(u64), vp, C.POINTER(sz), vp]),
     "lzx_zerocheck_prove": (st_t, [sz, vp, vp, vp, vp, vp, C.POINTER(u64), vp, vp]),
     "lzx_zerocheck_verify": (st_t, [vp, sz, sz, sz, vp, i32, u64, vp]),
+    "lzx_sumcheck_prove_batch": (st_t, [vp, sz, vp, vp, vp, vp, C.POINTER(u64), vp, C.POINTER(sz), vp]),
+    "lzx_sumcheck_prove_batch_star": (st_t, [vp, sz, vp, vp, vp, vp, C.POINTER(u64), vp, C.POINTER(sz),
+                                             C.POINTER(sz), vp]),
+    "lzx_sumcheck_verify_batch": (st_t, [vp, sz, sz, sz, sz, vp, sz, vp, i32, u64, vp]),
+    "lzx_sumcheck_verify_batch_star": (st_t, [vp, sz, sz, vp, sz, sz, vp, sz, vp, sz, vp, i32, u64, vp]),
     "lzx_vp_sum_over_hypercube": (st_t, [sz, sz, C.POINTER(vp), sz, vp, vp, vp, C.POINTER(u64), vp]),
 }
 

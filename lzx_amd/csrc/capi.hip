@@ -110,6 +110,9 @@ extern "C" const char *lzx_status_string(lzx_status s)
     case LZX_SUMCHECK_CLAIM_MISMATCH: return "SumcheckError::ClaimMismatch";
     case LZX_SUMCHECK_VIRTUAL_POLY: return "SumcheckError::VirtualPoly";
     case LZX_ZEROCHECK_FINAL_CLAIM_NONZERO: return "ZerocheckError::FinalClaimNonZero";
+    case LZX_BATCH_VARIABLE_COUNT_MISMATCH: return "BatchError::VariableCountMismatch";
+    case LZX_BATCH_EMPTY: return "BatchError::EmptyBatch";
+    case LZX_BATCH_CHALLENGE_COUNT: return "BatchError::ChallengeCount";
     case LZX_MOD50_LOG_N_OUT_OF_RANGE: return "Mod50Error::LogNOutOfRange";
     case LZX_MOD50_ADICITY_TOO_SMALL: return "Mod50Error::AdicityTooSmall";
     case LZX_MOD50_GENERATOR_INVALID: return "Mod50Error::GeneratorInvalid";

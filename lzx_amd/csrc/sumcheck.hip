@@ -1303,3 +1303,196 @@ extern "C" lzx_status lzx_zerocheck_verify(const uint64_t *rounds, size_t num_ro
     if (num_vars) memcpy(point_out, r.data(), num_vars * sizeof(uint64_t));
     return LZX_OK;
 }
+
+// ---- claim batching (batch.rs:33-296) -----------------------------------------
+
+namespace lzx {
+
+// Lift a table over r variables to r + k by ignoring its last k variables
+// (the low index bits): out[i] = f[i >> k] (batch.rs:130-148).
+__global__ void pad_table_kernel(const uint64_t *__restrict__ f, uint32_t k, uint64_t total, uint64_t *__restrict__ out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = f[i >> k];
+}
+
+// 2^k in the field (batch.rs:150-157 doubles k times; square-and-multiply gives the same element)
+static uint64_t gl_pow2(uint64_t k)
+{
+    uint64_t acc = 1, b = 2;
+    while (k) {
+        if (k & 1) acc = gl_mul(acc, b);
+        b = gl_mul(b, b);
+        k >>= 1;
+    }
+    return acc;
+}
+
+// BatchError::ChallengeCount stands for every transcript failure while drawing rho
+static lzx_status draw_rhos(lzx_transcript *tr, const char *label, size_t k, uint64_t *rhos)
+{
+    const lzx_status st = lzx_transcript_challenge_fields(tr, (const uint8_t *)label, strlen(label), k, rhos);
+    return st == LZX_TRANSCRIPT_REJECTION_BUDGET || st == LZX_TRANSCRIPT_MESSAGE_TOO_LARGE ? LZX_BATCH_CHALLENGE_COUNT
+                                                                                           : st;
+}
+
+// prove_batch (star = false) and prove_batch_star (star = true) share this: rho_i
+// from the transcript, the combined virtual polynomial over the union of the
+// claims' factors (rho_i c_t per term, a table passed twice folded once, lifted
+// tables built on the device), one device prove, the factor claims mapped back.
+static lzx_status prove_batch_impl(const lzx_batch_claim *claims, size_t k, bool star, lzx_transcript *tr,
+                                   uint64_t *rhos_out, uint64_t *rounds_out, uint64_t *challenges_out,
+                                   uint64_t *final_claim, uint64_t *factor_claims_out, size_t *num_vars_out,
+                                   size_t *degree_out, hipStream_t s)
+{
+    if (!tr || !final_claim || !degree_out) return LZX_INVALID_ARGUMENT;
+    if (k == 0) return LZX_BATCH_EMPTY; // batch.rs:37-39, 165-167
+    if (!claims || !rhos_out) return LZX_INVALID_ARGUMENT;
+    size_t N = 0, total_f = 0;
+    for (size_t i = 0; i < k; i++) {
+        const lzx_batch_claim &c = claims[i];
+        if ((c.nfactors && !c.factors_dev) || (c.nterms && (!c.coeffs || !c.term_off || !c.term_ids)) ||
+            c.claimed_sum >= GL_P)
+            return LZX_INVALID_ARGUMENT;
+        if (c.num_vars > 40) return LZX_UNSUPPORTED;
+        N = std::max(N, c.num_vars);
+        total_f += c.nfactors;
+    }
+    if (!star)
+        for (size_t i = 0; i < k; i++)
+            if (claims[i].num_vars != claims[0].num_vars) return LZX_BATCH_VARIABLE_COUNT_MISMATCH; // batch.rs:40-46
+    if ((N && (!rounds_out || !challenges_out)) || (total_f && !factor_claims_out)) return LZX_INVALID_ARGUMENT;
+    lzx_status st = draw_rhos(tr, star ? "batch-star-rho" : "batch-rho", k, rhos_out);
+    if (st != LZX_OK) return st;
+
+    std::vector<std::pair<const uint64_t *, uint32_t>> tabs; // distinct (table, lift)
+    std::vector<uint32_t> gmap(total_f);                      // global factor -> distinct table
+    std::vector<uint64_t> coeffs;
+    std::vector<uint32_t> off(1, 0), ids;
+    uint64_t combined = 0;
+    size_t g = 0;
+    for (size_t i = 0; i < k; i++) {
+        const lzx_batch_claim &c = claims[i];
+        const uint32_t lift = (uint32_t)(N - c.num_vars);
+        const uint64_t rho = rhos_out[i];
+        combined = gl_add(combined, gl_mul(gl_mul(rho, gl_pow2(lift)), c.claimed_sum));
+        const size_t base = g;
+        for (size_t f = 0; f < c.nfactors; f++) {
+            if (!c.factors_dev[f]) return LZX_INVALID_ARGUMENT;
+            const std::pair<const uint64_t *, uint32_t> key(c.factors_dev[f], lift);
+            size_t j = std::find(tabs.begin(), tabs.end(), key) - tabs.begin();
+            if (j == tabs.size()) tabs.push_back(key);
+            gmap[g++] = (uint32_t)j;
+        }
+        for (size_t t = 0; t < c.nterms; t++) {
+            if (c.term_off[t + 1] < c.term_off[t] || c.coeffs[t] >= GL_P) return LZX_INVALID_ARGUMENT;
+            if (c.term_off[t + 1] == c.term_off[t]) return LZX_SUMCHECK_VIRTUAL_POLY; // add_term: EmptyProduct
+            coeffs.push_back(gl_mul(rho, c.coeffs[t]));
+            for (uint32_t x = c.term_off[t]; x < c.term_off[t + 1]; x++) {
+                if (c.term_ids[x] >= c.nfactors) return LZX_SUMCHECK_VIRTUAL_POLY;
+                ids.push_back(gmap[base + c.term_ids[x]]);
+            }
+            off.push_back((uint32_t)ids.size());
+        }
+    }
+    if (tabs.size() > (size_t)SC_MAXF || coeffs.size() > (size_t)SC_MAXT) return LZX_UNSUPPORTED;
+    if (num_vars_out) *num_vars_out = N;
+
+    // lifted tables: 2^N entries each in scratch
+    std::vector<const uint64_t *> fac(tabs.size());
+    std::vector<uint64_t *> owned;
+    for (size_t j = 0; j < tabs.size() && st == LZX_OK; j++) {
+        fac[j] = tabs[j].first;
+        if (tabs[j].second == 0) continue;
+        uint64_t *dst = nullptr;
+        const uint64_t total = 1ULL << N;
+        hipError_t e = scratch_alloc((void **)&dst, total * sizeof(uint64_t), s);
+        if (e == hipSuccess) {
+            owned.push_back(dst);
+            const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 1 << 16));
+            hipLaunchKernelGGL(pad_table_kernel, dim3(blocks), dim3(256), 0, s, tabs[j].first, tabs[j].second, total,
+                               dst);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) st = device_fail(e, "batch lift");
+        fac[j] = dst;
+    }
+    std::vector<uint64_t> fclaims(std::max<size_t>(1, tabs.size()), 0);
+    if (st == LZX_OK)
+        st = prove_entry(N, tabs.size(), fac.data(), coeffs.size(), coeffs.data(), off.data(), ids.data(), combined,
+                         tr, nullptr, rounds_out, challenges_out, final_claim, fclaims.data(), degree_out, s);
+    for (uint64_t *p : owned) (void)scratch_free(p, s);
+    if (st == LZX_OK && !coeffs.empty())
+        for (size_t x = 0; x < total_f; x++) factor_claims_out[x] = fclaims[gmap[x]];
+    return st;
+}
+
+} // namespace lzx
+
+extern "C" lzx_status lzx_sumcheck_prove_batch(const lzx_batch_claim *claims, size_t nclaims,
+                                               lzx_transcript *transcript, uint64_t *rhos_out, uint64_t *rounds_out,
+                                               uint64_t *challenges_out, uint64_t *final_claim,
+                                               uint64_t *factor_claims_out, size_t *degree_out, void *stream)
+{
+    return prove_batch_impl(claims, nclaims, false, transcript, rhos_out, rounds_out, challenges_out, final_claim,
+                            factor_claims_out, nullptr, degree_out, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_sumcheck_prove_batch_star(const lzx_batch_claim *claims, size_t nclaims,
+                                                    lzx_transcript *transcript, uint64_t *rhos_out,
+                                                    uint64_t *rounds_out, uint64_t *challenges_out,
+                                                    uint64_t *final_claim, uint64_t *factor_claims_out,
+                                                    size_t *num_vars_out, size_t *degree_out, void *stream)
+{
+    return prove_batch_impl(claims, nclaims, true, transcript, rhos_out, rounds_out, challenges_out, final_claim,
+                            factor_claims_out, num_vars_out, degree_out, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_sumcheck_verify_batch(const uint64_t *rounds, size_t num_rounds, size_t round_len,
+                                                size_t num_vars, size_t max_degree, const uint64_t *claimed_sums,
+                                                size_t nclaims, lzx_transcript *transcript, int32_t has_expected,
+                                                uint64_t expected_final, uint64_t *point_out)
+{
+    if (!transcript || (nclaims && !claimed_sums)) return LZX_INVALID_ARGUMENT;
+    std::vector<uint64_t> rhos(std::max<size_t>(1, nclaims));
+    lzx_status st = draw_rhos(transcript, "batch-rho", nclaims, rhos.data());
+    if (st != LZX_OK) return st;
+    uint64_t combined = 0;
+    for (size_t i = 0; i < nclaims; i++) {
+        if (claimed_sums[i] >= GL_P) return LZX_INVALID_ARGUMENT;
+        combined = gl_add(combined, gl_mul(rhos[i], claimed_sums[i]));
+    }
+    uint64_t fin = 0;
+    return lzx_sumcheck_verify(rounds, num_rounds, round_len, num_vars, max_degree, combined, transcript,
+                               has_expected, expected_final, point_out, &fin);
+}
+
+extern "C" lzx_status lzx_sumcheck_verify_batch_star(const uint64_t *rounds, size_t num_rounds, size_t round_len,
+                                                     const size_t *num_vars, size_t n_num_vars, size_t max_degree,
+                                                     const uint64_t *claimed_sums, size_t nclaims,
+                                                     const uint64_t *rhos, size_t nrhos, lzx_transcript *transcript,
+                                                     int32_t has_expected, uint64_t expected_final,
+                                                     uint64_t *point_out)
+{
+    if (!transcript) return LZX_INVALID_ARGUMENT;
+    if (n_num_vars == 0 || n_num_vars != nclaims) return LZX_BATCH_VARIABLE_COUNT_MISMATCH; // batch.rs:243-248
+    if (!num_vars || !claimed_sums) return LZX_INVALID_ARGUMENT;
+    std::vector<uint64_t> drawn;
+    if (!rhos) { // verify_batch_star draws them; verify_batch_star_with_rhos is handed them
+        drawn.resize(nclaims);
+        const lzx_status st = draw_rhos(transcript, "batch-star-rho", nclaims, drawn.data());
+        if (st != LZX_OK) return st;
+        rhos = drawn.data();
+        nrhos = nclaims;
+    }
+    if (nrhos != nclaims) return LZX_BATCH_CHALLENGE_COUNT; // batch.rs:284-286
+    const size_t max_vars = *std::max_element(num_vars, num_vars + n_num_vars);
+    uint64_t combined = 0;
+    for (size_t i = 0; i < nclaims; i++) {
+        if (claimed_sums[i] >= GL_P || rhos[i] >= GL_P) return LZX_INVALID_ARGUMENT;
+        combined = gl_add(combined, gl_mul(gl_mul(rhos[i], gl_pow2(max_vars - num_vars[i])), claimed_sums[i]));
+    }
+    uint64_t fin = 0;
+    return lzx_sumcheck_verify(rounds, num_rounds, round_len, max_vars, max_degree, combined, transcript,
+                               has_expected, expected_final, point_out, &fin);
+}

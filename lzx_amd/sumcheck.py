@@ -247,3 +247,154 @@ def zerocheck_verify(rounds, num_vars: int, transcript: Transcript, expected_pol
                                      1 if expected_poly_eval is not None else 0, expected_poly_eval or 0,
                                      point.ctypes.data), "zerocheck::verify")
     return point[:num_vars]
+
+
+# ---- claim batching (crates/lattice-sumcheck/src/batch.rs:24-336) ------------------------
+
+class _BatchClaimC(C.Structure):
+    """lzx_batch_claim."""
+
+    _fields_ = [("num_vars", C.c_size_t), ("nfactors", C.c_size_t), ("factors_dev", C.c_void_p),
+                ("nterms", C.c_size_t), ("coeffs", C.c_void_p), ("term_off", C.c_void_p),
+                ("term_ids", C.c_void_p), ("claimed_sum", C.c_uint64)]
+
+
+@dataclass
+class BatchClaim:
+    """batch.rs:24-28: a virtual polynomial and its asserted sum."""
+
+    poly: VirtualPolynomial
+    claimed_sum: int
+
+
+@dataclass
+class PerClaimClaims:
+    """batch.rs:115-126."""
+
+    claimed_sum: int
+    num_vars: int
+    factor_claims: np.ndarray
+    point: np.ndarray
+
+
+@dataclass
+class BatchStarOutput:
+    """batch.rs:102-113: the single proof, the shared point, the rhos, the per-claim claims."""
+
+    output: SumcheckOutput
+    rhos: np.ndarray
+    per_claim: list
+
+    @property
+    def challenges(self):
+        return self.output.challenges
+
+    @property
+    def rounds(self):
+        return self.output.rounds
+
+
+def _batch_ffi(claims):
+    arr = (_BatchClaimC * max(1, len(claims)))()
+    keep = []
+    for i, c in enumerate(claims):
+        vp = c.poly
+        if any(f is None for f in vp.factors):
+            raise ValueError("batch claims need every factor table")
+        ptrs, coeffs, off, ids = vp._ffi()
+        keep += [ptrs, coeffs, off, ids]
+        arr[i] = _BatchClaimC(vp.num_vars, len(vp.factors), C.cast(ptrs, C.c_void_p).value, len(vp.terms),
+                              coeffs.ctypes.data, off.ctypes.data, ids.ctypes.data, int(c.claimed_sum) % GL_P)
+    return arr, keep
+
+
+def _prove_batch(claims, transcript, star, stream):
+    arr, keep = _batch_ffi(claims)
+    nv = max((c.poly.num_vars for c in claims), default=0)
+    d = max([max(c.poly.max_degree(), 1) for c in claims] or [1])
+    total_f = sum(len(c.poly.factors) for c in claims)
+    rhos = np.zeros(max(1, len(claims)), dtype=np.uint64)
+    rounds = np.zeros(max(1, nv * (d + 1)), dtype=np.uint64)
+    chals = np.zeros(max(1, nv), dtype=np.uint64)
+    fclaims = np.zeros(max(1, total_f), dtype=np.uint64)
+    fin, deg, nvo = C.c_uint64(), C.c_size_t(), C.c_size_t(nv)
+    if star:
+        st = lib().lzx_sumcheck_prove_batch_star(arr, len(claims), transcript.handle, rhos.ctypes.data,
+                                                 rounds.ctypes.data, chals.ctypes.data, C.byref(fin),
+                                                 fclaims.ctypes.data, C.byref(nvo), C.byref(deg),
+                                                 dev.stream_handle(stream))
+    else:
+        st = lib().lzx_sumcheck_prove_batch(arr, len(claims), transcript.handle, rhos.ctypes.data, rounds.ctypes.data,
+                                            chals.ctypes.data, C.byref(fin), fclaims.ctypes.data, C.byref(deg),
+                                            dev.stream_handle(stream))
+    check(st, "batch::prove_batch_star" if star else "batch::prove_batch")
+    del keep
+    n, w = nvo.value, deg.value + 1
+    has_terms = any(c.poly.terms for c in claims)
+    out = SumcheckOutput(rounds[: n * w].reshape(n, w), chals[:n], fin.value,
+                         fclaims[:total_f] if has_terms else np.zeros(0, dtype=np.uint64))
+    return out, rhos[: len(claims)]
+
+
+def prove_batch(claims, transcript: Transcript, stream=None):
+    """batch::prove_batch(&claims, &mut t) (batch.rs:33-80): (SumcheckOutput, rhos);
+    factor claims in global order (claim 0's factors first)."""
+    return _prove_batch(claims, transcript, False, stream)
+
+
+def prove_batch_star(claims, transcript: Transcript, stream=None) -> BatchStarOutput:
+    """batch::prove_batch_star (batch.rs:161-225): claims of any num_vars, one
+    sum-check over max num_vars rounds."""
+    out, rhos = _prove_batch(claims, transcript, True, stream)
+    per, base = [], 0
+    for c in claims:
+        nf = len(c.poly.factors)
+        per.append(PerClaimClaims(int(c.claimed_sum) % GL_P, c.poly.num_vars, out.factor_claims[base: base + nf],
+                                  out.challenges[: c.poly.num_vars]))
+        base += nf
+    return BatchStarOutput(out, rhos, per)
+
+
+def _rounds_arg(rounds):
+    r = np.ascontiguousarray(rounds, dtype=np.uint64)
+    nr = r.shape[0] if r.ndim == 2 else 0
+    rl = r.shape[1] if r.ndim == 2 else 0
+    return r, nr, rl
+
+
+def verify_batch(rounds, num_vars: int, max_degree: int, claimed_sums, transcript: Transcript,
+                 expected_combined_final=None) -> np.ndarray:
+    """batch::verify_batch (batch.rs:300-336): returns the point."""
+    r, nr, rl = _rounds_arg(rounds)
+    sums = np.array([int(x) % GL_P for x in claimed_sums] or [0], dtype=np.uint64)
+    point = np.zeros(max(1, nr), dtype=np.uint64)
+    check(lib().lzx_sumcheck_verify_batch(r.ctypes.data if r.size else None, nr, rl, num_vars, max_degree,
+                                          sums.ctypes.data, len(claimed_sums), transcript.handle,
+                                          1 if expected_combined_final is not None else 0,
+                                          expected_combined_final or 0, point.ctypes.data), "batch::verify_batch")
+    return point[:nr]
+
+
+def verify_batch_star(rounds, num_vars, max_degree: int, claimed_sums, transcript: Transcript,
+                      expected_combined_final=None, rhos=None) -> np.ndarray:
+    """batch::verify_batch_star (batch.rs:236-262), or verify_batch_star_with_rhos
+    (batch.rs:268-296) when rhos is given: returns the point."""
+    r, nr, rl = _rounds_arg(rounds)
+    nvs = np.array(list(num_vars) or [0], dtype=np.uintp)
+    sums = np.array([int(x) % GL_P for x in claimed_sums] or [0], dtype=np.uint64)
+    rh = None if rhos is None else np.array([int(x) for x in rhos] or [0], dtype=np.uint64)
+    point = np.zeros(max(1, nr), dtype=np.uint64)
+    check(lib().lzx_sumcheck_verify_batch_star(r.ctypes.data if r.size else None, nr, rl, nvs.ctypes.data,
+                                               len(num_vars), max_degree, sums.ctypes.data, len(claimed_sums),
+                                               rh.ctypes.data if rh is not None else None,
+                                               0 if rhos is None else len(rhos), transcript.handle,
+                                               1 if expected_combined_final is not None else 0,
+                                               expected_combined_final or 0, point.ctypes.data),
+          "batch::verify_batch_star")
+    return point[:nr]
+
+
+def verify_batch_star_with_rhos(rounds, num_vars, max_degree: int, claimed_sums, rhos, transcript: Transcript,
+                                expected_combined_final=None) -> np.ndarray:
+    """batch::verify_batch_star_with_rhos (batch.rs:268-296)."""
+    return verify_batch_star(rounds, num_vars, max_degree, claimed_sums, transcript, expected_combined_final, rhos)

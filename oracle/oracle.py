@@ -526,6 +526,83 @@ def zerocheck_verify(rounds, num_vars, transcript: Transcript, expected_poly_eva
 
 
 # ---------------------------------------------------------------------------
+# Claim batching (crates/lattice-sumcheck/src/batch.rs:33-336), composed from
+# the transcript and sumcheck_prove / sumcheck_verify above
+# ---------------------------------------------------------------------------
+
+BATCH_VARIABLE_COUNT_MISMATCH, BATCH_EMPTY, BATCH_CHALLENGE_COUNT = 16, 17, 18
+
+
+def gl_pow2(k):
+    """pow2 (batch.rs:150-157): 1 doubled k times."""
+    v = 1
+    for _ in range(k):
+        v = (v + v) % GL_P
+    return v
+
+
+def pad_factor(f, k):
+    """pad_factor (batch.rs:130-148): every entry repeated 2^k times consecutively."""
+    return np.repeat(np.ascontiguousarray(f, dtype=np.uint64), 1 << k)
+
+
+def sumcheck_prove_batch(claims, transcript: Transcript, star=False):
+    """prove_batch (batch.rs:33-80) or prove_batch_star (:161-225).  claims are
+    (num_vars, factors, terms, claimed_sum).  Returns (status, dict) with
+    rhos, num_vars and sumcheck_prove's outputs."""
+    if not claims:
+        return BATCH_EMPTY, None
+    nvs = [c[0] for c in claims]
+    n = max(nvs) if star else nvs[0]
+    if not star and any(v != n for v in nvs):                                   # :40-46
+        return BATCH_VARIABLE_COUNT_MISMATCH, None
+    st, rhos = transcript.challenge_fields(b"batch-star-rho" if star else b"batch-rho", len(claims))
+    if st:
+        return BATCH_CHALLENGE_COUNT, None
+    factors, terms, combined = [], [], 0
+    for ci, (nv, facs, tms, claimed) in enumerate(claims):
+        rho, k = int(rhos[ci]), n - nv
+        combined = (combined + rho * gl_pow2(k) % GL_P * (int(claimed) % GL_P)) % GL_P   # :58, :185-187
+        base = len(factors)
+        factors += [pad_factor(f, k) for f in facs]
+        terms += [(rho * (int(c) % GL_P) % GL_P, [base + i for i in ids]) for c, ids in tms]
+    st, out = sumcheck_prove(n, factors, terms, combined, transcript)
+    if st:
+        return st, None
+    out["rhos"] = np.asarray(rhos, dtype=np.uint64)[: len(claims)]
+    out["num_vars"] = n
+    out["combined_claim"] = combined
+    return 0, out
+
+
+def sumcheck_verify_batch(rounds, num_vars, max_degree, claimed_sums, transcript: Transcript, expected=None):
+    """verify_batch (batch.rs:300-336).  Returns (status, point)."""
+    st, rhos = transcript.challenge_fields(b"batch-rho", len(claimed_sums))
+    if st:
+        return BATCH_CHALLENGE_COUNT, None
+    comb = sum(int(r) * (int(c) % GL_P) for r, c in zip(rhos, claimed_sums)) % GL_P
+    st, point, _ = sumcheck_verify(rounds, num_vars, max_degree, comb, transcript, expected)
+    return st, point
+
+
+def sumcheck_verify_batch_star(rounds, num_vars, max_degree, claimed_sums, transcript: Transcript, expected=None,
+                               rhos=None):
+    """verify_batch_star (batch.rs:236-262); with rhos, verify_batch_star_with_rhos (:268-296)."""
+    if not num_vars or len(num_vars) != len(claimed_sums):
+        return BATCH_VARIABLE_COUNT_MISMATCH, None
+    if rhos is None:
+        st, rhos = transcript.challenge_fields(b"batch-star-rho", len(claimed_sums))
+        if st:
+            return BATCH_CHALLENGE_COUNT, None
+    if len(rhos) != len(claimed_sums):
+        return BATCH_CHALLENGE_COUNT, None
+    n = max(num_vars)
+    comb = sum(int(r) * gl_pow2(n - v) % GL_P * (int(c) % GL_P) for r, v, c in zip(rhos, num_vars, claimed_sums))
+    st, point, _ = sumcheck_verify(rounds, n, max_degree, comb % GL_P, transcript, expected)
+    return st, point
+
+
+# ---------------------------------------------------------------------------
 # The ~2^50 quadratic-slot ring (crates/lattice-ring/src/modulus50.rs:44-499)
 # ---------------------------------------------------------------------------
 

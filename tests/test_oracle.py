@@ -673,3 +673,125 @@ def test_gadget_decomposition_reference_cases():
         st, d = O.gadget_decompose(10, 21, int(x))
         assert st == 0 and sum(int(v) * 10**i for i, v in enumerate(d)) == int(x)
         assert all(-4 <= v <= 5 for v in d)
+
+
+# ---- claim batching (batch.rs:33-336) ------------------------------------------
+
+def _batch_vp(nv, rng):
+    """batch.rs:347-357 build(): factors f, g and P = 2 f g + 9 f."""
+    return (nv, [rng.integers(0, O.GL_P, 1 << nv, dtype=np.uint64) for _ in range(2)], [(2, [0, 1]), (9, [0])])
+
+
+def _vp_eval(vp, point):
+    _nv, facs, terms = vp
+    vals = [O.mle_evaluate(f, point) for f in facs]
+    tot = 0
+    for c, ids in terms:
+        p = c
+        for i in ids:
+            p = p * vals[i] % O.GL_P
+        tot = (tot + p) % O.GL_P
+    return tot
+
+
+def _claims(vps):
+    return [(nv, facs, terms, O.vp_sum_over_hypercube(nv, facs, terms)) for nv, facs, terms in vps]
+
+
+def test_batch_prove_verify_reference_cases():
+    # batch.rs:359-389 batch_prove_verify, :391-410 batch_wrong_sums_rejected, :412-437 mixed counts / empty
+    rng = np.random.default_rng(21)
+    vps = [_batch_vp(5, rng) for _ in range(3)]
+    cl = _claims(vps)
+    st, out = O.sumcheck_prove_batch(cl, O.Transcript(b"lzx-batch-test"))
+    assert st == 0 and len(out["rhos"]) == 3
+    r = [int(x) for x in out["challenges"]]
+    expected = sum(int(rho) * _vp_eval(vp, r) for rho, vp in zip(out["rhos"], vps)) % O.GL_P
+    assert out["final_claim"] == expected
+    st, point = O.sumcheck_verify_batch(out["rounds"], 5, 2, [c[3] for c in cl], O.Transcript(b"lzx-batch-test"),
+                                        expected)
+    assert st == 0 and [int(x) for x in point] == r
+    two = _claims([_batch_vp(4, rng) for _ in range(2)])
+    st, out = O.sumcheck_prove_batch(two, O.Transcript(b"lzx-batch-test"))
+    bad = [O.gl_add(two[0][3], 1), two[1][3]]
+    assert O.sumcheck_verify_batch(out["rounds"], 4, 2, bad, O.Transcript(b"lzx-batch-test"))[0] != 0
+    mixed = _claims([_batch_vp(3, rng), _batch_vp(4, rng)])
+    assert O.sumcheck_prove_batch(mixed, O.Transcript(b"t"))[0] == O.BATCH_VARIABLE_COUNT_MISMATCH
+    assert O.sumcheck_prove_batch([], O.Transcript(b"t"))[0] == O.BATCH_EMPTY
+
+
+def test_batch_star_reference_cases():
+    # batch.rs:443-479 row count, :481-514 terminal from per-claim openings, :516-533 wrong claim,
+    # :535-556 tampered round, :558-579 same-var batch star
+    rng = np.random.default_rng(22)
+    vps = [_batch_vp(nv, rng) for nv in (3, 5, 5, 4)]
+    cl = _claims(vps)
+    st, out = O.sumcheck_prove_batch(cl, O.Transcript(b"lzx-batch-star-test"), star=True)
+    assert st == 0 and out["rounds"].shape[0] == 5 and len(out["challenges"]) == 5
+    st, point = O.sumcheck_verify_batch_star(out["rounds"], [3, 5, 5, 4], 2, [c[3] for c in cl],
+                                             O.Transcript(b"lzx-batch-star-test"))
+    assert st == 0 and point.tolist() == out["challenges"].tolist()
+    r = [int(x) for x in out["challenges"]]
+    expected, base = 0, 0
+    for rho, vp in zip(out["rhos"], vps):
+        nv = vp[0]
+        # the lifted factor's claim is the original MLE at the truncated point
+        assert [int(x) for x in out["factor_claims"][base: base + 2]] == [O.mle_evaluate(f, r[:nv]) for f in vp[1]]
+        expected = (expected + int(rho) * _vp_eval(vp, r[:nv])) % O.GL_P
+        base += 2
+    assert out["final_claim"] == expected
+    assert O.sumcheck_verify_batch_star(out["rounds"], [3, 5, 5, 4], 2, [c[3] for c in cl],
+                                        O.Transcript(b"lzx-batch-star-test"), expected)[0] == 0
+    # the rhos handed in give the same result (verify_batch_star_with_rhos)
+    vt = O.Transcript(b"lzx-batch-star-test")
+    st_r, rh = vt.challenge_fields(b"batch-star-rho", 4)
+    assert st_r == 0 and rh.tolist() == out["rhos"].tolist()
+    assert O.sumcheck_verify_batch_star(out["rounds"], [3, 5, 5, 4], 2, [c[3] for c in cl], vt, expected, rh)[0] == 0
+    assert O.sumcheck_verify_batch_star(out["rounds"], [3, 5, 5, 4], 2, [c[3] for c in cl],
+                                        O.Transcript(b"x"), None, rh[:3])[0] == O.BATCH_CHALLENGE_COUNT
+    assert O.sumcheck_verify_batch_star(out["rounds"], [3, 5], 2, [c[3] for c in cl],
+                                        O.Transcript(b"x"))[0] == O.BATCH_VARIABLE_COUNT_MISMATCH
+    wc = _claims([_batch_vp(4, rng), _batch_vp(3, rng)])
+    st, out = O.sumcheck_prove_batch(wc, O.Transcript(b"lzx-batch-star-test"), star=True)
+    assert O.sumcheck_verify_batch_star(out["rounds"], [4, 3], 2, [O.gl_add(wc[0][3], 1), wc[1][3]],
+                                        O.Transcript(b"lzx-batch-star-test"))[0] != 0
+    tc = _claims([_batch_vp(4, rng), _batch_vp(4, rng)])
+    st, out = O.sumcheck_prove_batch(tc, O.Transcript(b"lzx-batch-star-test"), star=True)
+    tampered = out["rounds"].copy()
+    tampered[0, 0] = O.gl_add(int(tampered[0, 0]), 1)
+    assert O.sumcheck_verify_batch_star(tampered, [4, 4], 2, [c[3] for c in tc],
+                                        O.Transcript(b"lzx-batch-star-test"))[0] != 0
+    assert out["rounds"].shape[0] == 4
+    assert O.sumcheck_verify_batch_star(out["rounds"], [4, 4], 2, [c[3] for c in tc],
+                                        O.Transcript(b"lzx-batch-star-test"))[0] == 0
+
+
+def test_batch_star_rounds_closed_form():
+    """Each combined round message is sum_i rho_i g_i(t), g_i by hypercube
+    enumeration of claim i's own MLEs: 2^(N-r) sum_x P_i(r_<t, t, x) for rounds
+    t < r, and 2^(N-1-t) P_i(r_<r) once claim i's variables are bound (the lift
+    ignores the LAST variables)."""
+    rng = np.random.default_rng(23)
+    vps = [_batch_vp(nv, rng) for nv in (2, 4, 3)]
+    vps.append((4, [rng.integers(0, O.GL_P, 16, dtype=np.uint64) for _ in range(3)], [(5, [0, 1, 2]), (1, [2])]))
+    cl = _claims(vps)
+    st, out = O.sumcheck_prove_batch(cl, O.Transcript(b"closed-form"), star=True)
+    assert st == 0
+    N, d = 4, 3
+    r = [int(x) for x in out["challenges"]]
+    for t in range(N):
+        for X in range(d + 1):
+            want = 0
+            for rho, vp in zip(out["rhos"], vps):
+                nv = vp[0]
+                if t < nv:
+                    free = nv - 1 - t
+                    g = 0
+                    for b in range(1 << free):
+                        bits = [(b >> (free - 1 - i)) & 1 for i in range(free)]
+                        g = (g + _vp_eval(vp, r[:t] + [X] + bits)) % O.GL_P
+                    g = g * (1 << (N - nv)) % O.GL_P
+                else:
+                    g = _vp_eval(vp, r[:nv]) * (1 << (N - 1 - t)) % O.GL_P
+                want = (want + int(rho) * g) % O.GL_P
+            assert int(out["rounds"][t, X]) == want, (t, X)
