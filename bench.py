@@ -260,6 +260,80 @@ def bench_ring50(reps: int = 10):
             "mul_polys_per_s": round(polys / t_mul, 1)}
 
 
+def _timed_events(torch, fn, reps):
+    """Median seconds of fn over reps, HIP events on the launch stream (torch's current stream)."""
+    stream = torch.cuda.current_stream()
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e-3)
+    return statistics.median(ts)
+
+
+def bench_extension(reps: int = 10):
+    """The quadratic extension ring R_q[Y]/(Y^2+1), q = 3221225473: 2^17
+    elements of n = 64 (u32).  Algorithmic bytes per element: mul reads a, b
+    and writes the product (3 x 512 B); norm reads 512 B and writes 256 B;
+    inverse reads 512 B and writes 512 B + 1 flag byte."""
+    import torch
+
+    from lzx_amd import device as D
+    from lzx_amd import extension as E
+    from lzx_amd import ring as R
+
+    count, n = 1 << 17, 64
+    q = R.Modulus32.Q_32.q
+    ring = R.RingConfig(R.Modulus32.Q_32, 6)
+    rng = np.random.default_rng(0xE2)
+    a = D.to_device(rng.integers(0, q, count * 2 * n, dtype=np.uint64).astype(np.uint32))
+    b = D.to_device(rng.integers(0, q, count * 2 * n, dtype=np.uint64).astype(np.uint32))
+    out = D.empty(count * 2 * n, np.uint32)
+    nrm = D.empty(count * n, np.uint32)
+    t_mul = _timed_events(torch, lambda: E.mul_batch(ring, a, b, out), reps)
+    t_norm = _timed_events(torch, lambda: E.norm_batch(ring, a, nrm), reps)
+    t_inv = _timed_events(torch, lambda: E.inverse_batch(ring, a, out), reps)
+    gbs = lambda byts, t: round(byts * count / t / 1e9, 1)
+    return {"workload": "R_q[Y]/(Y^2+1), q=3221225473, 2^17 elements x n=64, u32",
+            "mul_us": round(t_mul * 1e6, 2), "mul_gbs": gbs(1536, t_mul), "mul_per_s": round(count / t_mul, 1),
+            "mul_hbm_frac": round(1536 * count / t_mul / 1e9 / HBM_PEAK_GBS, 4),
+            "norm_us": round(t_norm * 1e6, 2), "norm_gbs": gbs(768, t_norm),
+            "inverse_us": round(t_inv * 1e6, 2), "inverse_gbs": gbs(1025, t_inv)}
+
+
+def bench_packing(reps: int = 10):
+    """Witness packing, HBM-bound maps: 2^24 Goldilocks values into 22-bit limbs
+    (8 B read + 12 B written per value), balanced base-16 digits as i8 (17 per
+    value: 8 B read + 17 B written), and 2^30 bits at 31 per coefficient
+    (31 B read + 4 B written per coefficient); ring degree 64, q = 3221225473."""
+    import torch
+
+    from lzx_amd import device as D
+    from lzx_amd import packing as PK
+    from lzx_amd import ring as R
+
+    cfg = R.RingConfig(R.Modulus32.Q_32, 6)
+    rng = np.random.default_rng(0xB7)
+    nv = 1 << 24
+    vals = D.to_device(rng.integers(0, 0xFFFFFFFF00000001, nv, dtype=np.uint64))
+    t_pack = _timed_events(torch, lambda: PK.pack_field_elements(cfg, vals), reps)
+    g = PK.GadgetDecomposition(16, 17)
+    t_dec = _timed_events(torch, lambda: g.decompose_batch(vals, digit_bytes=1), reps)
+    nbits = 1 << 30
+    bits = torch.randint(0, 2, (nbits,), dtype=torch.uint8, device="cuda")
+    t_bits = _timed_events(torch, lambda: PK.pack_bits(cfg, bits), reps)
+    coeffs = -(-nbits // 31)
+    return {"workload": "witness packing, 2^24 values / 2^30 bits, n=64, q=3221225473",
+            "pack_field_us": round(t_pack * 1e6, 2), "pack_field_gbs": round(20 * nv / t_pack / 1e9, 1),
+            "decompose_i8_us": round(t_dec * 1e6, 2), "decompose_i8_gbs": round(25 * nv / t_dec / 1e9, 1),
+            "pack_bits_us": round(t_bits * 1e6, 2), "pack_bits_gbs": round(35 * coeffs / t_bits / 1e9, 1)}
+
+
 def bench_uneven_batch(reps: int = 5, autotune: bool = True):
     """The uneven commitment batch: 512 jobs (workloads.uneven_batch_jobs(0xC5)),
     n in {64, 128, 256}, q in {3221225473, 2013265921, 1125899437080577},
@@ -349,6 +423,8 @@ def main():
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven commitment batch leg")
     ap.add_argument("--only-sumcheck", action="store_true", help="run only the 2^24 sum-check prove leg")
     ap.add_argument("--only-ring50", action="store_true", help="run only the 50-bit quadratic-slot ring leg")
+    ap.add_argument("--only-extension", action="store_true", help="run only the R_q[Y]/(Y^2+1) leg")
+    ap.add_argument("--only-packing", action="store_true", help="run only the witness-packing leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch leg: use cached (LZX_AUTOTUNE_CACHE) or default chunks instead of tuning")
     ap.add_argument("--copy-calib", action="store_true",
@@ -364,6 +440,12 @@ def main():
     D.require_gpu()
     if args.only_ring50:
         print(json.dumps({"ring50": bench_ring50()}), flush=True)
+        return
+    if args.only_extension:
+        print(json.dumps({"extension": bench_extension()}), flush=True)
+        return
+    if args.only_packing:
+        print(json.dumps({"packing": bench_packing()}), flush=True)
         return
     if args.only_sumcheck:
         print(json.dumps({"sumcheck_prove": bench_sumcheck(reps=5)}), flush=True)
@@ -403,7 +485,8 @@ def main():
     }
     if not args.only_ntt:
         line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
-                             "uneven_batch": bench_uneven_batch(), "ring50": bench_ring50()}
+                             "uneven_batch": bench_uneven_batch(), "ring50": bench_ring50(),
+                             "extension": bench_extension(), "packing": bench_packing()}
         if not args.no_cpu:
             line["secondary"]["cpu_baseline"] = cpu_baseline_secondary()
     print(json.dumps(line), flush=True)

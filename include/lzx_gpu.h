@@ -1,16 +1,23 @@
 /*
  * lzx_gpu.h -- C ABI of the MI355X (gfx950) lattice-commitment hot path.
  *
- * Drop-in boundary for three Rust items of idrees2516/lzx (paths relative to
+ * Drop-in boundary for these Rust items of idrees2516/lzx (paths relative to
  * the reference's crates/):
  *   - the negacyclic NTT            lattice-ring/src/ntt.rs, ring.rs
+ *   - the ~2^50 quadratic-slot ring lattice-ring/src/modulus50.rs
+ *   - the extension R_q[Y]/(Y^2+1)  lattice-ring/src/extension.rs
+ *   - witness packing               lattice-ring/src/packing.rs,
+ *                                   lattice-commitment/src/sparse.rs,
+ *                                   lattice-core/src/decomposition.rs
  *   - the Ajtai commitment A*s      lattice-commitment/src/ajtai.rs
  *   - the Goldilocks sum-check      lattice-sumcheck/src/sumcheck.rs with the
- *     prover and its Fiat-Shamir     lattice-core/src/transcript.rs, keccak.rs
+ *     prover and its Fiat-Shamir     lattice-core/src/transcript.rs, keccak.rs,
+ *     zerocheck, batching, split eq  zerocheck.rs, batch.rs, fastprover.rs
  *
  * Conventions
- *   - Every entry point returns lzx_status; values 1..29 map one-to-one onto
- *     the reference's error enums (noted per code), 30.. are ABI/device codes.
+ *   - Every entry point returns lzx_status; values 1..29 and 40..58 map
+ *     one-to-one onto the reference's error enums (noted per code), 30, 31 and
+ *     100.. are ABI/device codes.
  *     Nothing unwinds across the ABI.
  *   - Pointers named *_dev are device (HBM) pointers; *_host are host pointers.
  *     `stream` is a hipStream_t passed as void* (NULL = the null stream).
@@ -125,6 +132,37 @@ lzx_status lzx_ntt_inverse_host(const lzx_ntt_plan *plan, uint32_t *a_host, size
  * ring.rs:309-340).  Device buffers, [poly][n]. */
 lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
                               uint32_t *out_dev, size_t len, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Quadratic extension ring R_q[Y]/(Y^2+1) (lattice-ring/src/extension.rs).
+ * An element c0 + c1 Y is [c0 | c1], n canonical coefficients each; a batch
+ * of count elements is [count][2][n] in device memory (the order of
+ * Rq2::to_bytes). The plan names q and n. Outputs may alias inputs.
+ * ------------------------------------------------------------------------- */
+
+/* Rq2::mul (extension.rs:115-124): out = a * b, count elements. */
+lzx_status lzx_rq2_mul_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                             uint32_t *out_dev, size_t count, void *stream);
+
+/* Rq2::norm (extension.rs:142-147): out[count][n] = c0^2 + c1^2. */
+lzx_status lzx_rq2_norm_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev, size_t count,
+                              void *stream);
+
+/* Rq2::inverse (extension.rs:151-167): ok_dev[i] = 1 and out[i] = a[i]^{-1}
+ * when every NTT slot of N(a[i]) is non-zero, else ok_dev[i] = 0 (None) and
+ * out[i] = 0. */
+lzx_status lzx_rq2_inverse_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev,
+                                 uint8_t *ok_dev, size_t count, void *stream);
+
+/* Rq2::{add, sub, neg, conjugate} (extension.rs:91-139), coefficient-wise. */
+lzx_status lzx_rq2_add_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                             uint32_t *out_dev, size_t count, void *stream);
+lzx_status lzx_rq2_sub_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                             uint32_t *out_dev, size_t count, void *stream);
+lzx_status lzx_rq2_neg_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev, size_t count,
+                             void *stream);
+lzx_status lzx_rq2_conjugate_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev,
+                                   size_t count, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Ajtai commitment t = A * s over R_q (lattice-commitment/src/ajtai.rs).

@@ -29,6 +29,12 @@ HIPCC_FLAGS = [
 ]
 
 
+# per-source additions: the fused extension-ring kernel unrolls two register
+# NTTs and its slot loops in one body, past the default pragma-unroll budget
+# (a partly unrolled loop would index the register arrays at run time: scratch)
+EXTRA_FLAGS = {"rq2.hip": ["-mllvm", "-pragma-unroll-threshold=200000"]}
+
+
 def _hipcc() -> str:
     for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
         if cand and os.path.exists(cand):
@@ -38,7 +44,8 @@ def _hipcc() -> str:
 
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-I", str(PKG.parent / "include"), "-c", str(src), "-o", str(obj)]
+    cmd = [_hipcc(), *HIPCC_FLAGS, *EXTRA_FLAGS.get(src.name, []), "-I", str(PKG.parent / "include"), "-c", str(src),
+           "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr}")

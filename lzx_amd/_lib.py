@@ -36,6 +36,14 @@ PROTOTYPES = {
     "lzx_ntt_forward_host": (st_t, [vp, vp, sz]),
     "lzx_ntt_inverse_host": (st_t, [vp, vp, sz]),
     "lzx_ring_mul_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    # Quadratic extension ring R_q[Y]/(Y^2+1)
+    "lzx_rq2_mul_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    "lzx_rq2_norm_batch": (st_t, [vp, vp, vp, sz, vp]),
+    "lzx_rq2_inverse_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    "lzx_rq2_add_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    "lzx_rq2_sub_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    "lzx_rq2_neg_batch": (st_t, [vp, vp, vp, sz, vp]),
+    "lzx_rq2_conjugate_batch": (st_t, [vp, vp, vp, sz, vp]),
     # Quadratic-slot ring over ~2^50 moduli
     "lzx_quad_plan_create": (st_t, [u64, u32, u64, u32, C.POINTER(vp)]),
     "lzx_quad_plan_destroy": (None, [vp]),

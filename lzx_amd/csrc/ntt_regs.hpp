@@ -1,0 +1,80 @@
+// One negacyclic NTT held in a lane's VGPRs (the butterflies the batched NTT
+// and the fused extension-ring kernels share); tables as laid out by TabLayout.
+#pragma once
+
+#include "ntt.hpp"
+
+namespace lzx {
+
+__host__ __device__ constexpr uint32_t brv_bits(uint32_t x, uint32_t bits)
+{
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < bits; i++) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t brv_dev(uint32_t x, uint32_t bits)
+{
+    return bits == 0 ? 0 : (__brev(x) >> (32 - bits));
+}
+
+// Forward (merged psi twiddles, CT from natural order, output permuted to
+// natural-order evaluations) and inverse (GS, n^{-1} folded into the last
+// level) of one polynomial held in VGPRs.  a[] is the working order: the
+// forward reads coefficients into a[c]; the inverse reads evaluation k into
+// a[brv(k)].
+template <int LOG_N, int AR, int MODE>
+__device__ __forceinline__ void ntt_regs(uint32_t (&a)[1u << LOG_N], const uint32_t *__restrict__ tab)
+{
+    constexpr uint32_t N = 1u << LOG_N;
+    const QConst Q{tab[0], tab[1], tab[5], tab[6]};
+    if (MODE == NTT_FWD) {
+#pragma unroll
+        for (uint32_t len = N / 2; len >= 1; len >>= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                const uint32_t w = tab[TabLayout::fwd_w(N) + k];
+                const uint32_t wp = tab[TabLayout::fwd_wp(N) + k];
+#pragma unroll
+                for (uint32_t j = s; j < s + len; j++) {
+                    const uint32_t t = mulq<AR>(a[j + len], w, wp, Q);
+                    const uint32_t u = a[j];
+                    a[j] = addq<AR>(u, t, Q);
+                    a[j + len] = subq<AR>(u, t, Q);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t len = 1; len <= N / 2; len <<= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                if (len == N / 2) {
+                    const uint32_t nw = tab[8], nwp = tab[9], lw = tab[10], lwp = tab[11];
+#pragma unroll
+                    for (uint32_t j = s; j < s + len; j++) {
+                        const uint32_t u = a[j], v = a[j + len];
+                        a[j] = mulq<AR>(addq<AR>(u, v, Q), nw, nwp, Q);
+                        a[j + len] = mulq<AR>(subq<AR>(u, v, Q), lw, lwp, Q);
+                    }
+                } else {
+                    const uint32_t w = tab[TabLayout::inv_w(N) + k];
+                    const uint32_t wp = tab[TabLayout::inv_wp(N) + k];
+#pragma unroll
+                    for (uint32_t j = s; j < s + len; j++) {
+                        const uint32_t u = a[j], v = a[j + len];
+                        a[j] = addq<AR>(u, v, Q);
+                        a[j + len] = mulq<AR>(subq<AR>(u, v, Q), w, wp, Q);
+                    }
+                }
+            }
+        }
+    }
+}
+
+} // namespace lzx

@@ -1,0 +1,365 @@
+// Batched arithmetic in R_q[Y]/(Y^2+1) (crates/lattice-ring/src/extension.rs:27-176).
+//
+// An element c0 + c1 Y is stored as [c0 | c1], n canonical u32 coefficients
+// each, so a batch is [count][2][n] (the reference's to_bytes order).
+//
+// rq2_kernel (n = 4 .. 64): one 64-lane wave owns a tile of 32 elements, lane
+// 2e + h holding half h of element e. Both operands stream through LDS tiles
+// whose row l is exactly lane l's polynomial, so global loads and stores are
+// coalesced and the per-lane row reads are bank-conflict free (rows padded to
+// n + 1 words). Each lane transforms its halves in VGPRs (the shared register
+// NTT), combines slot-wise with its partner lane (lane ^ 1, one DPP move per
+// value) and inverts its result half:
+//   mul      c0 = a0 b0 - a1 b1 (even lane), c1 = a0 b1 + a1 b0 (odd lane),
+//            two slot products per lane (in the NTT domain the Karatsuba
+//            saving is one slot product; the result is the same ring element);
+//   norm     N = a0^2 + a1^2 (extension.rs:142-147), written by the even lane;
+//   inverse  conj(z) N(z)^{-1} (extension.rs:151-167) with every slot of N
+//            inverted by one exponentiation (prefix products, Montgomery's
+//            batch inversion); an element with a zero slot has no inverse:
+//            its flag is 0 and its output is zero.
+// Nothing but the inputs and the outputs touches HBM.
+//
+// Other n (1, 2 and 128 .. 8192): the tuned batched NTT launches on scratch
+// copies, rq2_slot_kernel for the slot-wise step, and the inverse launch.
+#include <algorithm>
+
+#include "ntt.hpp"
+#include "ntt_regs.hpp"
+#include "runtime.hpp"
+
+namespace lzx {
+
+enum Rq2Mode { RQ2_MUL = 0, RQ2_NORM = 1, RQ2_INV = 2 };
+enum Rq2Linear { RQ2_ADD = 0, RQ2_SUB = 1, RQ2_NEG = 2, RQ2_CONJ = 3 };
+constexpr uint32_t RQ2_TILE = 32;        // elements per wave tile
+constexpr uint32_t RQ2_MAX_BLOCKS = 8192; // grid-stride beyond this
+
+// the partner lane's value (lane ^ 1): DPP quad_perm [1, 0, 3, 2]
+__device__ __forceinline__ uint32_t rq2_swap(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+}
+
+__device__ __forceinline__ uint32_t rq2_mul(uint32_t a, uint32_t b, uint32_t q, uint64_t barrett)
+{
+    return reduce_u64((uint64_t)a * b, q, barrett);
+}
+
+__device__ __forceinline__ uint32_t rq2_pow(uint32_t a, uint32_t e, uint32_t q, uint64_t barrett)
+{
+    uint32_t acc = 1;
+    for (; e; e >>= 1) {
+        if (e & 1) acc = rq2_mul(acc, a, q, barrett);
+        a = rq2_mul(a, a, q, barrett);
+    }
+    return acc;
+}
+
+// out may alias a (in-place use): each tile is read whole before it is written
+template <int LOG_N, int AR, int MODE>
+__global__ __launch_bounds__(64) void rq2_kernel(const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
+                                                 const uint32_t *__restrict__ tab, uint64_t barrett, size_t count)
+{
+    constexpr uint32_t N = 1u << LOG_N;
+    constexpr uint32_t S = N + 1;
+    __shared__ uint32_t ta[64 * S];
+    __shared__ uint32_t tb[MODE == RQ2_MUL ? 64 * S : 1];
+    const QConst Q{tab[0], tab[1], tab[5], tab[6]};
+    const uint32_t q = Q.q;
+    const uint32_t lane = threadIdx.x, odd = lane & 1;
+    const size_t tiles = (count + RQ2_TILE - 1) / RQ2_TILE;
+    for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const size_t e0 = t * RQ2_TILE;
+        const uint32_t ne = count - e0 < RQ2_TILE ? (uint32_t)(count - e0) : RQ2_TILE;
+        const uint32_t words = 2 * ne * N; // rows 0 .. 2 ne - 1
+        const uint32_t *ga = a + e0 * 2 * N;
+        for (uint32_t i = lane; i < words; i += 64) ta[(i >> LOG_N) * S + (i & (N - 1))] = ga[i];
+        if (MODE == RQ2_MUL) {
+            const uint32_t *gb = b + e0 * 2 * N;
+            for (uint32_t i = lane; i < words; i += 64) tb[(i >> LOG_N) * S + (i & (N - 1))] = gb[i];
+        }
+        __syncthreads();
+        // every lane computes (rows past 2 ne hold stale values and are not stored),
+        // so the partner exchange always sees both lanes of a pair
+        uint32_t x[N];
+        const uint32_t *row = ta + lane * S;
+#pragma unroll
+        for (uint32_t c = 0; c < N; c++) x[c] = row[c];
+        ntt_regs<LOG_N, AR, NTT_FWD>(x, tab); // x[j]: evaluation brv(j), the order the inverse reads
+        bool good = true;
+        if (MODE == RQ2_MUL) {
+            uint32_t y[N];
+            const uint32_t *rb = tb + lane * S;
+#pragma unroll
+            for (uint32_t c = 0; c < N; c++) y[c] = rb[c];
+            ntt_regs<LOG_N, AR, NTT_FWD>(y, tab);
+#pragma unroll
+            for (uint32_t j = 0; j < N; j++) {
+                const uint32_t pa = rq2_swap(x[j]), pb = rq2_swap(y[j]);
+                // even: x y - pa pb = a0 b0 - a1 b1; odd: pa y + x pb = a0 b1 + a1 b0
+                const uint32_t p = rq2_mul(odd ? pa : x[j], y[j], q, barrett);
+                const uint32_t r = rq2_mul(odd ? x[j] : pa, pb, q, barrett);
+                x[j] = odd ? addq<AR>(p, r, Q) : subq<AR>(p, r, Q);
+            }
+        } else if (MODE == RQ2_NORM) {
+#pragma unroll
+            for (uint32_t j = 0; j < N; j++) {
+                const uint32_t s = rq2_mul(x[j], x[j], q, barrett);
+                x[j] = addq<AR>(s, rq2_swap(s), Q);
+            }
+        } else {
+            // pre[j] = N_0 ... N_{j-1}, kept in this lane's tile row (read already)
+            uint32_t *pre = ta + lane * S;
+            uint32_t acc = 1;
+#pragma unroll
+            for (uint32_t j = 0; j < N; j++) {
+                const uint32_t s = rq2_mul(x[j], x[j], q, barrett);
+                pre[j] = acc;
+                acc = rq2_mul(acc, addq<AR>(s, rq2_swap(s), Q), q, barrett);
+            }
+            good = acc != 0; // q is prime: the product vanishes iff a slot does
+            uint32_t inv = rq2_pow(acc, q - 2, q, barrett);
+#pragma unroll
+            for (uint32_t jj = 0; jj < N; jj++) {
+                const uint32_t j = N - 1 - jj;
+                const uint32_t s = rq2_mul(x[j], x[j], q, barrett);
+                const uint32_t nj = addq<AR>(s, rq2_swap(s), Q);
+                const uint32_t ninv = rq2_mul(inv, pre[j], q, barrett);
+                inv = rq2_mul(inv, nj, q, barrett);
+                const uint32_t v = odd ? (x[j] ? q - x[j] : 0) : x[j]; // conj: -c1 on the odd lane
+                x[j] = rq2_mul(v, ninv, q, barrett);
+            }
+        }
+        ntt_regs<LOG_N, AR, NTT_INV>(x, tab);
+        // results back through the tile (every lane has read its rows already)
+        const uint32_t orow = MODE == RQ2_NORM ? lane >> 1 : lane;
+        const bool writes = lane < 2 * ne && (MODE != RQ2_NORM || !odd);
+        if (writes) {
+            uint32_t *w = ta + orow * S;
+#pragma unroll
+            for (uint32_t c = 0; c < N; c++) w[c] = x[c];
+        }
+        if (MODE == RQ2_INV && lane < 2 * ne && !odd) ok[e0 + (lane >> 1)] = good ? 1 : 0;
+        __syncthreads();
+        const uint32_t owords = MODE == RQ2_NORM ? ne * N : words;
+        uint32_t *go = out + e0 * (MODE == RQ2_NORM ? N : 2 * N);
+        for (uint32_t i = lane; i < owords; i += 64) go[i] = ta[(i >> LOG_N) * S + (i & (N - 1))];
+        __syncthreads(); // the next tile overwrites ta
+    }
+}
+
+// Slot-wise step of the composed path on NTT-domain batches: A, B are
+// [count][2][n]; MUL writes C = A B to out ([count][2][n]); NORM writes
+// A0^2 + A1^2 to out ([count][n]); INV_FLAG clears ok[e] when a slot of N
+// vanishes; INV writes ok[e] ? (A0, -A1) N^{-1} : 0 to out.
+enum Rq2Slot { SLOT_MUL = 0, SLOT_NORM = 1, SLOT_INV_FLAG = 2, SLOT_INV = 3 };
+
+__global__ void rq2_slot_kernel(int mode, const uint32_t *__restrict__ A, const uint32_t *__restrict__ B,
+                                uint32_t *__restrict__ out, uint8_t *__restrict__ ok, uint32_t log_n, size_t count,
+                                uint32_t q, uint64_t barrett)
+{
+    const uint32_t n = 1u << log_n;
+    const size_t total = count << log_n;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = i >> log_n, j = i & (n - 1);
+        const uint32_t *ae = A + e * 2 * n;
+        const uint32_t a0 = ae[j], a1 = ae[n + j];
+        if (mode == SLOT_MUL) {
+            const uint32_t *be = B + e * 2 * n;
+            const uint32_t b0 = be[j], b1 = be[n + j];
+            const uint32_t p = rq2_mul(a0, b0, q, barrett), r = rq2_mul(a1, b1, q, barrett);
+            const uint32_t u = rq2_mul(a0, b1, q, barrett), v = rq2_mul(a1, b0, q, barrett);
+            out[e * 2 * n + j] = p >= r ? p - r : p + (q - r);
+            const uint64_t s = (uint64_t)u + v;
+            out[e * 2 * n + n + j] = (uint32_t)(s >= q ? s - q : s);
+            continue;
+        }
+        const uint64_t s = (uint64_t)rq2_mul(a0, a0, q, barrett) + rq2_mul(a1, a1, q, barrett);
+        const uint32_t nj = (uint32_t)(s >= q ? s - q : s);
+        if (mode == SLOT_NORM) {
+            out[i] = nj;
+        } else if (mode == SLOT_INV_FLAG) {
+            if (nj == 0) ok[e] = 0; // every writer stores 0
+        } else {
+            const uint32_t ninv = ok[e] ? rq2_pow(nj, q - 2, q, barrett) : 0;
+            out[e * 2 * n + j] = rq2_mul(a0, ninv, q, barrett);
+            out[e * 2 * n + n + j] = rq2_mul(a1 ? q - a1 : 0, ninv, q, barrett);
+        }
+    }
+}
+
+__global__ void rq2_fill_u8_kernel(uint8_t *__restrict__ p, size_t n, uint8_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// add / sub / neg / conjugate (extension.rs:91-139), coefficient-wise on [count][2][n]
+__global__ void rq2_linear_kernel(int op, const uint32_t *a, const uint32_t *b, uint32_t *out, uint32_t log_n,
+                                  size_t count, uint32_t q)
+{
+    const size_t total = count << (log_n + 1);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t x = a[i];
+        uint32_t r;
+        if (op == RQ2_ADD) {
+            const uint64_t s = (uint64_t)x + b[i];
+            r = (uint32_t)(s >= q ? s - q : s);
+        } else if (op == RQ2_SUB) {
+            const uint32_t y = b[i];
+            r = x >= y ? x - y : x + (q - y);
+        } else {
+            const bool neg = op == RQ2_NEG || ((i >> log_n) & 1); // conjugate negates c1 only
+            r = neg && x ? q - x : x;
+        }
+        out[i] = r;
+    }
+}
+
+static uint32_t rq2_grid(size_t work, uint32_t block)
+{
+    return (uint32_t)std::max<size_t>(1, std::min<size_t>((work + block - 1) / block, 65536));
+}
+
+template <int LOG_N, int AR>
+static void launch_fused(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
+                         const lzx_ntt_plan *plan, size_t count, hipStream_t s)
+{
+    const uint32_t blocks = (uint32_t)std::min<size_t>((count + RQ2_TILE - 1) / RQ2_TILE, RQ2_MAX_BLOCKS);
+    const uint64_t bar = plan->mod.barrett;
+    if (mode == RQ2_MUL)
+        hipLaunchKernelGGL((rq2_kernel<LOG_N, AR, RQ2_MUL>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
+                           plan->dev_tab, bar, count);
+    else if (mode == RQ2_NORM)
+        hipLaunchKernelGGL((rq2_kernel<LOG_N, AR, RQ2_NORM>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
+                           plan->dev_tab, bar, count);
+    else
+        hipLaunchKernelGGL((rq2_kernel<LOG_N, AR, RQ2_INV>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
+                           plan->dev_tab, bar, count);
+}
+
+template <int AR>
+static void dispatch_fused(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
+                           const lzx_ntt_plan *plan, size_t count, hipStream_t s)
+{
+    switch (plan->log_n) {
+    case 2: launch_fused<2, AR>(mode, a, b, out, ok, plan, count, s); break;
+    case 3: launch_fused<3, AR>(mode, a, b, out, ok, plan, count, s); break;
+    case 4: launch_fused<4, AR>(mode, a, b, out, ok, plan, count, s); break;
+    case 5: launch_fused<5, AR>(mode, a, b, out, ok, plan, count, s); break;
+    default: launch_fused<6, AR>(mode, a, b, out, ok, plan, count, s); break;
+    }
+}
+
+// The composed path: scratch copies of the operands, forward launches, the
+// slot-wise kernel into out, the inverse launch on out.
+static lzx_status rq2_composed(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
+                               const lzx_ntt_plan *plan, size_t count, hipStream_t s)
+{
+    const uint32_t n = plan->n, log_n = plan->log_n;
+    const size_t elem = 2 * (size_t)n, nops = mode == RQ2_MUL ? 2 : 1;
+    uint32_t *tmp = nullptr;
+    LZX_HIP(scratch_alloc((void **)&tmp, nops * count * elem * sizeof(uint32_t), s));
+    uint32_t *A = tmp, *B = mode == RQ2_MUL ? tmp + count * elem : nullptr;
+    lzx_status st = LZX_OK;
+    hipError_t e = hipMemcpyAsync(A, a, count * elem * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && B) e = hipMemcpyAsync(B, b, count * elem * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) st = device_fail(e, "extension-ring staging copy");
+    if (st == LZX_OK) st = ntt_launch(&plan, 1, tmp, nops * count * 2, NTT_FWD, 0, s);
+    const uint32_t g = rq2_grid(count * n, 256);
+    const uint64_t bar = plan->mod.barrett;
+    auto slot = [&](int m) {
+        if (st != LZX_OK) return;
+        hipLaunchKernelGGL(rq2_slot_kernel, dim3(g), dim3(256), 0, s, m, A, B, out, ok, log_n, count, plan->q, bar);
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess) st = device_fail(le, "rq2_slot_kernel");
+    };
+    if (mode == RQ2_MUL) {
+        slot(SLOT_MUL);
+    } else if (mode == RQ2_NORM) {
+        slot(SLOT_NORM);
+    } else if (st == LZX_OK) {
+        hipLaunchKernelGGL(rq2_fill_u8_kernel, dim3(rq2_grid(count, 256)), dim3(256), 0, s, ok, count, (uint8_t)1);
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess) st = device_fail(le, "rq2_fill_u8_kernel");
+        slot(SLOT_INV_FLAG);
+        slot(SLOT_INV);
+    }
+    if (st == LZX_OK) st = ntt_launch(&plan, 1, out, count * (mode == RQ2_NORM ? 1 : 2), NTT_INV, 0, s);
+    (void)scratch_free(tmp, s);
+    return st;
+}
+
+static lzx_status rq2_run(int mode, const lzx_ntt_plan *plan, const uint32_t *a, const uint32_t *b, uint32_t *out,
+                          uint8_t *ok, size_t count, void *stream)
+{
+    if (!plan || !a || !out || (mode == RQ2_MUL && !b) || (mode == RQ2_INV && !ok)) return LZX_INVALID_ARGUMENT;
+    if (count == 0) return LZX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (plan->log_n >= 2 && plan->log_n <= 6) {
+        if (plan->arith == ARITH_MONT)
+            dispatch_fused<ARITH_MONT>(mode, a, b, out, ok, plan, count, s);
+        else
+            dispatch_fused<ARITH_SHOUP>(mode, a, b, out, ok, plan, count, s);
+        LZX_LAUNCH_CHECK("rq2_kernel");
+        return LZX_OK;
+    }
+    return rq2_composed(mode, a, b, out, ok, plan, count, s);
+}
+
+static lzx_status rq2_linear(int op, const lzx_ntt_plan *plan, const uint32_t *a, const uint32_t *b, uint32_t *out,
+                             size_t count, void *stream)
+{
+    if (!plan || !a || !out || (op <= RQ2_SUB && !b)) return LZX_INVALID_ARGUMENT;
+    if (count == 0) return LZX_OK;
+    hipLaunchKernelGGL(rq2_linear_kernel, dim3(rq2_grid(count * 2 * plan->n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, op, a, b, out, plan->log_n, count, plan->q);
+    LZX_LAUNCH_CHECK("rq2_linear_kernel");
+    return LZX_OK;
+}
+
+} // namespace lzx
+
+using namespace lzx;
+
+extern "C" lzx_status lzx_rq2_mul_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                                        uint32_t *out_dev, size_t count, void *stream)
+{
+    return rq2_run(RQ2_MUL, plan, a_dev, b_dev, out_dev, nullptr, count, stream);
+}
+
+extern "C" lzx_status lzx_rq2_norm_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev,
+                                         size_t count, void *stream)
+{
+    return rq2_run(RQ2_NORM, plan, a_dev, nullptr, out_dev, nullptr, count, stream);
+}
+
+extern "C" lzx_status lzx_rq2_inverse_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev,
+                                            uint8_t *ok_dev, size_t count, void *stream)
+{
+    return rq2_run(RQ2_INV, plan, a_dev, nullptr, out_dev, ok_dev, count, stream);
+}
+
+extern "C" lzx_status lzx_rq2_add_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                                        uint32_t *out_dev, size_t count, void *stream)
+{
+    return rq2_linear(RQ2_ADD, plan, a_dev, b_dev, out_dev, count, stream);
+}
+
+extern "C" lzx_status lzx_rq2_sub_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
+                                        uint32_t *out_dev, size_t count, void *stream)
+{
+    return rq2_linear(RQ2_SUB, plan, a_dev, b_dev, out_dev, count, stream);
+}
+
+extern "C" lzx_status lzx_rq2_neg_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev,
+                                        size_t count, void *stream)
+{
+    return rq2_linear(RQ2_NEG, plan, a_dev, nullptr, out_dev, count, stream);
+}
+
+extern "C" lzx_status lzx_rq2_conjugate_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev,
+                                              size_t count, void *stream)
+{
+    return rq2_linear(RQ2_CONJ, plan, a_dev, nullptr, out_dev, count, stream);
+}

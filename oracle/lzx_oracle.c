@@ -331,6 +331,79 @@ int or_ring_mul(const or_ntt *t, const uint32_t *a, const uint32_t *b, uint32_t 
 }
 
 /* ------------------------------------------------------------------------ */
+/* R_q[Y]/(Y^2+1): lattice-ring/src/extension.rs:27-176; an element is      */
+/* [c0 | c1], n coefficients each                                             */
+/* ------------------------------------------------------------------------ */
+
+static void or_poly_add(const or_mod32 *m, const uint32_t *a, const uint32_t *b, uint32_t *o, uint32_t n)
+{
+    for (uint32_t i = 0; i < n; i++) o[i] = or_add(m, a[i], b[i]);
+}
+
+static void or_poly_sub(const or_mod32 *m, const uint32_t *a, const uint32_t *b, uint32_t *o, uint32_t n)
+{
+    for (uint32_t i = 0; i < n; i++) o[i] = or_sub(m, a[i], b[i]);
+}
+
+/* Rq2::mul (extension.rs:115-124): p0 = a0 b0, p1 = a1 b1, pc = (a0+a1)(b0+b1);
+ * c0 = p0 - p1, c1 = pc - p0 - p1. */
+int or_rq2_mul(const or_ntt *t, const uint32_t *a, const uint32_t *b, uint32_t *out)
+{
+    const uint32_t n = t->n;
+    uint32_t *w = (uint32_t *)malloc(sizeof(uint32_t) * 5 * n);
+    uint32_t *p0 = w, *p1 = w + n, *pc = w + 2 * n, *sa = w + 3 * n, *sb = w + 4 * n;
+    or_ring_mul(t, a, b, p0);
+    or_ring_mul(t, a + n, b + n, p1);
+    or_poly_add(&t->mod, a, a + n, sa, n);
+    or_poly_add(&t->mod, b, b + n, sb, n);
+    or_ring_mul(t, sa, sb, pc);
+    or_poly_sub(&t->mod, p0, p1, out, n);
+    or_poly_sub(&t->mod, pc, p0, out + n, n);
+    or_poly_sub(&t->mod, out + n, p1, out + n, n);
+    free(w);
+    return OR_OK;
+}
+
+/* Rq2::norm (extension.rs:142-147): c0^2 + c1^2, one R_q element. */
+int or_rq2_norm(const or_ntt *t, const uint32_t *a, uint32_t *out)
+{
+    const uint32_t n = t->n;
+    uint32_t *sq = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    or_ring_mul(t, a, a, out);
+    or_ring_mul(t, a + n, a + n, sq);
+    or_poly_add(&t->mod, out, sq, out, n);
+    free(sq);
+    return OR_OK;
+}
+
+/* Rq2::inverse (extension.rs:151-167): conj(z) N(z)^{-1} with N(z)^{-1} taken
+ * slot-wise in the NTT domain; returns 1 and writes out when every slot of
+ * N(z) is non-zero, else 0 (None) and leaves out untouched. */
+int or_rq2_inverse(const or_ntt *t, const uint32_t *a, uint32_t *out)
+{
+    const uint32_t n = t->n;
+    uint32_t *w = (uint32_t *)malloc(sizeof(uint32_t) * 3 * n);
+    uint32_t *nrm = w, *c1n = w + n, *res = w + 2 * n;
+    or_rq2_norm(t, a, nrm);
+    or_ntt_forward(t, nrm, n);
+    for (uint32_t i = 0; i < n; i++) {
+        if (nrm[i] == 0) {
+            free(w);
+            return 0;
+        }
+        nrm[i] = or_inv(&t->mod, nrm[i]);
+    }
+    or_ntt_inverse(t, nrm, n);
+    for (uint32_t i = 0; i < n; i++) c1n[i] = or_neg(&t->mod, a[n + i]);
+    or_ring_mul(t, a, nrm, res);
+    memcpy(out, res, sizeof(uint32_t) * n);
+    or_ring_mul(t, c1n, nrm, res);
+    memcpy(out + n, res, sizeof(uint32_t) * n);
+    free(w);
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Keccak-f[1600] / SHA3 / SHAKE256: lattice-core/src/keccak.rs:9-373       */
 /* ------------------------------------------------------------------------ */
 

@@ -70,6 +70,9 @@ def lib():
         "or_ntt_forward_batch": (C.c_int, [vp, _u32p, sz]),
         "or_ntt_inverse_batch": (C.c_int, [vp, _u32p, sz]),
         "or_ring_mul": (C.c_int, [vp, _u32p, _u32p, _u32p]),
+        "or_rq2_mul": (C.c_int, [vp, _u32p, _u32p, _u32p]),
+        "or_rq2_norm": (C.c_int, [vp, _u32p, _u32p]),
+        "or_rq2_inverse": (C.c_int, [vp, _u32p, _u32p]),
         "or_keccak_f1600": (None, [_u64p]),
         "or_shake256": (None, [_u8p, sz, C.c_void_p, sz]),
         "or_sha3_256": (None, [_u8p, sz, C.c_void_p]),
@@ -226,6 +229,38 @@ class Ntt:
                               np.ascontiguousarray(b.reshape(-1, self.n)[i]), o)
             out.reshape(-1, self.n)[i] = o
         return out
+
+    # R_q[Y]/(Y^2+1) (extension.rs:115-167): elements are rows [c0 | c1] of 2n
+    def rq2_mul(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, 2 * self.n)
+        b = np.ascontiguousarray(b, dtype=np.uint32).reshape(-1, 2 * self.n)
+        out = np.zeros_like(a)
+        for i in range(a.shape[0]):
+            o = np.zeros(2 * self.n, dtype=np.uint32)
+            lib().or_rq2_mul(self.h, np.ascontiguousarray(a[i]), np.ascontiguousarray(b[i]), o)
+            out[i] = o
+        return out
+
+    def rq2_norm(self, a):
+        a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, 2 * self.n)
+        out = np.zeros((a.shape[0], self.n), dtype=np.uint32)
+        for i in range(a.shape[0]):
+            o = np.zeros(self.n, dtype=np.uint32)
+            lib().or_rq2_norm(self.h, np.ascontiguousarray(a[i]), o)
+            out[i] = o
+        return out
+
+    def rq2_inverse(self, a):
+        """(ok[count], out[count][2n]); rows with ok = 0 (no inverse) are zero."""
+        a = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, 2 * self.n)
+        out = np.zeros_like(a)
+        ok = np.zeros(a.shape[0], dtype=np.uint8)
+        for i in range(a.shape[0]):
+            o = np.zeros(2 * self.n, dtype=np.uint32)
+            ok[i] = lib().or_rq2_inverse(self.h, np.ascontiguousarray(a[i]), o)
+            if ok[i]:
+                out[i] = o
+        return ok, out
 
 
 # ---------------------------------------------------------------------------

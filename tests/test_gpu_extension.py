@@ -1,0 +1,134 @@
+"""GPU parity of the quadratic extension ring R_q[Y]/(Y^2+1)
+(crates/lattice-ring/src/extension.rs:27-443) against the oracle's
+restatement (pinned in test_oracle.py by schoolbook products and the
+reference's cases)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ext():
+    from lzx_amd import device as D
+    from lzx_amd import extension as E
+    from lzx_amd import ring as R
+    D.require_gpu()
+    return E, D, R
+
+
+MODS = {"q32": (3221225473, 30, 5), "q31": (2013265921, 27, 31), "q12289": (12289, 12, 11)}
+
+
+def _setup(R, name, log_n):
+    q, adic, g = MODS[name]
+    ring = R.RingConfig(R.Modulus32(q, adic, g), log_n)
+    return ring, O.Ntt(q, adic, g, log_n), q
+
+
+@pytest.mark.parametrize("name", ["q32", "q31"])
+@pytest.mark.parametrize("log_n,count", [(0, 5), (1, 7), (2, 33), (3, 64), (4, 31), (5, 100), (6, 1000), (7, 9),
+                                         (10, 3)])
+def test_rq2_mul_norm_inverse_match_oracle(ext, name, log_n, count):
+    E, D, R = ext
+    ring, ntt, q = _setup(R, name, log_n)
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n * 100 + count)
+    a = rng.integers(0, q, (count, 2 * n), dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, q, (count, 2 * n), dtype=np.uint64).astype(np.uint32)
+    da, db = D.to_device(a), D.to_device(b)
+    got = D.to_host(E.mul_batch(ring, da, db), np.uint32).reshape(count, 2 * n)
+    assert np.array_equal(got, ntt.rq2_mul(a, b))
+    got = D.to_host(E.norm_batch(ring, da), np.uint32).reshape(count, n)
+    assert np.array_equal(got, ntt.rq2_norm(a))
+    ok, inv = E.inverse_batch(ring, da)
+    wok, winv = ntt.rq2_inverse(a)
+    assert D.to_host(ok, np.uint8).tolist() == wok.tolist()
+    assert np.array_equal(D.to_host(inv, np.uint32).reshape(count, 2 * n), winv)
+    # in place: out aliases the first operand
+    E.mul_batch(ring, da, db, out=da)
+    assert np.array_equal(D.to_host(da, np.uint32).reshape(count, 2 * n), ntt.rq2_mul(a, b))
+
+
+@pytest.mark.parametrize("log_n", [4, 6, 8])
+def test_rq2_inverse_flags_zero_norm_slots(ext, log_n):
+    # over q = 12289 a norm slot vanishes often enough to see both outcomes in one batch
+    E, D, R = ext
+    ring, ntt, q = _setup(R, "q12289", log_n)
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n)
+    a = rng.integers(0, q, (600, 2 * n), dtype=np.uint64).astype(np.uint32)
+    ev = rng.integers(1, q, n, dtype=np.uint64).astype(np.uint32)
+    ev[3] = 0
+    a[0, :n] = ntt.inverse(ev)[1]  # c0 with a zero slot, c1 = 0: N has a zero slot
+    a[0, n:] = 0
+    ok, inv = E.inverse_batch(ring, D.to_device(a))
+    wok, winv = ntt.rq2_inverse(a)
+    okh = D.to_host(ok, np.uint8)
+    assert okh.tolist() == wok.tolist() and okh[0] == 0 and okh.sum() > 0
+    assert np.array_equal(D.to_host(inv, np.uint32).reshape(600, 2 * n), winv)
+    assert not D.to_host(inv, np.uint32).reshape(600, 2 * n)[okh == 0].any()
+
+
+def test_rq2_linear_ops(ext):
+    E, D, R = ext
+    ring, _ntt, q = _setup(R, "q32", 6)
+    rng = np.random.default_rng(2)
+    a = rng.integers(0, q, (77, 128), dtype=np.uint64)
+    b = rng.integers(0, q, (77, 128), dtype=np.uint64)
+    da, db = D.to_device(a.astype(np.uint32)), D.to_device(b.astype(np.uint32))
+    h = lambda t: D.to_host(t, np.uint32).reshape(77, 128).astype(np.uint64)
+    assert np.array_equal(h(E.add_batch(ring, da, db)), (a + b) % q)
+    assert np.array_equal(h(E.sub_batch(ring, da, db)), (a + q - b) % q)
+    assert np.array_equal(h(E.neg_batch(ring, da)), (q - a) % q)
+    conj = a.copy()
+    conj[:, 64:] = (q - conj[:, 64:]) % q
+    assert np.array_equal(h(E.conjugate_batch(ring, da)), conj)
+
+
+def test_rq2_reference_cases(ext):
+    # extension.rs:263-391 through the element API
+    E, D, R = ext
+    ring, ntt, q = _setup(R, "q32", 6)
+    rng = np.random.default_rng(7)
+    el = lambda: E.Rq2(R.RingElement(ring, rng.integers(0, q, 64, dtype=np.uint64).astype(np.uint32)),
+                       R.RingElement(ring, rng.integers(0, q, 64, dtype=np.uint64).astype(np.uint32)))
+    y2 = E.Rq2.y(ring).square()
+    assert y2.c0.coeffs[0] == q - 1 and not y2.c0.coeffs[1:].any() and y2.c1.is_zero()
+    a, b, c = el(), el(), el()
+    assert a.mul(b) == b.mul(a)
+    assert a.mul(b).mul(c) == a.mul(b.mul(c))
+    assert a.mul(b.add(c)) == a.mul(b).add(a.mul(c))
+    assert a.mul(E.Rq2.one(ring)) == a and a.mul(E.Rq2.zero(ring)).is_zero() and a.add(a.neg()).is_zero()
+    assert a.conjugate().conjugate() == a
+    zc = a.mul(a.conjugate())
+    assert zc.c1.is_zero() and zc.c0 == a.norm()
+    assert a.mul(b).norm() == a.norm().mul(b.norm())
+    inv = a.inverse()
+    assert inv is not None and a.mul(inv) == E.Rq2.one(ring)
+    assert E.Rq2.zero(ring).inverse() is None
+    assert a.to_bytes() == a.c0.to_bytes() + a.c1.to_bytes()
+    s0 = a.c0.mul(b.c0).coeffs.astype(np.int64) - a.c1.mul(b.c1).coeffs
+    s1 = a.c0.mul(b.c1).coeffs.astype(np.uint64) + a.c1.mul(b.c0).coeffs
+    m = a.mul(b)
+    assert m.c0.coeffs.tolist() == (s0 % q).tolist() and m.c1.coeffs.tolist() == (s1 % q).tolist()
+    from lzx_amd._lib import LzxError
+    with pytest.raises(LzxError) as e:
+        E.Rq2(a.c0, R.RingElement(R.RingConfig(R.Modulus32.Q_32, 5), np.zeros(32, dtype=np.uint32)))
+    assert e.value.code == 3
+
+
+def test_rq2_full_batch_sampled(ext):
+    # 2^17 elements at n = 64 (64 MiB per operand): 512 sampled rows against the oracle
+    E, D, R = ext
+    ring, ntt, q = _setup(R, "q32", 6)
+    count = 1 << 17
+    rng = np.random.default_rng(17)
+    a = rng.integers(0, q, (count, 128), dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, q, (count, 128), dtype=np.uint64).astype(np.uint32)
+    got = D.to_host(E.mul_batch(ring, D.to_device(a), D.to_device(b)), np.uint32).reshape(count, 128)
+    rows = np.sort(rng.choice(count, 512, replace=False))
+    rows[-1] = count - 1
+    assert np.array_equal(got[rows], ntt.rq2_mul(a[rows], b[rows]))

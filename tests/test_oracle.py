@@ -795,3 +795,94 @@ def test_batch_star_rounds_closed_form():
                     g = _vp_eval(vp, r[:nv]) * (1 << (N - 1 - t)) % O.GL_P
                 want = (want + int(rho) * g) % O.GL_P
             assert int(out["rounds"][t, X]) == want, (t, X)
+
+
+# ---- R_q[Y]/(Y^2+1) (extension.rs:27-176) ----------------------------------------
+
+def _negacyclic(q, a, b):
+    """a(X) b(X) mod (X^n + 1, q) by Python integers."""
+    n = len(a)
+    out = [0] * n
+    for i in range(n):
+        for j in range(n):
+            k, v = i + j, int(a[i]) * int(b[j])
+            if k < n:
+                out[k] += v
+            else:
+                out[k - n] -= v
+    return np.array([x % q for x in out], dtype=np.uint32)
+
+
+def _rq2_elem(rng, q, n):
+    return rng.integers(0, q, 2 * n, dtype=np.uint64).astype(np.uint32)
+
+
+def _rq2_small(q, n, tag):
+    """extension.rs:425-441 helper: xof("rq2-test", tag, 2n) pairs, u16 mod 5 - 2."""
+    b = O.xof(b"rq2-test", tag, 2 * n)
+    return np.array([(int.from_bytes(b[2 * i: 2 * i + 2], "little") % 5 - 2) % q for i in range(n)], dtype=np.uint32)
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 3, 6])
+def test_rq2_mul_closed_form(log_n):
+    # (a0 + a1 Y)(b0 + b1 Y) = (a0 b0 - a1 b1) + (a0 b1 + a1 b0) Y by schoolbook products (extension.rs:366-391)
+    q, n = O.Q_32[0], 1 << log_n
+    ntt = O.Ntt(*O.Q_32, log_n)
+    rng = np.random.default_rng(log_n)
+    for _ in range(3):
+        a, b = _rq2_elem(rng, q, n), _rq2_elem(rng, q, n)
+        got = ntt.rq2_mul(a, b)[0]
+        c0 = (_negacyclic(q, a[:n], b[:n]).astype(np.int64) - _negacyclic(q, a[n:], b[n:])) % q
+        c1 = (_negacyclic(q, a[:n], b[n:]).astype(np.int64) + _negacyclic(q, a[n:], b[:n])) % q
+        assert got[:n].tolist() == c0.tolist() and got[n:].tolist() == c1.tolist()
+        nrm = (_negacyclic(q, a[:n], a[:n]).astype(np.int64) + _negacyclic(q, a[n:], a[n:])) % q
+        assert ntt.rq2_norm(a)[0].tolist() == nrm.tolist()
+
+
+def test_rq2_reference_cases():
+    # extension.rs:263-366: Y^2 = -1, ring axioms, conjugation and the multiplicative norm, inverse via the norm
+    q, n = O.Q_32[0], 64
+    ntt = O.Ntt(*O.Q_32, 6)
+    rng = np.random.default_rng(64)
+    y = np.zeros(2 * n, dtype=np.uint32)
+    y[n] = 1
+    y2 = ntt.rq2_mul(y, y)[0]
+    assert y2[0] == q - 1 and not y2[1:].any()
+    one = np.zeros(2 * n, dtype=np.uint32)
+    one[0] = 1
+    a, b, c = (_rq2_elem(rng, q, n) for _ in range(3))
+    mul = lambda x, y_: ntt.rq2_mul(x, y_)[0]
+    add = lambda x, y_: ((x.astype(np.uint64) + y_) % q).astype(np.uint32)
+    assert mul(a, b).tolist() == mul(b, a).tolist()
+    assert mul(mul(a, b), c).tolist() == mul(a, mul(b, c)).tolist()
+    assert mul(a, add(b, c)).tolist() == add(mul(a, b), mul(a, c)).tolist()
+    assert mul(a, one).tolist() == a.tolist() and not mul(a, np.zeros_like(a)).any()
+    conj = a.copy()
+    conj[n:] = (q - conj[n:].astype(np.uint64)) % q
+    zc = mul(a, conj)
+    assert not zc[n:].any() and zc[:n].tolist() == ntt.rq2_norm(a)[0].tolist()
+    assert ntt.rq2_norm(mul(a, b))[0].tolist() == ntt.ring_mul(ntt.rq2_norm(a)[0], ntt.rq2_norm(b)[0]).tolist()
+    small = np.concatenate([_rq2_small(q, n, b"i0"), _rq2_small(q, n, b"i1")])
+    ok, inv = ntt.rq2_inverse(small)
+    if ok[0]:
+        assert mul(small, inv[0]).tolist() == one.tolist()
+    ok, inv = ntt.rq2_inverse(a)  # a random element: its norm has a zero slot with probability ~n/q
+    assert ok[0] == 1 and mul(a, inv[0]).tolist() == one.tolist()
+    assert ntt.rq2_inverse(np.zeros(2 * n, dtype=np.uint32))[0][0] == 0
+
+
+def test_rq2_inverse_refuses_a_zero_norm_slot():
+    # N(z) with one NTT slot 0 has no slot-wise inverse (extension.rs:151-167 returns None)
+    q, n = O.Q_32[0], 16
+    ntt = O.Ntt(*O.Q_32, 4)
+    rng = np.random.default_rng(3)
+    ev = rng.integers(1, q, n, dtype=np.uint64).astype(np.uint32)
+    ev[5] = 0
+    _, c0 = ntt.inverse(ev)  # c0 has a zero evaluation, so N = c0^2 + 0 has one too
+    z = np.concatenate([c0, np.zeros(n, dtype=np.uint32)])
+    assert ntt.rq2_inverse(z)[0][0] == 0
+    z[n] = 1  # N = c0^2 + 1: slot 5 becomes 1, and the element is invertible unless another slot hits -1
+    ok, inv = ntt.rq2_inverse(z)
+    one = np.zeros(2 * n, dtype=np.uint32)
+    one[0] = 1
+    assert ok[0] == 1 and ntt.rq2_mul(z, inv[0])[0].tolist() == one.tolist()
