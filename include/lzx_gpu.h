@@ -15,7 +15,7 @@
  *     zerocheck, batching, split eq  zerocheck.rs, batch.rs, fastprover.rs
  *
  * Conventions
- *   - Every entry point returns lzx_status; values 1..29 and 40..58 map
+ *   - Every entry point returns lzx_status; values 1..29 and 40..62 map
  *     one-to-one onto the reference's error enums (noted per code), 30, 31 and
  *     100.. are ABI/device codes.
  *     Nothing unwinds across the ABI.
@@ -68,6 +68,9 @@ enum {
     LZX_DECOMP_DIGIT_OUT_OF_RANGE = 56, /* DecompositionError::DigitOutOfRange decomposition.rs:111 */
     LZX_DECOMP_VALUE_OUT_OF_RANGE = 57, /* DecompositionError::ValueOutOfRange decomposition.rs:112 */
     LZX_DECOMP_VALUE_TOO_LARGE = 58,    /* DecompositionError::ValueTooLarge   decomposition.rs:113 */
+    LZX_SHORT_CHALLENGE_REJECTION_BUDGET = 60, /* ShortChallengeError::RejectionBudgetExceeded short_challenge.rs:84 */
+    LZX_SHORT_CHALLENGE_INVALID_PARAMETERS = 61, /* ShortChallengeError::InvalidParameters short_challenge.rs:86 */
+    LZX_SHORT_CHALLENGE_GAMMA_CAP_EXCEEDED = 62, /* ShortChallengeError::GammaCapExceeded short_challenge.rs:88 */
     LZX_TRANSCRIPT_REJECTION_BUDGET = 20, /* TranscriptError::RejectionBudgetExceeded transcript.rs:16 */
     LZX_TRANSCRIPT_MESSAGE_TOO_LARGE = 21, /* TranscriptError::MessageTooLarge transcript.rs:18 */
     LZX_INVALID_ARGUMENT = 30,          /* null handle/pointer or malformed shape */
@@ -153,6 +156,46 @@ lzx_status lzx_rq2_norm_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, u
  * out[i] = 0. */
 lzx_status lzx_rq2_inverse_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, uint32_t *out_dev,
                                  uint8_t *ok_dev, size_t count, void *stream);
+
+/* Short ring-element challenges (lattice-core/src/short_challenge.rs:47-89):
+ * ShortChallengeFamily as `family` with its parameters; unused fields are ignored. */
+enum {
+    LZX_SC_FIXED_WEIGHT = 0,           /* weight, amplitude */
+    LZX_SC_BIASED_TERNARY = 1,         /* p_nonzero_permille */
+    LZX_SC_SMALL_SET = 2,              /* values[nvalues] */
+    LZX_SC_FIXED_WEIGHT_SMALL_SET = 3  /* weight, values[nvalues] */
+};
+typedef struct {
+    size_t n;
+    int32_t family;
+    size_t weight;
+    int64_t amplitude;
+    uint32_t p_nonzero_permille;
+    const int64_t *values;
+    size_t nvalues;
+} lzx_short_challenge_spec;
+
+/* ShortChallengeSpec::sample (short_challenge.rs:183-205): coeffs_out[n]
+ * signed coefficients and *gamma_out = Γ_C = ceil(sqrt(sum c_i^2)).  Host code. */
+lzx_status lzx_short_challenge_sample(const lzx_short_challenge_spec *spec, const uint8_t *seed, size_t seed_len,
+                                      int64_t *coeffs_out, uint64_t *gamma_out);
+
+/* ShortChallengeSpec::sample_with_gamma_cap (short_challenge.rs:208-236):
+ * tries seed || u64le(attempt) for attempt = 0 .. retries until Γ_C <= cap;
+ * LZX_SHORT_CHALLENGE_GAMMA_CAP_EXCEEDED (with the last Γ in *gamma_out) otherwise. */
+lzx_status lzx_short_challenge_sample_with_gamma_cap(const lzx_short_challenge_spec *spec, const uint8_t *seed,
+                                                     size_t seed_len, uint64_t cap, size_t retries,
+                                                     int64_t *coeffs_out, uint64_t *gamma_out);
+
+/* sample_rq2_challenge (extension.rs:180-216) for RingConfig(q, log_n):
+ * spec->n must be 2n; out_host[2n] = [c0 | c1] (even stream positions to c0,
+ * odd to c1, reduced mod q); *g0, *g1 the per-half Γ.  Host code. */
+lzx_status lzx_rq2_sample_challenge(uint32_t q, uint32_t log_n, const lzx_short_challenge_spec *spec,
+                                    const uint8_t *seed, size_t seed_len, uint32_t *out_host, uint64_t *g0,
+                                    uint64_t *g1);
+
+/* ceil_sqrt_u128 (short_challenge.rs:293-340): min(ceil(sqrt(hi 2^64 + lo)), 2^64 - 1). */
+uint64_t lzx_ceil_sqrt_u128(uint64_t lo, uint64_t hi);
 
 /* Rq2::{add, sub, neg, conjugate} (extension.rs:91-139), coefficient-wise. */
 lzx_status lzx_rq2_add_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,

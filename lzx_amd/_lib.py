@@ -40,6 +40,10 @@ PROTOTYPES = {
     "lzx_rq2_mul_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
     "lzx_rq2_norm_batch": (st_t, [vp, vp, vp, sz, vp]),
     "lzx_rq2_inverse_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
+    "lzx_short_challenge_sample": (st_t, [vp, vp, sz, vp, C.POINTER(u64)]),
+    "lzx_short_challenge_sample_with_gamma_cap": (st_t, [vp, vp, sz, u64, sz, vp, C.POINTER(u64)]),
+    "lzx_rq2_sample_challenge": (st_t, [u32, u32, vp, vp, sz, vp, C.POINTER(u64), C.POINTER(u64)]),
+    "lzx_ceil_sqrt_u128": (u64, [u64, u64]),
     "lzx_rq2_add_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
     "lzx_rq2_sub_batch": (st_t, [vp, vp, vp, vp, sz, vp]),
     "lzx_rq2_neg_batch": (st_t, [vp, vp, vp, sz, vp]),

@@ -113,6 +113,9 @@ extern "C" const char *lzx_status_string(lzx_status s)
     case LZX_BATCH_VARIABLE_COUNT_MISMATCH: return "BatchError::VariableCountMismatch";
     case LZX_BATCH_EMPTY: return "BatchError::EmptyBatch";
     case LZX_BATCH_CHALLENGE_COUNT: return "BatchError::ChallengeCount";
+    case LZX_SHORT_CHALLENGE_REJECTION_BUDGET: return "ShortChallengeError::RejectionBudgetExceeded";
+    case LZX_SHORT_CHALLENGE_INVALID_PARAMETERS: return "ShortChallengeError::InvalidParameters";
+    case LZX_SHORT_CHALLENGE_GAMMA_CAP_EXCEEDED: return "ShortChallengeError::GammaCapExceeded";
     case LZX_MOD50_LOG_N_OUT_OF_RANGE: return "Mod50Error::LogNOutOfRange";
     case LZX_MOD50_ADICITY_TOO_SMALL: return "Mod50Error::AdicityTooSmall";
     case LZX_MOD50_GENERATOR_INVALID: return "Mod50Error::GeneratorInvalid";

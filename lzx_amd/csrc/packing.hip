@@ -43,7 +43,52 @@ __global__ __launch_bounds__(PK_BLOCK) void pack_field_kernel(const uint64_t *__
             const uint64_t raw = v[g / 3], j = g % 3;
             l = j == 0 ? raw & PK_LIMB_MASK : j == 1 ? (raw >> PK_LIMB_BITS) & PK_LIMB_MASK : raw >> (2 * PK_LIMB_BITS);
         }
-        out[g] = (uint32_t)(l % q);
+        out[g] = (uint32_t)(l < q ? l : l % q); // limbs are < 2^22: reduced only for small q
+    }
+}
+
+// Four values per thread: two 16-byte loads, three 16-byte stores of their 12
+// limbs (output words 12 t .. 12 t + 11); words past the last full group of 12
+// take the one-word form.  v and out are 16-byte aligned.
+__global__ __launch_bounds__(PK_BLOCK) void pack_field_wide_kernel(const uint64_t *__restrict__ v, size_t count,
+                                                                   uint32_t q, uint32_t *__restrict__ out,
+                                                                   size_t total)
+{
+    const size_t groups = total / 12;
+    PK_FOR(t, groups)
+    {
+        uint64_t x[4];
+        if (4 * t + 3 < count) {
+            const ulonglong2 a = ((const ulonglong2 *)v)[2 * t], b = ((const ulonglong2 *)v)[2 * t + 1];
+            x[0] = a.x, x[1] = a.y, x[2] = b.x, x[3] = b.y;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) x[k] = 4 * t + k < count ? v[4 * t + k] : 0;
+        }
+        uint32_t w[12];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            w[3 * k] = (uint32_t)(x[k] & PK_LIMB_MASK);
+            w[3 * k + 1] = (uint32_t)((x[k] >> PK_LIMB_BITS) & PK_LIMB_MASK);
+            w[3 * k + 2] = (uint32_t)(x[k] >> (2 * PK_LIMB_BITS));
+        }
+        if (q <= PK_LIMB_MASK) // limbs are < 2^22: reduced only for small q
+#pragma unroll
+            for (int k = 0; k < 12; k++) w[k] %= q;
+        uint4 *o = (uint4 *)(out + 12 * t);
+        o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        o[2] = make_uint4(w[8], w[9], w[10], w[11]);
+    }
+    PK_FOR(r, total - 12 * groups)
+    {
+        const size_t g = 12 * groups + r;
+        uint64_t l = 0;
+        if (g < 3 * count) {
+            const uint64_t raw = v[g / 3], j = g % 3;
+            l = j == 0 ? raw & PK_LIMB_MASK : j == 1 ? (raw >> PK_LIMB_BITS) & PK_LIMB_MASK : raw >> (2 * PK_LIMB_BITS);
+        }
+        out[g] = (uint32_t)(l < q ? l : l % q);
     }
 }
 
@@ -82,6 +127,50 @@ __global__ __launch_bounds__(PK_BLOCK) void pack_bits_kernel(const uint8_t *__re
     }
 }
 
+// The same packing from 16-byte loads: a block owns chunks of PK_BLOCK
+// coefficients (31 * 256 = 7936 input bytes = 496 aligned 16-byte pieces,
+// since the chunk start 7936 c is 16-byte aligned), compresses each piece to a
+// 16-bit mask in LDS, and each thread cuts its 31-bit window out of the masks.
+constexpr uint32_t PB_BYTES = PK_BLOCK * PK_BITS_PER_COEFF; // 7936
+constexpr uint32_t PB_PIECES = PB_BYTES / 16;               // 496
+
+__device__ __forceinline__ uint32_t nz4(uint32_t w) // bit t = (byte t != 0)
+{
+    return ((w & 0xFFu) != 0) | (((w >> 8) & 0xFFu) != 0) << 1 | (((w >> 16) & 0xFFu) != 0) << 2 |
+           ((w >> 24) != 0) << 3;
+}
+
+__global__ __launch_bounds__(PK_BLOCK) void pack_bits_wide_kernel(const uint8_t *__restrict__ bits, size_t nbits,
+                                                                  uint32_t *__restrict__ out, size_t total)
+{
+    __shared__ uint16_t m16[PB_PIECES + 1]; // + 1 zero piece: the last window reads one past its bits
+    const size_t chunks = (total + PK_BLOCK - 1) / PK_BLOCK;
+    if (threadIdx.x == 0) m16[PB_PIECES] = 0;
+    for (size_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const size_t b0 = c * PB_BYTES;
+        for (uint32_t k = threadIdx.x; k < PB_PIECES; k += PK_BLOCK) {
+            const size_t off = b0 + 16 * (size_t)k;
+            uint32_t m = 0;
+            if (off + 16 <= nbits) {
+                const uint4 v = *(const uint4 *)(bits + off);
+                m = nz4(v.x) | nz4(v.y) << 4 | nz4(v.z) << 8 | nz4(v.w) << 12;
+            } else {
+                for (uint32_t t = 0; t < 16; t++)
+                    if (off + t < nbits && bits[off + t]) m |= 1u << t;
+            }
+            m16[k] = (uint16_t)m;
+        }
+        __syncthreads();
+        const size_t g = c * PK_BLOCK + threadIdx.x;
+        if (g < total) {
+            const uint32_t bit = threadIdx.x * PK_BITS_PER_COEFF, k = bit >> 4, sh = bit & 15;
+            const uint64_t w = (uint64_t)m16[k] | (uint64_t)m16[k + 1] << 16 | (uint64_t)m16[k + 2] << 32;
+            out[g] = (uint32_t)(w >> sh) & 0x7FFFFFFFu;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(PK_BLOCK) void unpack_bits_kernel(const uint32_t *__restrict__ el, size_t nelems,
                                                                uint32_t n, uint8_t *__restrict__ bits, size_t nbits)
 {
@@ -93,23 +182,58 @@ __global__ __launch_bounds__(PK_BLOCK) void unpack_bits_kernel(const uint32_t *_
     }
 }
 
-// balanced digits: d = r if r <= b/2 else r - b; the next remainder is
-// (rem - d) / b = rem / b, plus one when d borrowed b (exact in u64)
-template <typename DT>
-__global__ __launch_bounds__(PK_BLOCK) void decompose_kernel(const uint64_t *__restrict__ x, size_t count, uint64_t base,
-                                                             uint32_t nd, DT *__restrict__ digits, uint32_t *flag)
+// x = q b + r by a Barrett estimate: with m = floor((2^64 - 1) / b), the
+// high product floor(x m / 2^64) is floor(x / b) or at most 2 below it.
+__device__ __forceinline__ uint64_t div_barrett(uint64_t x, uint64_t b, uint64_t m, uint64_t &r)
 {
+    uint64_t q = __umul64hi(x, m);
+    r = x - q * b;
+    if (r >= b) r -= b, q++;
+    if (r >= b) r -= b, q++;
+    return q;
+}
+
+// balanced digits: d = r if r <= b/2 else r - b; the next remainder is
+// (rem - d) / b = rem / b, plus one when d borrowed b (exact in u64).
+// STAGE: a block's PK_BLOCK values stage their digits in LDS (dynamic, one
+// PK_BLOCK * nd * sizeof(DT) tile) and leave as coalesced 4-byte stores.
+// POW2: base = 2^m, so the quotient and remainder are a shift and a mask.
+template <typename DT, bool STAGE, bool POW2>
+__global__ __launch_bounds__(PK_BLOCK) void decompose_kernel(const uint64_t *__restrict__ x, size_t count, uint64_t base,
+                                                             uint64_t m, uint32_t nd, DT *__restrict__ digits,
+                                                             uint32_t *flag)
+{
+    extern __shared__ uint8_t pk_stage[];
     const uint64_t half = base / 2;
-    PK_FOR(i, count)
-    {
-        uint64_t rem = x[i];
-        for (uint32_t k = 0; k < nd; k++) {
-            const uint64_t r = rem % base;
-            const bool borrow = r > half;
-            digits[i * nd + k] = (DT)(borrow ? (int64_t)r - (int64_t)base : (int64_t)r);
-            rem = rem / base + (borrow ? 1 : 0);
+    const size_t chunks = (count + PK_BLOCK - 1) / PK_BLOCK;
+    for (size_t c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const size_t i = c * PK_BLOCK + threadIdx.x;
+        DT *dst = STAGE ? (DT *)pk_stage + (size_t)threadIdx.x * nd : digits + i * nd;
+        if (i < count) {
+            uint64_t rem = x[i];
+            for (uint32_t k = 0; k < nd; k++) {
+                uint64_t r, qt;
+                if (POW2)
+                    r = rem & (base - 1), qt = rem >> m;
+                else
+                    qt = div_barrett(rem, base, m, r);
+                const bool borrow = r > half;
+                dst[k] = (DT)(borrow ? (int64_t)r - (int64_t)base : (int64_t)r);
+                rem = qt + (borrow ? 1 : 0);
+            }
+            if (rem) atomicOr(flag, 1u); // DecompositionError::ValueTooLarge
         }
-        if (rem) atomicOr(flag, 1u); // DecompositionError::ValueTooLarge
+        if (STAGE) {
+            __syncthreads();
+            const size_t nv = count - c * PK_BLOCK < PK_BLOCK ? count - c * PK_BLOCK : PK_BLOCK;
+            const size_t bytes = nv * nd * sizeof(DT);
+            // 4-byte aligned: the base is, and PK_BLOCK * nd * sizeof(DT) is a multiple of 4
+            uint8_t *g = (uint8_t *)(digits + c * PK_BLOCK * nd);
+            const uint32_t *s4 = (const uint32_t *)pk_stage;
+            for (size_t w = threadIdx.x; w < bytes / 4; w += PK_BLOCK) ((uint32_t *)g)[w] = s4[w];
+            for (size_t b = (bytes & ~(size_t)3) + threadIdx.x; b < bytes; b += PK_BLOCK) g[b] = pk_stage[b];
+            __syncthreads();
+        }
     }
 }
 
@@ -185,14 +309,18 @@ __global__ __launch_bounds__(PK_BLOCK) void scatter_set_kernel(uint32_t *__restr
     PK_FOR(i, nnz) s[idx[i]] = val[i];
 }
 
-// flag word: allocated, cleared, launched into, read back
+// flag word: allocated, cleared, launched into, read back through a pinned
+// staging word (a pageable copy would be a slower synchronous copy)
 struct Flag {
     uint32_t *dev = nullptr;
+    HostStage hs{};
+    bool inflight = false;
     hipStream_t s;
     explicit Flag(hipStream_t st) : s(st) {}
     lzx_status init()
     {
         hipError_t e = scratch_alloc((void **)&dev, sizeof(uint32_t), s);
+        if (e == hipSuccess) e = stage_acquire(hs, sizeof(uint32_t));
         if (e == hipSuccess) {
             hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(PK_BLOCK), 0, s, dev, (size_t)1);
             e = hipGetLastError();
@@ -201,13 +329,17 @@ struct Flag {
     }
     lzx_status read(uint32_t *v)
     {
-        hipError_t e = hipMemcpyAsync(v, dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        hipError_t e = hipMemcpyAsync(hs.p, dev, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        inflight = e == hipSuccess;
         if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess) inflight = false, *v = *(const uint32_t *)hs.p;
         return e == hipSuccess ? LZX_OK : device_fail(e, "packing flag readback");
     }
     ~Flag()
     {
         if (dev) (void)scratch_free(dev, s);
+        if (inflight) (void)hipStreamSynchronize(s); // the pinned word may still be written
+        stage_release(hs);
     }
 };
 
@@ -225,8 +357,12 @@ extern "C" lzx_status lzx_pack_field_elements(const uint64_t *values_dev, size_t
 {
     if (!out_dev || (count && !values_dev) || n == 0 || q < 2) return LZX_INVALID_ARGUMENT;
     const size_t total = lzx_pack_field_elements_len(count, n) * n;
-    hipLaunchKernelGGL(pack_field_kernel, dim3(pk_grid(total)), dim3(PK_BLOCK), 0, (hipStream_t)stream, values_dev,
-                       count, q, out_dev, total);
+    if ((uintptr_t)values_dev % 16 == 0 && (uintptr_t)out_dev % 16 == 0)
+        hipLaunchKernelGGL(pack_field_wide_kernel, dim3(pk_grid(total / 12 + 1)), dim3(PK_BLOCK), 0,
+                           (hipStream_t)stream, values_dev, count, q, out_dev, total);
+    else
+        hipLaunchKernelGGL(pack_field_kernel, dim3(pk_grid(total)), dim3(PK_BLOCK), 0, (hipStream_t)stream,
+                           values_dev, count, q, out_dev, total);
     LZX_LAUNCH_CHECK("pack_field_kernel");
     return LZX_OK;
 }
@@ -260,8 +396,12 @@ extern "C" lzx_status lzx_pack_bits(const uint8_t *bits_dev, size_t nbits, uint3
 {
     if (!out_dev || (nbits && !bits_dev) || n == 0) return LZX_INVALID_ARGUMENT;
     const size_t total = lzx_pack_bits_len(nbits, n) * n;
-    hipLaunchKernelGGL(pack_bits_kernel, dim3(pk_grid(total)), dim3(PK_BLOCK), 0, (hipStream_t)stream, bits_dev, nbits,
-                       out_dev, total);
+    if ((uintptr_t)bits_dev % 16 == 0)
+        hipLaunchKernelGGL(pack_bits_wide_kernel, dim3(pk_grid(total)), dim3(PK_BLOCK), 0, (hipStream_t)stream,
+                           bits_dev, nbits, out_dev, total);
+    else
+        hipLaunchKernelGGL(pack_bits_kernel, dim3(pk_grid(total)), dim3(PK_BLOCK), 0, (hipStream_t)stream, bits_dev,
+                           nbits, out_dev, total);
     LZX_LAUNCH_CHECK("pack_bits_kernel");
     return LZX_OK;
 }
@@ -288,12 +428,35 @@ extern "C" lzx_status lzx_gadget_decompose(const uint64_t *x_dev, size_t count, 
     Flag f(s);
     lzx_status st = f.init();
     if (st != LZX_OK) return st;
-    if (digit_bytes == 1)
-        hipLaunchKernelGGL(decompose_kernel<int8_t>, dim3(pk_grid(count)), dim3(PK_BLOCK), 0, s, x_dev, count, base,
-                           num_digits, (int8_t *)digits_dev, f.dev);
+    const bool pow2 = (base & (base - 1)) == 0;
+    uint64_t m = UINT64_MAX / base; // Barrett multiplier, or the shift for a power of two
+    if (pow2)
+        for (m = 0; (1ULL << m) != base; m++) {
+        }
+    const size_t stage = (size_t)PK_BLOCK * num_digits * digit_bytes;
+    const bool staged = stage <= 32768 && (uintptr_t)digits_dev % 4 == 0;
+    const dim3 g(pk_grid(count));
+    const size_t sh = staged ? stage : 0;
+#define PK_DECOMPOSE(DT, ST, P2)                                                                                    \
+    hipLaunchKernelGGL((decompose_kernel<DT, ST, P2>), g, dim3(PK_BLOCK), sh, s, x_dev, count, base, m, num_digits,    \
+                       (DT *)digits_dev, f.dev)
+    if (digit_bytes == 1 && staged && pow2)
+        PK_DECOMPOSE(int8_t, true, true);
+    else if (digit_bytes == 1 && staged)
+        PK_DECOMPOSE(int8_t, true, false);
+    else if (digit_bytes == 1 && pow2)
+        PK_DECOMPOSE(int8_t, false, true);
+    else if (digit_bytes == 1)
+        PK_DECOMPOSE(int8_t, false, false);
+    else if (staged && pow2)
+        PK_DECOMPOSE(int64_t, true, true);
+    else if (staged)
+        PK_DECOMPOSE(int64_t, true, false);
+    else if (pow2)
+        PK_DECOMPOSE(int64_t, false, true);
     else
-        hipLaunchKernelGGL(decompose_kernel<int64_t>, dim3(pk_grid(count)), dim3(PK_BLOCK), 0, s, x_dev, count, base,
-                           num_digits, (int64_t *)digits_dev, f.dev);
+        PK_DECOMPOSE(int64_t, false, false);
+#undef PK_DECOMPOSE
     LZX_LAUNCH_CHECK("decompose_kernel");
     uint32_t v = 0;
     st = f.read(&v);

@@ -166,3 +166,16 @@ class Rq2:
 
     def __eq__(self, other) -> bool:
         return isinstance(other, Rq2) and self.c0 == other.c0 and self.c1 == other.c1
+
+
+def sample_rq2_challenge(ring: RingConfig, spec, seed: bytes):
+    """sample_rq2_challenge (extension.rs:180-216): (Rq2, g0, g1); spec.n must be 2n."""
+    import ctypes as C
+
+    n = ring.n()
+    s = spec._c()
+    out = np.zeros(2 * n, dtype=np.uint32)
+    g0, g1 = C.c_uint64(), C.c_uint64()
+    check(lib().lzx_rq2_sample_challenge(ring.modulus.q, ring.log_n, C.byref(s), seed, len(seed), out.ctypes.data,
+                                         C.byref(g0), C.byref(g1)), "sample_rq2_challenge")
+    return Rq2._from(ring, out), g0.value, g1.value

@@ -1472,3 +1472,190 @@ int or_crt_unpack(uint32_t n, const uint32_t *packed, size_t elems, const uint64
         for (size_t i = 0; i < k && got < expected; i++) out[got++] = packed[e * n] % moduli[i];
     return got != expected ? 50 : 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Short ring-element challenges: lattice-core/src/short_challenge.rs        */
+/* ------------------------------------------------------------------------ */
+
+enum { OR_SC_FW = 0, OR_SC_BIASED = 1, OR_SC_SMALLSET = 2, OR_SC_FWSET = 3 };
+enum { OR_SC_OK = 0, OR_SC_REJECTION = 60, OR_SC_INVALID = 61, OR_SC_GAMMA_CAP = 62 };
+
+/* ceil_sqrt / ceil_sqrt_u128 (short_challenge.rs:281-340): min(ceil(sqrt(x)), 2^64 - 1),
+ * by bisection on exact 128-bit squares. */
+uint64_t or_ceil_sqrt_u128(uint64_t lo, uint64_t hi)
+{
+    const unsigned __int128 x = ((unsigned __int128)hi << 64) | lo;
+    if (x == 0) return 0;
+    /* smallest g with g^2 >= x, searched in [1, 2^64] */
+    unsigned __int128 a = 1, b = (unsigned __int128)1 << 64;
+    while (a < b) {
+        const unsigned __int128 m = a + (b - a) / 2;
+        const int ge = m >= ((unsigned __int128)1 << 64) ? 1 : m * m >= x;
+        if (ge)
+            b = m;
+        else
+            a = m + 1;
+    }
+    return a >= ((unsigned __int128)1 << 64) ? UINT64_MAX : (uint64_t)a;
+}
+
+/* stream (short_challenge.rs:343-352): SHAKE256("SHORTCHAL" || u32le|label| || label
+ * || u32le|seed| || seed || u64le attempt). */
+static void or_sc_stream(const char *label, const uint8_t *seed, size_t slen, uint8_t *out, size_t bytes)
+{
+    const size_t llen = strlen(label), blen = 9 + 4 + llen + 4 + slen + 8;
+    uint8_t *buf = (uint8_t *)malloc(blen);
+    memcpy(buf, "SHORTCHAL", 9);
+    put_u32le(buf + 9, (uint32_t)llen);
+    memcpy(buf + 13, label, llen);
+    put_u32le(buf + 13 + llen, (uint32_t)slen);
+    if (slen) memcpy(buf + 17 + llen, seed, slen);
+    put_u64le(buf + 17 + llen + slen, 0);
+    or_shake256(buf, blen, out, bytes);
+    free(buf);
+}
+
+typedef struct {
+    const uint8_t *s;
+    size_t len, pos;
+} or_cursor;
+
+/* UniformCursor::next_index (short_challenge.rs:361-388); returns 0 and sets *ok = 0 when the stream runs dry. */
+static uint64_t or_next_index(or_cursor *c, uint64_t range, int *ok)
+{
+    if (range == 0) {
+        *ok = 0;
+        return 0;
+    }
+    if (range <= 1) return 0;
+    if (range > UINT32_MAX) {
+        const uint64_t lim = (UINT64_MAX / range) * range;
+        for (;;) {
+            if (c->pos + 8 > c->len) {
+                *ok = 0;
+                return 0;
+            }
+            const uint64_t raw = get_u64le(c->s + c->pos);
+            c->pos += 8;
+            if (raw < lim) return raw % range;
+        }
+    }
+    const uint64_t limit = (((uint64_t)UINT32_MAX + 1) / range) * range;
+    for (;;) {
+        if (c->pos + 4 > c->len) {
+            *ok = 0;
+            return 0;
+        }
+        const uint64_t raw = get_u32le(c->s + c->pos);
+        c->pos += 4;
+        if (raw < limit) return raw % range;
+    }
+}
+
+static int or_sc_validate(size_t n, int family, size_t weight, int64_t amplitude, uint32_t permille, size_t nvalues)
+{
+    if (n == 0 || n > ((size_t)1 << 20)) return 0;
+    switch (family) {
+    case OR_SC_FW: return !(weight == 0 || weight > n || amplitude <= 0);
+    case OR_SC_BIASED: return !(permille == 0 || permille > 1000);
+    case OR_SC_SMALLSET: return !(nvalues == 0 || nvalues > 1024);
+    case OR_SC_FWSET: return !(weight == 0 || weight > n || nvalues == 0 || nvalues > 1024);
+    default: return 0;
+    }
+}
+
+/* ShortChallengeSpec::sample (short_challenge.rs:183-205, samplers :392-497); *gamma = Γ_C. */
+int or_short_challenge_sample(size_t n, int family, size_t weight, int64_t amplitude, uint32_t permille,
+                              const int64_t *values, size_t nvalues, const uint8_t *seed, size_t slen, int64_t *out,
+                              uint64_t *gamma)
+{
+    if (!or_sc_validate(n, family, weight, amplitude, permille, nvalues)) return OR_SC_INVALID;
+    size_t bytes = 0;
+    const char *label = "";
+    switch (family) {
+    case OR_SC_FW: bytes = weight * 24 + 32, label = "fw"; break;
+    case OR_SC_BIASED: bytes = n * 6 + 32, label = "bias3"; break;
+    case OR_SC_SMALLSET: bytes = n * 8 + 32, label = "smallset"; break;
+    default: bytes = weight * 32 + 32, label = "fwset"; break;
+    }
+    uint8_t *s = (uint8_t *)malloc(bytes);
+    or_sc_stream(label, seed, slen, s, bytes);
+    or_cursor c = {s, bytes, 0};
+    int ok = 1;
+    for (size_t i = 0; i < n; i++) out[i] = 0;
+    if (family == OR_SC_FW || family == OR_SC_FWSET) {
+        uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * n); /* partial Fisher-Yates */
+        for (size_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+        for (size_t slot = 0; slot < weight && ok; slot++) {
+            const uint64_t pos = or_next_index(&c, n - slot, &ok);
+            if (!ok) break;
+            const uint32_t at = idx[pos];
+            const uint32_t t = idx[pos];
+            idx[pos] = idx[n - 1 - slot];
+            idx[n - 1 - slot] = t;
+            if (family == OR_SC_FW) {
+                const uint64_t sign = or_next_index(&c, 2, &ok);
+                if (ok) out[at] = sign == 0 ? amplitude : -amplitude;
+            } else {
+                const uint64_t vp = or_next_index(&c, nvalues, &ok);
+                if (ok) out[at] = values[vp];
+            }
+        }
+        free(idx);
+    } else if (family == OR_SC_BIASED) {
+        for (size_t i = 0; i < n && ok; i++) {
+            if (c.pos + 4 > c.len) {
+                ok = 0;
+                break;
+            }
+            const uint32_t raw = get_u32le(c.s + c.pos);
+            c.pos += 4;
+            if ((raw & 0x3FF) >= permille) continue; /* zero */
+            if (c.pos + 4 > c.len) {
+                ok = 0;
+                break;
+            }
+            const uint32_t sraw = get_u32le(c.s + c.pos);
+            c.pos += 4;
+            out[i] = (sraw & 1) == 0 ? 1 : -1;
+        }
+    } else {
+        for (size_t i = 0; i < n && ok; i++) {
+            const uint64_t vp = or_next_index(&c, nvalues, &ok);
+            if (ok) out[i] = values[vp];
+        }
+    }
+    free(s);
+    if (!ok) return OR_SC_REJECTION;
+    unsigned __int128 l2 = 0; /* certified_gamma (:271-277) */
+    for (size_t i = 0; i < n; i++) l2 += (unsigned __int128)((__int128)out[i] * out[i]);
+    *gamma = or_ceil_sqrt_u128((uint64_t)l2, (uint64_t)(l2 >> 64));
+    return OR_SC_OK;
+}
+
+/* sample_rq2_challenge (extension.rs:180-216): spec.n must be 2n; even stream
+ * positions -> c0, odd -> c1, each reduced mod q; g0, g1 the per-half Γ. */
+int or_rq2_sample_challenge(uint32_t q, uint32_t n, size_t spec_n, int family, size_t weight, int64_t amplitude,
+                            uint32_t permille, const int64_t *values, size_t nvalues, const uint8_t *seed, size_t slen,
+                            uint32_t *out, uint64_t *g0, uint64_t *g1)
+{
+    if (spec_n != 2 * (size_t)n) return OR_SC_INVALID;
+    int64_t *c = (int64_t *)malloc(sizeof(int64_t) * spec_n);
+    uint64_t gam = 0;
+    const int st =
+        or_short_challenge_sample(spec_n, family, weight, amplitude, permille, values, nvalues, seed, slen, c, &gam);
+    if (st == OR_SC_OK) {
+        unsigned __int128 l0 = 0, l1 = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const int64_t a = c[2 * i], b = c[2 * i + 1];
+            out[i] = (uint32_t)(((a % (int64_t)q) + q) % q);
+            out[n + i] = (uint32_t)(((b % (int64_t)q) + q) % q);
+            l0 += (unsigned __int128)((__int128)a * a);
+            l1 += (unsigned __int128)((__int128)b * b);
+        }
+        *g0 = or_ceil_sqrt_u128((uint64_t)l0, (uint64_t)(l0 >> 64));
+        *g1 = or_ceil_sqrt_u128((uint64_t)l1, (uint64_t)(l1 >> 64));
+    }
+    free(c);
+    return st;
+}

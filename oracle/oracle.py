@@ -77,6 +77,11 @@ def lib():
         "or_shake256": (None, [_u8p, sz, C.c_void_p, sz]),
         "or_sha3_256": (None, [_u8p, sz, C.c_void_p]),
         "or_xof": (None, [_u8p, sz, _u8p, sz, C.c_void_p, sz]),
+        "or_ceil_sqrt_u128": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+        "or_short_challenge_sample": (C.c_int, [sz, C.c_int, sz, C.c_int64, C.c_uint32, vp, sz, vp, sz, vp,
+                                                C.POINTER(C.c_uint64)]),
+        "or_rq2_sample_challenge": (C.c_int, [C.c_uint32, C.c_uint32, sz, C.c_int, sz, C.c_int64, C.c_uint32, vp,
+                                              sz, vp, sz, vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "or_uniform_from_seed": (None, [u32, u32, _u8p, sz, _u8p, sz, u64, _u32p]),
         "or_sample_small_secret": (None, [u32, u32, sz, u32, _u8p, sz, _u32p]),
         "or_ajtai_from_seed": (C.c_int, [u32, u32, u32, u32, sz, sz, _u8p, vp, vp]),
@@ -755,3 +760,57 @@ def crt_unpack(n, packed, moduli, expected):
     out = np.zeros(max(1, expected), dtype=np.uint64)
     st = lib().or_crt_unpack(n, pk if pk.size else np.zeros(1, np.uint32), pk.size // n, m, m.size, expected, out)
     return st, out[:expected]
+
+
+# ---------------------------------------------------------------------------
+# Short ring-element challenges (crates/lattice-core/src/short_challenge.rs)
+# ---------------------------------------------------------------------------
+
+SC_FIXED_WEIGHT, SC_BIASED_TERNARY, SC_SMALL_SET, SC_FIXED_WEIGHT_SMALL_SET = 0, 1, 2, 3
+SC_REJECTION_BUDGET, SC_INVALID_PARAMETERS, SC_GAMMA_CAP_EXCEEDED = 60, 61, 62
+
+
+def ceil_sqrt_u128(x: int) -> int:
+    return lib().or_ceil_sqrt_u128(x & ((1 << 64) - 1), x >> 64)
+
+
+def _spec_args(family, weight=0, amplitude=0, permille=0, values=()):
+    vals = np.ascontiguousarray(list(values) or [0], dtype=np.int64)
+    return family, weight, amplitude, permille, vals, len(values)
+
+
+def short_challenge_sample(n, family, seed: bytes, weight=0, amplitude=0, permille=0, values=()):
+    """ShortChallengeSpec::sample (short_challenge.rs:183-205): (status, coefficients, gamma_c)."""
+    fam, w, a, pm, vals, nv = _spec_args(family, weight, amplitude, permille, values)
+    out = np.zeros(max(1, n), dtype=np.int64)
+    g = C.c_uint64(0)
+    sb = (C.c_uint8 * max(1, len(seed))).from_buffer_copy(seed or b"\0")
+    st = lib().or_short_challenge_sample(n, fam, w, a, pm, vals.ctypes.data, nv, C.cast(sb, C.c_void_p), len(seed),
+                                         out.ctypes.data, C.byref(g))
+    return st, out[:n], g.value
+
+
+def short_challenge_sample_with_gamma_cap(n, family, seed: bytes, cap, retries, **kw):
+    """sample_with_gamma_cap (short_challenge.rs:208-236): (status, coefficients, gamma_c)."""
+    if cap == 0:
+        return SC_INVALID_PARAMETERS, None, 0
+    last = 0
+    for attempt in range(retries + 1):
+        st, c, g = short_challenge_sample(n, family, seed + attempt.to_bytes(8, "little"), **kw)
+        if st:
+            return st, None, 0
+        last = g
+        if g <= cap:
+            return 0, c, g
+    return SC_GAMMA_CAP_EXCEEDED, None, last
+
+
+def rq2_sample_challenge(q, n, spec_n, family, seed: bytes, weight=0, amplitude=0, permille=0, values=()):
+    """sample_rq2_challenge (extension.rs:180-216): (status, [c0 | c1] u32, g0, g1)."""
+    fam, w, a, pm, vals, nv = _spec_args(family, weight, amplitude, permille, values)
+    out = np.zeros(2 * n, dtype=np.uint32)
+    g0, g1 = C.c_uint64(0), C.c_uint64(0)
+    sb = (C.c_uint8 * max(1, len(seed))).from_buffer_copy(seed or b"\0")
+    st = lib().or_rq2_sample_challenge(q, n, spec_n, fam, w, a, pm, vals.ctypes.data, nv, C.cast(sb, C.c_void_p),
+                                       len(seed), out.ctypes.data, C.byref(g0), C.byref(g1))
+    return st, out, g0.value, g1.value

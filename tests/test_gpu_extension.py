@@ -132,3 +132,93 @@ def test_rq2_full_batch_sampled(ext):
     rows = np.sort(rng.choice(count, 512, replace=False))
     rows[-1] = count - 1
     assert np.array_equal(got[rows], ntt.rq2_mul(a[rows], b[rows]))
+
+
+# ---- short challenges and sample_rq2_challenge (library host code against the oracle) ----
+
+SPECS = [(256, "fw", dict(weight=23, amplitude=1)), (64, "fw", dict(weight=63, amplitude=3)),
+         (1 << 16, "fw", dict(weight=300, amplitude=5)), (512, "bt", dict(permille=500)),
+         (100, "bt", dict(permille=1000)), (64, "ss", dict(values=[-2, -1, 0, 1, 2])),
+         (1000, "ss", dict(values=list(range(-500, 501)))), (256, "fws", dict(weight=128, values=[-1, 1, 2])),
+         (7, "fws", dict(weight=7, values=[9]))]
+
+
+def _spec(SC, n, fam, kw):
+    f = {"fw": lambda: SC.FixedWeight(kw["weight"], kw["amplitude"]), "bt": lambda: SC.BiasedTernary(kw["permille"]),
+         "ss": lambda: SC.SmallSet(tuple(kw["values"])),
+         "fws": lambda: SC.FixedWeightSmallSet(kw["weight"], tuple(kw["values"]))}[fam]()
+    return SC.ShortChallengeSpec(n, f)
+
+
+OFAM = {"fw": O.SC_FIXED_WEIGHT, "bt": O.SC_BIASED_TERNARY, "ss": O.SC_SMALL_SET, "fws": O.SC_FIXED_WEIGHT_SMALL_SET}
+
+
+@pytest.mark.parametrize("n,fam,kw", SPECS)
+def test_short_challenge_sample_matches_oracle(n, fam, kw):
+    from lzx_amd import short_challenge as SC
+    from lzx_amd._lib import LzxError
+    spec = _spec(SC, n, fam, kw)
+    for seed in (b"", b"seed-a", bytes(range(200))):
+        st, want, g = O.short_challenge_sample(n, OFAM[fam], seed, **kw)
+        if st:  # e.g. the biased ternary at p = 1/2 can outrun its n * 6 + 32 byte stream
+            with pytest.raises(LzxError) as e:
+                spec.sample(seed)
+            assert e.value.code == st
+            continue
+        c = spec.sample(seed)
+        assert c.coefficients.tolist() == want.tolist() and c.gamma_c == g
+
+
+def test_short_challenge_cap_errors_and_bookkeeping():
+    # short_challenge.rs:668-886 through the library
+    from lzx_amd import short_challenge as SC
+    from lzx_amd._lib import LzxError
+    pk = SC.pikkufold_spec()
+    c = pk.sample(b"seed-a")
+    assert c.hamming_weight() == 23 and c.l_inf() == 1 and c.gamma_c == 5 and pk.gamma_c_worst_case() == 5
+    assert pk.entropy_bits() > 100.0
+    with pytest.raises(LzxError) as e:
+        pk.sample_with_gamma_cap(b"x", 4, 4)
+    assert e.value.code == 62
+    assert pk.sample_with_gamma_cap(b"x", 5, 4).gamma_c == 5
+    lf = SC.latticefold_plus_spec()
+    st, want, g = O.short_challenge_sample_with_gamma_cap(64, O.SC_SMALL_SET, b"t", 10, 200, values=[-2, -1, 0, 1, 2])
+    if st == 0:
+        got = lf.sample_with_gamma_cap(b"t", 10, 200)
+        assert got.coefficients.tolist() == want.tolist() and got.gamma_c == g <= 10
+    assert lf.gamma_c_worst_case() == 16 and abs(lf.entropy_bits() - 64 * np.log2(5)) < 1e-9
+    assert abs(SC.cyclo_spec().entropy_bits() - 768.0) < 1.0 and SC.protogalattice_spec(256).entropy_bits() > 128
+    assert SC.symphony_spec(512).entropy_bits() == 512 * np.log2(5) and SC.rokoko_spec(1024).entropy_bits() > 100
+    pk.require_entropy(90.0)
+    with pytest.raises(LzxError):
+        pk.require_entropy(200.0)
+    for bad in (SC.ShortChallengeSpec(0, SC.FixedWeight(1, 1)), SC.ShortChallengeSpec(8, SC.FixedWeight(9, 1)),
+                SC.ShortChallengeSpec(8, SC.SmallSet(())), SC.ShortChallengeSpec(8, SC.BiasedTernary(1001))):
+        with pytest.raises(LzxError) as e:
+            bad.sample(b"s")
+        assert e.value.code == 61
+        with pytest.raises(LzxError):
+            bad.validate()
+    M = (1 << 64) - 1
+    for x in (0, 1, 2, 3, 24, 25, 26, 1 << 96, (1 << 96) + 1, M * M, M * M + 1, (1 << 128) - 1):
+        assert SC.ceil_sqrt(x) == O.ceil_sqrt_u128(x)
+
+
+def test_rq2_sample_challenge_matches_oracle(ext):
+    # extension.rs:393-441 through the library, against the oracle
+    E, D, R = ext
+    from lzx_amd import short_challenge as SC
+    from lzx_amd._lib import LzxError
+    ring, _ntt, q = _setup(R, "q32", 6)
+    spec = SC.ShortChallengeSpec(128, SC.FixedWeight(23, 1))
+    z, g0, g1 = E.sample_rq2_challenge(ring, spec, b"chal-seed")
+    st, want, w0, w1 = O.rq2_sample_challenge(q, 64, 128, O.SC_FIXED_WEIGHT, b"chal-seed", weight=23, amplitude=1)
+    assert st == 0 and np.concatenate([z.c0.coeffs, z.c1.coeffs]).tolist() == want.tolist() and (g0, g1) == (w0, w1)
+    assert int(np.count_nonzero(z.c0.coeffs)) + int(np.count_nonzero(z.c1.coeffs)) == 23 and g0 >= 1 and g1 >= 1
+    with pytest.raises(LzxError) as e:
+        E.sample_rq2_challenge(ring, SC.ShortChallengeSpec(64, SC.FixedWeight(8, 1)), b"x")
+    assert e.value.code == 61
+    sym = SC.symphony_spec(128)
+    z, _, _ = E.sample_rq2_challenge(ring, sym, b"sym-seed")
+    for c in np.concatenate([z.c0.coeffs, z.c1.coeffs]).astype(np.int64):
+        assert min(c, q - c) <= 2

@@ -886,3 +886,94 @@ def test_rq2_inverse_refuses_a_zero_norm_slot():
     one = np.zeros(2 * n, dtype=np.uint32)
     one[0] = 1
     assert ok[0] == 1 and ntt.rq2_mul(z, inv[0])[0].tolist() == one.tolist()
+
+
+# ---- short challenges (short_challenge.rs) and sample_rq2_challenge (extension.rs:180-216) ----
+
+FW, BT, SS, FWS = O.SC_FIXED_WEIGHT, O.SC_BIASED_TERNARY, O.SC_SMALL_SET, O.SC_FIXED_WEIGHT_SMALL_SET
+
+
+def test_short_challenge_reference_cases():
+    # short_challenge.rs:668-782
+    st, c, g = O.short_challenge_sample(256, FW, b"seed-a", weight=23, amplitude=1)
+    assert st == 0 and len(c) == 256 and (c != 0).sum() == 23 and abs(c).max() == 1 and g == 5
+    assert O.short_challenge_sample(256, FW, b"seed-a", weight=23, amplitude=1)[1].tolist() == c.tolist()
+    assert O.short_challenge_sample(256, FW, b"seed-b", weight=23, amplitude=1)[1].tolist() != c.tolist()
+    for i in range(8):  # dense weight: Fisher-Yates never collides; ceil(sqrt(63 * 9)) = 24
+        st, c, g = O.short_challenge_sample(64, FW, bytes([i]) * 8, weight=63, amplitude=3)
+        assert st == 0 and (c != 0).sum() == 63 and abs(c).max() == 3 and g == 24
+    st, c, _ = O.short_challenge_sample(512, BT, b"cyclo-seed", permille=500)
+    assert st == 0 and abs(c).max() <= 1 and 200 < (c != 0).sum() < 312
+    st, c, _ = O.short_challenge_sample(64, SS, b"lfplus", values=[-2, -1, 0, 1, 2])
+    assert st == 0 and c.min() >= -2 and c.max() <= 2
+    st, c, _ = O.short_challenge_sample(256, FWS, b"pg-seed", weight=128, values=[-1, 1, 2])
+    assert st == 0 and (c != 0).sum() == 128 and set(c.tolist()) <= {0, -1, 1, 2}
+    st, c, _ = O.short_challenge_sample(512, SS, b"sym", values=[0, 1, -1, 2, -2])
+    assert st == 0 and abs(c).max() <= 2
+    st, c, _ = O.short_challenge_sample(1024, FW, b"rok", weight=22, amplitude=1)
+    assert st == 0 and (c != 0).sum() == 22 and abs(c).max() == 1
+    assert (O.short_challenge_sample(256, FW, b"hw", weight=10, amplitude=1)[1] != 0).sum() == 10
+
+
+def test_short_challenge_gamma_cap_and_validation():
+    # short_challenge.rs:753-775 (cap), :809-852 (ceil sqrt), :854-886 (validation)
+    st, _, g = O.short_challenge_sample_with_gamma_cap(256, FW, b"x", 4, 4, weight=23, amplitude=1)
+    assert st == O.SC_GAMMA_CAP_EXCEEDED and g == 5
+    assert O.short_challenge_sample_with_gamma_cap(256, FW, b"x", 5, 4, weight=23, amplitude=1)[0] == 0
+    st, c, g = O.short_challenge_sample_with_gamma_cap(64, SS, b"t", 10, 200, values=[-2, -1, 0, 1, 2])
+    assert (st == 0 and g <= 10 and int((c.astype(object) ** 2).sum()) <= 100) or st == O.SC_GAMMA_CAP_EXCEEDED
+    for x, want in [(0, 0), (1, 1), (2, 2), (3, 2), (4, 2), (5, 3), (24, 5), (25, 5), (26, 6)]:
+        assert O.ceil_sqrt_u128(x) == want
+    M = (1 << 64) - 1
+    assert O.ceil_sqrt_u128((1 << 128) - 1) == M and O.ceil_sqrt_u128(M * M) == M and O.ceil_sqrt_u128(M * M + 1) == M
+    assert O.ceil_sqrt_u128(1 << 96) == 1 << 48 and O.ceil_sqrt_u128((1 << 96) + 1) == (1 << 48) + 1
+    rng = np.random.default_rng(5)
+    for _ in range(200):  # the ceiling property (g - 1)^2 < x <= g^2 against Python's isqrt
+        x = int(rng.integers(1, 1 << 62)) * int(rng.integers(1, 1 << 62))
+        import math
+        r = math.isqrt(x)
+        assert O.ceil_sqrt_u128(x) == (r if r * r == x else r + 1)
+    assert O.short_challenge_sample(0, FW, b"s", weight=1, amplitude=1)[0] == O.SC_INVALID_PARAMETERS
+    assert O.short_challenge_sample(8, FW, b"s", weight=9, amplitude=1)[0] == O.SC_INVALID_PARAMETERS
+    assert O.short_challenge_sample(8, SS, b"s", values=[])[0] == O.SC_INVALID_PARAMETERS
+    assert O.short_challenge_sample(8, BT, b"s", permille=1001)[0] == O.SC_INVALID_PARAMETERS
+    assert O.short_challenge_sample(8, FW, b"s", weight=2, amplitude=0)[0] == O.SC_INVALID_PARAMETERS
+
+
+def test_short_challenge_positions_uniform_and_stream_framing():
+    # short_challenge.rs:888-918: 200 draws of weight 4 over n = 16, every position within 20..80 hits
+    counts = np.zeros(16, dtype=np.int64)
+    for i in range(200):
+        st, c, _ = O.short_challenge_sample(16, FW, i.to_bytes(8, "little"), weight=4, amplitude=1)
+        counts += c != 0
+    assert counts.sum() == 800 and counts.min() >= 20 and counts.max() < 80
+    # the first draw from hashlib's SHAKE-256 over the framed input (short_challenge.rs:343-352):
+    # position = first u32 below floor(2^32 / n) n, reduced mod n; sign = next u32 below 2^32, mod 2
+    seed, n = b"frame", 96
+    buf = b"SHORTCHAL" + (2).to_bytes(4, "little") + b"fw" + len(seed).to_bytes(4, "little") + seed + bytes(8)
+    s = hashlib.shake_256(buf).digest(24 + 32)
+    words = [int.from_bytes(s[4 * k: 4 * k + 4], "little") for k in range(len(s) // 4)]
+    lim = (1 << 32) // n * n
+    k = next(i for i, w in enumerate(words) if w < lim)
+    pos, sign = words[k] % n, words[k + 1] % 2
+    st, c, _ = O.short_challenge_sample(n, FW, seed, weight=1, amplitude=7)
+    assert st == 0 and np.flatnonzero(c).tolist() == [pos] and c[pos] == (7 if sign == 0 else -7)
+
+
+def test_rq2_sample_challenge_reference_cases():
+    # extension.rs:393-423 short_challenge_sampling_and_norms, :425-441 symphony_family_sampling
+    q, n = O.Q_32[0], 64
+    st, z, g0, g1 = O.rq2_sample_challenge(q, n, 2 * n, FW, b"chal-seed", weight=23, amplitude=1)
+    assert st == 0 and O.rq2_sample_challenge(q, n, 2 * n, FW, b"chal-seed", weight=23, amplitude=1)[1].tolist() == \
+        z.tolist()
+    assert (z != 0).sum() == 23
+    bal = np.where(z <= q // 2, z.astype(np.int64), z.astype(np.int64) - q)
+    assert abs(bal).max() <= 1 and g0 >= 1 and g1 >= 1
+    # the halves are the even and odd positions of the 2n-slot challenge, with per-half Gamma
+    _, c, _ = O.short_challenge_sample(2 * n, FW, b"chal-seed", weight=23, amplitude=1)
+    assert bal[:n].tolist() == c[0::2].tolist() and bal[n:].tolist() == c[1::2].tolist()
+    assert g0 == O.ceil_sqrt_u128(int((c[0::2] ** 2).sum())) and g1 == O.ceil_sqrt_u128(int((c[1::2] ** 2).sum()))
+    assert O.rq2_sample_challenge(q, n, n, FW, b"x", weight=8, amplitude=1)[0] == O.SC_INVALID_PARAMETERS
+    st, z, _, _ = O.rq2_sample_challenge(q, n, 2 * n, SS, b"sym-seed", values=[0, 1, -1, 2, -2])
+    bal = np.where(z <= q // 2, z.astype(np.int64), z.astype(np.int64) - q)
+    assert st == 0 and abs(bal).max() <= 2
