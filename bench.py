@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per forward launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (each scaled by the reading for a 64 MiB copy kernel): 65,834 KiB
 # read + 65,536 KiB written; the rocprofv3 summary is kept under profiles/.
-NTT_FWD_TRAFFIC_BYTES = (65834 + 65536) * 1024
+NTT_FWD_TRAFFIC_BYTES = (65812 + 65536) * 1024
 POLYS = 1 << 16
 LOG_N = 6
 N = 1 << LOG_N
@@ -475,7 +475,7 @@ def main():
         "config": {"workload": "batched negacyclic NTT forward+inverse, 4 CRT primes x 2^16 polys x d=64, u32",
                    "polys_per_prime": POLYS, "degree": N, "primes": [m.q for m in __import__(
                        "lzx_amd.ring", fromlist=["CRT_PRIMES"]).CRT_PRIMES], "roundtrip_exact": r["roundtrip_exact"]},
-        "roofline": {"bound": "hbm", "kernel": "ntt_dma_kernel<6,ARITH_SHOUP,NTT_FWD>",
+        "roofline": {"bound": "hbm", "kernel": "ntt_pipe_kernel<6,ARITH_SHOUP,NTT_FWD>",
                      "achieved": round(r["fwd_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["fwd_gbs"] / HBM_PEAK_GBS, 4), "traffic": NTT_FWD_TRAFFIC_BYTES,
                      "alg_bytes": r["bytes_per_pass"],

@@ -9,12 +9,13 @@
 //            omega = psi^2), whose intermediate state is part of the contract.
 //
 // Kernels
-//   ntt_dma_kernel  n in [4, 64] (default): one polynomial per lane, all
+//   ntt_pipe_kernel n in [4, 64] (default): one polynomial per lane, all
 //       log n levels in VGPRs (no cross-lane traffic), twiddles wave-uniform
-//       (scalar loads); persistent waves stream 64-polynomial tiles through a
-//       double-buffered LDS tile fed by global_load_lds (see below).  psi is
-//       folded into the twiddles; the forward permutes to natural order on the
-//       way back to LDS and the inverse folds n^{-1} into its last level.
+//       (scalar loads from the constant address space); persistent waves
+//       stream 64-polynomial tiles, the next tile's loads in flight in VGPRs
+//       while the current one is transformed (see below).  psi is folded into
+//       the twiddles; the forward permutes to natural order on the way back to
+//       LDS and the inverse folds n^{-1} into its last level.
 //   ntt_reg_kernel  the same butterflies with a plain LDS-staged tile per
 //       block, for buffers that are not 16-byte aligned.
 //   ntt_lds_kernel  any n up to 2^13 and the incomplete transforms: a tile of
@@ -58,47 +59,60 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
     for (uint32_t i = tid; i < nw; i += 64) g[i] = tile[(i / N) * S + (i % N)];
 }
 
-// Persistent, double-buffered register kernel (default for n in [4, 64]).
-// A tile is 64 polynomials of C = n/4 16-byte chunks.  Tiles stream from HBM
-// straight into LDS with global_load_lds_dwordx4 (fully coalesced: each
-// instruction moves 1 KiB of consecutive polynomials) while the wave runs the
-// butterflies of the previous tile in VGPRs.  Chunk c of polynomial p lives at
-// LDS slot C*p + ((c + p) mod C), so the transposed per-lane reads (lane =
-// polynomial) and writes are bank-conflict free.  Results go back through the
-// same slots and leave as coalesced 16-byte stores.  One wave per SIMD
-// (grid = 4 x CUs) keeps 32 KiB of LDS per wave.
+// Persistent register kernel (default for n in [4, 64]).  A tile is 64
+// polynomials of C = n/4 16-byte chunks.  Each lane loads C chunks of the next
+// tile with coalesced 16-byte loads (one instruction moves 1 KiB of
+// consecutive polynomials) into VGPRs while the wave runs the butterflies of
+// the current tile; at the top of the next iteration the chunks go to LDS,
+// where chunk c of polynomial p lives at slot C*p + ((c + p) mod C), so the
+// transposed per-lane reads (lane = polynomial) and writes are bank-conflict
+// free.  Results go back through the same slots and leave as coalesced
+// 16-byte stores.  One 16 KiB tile per wave: at n = 64 (Shoup) two waves per
+// SIMD are resident, and the grid takes exactly the resident waves.
+// Prefetching into VGPRs (not global_load_lds) lets the compiler wait only for
+// the registers it stages; an LDS DMA would make it wait for every DMA before
+// any LDS access, the next tile's included.
 template <uint32_t C>
 __device__ __forceinline__ uint32_t tile_slot(uint32_t p, uint32_t c)
 {
     return C * p + ((c + p) & (C - 1));
 }
 
-template <uint32_t C, bool PARTIAL>
-__device__ __forceinline__ void tile_dma_n(const uint32_t *__restrict__ g, uint32_t np, uint32_t *T, uint32_t lane)
+// The pipe kernel's blocks are one wave: lanes see each other's LDS writes in
+// issue order (a wave's LDS operations execute in order), so a wave-scope
+// fence and a code-motion barrier order them.  A workgroup barrier's release
+// fence would also wait for the wave's in-flight global stores.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Chunk i of a lane's prefetch: the 16-byte piece of LDS slot s = 64 i + lane,
+// i.e. chunk ((s mod C) - p) mod C of polynomial p = s / C (the rotated layout
+// of tile_slot), read from HBM with the same coalesced addressing.  Slots past
+// the tile's last polynomial reload polynomial 0 (in bounds; never stored), so
+// every load is unconditional and the array stays in VGPRs.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4))); // promotable to VGPRs across loop iterations
+
+template <uint32_t C>
+__device__ __forceinline__ void tile_fetch(const uint32_t *__restrict__ g, uint32_t np, u32x4 (&r)[C], uint32_t lane)
 {
 #pragma unroll
     for (uint32_t i = 0; i < C; i++) {
         const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
-        if (!PARTIAL || p < np) __builtin_amdgcn_global_load_lds(g + (p * C + c) * 4, T + 256 * i, 16, 0, 0);
+        const uint32_t pp = p < np ? p : 0;
+        r[i] = *reinterpret_cast<const u32x4 *>(g + (pp * C + c) * 4);
     }
 }
 
-// full tiles skip the per-lane bound (an exec-masked branch per instruction)
-template <uint32_t C>
-__device__ __forceinline__ void tile_dma(const uint32_t *__restrict__ g, uint32_t np, uint32_t *T, uint32_t lane)
-{
-    if (np == 64)
-        tile_dma_n<C, false>(g, np, T, lane);
-    else
-        tile_dma_n<C, true>(g, np, T, lane);
-}
-
 template <int LOG_N, int AR, int MODE>
-__global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
-                                                     uint32_t blocks_per_plan, uint32_t total)
+__global__ __launch_bounds__(64) void ntt_pipe_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
+                                                      uint32_t blocks_per_plan, uint32_t total)
 {
     constexpr uint32_t N = 1u << LOG_N, C = N / 4;
-    __shared__ __attribute__((aligned(16))) uint32_t lds[2][64 * N];
+    __shared__ __attribute__((aligned(16))) uint32_t T[64 * N];
     const uint32_t lane = threadIdx.x;
     // each wave owns a contiguous range of tiles, so it stays on one modulus
     // (one twiddle table in the scalar cache) for most of its tiles
@@ -113,18 +127,21 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
         return data + ((size_t)plan * polys + p0) * N;
     };
     uint32_t np, plan;
-    uint32_t *g = locate(b, np, plan);
-    tile_dma<C>(g, np, lds[0], lane);
-    uint32_t cur = 0;
+    u32x4 nx[C]; // the next tile, in flight while the current one is transformed
+    tile_fetch<C>(locate(b, np, plan), np, nx, lane);
+    // The compiler's waits before staging nx count the vector memory ops issued
+    // after each load; on the steady path those include the previous tile's C
+    // stores, so the stores stay in flight.  Every other way into the loop
+    // header (this prologue, a partial tile) ends with nothing pending, so it
+    // cannot lower those counts.
+    constexpr int WAIT_VM0 = 0x0F70; // s_waitcnt vmcnt(0), lgkmcnt and expcnt free
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
     for (; b < b_end; b++) {
-        // vmcnt(0): this tile has landed in LDS.  The asm clobber makes the
-        // compiler treat LDS as rewritten (it does not model the DMA writes).
-        __builtin_amdgcn_s_waitcnt(0x0F70);
-        asm volatile("" ::: "memory");
-        __syncthreads();
-        g = locate(b, np, plan);
+        uint32_t *g = locate(b, np, plan);
         const uint32_t *__restrict__ tab = tabs.p[plan];
-        uint32_t *T = lds[cur];
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = nx[i];
+        wave_sync();
         uint32_t a[N];
 #pragma unroll
         for (uint32_t c4 = 0; c4 < C; c4++) {
@@ -133,14 +150,13 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
 #pragma unroll
             for (uint32_t e = 0; e < 4; e++) a[MODE == NTT_FWD ? 4 * c4 + e : brv_bits(4 * c4 + e, LOG_N)] = vv[e];
         }
-        const uint32_t bn = b + 1;
-        if (bn < b_end) {
+        if (b + 1 < b_end) {
             uint32_t npn, pn;
-            const uint32_t *gn = locate(bn, npn, pn);
-            tile_dma<C>(gn, npn, lds[cur ^ 1], lane);
+            const uint32_t *gn = locate(b + 1, npn, pn);
+            tile_fetch<C>(gn, npn, nx, lane);
         }
         ntt_regs<LOG_N, AR, MODE>(a, tab);
-        __syncthreads();
+        wave_sync(); // every lane has read its row before the results overwrite the tile
 #pragma unroll
         for (uint32_t c4 = 0; c4 < C; c4++) {
             uint32_t vv[4];
@@ -148,22 +164,23 @@ __global__ __launch_bounds__(64) void ntt_dma_kernel(uint32_t *__restrict__ data
             for (uint32_t e = 0; e < 4; e++) vv[e] = a[MODE == NTT_FWD ? brv_bits(4 * c4 + e, LOG_N) : 4 * c4 + e];
             *reinterpret_cast<uint4 *>(T + 4 * tile_slot<C>(lane, c4)) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
         }
-        __syncthreads();
+        wave_sync();
         if (np == 64) {
 #pragma unroll
             for (uint32_t i = 0; i < C; i++) {
                 const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
-                *reinterpret_cast<uint4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const uint4 *>(T + 4 * s);
+                *reinterpret_cast<u32x4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const u32x4 *>(T + 4 * s);
             }
         } else {
 #pragma unroll
             for (uint32_t i = 0; i < C; i++) {
                 const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
                 if (p < np)
-                    *reinterpret_cast<uint4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const uint4 *>(T + 4 * s);
+                    *reinterpret_cast<u32x4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const u32x4 *>(T + 4 * s);
             }
+            __builtin_amdgcn_s_waitcnt(WAIT_VM0);
         }
-        cur ^= 1;
+        wave_sync(); // the stores have read the tile before the next one is staged
     }
 }
 
@@ -334,17 +351,37 @@ __global__ void pointwise_mul_kernel(uint32_t *__restrict__ out, const uint32_t 
 // host side
 // ---------------------------------------------------------------------------
 
+// Resident one-wave blocks per CU of a persistent pipe kernel (its VGPR and
+// LDS use), measured once per instantiation.
+template <int LOG_N, int AR, int MODE>
+static uint32_t pipe_blocks_per_cu()
+{
+    static const uint32_t nb = [] {
+        int v = 0;
+        const hipError_t e =
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, ntt_pipe_kernel<LOG_N, AR, MODE>, 64, 0);
+        return (uint32_t)(e == hipSuccess && v > 0 ? v : 4);
+    }();
+    return nb;
+}
+
+// grid: for the persistent pipe kernel, the number of CUs (the launch takes
+// as many waves as stay resident); otherwise one block per tile.
 template <int LOG_N, int AR>
 static void launch_reg(NttMode mode, bool dma, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys,
                        uint32_t bpp, uint32_t total, hipStream_t stream)
 {
     if (dma) {
-        if (mode == NTT_FWD)
-            hipLaunchKernelGGL((ntt_dma_kernel<LOG_N, AR, NTT_FWD>), grid, dim3(64), 0, stream, data, ts, polys, bpp,
+        const uint32_t cus = grid.x;
+        if (mode == NTT_FWD) {
+            const dim3 g(std::min<uint32_t>(total, cus * pipe_blocks_per_cu<LOG_N, AR, NTT_FWD>()));
+            hipLaunchKernelGGL((ntt_pipe_kernel<LOG_N, AR, NTT_FWD>), g, dim3(64), 0, stream, data, ts, polys, bpp,
                                total);
-        else
-            hipLaunchKernelGGL((ntt_dma_kernel<LOG_N, AR, NTT_INV>), grid, dim3(64), 0, stream, data, ts, polys, bpp,
+        } else {
+            const dim3 g(std::min<uint32_t>(total, cus * pipe_blocks_per_cu<LOG_N, AR, NTT_INV>()));
+            hipLaunchKernelGGL((ntt_pipe_kernel<LOG_N, AR, NTT_INV>), g, dim3(64), 0, stream, data, ts, polys, bpp,
                                total);
+        }
     } else {
         if (mode == NTT_FWD)
             hipLaunchKernelGGL((ntt_reg_kernel<LOG_N, AR, NTT_FWD>), grid, dim3(64), 0, stream, data, ts, polys, bpp);
@@ -435,7 +472,7 @@ lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t 
             const uint32_t bpp = (uint32_t)((polys + 63) / 64);
             const uint32_t total = (uint32_t)(bpp * cnt);
             const bool dma = plans[i]->stage_dma;
-            const dim3 grid(dma ? std::min<uint32_t>(total, (uint32_t)plans[i]->persistent_waves) : total);
+            const dim3 grid(dma ? (uint32_t)plans[i]->cus : total);
             dispatch_reg(arith, log_n, mode, dma, grid, chunk, ts, (uint32_t)polys, bpp, total, stream);
         } else {
             const uint32_t ppb = std::max<uint32_t>(1u, 2048u >> log_n);
@@ -575,7 +612,7 @@ extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint
     hipError_t e = hipGetDevice(&p->device);
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
-    p->persistent_waves = 4 * std::max(cus, 1); // one wave per SIMD
+    p->cus = std::max(cus, 1);
     if (e == hipSuccess) e = hipMalloc(&p->dev_tab, p->host_tab.size() * sizeof(uint32_t));
     if (e == hipSuccess)
         e = hipMemcpy(p->dev_tab, p->host_tab.data(), p->host_tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);

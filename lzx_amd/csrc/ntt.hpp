@@ -54,7 +54,7 @@ struct lzx_ntt_plan {
     uint32_t prime_c;  // q = prime_c * 2^prime_k + 1
     uint32_t prime_k;
     bool stage_dma;       // persistent DMA-staged register kernel (else LDS-staged blocks)
-    int persistent_waves; // grid of the persistent kernel: one wave per SIMD
+    int cus;              // compute units of the plan's device (the persistent kernel's grid scale)
     lzx::Mod32 mod;
     std::vector<uint32_t> host_tab; // same words as dev_tab
     uint32_t *dev_tab;

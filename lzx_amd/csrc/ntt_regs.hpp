@@ -27,8 +27,12 @@ __device__ __forceinline__ uint32_t brv_dev(uint32_t x, uint32_t bits)
 // forward reads coefficients into a[c]; the inverse reads evaluation k into
 // a[brv(k)].
 template <int LOG_N, int AR, int MODE>
-__device__ __forceinline__ void ntt_regs(uint32_t (&a)[1u << LOG_N], const uint32_t *__restrict__ tab)
+__device__ __forceinline__ void ntt_regs(uint32_t (&a)[1u << LOG_N], const uint32_t *__restrict__ table)
 {
+    // the tables are read-only for a kernel's life: reading them through the
+    // constant address space makes the wave-uniform twiddles scalar loads
+    // (counted by lgkmcnt, so they never wait behind the tile's DMA loads)
+    const __attribute__((address_space(4))) uint32_t *tab = (const __attribute__((address_space(4))) uint32_t *)table;
     constexpr uint32_t N = 1u << LOG_N;
     const QConst Q{tab[0], tab[1], tab[5], tab[6]};
     if (MODE == NTT_FWD) {
