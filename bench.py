@@ -54,22 +54,28 @@ def bench_ntt(steps: int, warmup: int):
         R.forward_multi(tabs, data, POLYS, stream)
         R.inverse_multi(tabs, data, POLYS, stream)
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-           torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    # the timed region: K steps back to back, bracketed by one pair of events
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record(stream)
+    for _ in range(steps):
+        R.forward_multi(tabs, data, POLYS, stream)
+        R.inverse_multi(tabs, data, POLYS, stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(end)
+    # per-launch durations for the roofline: the same launches, each between its own events
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     for e0, e1, e2 in ev:
         e0.record(stream)
         R.forward_multi(tabs, data, POLYS, stream)
         e1.record(stream)
         R.inverse_multi(tabs, data, POLYS, stream)
         e2.record(stream)
-    end.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    total_ms = start.elapsed_time(end)
     fwd_ms = statistics.mean(a.elapsed_time(b) for a, b, _ in ev)
     inv_ms = statistics.mean(b.elapsed_time(c) for _, b, c in ev)
     # round trip must restore the input exactly after every step
@@ -132,6 +138,47 @@ def _median_gpu_ms(torch, fn, reps, warmup=1):
     return statistics.median(times)
 
 
+def achievable_copy_gbs(torch, nbytes: int = 134217728, reps: int = 20) -> float:
+    """Achievable HBM rate on this GPU: a device-to-device copy of nbytes
+    (read + write counted), median of reps, HIP events on the launch stream."""
+    a = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        b.copy_(a)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e-3)
+    return 2 * nbytes / statistics.median(ts) / 1e9
+
+
+def achievable_int8_tops(torch, dim: int = 8192, reps: int = 10):
+    """Achievable dense int8 matrix rate of the library GEMM (torch._int_mm on
+    dim^3 int8 operands); (TOP/s, note)."""
+    try:
+        a = torch.randint(-128, 127, (dim, dim), dtype=torch.int8, device="cuda")
+        b = torch.randint(-128, 127, (dim, dim), dtype=torch.int8, device="cuda")
+        torch._int_mm(a, b)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            torch._int_mm(a, b)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return 2 * dim ** 3 / statistics.median(ts) / 1e12, f"torch._int_mm {dim}^3"
+    except Exception as e:  # the library GEMM is optional here; report why it is missing
+        return None, f"torch._int_mm unavailable: {type(e).__name__}"
+
+
 def bench_ajtai(reps: int = 5):
     """Ajtai commit, d=64, k=16, m=2^16, W=64 small-digit witnesses (digits in
     [-8, 8), P(0) = 17/32), A from seed [0x43; 32]: NTT-domain path vs int8
@@ -162,7 +209,14 @@ def bench_ajtai(reps: int = 5):
             "int8_path": {"ms": round(ms_i8, 4), "ring_macs_per_s": rmacs / (ms_i8 * 1e-3),
                           "int8_tops": round(i8_ops / (ms_i8 * 1e-3) / 1e12, 1),
                           "int8_mfma_frac_of_5000_tops": round(i8_ops / (ms_i8 * 1e-3) / 5e15, 4)},
-            "int8_over_ntt": round(ms_ntt / ms_i8, 3), "paths_bit_identical": same}
+            "int8_over_ntt": round(ms_ntt / ms_i8, 3), "paths_bit_identical": same,
+            "int8_achievable": _int8_achievable(torch, i8_ops / (ms_i8 * 1e-3) / 1e12)}
+
+
+def _int8_achievable(torch, tops):
+    peak, note = achievable_int8_tops(torch)
+    return {"tops": None if peak is None else round(peak, 1), "how": note,
+            "int8_path_frac": None if peak is None else round(tops / peak, 4)}
 
 
 def bench_sumcheck(reps: int = 3):
@@ -483,6 +537,10 @@ def main():
                      "inv_gbs": round(r["inv_gbs"], 2)},
         "cpu_baseline": None if args.no_cpu else cpu_baseline_ntt(),
     }
+    copy_gbs = achievable_copy_gbs(torch)
+    line["roofline"]["achievable_peak"] = {"value": round(copy_gbs, 1), "unit": "GB/s",
+                                           "how": "device-to-device copy of 128 MiB (read + write)",
+                                           "frac": round(r["fwd_gbs"] / copy_gbs, 4)}
     if not args.only_ntt:
         line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
                              "uneven_batch": bench_uneven_batch(), "ring50": bench_ring50(),

@@ -403,6 +403,28 @@ lzx_status lzx_zerocheck_verify(const uint64_t *rounds, size_t num_rounds, size_
                                 lzx_transcript *transcript, int32_t has_expected, uint64_t expected_poly_eval,
                                 uint64_t *point_out);
 
+/* Round-level sum-check session: the rounds of sumcheck::prove
+ * (sumcheck.rs:100-143) one call at a time, with the transcript steps left to
+ * the caller (absorb the round, draw r, check g(0)+g(1), interpolate).
+ * create evaluates round 0 on the device; round_eval copies g_j(0..d) out
+ * (round_len() = degree + 1 values; the zero polynomial gives 2 zeros);
+ * fold(r) binds the round's variable to r and evaluates the next round;
+ * final, once every variable is bound, returns the factor claims and
+ * P(r) (sumcheck.rs:139-154).  round_eval/fold past the last variable and
+ * final before it return LZX_SUMCHECK_BAD_ROUND_SHAPE.  The caller's tables
+ * must stay valid until the first fold; the session owns its folded copies.
+ * Calls on one session are sequential. */
+typedef struct lzx_sumcheck_session lzx_sumcheck_session;
+lzx_status lzx_sumcheck_session_create(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev,
+                                       size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
+                                       const uint32_t *term_ids, void *stream, lzx_sumcheck_session **out);
+size_t lzx_sumcheck_session_round_len(const lzx_sumcheck_session *session);
+lzx_status lzx_sumcheck_session_round_eval(lzx_sumcheck_session *session, uint64_t *evals_out);
+lzx_status lzx_sumcheck_session_fold(lzx_sumcheck_session *session, uint64_t r);
+lzx_status lzx_sumcheck_session_final(lzx_sumcheck_session *session, uint64_t *factor_claims_out,
+                                      uint64_t *final_claim);
+void lzx_sumcheck_session_destroy(lzx_sumcheck_session *session);
+
 /* One claim of a batch (batch.rs:24-28 BatchClaim): a virtual polynomial in
  * the lzx_sumcheck_prove form (device tables of 2^num_vars canonical u64,
  * terms in CSR form) and its asserted sum. */

@@ -107,6 +107,12 @@ PROTOTYPES = {
                                      C.POINTER(u64), vp, C.POINTER(sz), vp]),
     "lzx_zerocheck_prove": (st_t, [sz, vp, vp, vp, vp, vp, C.POINTER(u64), vp, vp]),
     "lzx_zerocheck_verify": (st_t, [vp, sz, sz, sz, vp, i32, u64, vp]),
+    "lzx_sumcheck_session_create": (st_t, [sz, sz, C.POINTER(vp), sz, vp, vp, vp, vp, C.POINTER(vp)]),
+    "lzx_sumcheck_session_round_len": (sz, [vp]),
+    "lzx_sumcheck_session_round_eval": (st_t, [vp, vp]),
+    "lzx_sumcheck_session_fold": (st_t, [vp, u64]),
+    "lzx_sumcheck_session_final": (st_t, [vp, vp, C.POINTER(u64)]),
+    "lzx_sumcheck_session_destroy": (None, [vp]),
     "lzx_sumcheck_prove_batch": (st_t, [vp, sz, vp, vp, vp, vp, C.POINTER(u64), vp, C.POINTER(sz), vp]),
     "lzx_sumcheck_prove_batch_star": (st_t, [vp, sz, vp, vp, vp, vp, C.POINTER(u64), vp, C.POINTER(sz),
                                              C.POINTER(sz), vp]),

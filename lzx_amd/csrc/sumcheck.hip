@@ -1496,3 +1496,184 @@ extern "C" lzx_status lzx_sumcheck_verify_batch_star(const uint64_t *rounds, siz
     return lzx_sumcheck_verify(rounds, num_rounds, round_len, max_vars, max_degree, combined, transcript,
                                has_expected, expected_final, point_out, &fin);
 }
+
+// ---- round-level session ---------------------------------------------------------
+// The prover's rounds one call at a time, for callers that drive their own
+// transcript between rounds (sumcheck.rs:62-162 with the absorb/challenge
+// steps left to the caller): the session evaluates the current round's
+// g_j(0..d) on the device and folds with the caller's challenge.
+
+namespace lzx {
+
+// The last fold: each factor's two entries become its claim (sumcheck.rs:139-143).
+__global__ void sc_last_fold_kernel(ScTables tabs, uint32_t nf, const uint64_t *__restrict__ rp,
+                                    uint64_t *__restrict__ claims)
+{
+    const uint32_t f = threadIdx.x;
+    if (f < nf) {
+        const uint64_t a = tabs.src[f][0], b = tabs.src[f][1], r = *rp;
+        claims[f] = gl_add(a, gl_mul(gl_sub(b, a), r));
+    }
+}
+
+} // namespace lzx
+
+struct lzx_sumcheck_session {
+    lzx::ScProblem P;
+    lzx::ScKernels K;
+    lzx::ScTables tabs;
+    hipStream_t stream;
+    size_t width, round;
+    uint64_t half;             // pairs of the current round (0 once every variable is bound)
+    uint8_t *dev = nullptr;    // folded tables | block partials | evals[width] | r | claims[nf]
+    uint64_t *partial, *evals, *rdev, *claims;
+    bool zero;                 // no terms: the zero polynomial's degree-1 zero rounds (sumcheck.rs:67-92)
+};
+
+namespace lzx {
+
+static lzx_status session_reduce(lzx_sumcheck_session *S, uint32_t blocks)
+{
+    hipLaunchKernelGGL(sc_reduce_kernel, dim3(1), dim3(SC_BLOCK), 0, S->stream, S->partial, blocks,
+                       (uint32_t)S->width, S->evals);
+    LZX_LAUNCH_CHECK("sc_reduce_kernel");
+    return LZX_OK;
+}
+
+} // namespace lzx
+
+extern "C" lzx_status lzx_sumcheck_session_create(size_t num_vars, size_t nfactors, const uint64_t *const *factors_dev,
+                                                  size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
+                                                  const uint32_t *term_ids, void *stream,
+                                                  lzx_sumcheck_session **out)
+{
+    if (!out) return LZX_INVALID_ARGUMENT;
+    *out = nullptr;
+    lzx_sumcheck_session *S = new lzx_sumcheck_session();
+    S->stream = (hipStream_t)stream;
+    S->round = 0;
+    S->zero = nterms == 0;
+    if (S->zero) {
+        memset(&S->P, 0, sizeof(S->P));
+        S->P.num_vars = num_vars;
+        S->width = 2;
+        S->half = num_vars ? 1ULL << (num_vars - 1) : 0;
+        *out = S;
+        return LZX_OK;
+    }
+    lzx_status st = build_problem(num_vars, nfactors, factors_dev, nterms, coeffs, term_off, term_ids, S->P);
+    if (st != LZX_OK) {
+        delete S;
+        return st;
+    }
+    S->K = pick_kernels(S->P.nf, S->P.d);
+    S->width = S->P.d + 1;
+    const size_t nf = S->P.nf, N = (size_t)1 << num_vars, halfN = num_vars ? N / 2 : 1;
+    const size_t tab_bytes = std::max<size_t>(1, nf) * halfN * 8, part_bytes = (size_t)SC_MAX_BLOCKS * S->width * 8;
+    const size_t bytes = tab_bytes + part_bytes + (S->width + 1 + std::max<size_t>(1, nf)) * 8;
+    const hipError_t e = scratch_alloc((void **)&S->dev, bytes, S->stream);
+    if (e != hipSuccess) {
+        delete S;
+        return device_fail(e, "sumcheck session scratch");
+    }
+    S->partial = (uint64_t *)(S->dev + tab_bytes);
+    S->evals = S->partial + (size_t)SC_MAX_BLOCKS * S->width;
+    S->rdev = S->evals + S->width;
+    S->claims = S->rdev + 1;
+    S->tabs = ScTables{};
+    for (size_t f = 0; f < nf; f++) {
+        S->tabs.src[f] = factors_dev[f];
+        S->tabs.dst[f] = (uint64_t *)S->dev + f * halfN;
+    }
+    S->half = num_vars ? N / 2 : 0;
+    if (num_vars) { // round 0's evaluations over the caller's tables
+        const uint32_t g = sc_grid(S->half);
+        S->K.eval(dim3(g), S->stream, S->tabs, S->P.T, S->half, ScEqArgs{}, S->partial);
+        hipError_t le = hipGetLastError();
+        if (le == hipSuccess) st = session_reduce(S, g);
+        else st = device_fail(le, "sc_eval_kernel");
+    } else { // no variables: the claims are the tables' single entries
+        for (size_t f = 0; f < nf && st == LZX_OK; f++) {
+            const hipError_t ce =
+                hipMemcpyAsync(S->claims + f, factors_dev[f], 8, hipMemcpyDeviceToDevice, S->stream);
+            if (ce != hipSuccess) st = device_fail(ce, "session claims");
+        }
+    }
+    if (st != LZX_OK) {
+        (void)scratch_free(S->dev, S->stream);
+        delete S;
+        return st;
+    }
+    *out = S;
+    return LZX_OK;
+}
+
+extern "C" size_t lzx_sumcheck_session_round_len(const lzx_sumcheck_session *S) { return S ? S->width : 0; }
+
+extern "C" lzx_status lzx_sumcheck_session_round_eval(lzx_sumcheck_session *S, uint64_t *evals_out)
+{
+    if (!S || !evals_out) return LZX_INVALID_ARGUMENT;
+    if (S->half == 0) return LZX_SUMCHECK_BAD_ROUND_SHAPE; // every variable is bound
+    if (S->zero) {
+        evals_out[0] = evals_out[1] = 0;
+        return LZX_OK;
+    }
+    LZX_HIP(hipMemcpyAsync(evals_out, S->evals, S->width * 8, hipMemcpyDeviceToHost, S->stream));
+    LZX_HIP(hipStreamSynchronize(S->stream));
+    return LZX_OK;
+}
+
+extern "C" lzx_status lzx_sumcheck_session_fold(lzx_sumcheck_session *S, uint64_t r)
+{
+    if (!S || r >= GL_P) return LZX_INVALID_ARGUMENT;
+    if (S->half == 0) return LZX_SUMCHECK_BAD_ROUND_SHAPE;
+    S->round++;
+    if (S->zero) {
+        S->half >>= 1;
+        return LZX_OK;
+    }
+    LZX_HIP(hipMemcpyAsync(S->rdev, &r, 8, hipMemcpyHostToDevice, S->stream));
+    if (S->half == 1) {
+        hipLaunchKernelGGL(sc_last_fold_kernel, dim3(1), dim3(64), 0, S->stream, S->tabs, (uint32_t)S->P.nf, S->rdev,
+                           S->claims);
+        LZX_LAUNCH_CHECK("sc_last_fold_kernel");
+        S->half = 0;
+    } else {
+        const uint32_t g = sc_grid(S->half / 2);
+        S->K.fold(dim3(g), S->stream, S->tabs, S->P.T, S->half, S->rdev, ScEqArgs{}, S->partial);
+        LZX_LAUNCH_CHECK("sc_fold_eval_kernel");
+        for (size_t f = 0; f < S->P.nf; f++) S->tabs.src[f] = S->tabs.dst[f]; // later rounds fold in place
+        S->half >>= 1;
+        const lzx_status st = session_reduce(S, g);
+        if (st != LZX_OK) return st;
+    }
+    // r is pageable host memory: the copy must land before the call returns
+    LZX_HIP(hipStreamSynchronize(S->stream));
+    return LZX_OK;
+}
+
+extern "C" lzx_status lzx_sumcheck_session_final(lzx_sumcheck_session *S, uint64_t *factor_claims_out,
+                                                 uint64_t *final_claim)
+{
+    if (!S || !final_claim) return LZX_INVALID_ARGUMENT;
+    if (S->half != 0) return LZX_SUMCHECK_BAD_ROUND_SHAPE; // variables left to bind
+    if (S->zero) {
+        *final_claim = 0;
+        return LZX_OK;
+    }
+    if (S->P.nf && !factor_claims_out) return LZX_INVALID_ARGUMENT;
+    LZX_HIP(hipMemcpyAsync(factor_claims_out, S->claims, S->P.nf * 8, hipMemcpyDeviceToHost, S->stream));
+    LZX_HIP(hipStreamSynchronize(S->stream));
+    *final_claim = host_eval_terms(S->P, factor_claims_out); // sumcheck.rs:144-154
+    return LZX_OK;
+}
+
+extern "C" void lzx_sumcheck_session_destroy(lzx_sumcheck_session *S)
+{
+    if (!S) return;
+    if (S->dev) {
+        (void)scratch_free(S->dev, S->stream);
+        (void)hipStreamSynchronize(S->stream);
+    }
+    delete S;
+}

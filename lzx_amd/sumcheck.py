@@ -144,8 +144,9 @@ def _prove(vp: VirtualPolynomial, claim: int, transcript, challenges, stream):
                                                       fclaims.ctypes.data, C.byref(deg), dev.stream_handle(stream))
     check(st, "sumcheck::prove")
     w = deg.value + 1
+    # the zero polynomial's proof carries no factor claims (sumcheck.rs:86-91)
     return SumcheckOutput(rounds[: vp.num_vars * w].reshape(vp.num_vars, w), chals[: vp.num_vars], fin.value,
-                          fclaims[: len(vp.factors)])
+                          fclaims[: len(vp.factors) if vp.terms else 0])
 
 
 def prove(vp: VirtualPolynomial, claim: int, transcript: Transcript, stream=None) -> SumcheckOutput:
@@ -183,7 +184,7 @@ def prove_fast_with_eq(vp: VirtualPolynomial, claim: int, transcript: Transcript
           "fastprover::prove_fast_with_eq")
     wd = deg.value + 1
     return SumcheckOutput(rounds[: vp.num_vars * wd].reshape(vp.num_vars, wd), chals[: vp.num_vars], fin.value,
-                          fclaims[: len(vp.factors)])
+                          fclaims[: len(vp.factors) if vp.terms else 0])
 
 
 def prove_with_challenges(vp: VirtualPolynomial, claim: int, challenges, stream=None) -> SumcheckOutput:
@@ -398,3 +399,86 @@ def verify_batch_star_with_rhos(rounds, num_vars, max_degree: int, claimed_sums,
                                 expected_combined_final=None) -> np.ndarray:
     """batch::verify_batch_star_with_rhos (batch.rs:268-296)."""
     return verify_batch_star(rounds, num_vars, max_degree, claimed_sums, transcript, expected_combined_final, rhos)
+
+
+# ---- round-level session ------------------------------------------------------------
+
+class SumcheckSession:
+    """The prover's rounds one call at a time (sumcheck.rs:100-143), the
+    transcript steps left to the caller: round_eval() gives g_j(0..d) for the
+    current round, fold(r) binds its variable, final() the factor claims and
+    P(r) once every variable is bound."""
+
+    def __init__(self, vp: VirtualPolynomial, stream=None):
+        if any(f is None for f in vp.factors):
+            raise ValueError("a session needs every factor table")
+        self._vp = vp  # the caller's tables are read until the first fold
+        ptrs, coeffs, off, ids = vp._ffi()
+        self._h = C.c_void_p()
+        check(lib().lzx_sumcheck_session_create(vp.num_vars, len(vp.factors), ptrs, len(vp.terms), coeffs.ctypes.data,
+                                                off.ctypes.data, ids.ctypes.data, dev.stream_handle(stream),
+                                                C.byref(self._h)), "sumcheck session")
+        self.width = int(lib().lzx_sumcheck_session_round_len(self._h))
+        self.nfactors = len(vp.factors)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().lzx_sumcheck_session_destroy(h)
+            self._h = None
+
+    def round_eval(self) -> np.ndarray:
+        out = np.zeros(self.width, dtype=np.uint64)
+        check(lib().lzx_sumcheck_session_round_eval(self._h, out.ctypes.data), "sumcheck session round_eval")
+        return out
+
+    def fold(self, r: int) -> None:
+        check(lib().lzx_sumcheck_session_fold(self._h, int(r)), "sumcheck session fold")
+
+    def final(self):
+        """(factor_claims, P(r))."""
+        claims = np.zeros(max(1, self.nfactors), dtype=np.uint64)
+        fin = C.c_uint64()
+        check(lib().lzx_sumcheck_session_final(self._h, claims.ctypes.data, C.byref(fin)), "sumcheck session final")
+        return claims[: self.nfactors], fin.value
+
+
+def interpolate_at(evals, r: int) -> int:
+    """interpolate_at (sumcheck.rs:286-306): the degree-(len-1) polynomial through
+    (i, evals[i]), i = 0.., evaluated at r, in the Goldilocks field."""
+    n, r = len(evals), int(r) % GL_P
+    acc = 0
+    for i in range(n):
+        num, den = 1, 1
+        for j in range(n):
+            if j != i:
+                num = num * ((r - j) % GL_P) % GL_P
+                den = den * ((i - j) % GL_P) % GL_P
+        acc = (acc + int(evals[i]) * num % GL_P * pow(den, GL_P - 2, GL_P)) % GL_P
+    return acc
+
+
+def prove_with_session(vp: VirtualPolynomial, claim: int, transcript: Transcript, stream=None) -> SumcheckOutput:
+    """sumcheck::prove (sumcheck.rs:62-162) driven round by round through a
+    session: per round absorb "sumcheck-round", draw "sumcheck-challenge",
+    check g(0) + g(1) against the running claim, interpolate, fold."""
+    S = SumcheckSession(vp, stream)
+    m = vp.num_vars
+    rounds = np.zeros((m, S.width), dtype=np.uint64)
+    chals = np.zeros(m, dtype=np.uint64)
+    current = int(claim) % GL_P
+    if not vp.terms and current != 0:
+        check(13, "sumcheck::prove")  # SumcheckError::ClaimMismatch
+    for j in range(m):
+        ev = S.round_eval()
+        transcript.append_field_slice(b"sumcheck-round", ev)
+        r = transcript.challenge_field(b"sumcheck-challenge")
+        if (int(ev[0]) + int(ev[1])) % GL_P != current:
+            check(13, "sumcheck::prove")  # SumcheckError::ClaimMismatch
+        current = interpolate_at(ev, r)
+        S.fold(r)
+        rounds[j], chals[j] = ev, r
+    claims, fin = S.final() if vp.terms else (np.zeros(0, dtype=np.uint64), 0)
+    if vp.terms and fin != current:
+        check(12, "sumcheck::prove")  # SumcheckError::FinalCheckFailed
+    return SumcheckOutput(rounds, chals, fin, claims)

@@ -321,3 +321,65 @@ def test_zerocheck_nonzero_table_with_vanishing_point(sc, nv):
     with pytest.raises(LzxError) as e:
         S.zerocheck_prove(D.to_device(p), nv, S.Transcript(b"zc-vanish"))
     assert e.value.code == O.ZEROCHECK_FINAL_CLAIM_NONZERO
+
+
+# ---- round-level session -------------------------------------------------------------
+
+SESSION_SHAPES = [(1, 2, [(1, [0, 1])]), (5, 3, [(3, [0, 1]), (5, [1, 2, 0]), (11, [2])]),
+                  (13, 4, [(1, [0, 1, 2]), (P - 1, [0, 3])]), (16, 2, [(7, [0, 0, 1])]),
+                  (9, 8, [(i + 1, [i, (i + 3) % 8]) for i in range(8)])]
+
+
+@pytest.mark.parametrize("nv,nf,terms", SESSION_SHAPES)
+def test_session_rounds_match_oracle(sc, nv, nf, terms):
+    S, D = sc
+    rng = np.random.default_rng(nv * 7 + nf)
+    tables = [rng.integers(0, P, 1 << nv, dtype=np.uint64) for _ in range(nf)]
+    claim = O.vp_sum_over_hypercube(nv, tables, terms)
+    st, ref = O.sumcheck_prove(nv, tables, terms, claim, O.Transcript(b"lzx-session"))
+    assert st == 0
+    sess = S.SumcheckSession(_vp(S, D, nv, tables, terms))
+    for j in range(nv):
+        assert sess.round_eval().tolist() == ref["rounds"][j].tolist(), j
+        sess.fold(int(ref["challenges"][j]))
+    claims, fin = sess.final()
+    assert claims.tolist() == ref["factor_claims"].tolist() and fin == ref["final_claim"]
+
+
+@pytest.mark.parametrize("nv,nf,terms", SESSION_SHAPES[:4] + [(0, 2, [(4, [0, 1])]), (4, 2, [])])
+def test_prove_with_session_equals_device_transcript_prove(sc, nv, nf, terms):
+    S, D = sc
+    rng = np.random.default_rng(nv + 100)
+    tables = [rng.integers(0, P, 1 << nv, dtype=np.uint64) for _ in range(nf)]
+    claim = O.vp_sum_over_hypercube(nv, tables, terms) if terms else 0
+    ta, tb = S.Transcript(b"lzx-session-eq"), S.Transcript(b"lzx-session-eq")
+    a = S.prove_with_session(_vp(S, D, nv, tables, terms), claim, ta)
+    b = S.prove(_vp(S, D, nv, tables, terms), claim, tb)
+    assert a.rounds.tolist() == b.rounds.tolist() and a.challenges.tolist() == b.challenges.tolist()
+    assert a.final_claim == b.final_claim and a.factor_claims.tolist() == b.factor_claims.tolist()
+    assert ta.challenge_field(b"next") == tb.challenge_field(b"next")  # both transcripts end in one state
+
+
+def test_session_errors(sc):
+    from lzx_amd._lib import LzxError
+    S, D = sc
+    rng = np.random.default_rng(3)
+    tables = [rng.integers(0, P, 8, dtype=np.uint64) for _ in range(2)]
+    sess = S.SumcheckSession(_vp(S, D, 3, tables, [(1, [0, 1])]))
+    with pytest.raises(LzxError) as e:  # variables left to bind
+        sess.final()
+    assert e.value.code == 10
+    for r in (5, 6, 7):
+        sess.round_eval()
+        sess.fold(r)
+    for call in (sess.round_eval, lambda: sess.fold(1)):
+        with pytest.raises(LzxError) as e:
+            call()
+        assert e.value.code == 10
+    with pytest.raises(LzxError) as e:  # a challenge outside the field
+        S.SumcheckSession(_vp(S, D, 3, tables, [(1, [0, 1])])).fold(P)
+    assert e.value.code == 30
+    tampered = 1 + O.vp_sum_over_hypercube(3, tables, [(1, [0, 1])])
+    with pytest.raises(LzxError) as e:  # sumcheck.rs:120-123
+        S.prove_with_session(_vp(S, D, 3, tables, [(1, [0, 1])]), tampered, S.Transcript(b"x"))
+    assert e.value.code == 13
