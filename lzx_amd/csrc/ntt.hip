@@ -72,41 +72,6 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
 // Prefetching into VGPRs (not global_load_lds) lets the compiler wait only for
 // the registers it stages; an LDS DMA would make it wait for every DMA before
 // any LDS access, the next tile's included.
-template <uint32_t C>
-__device__ __forceinline__ uint32_t tile_slot(uint32_t p, uint32_t c)
-{
-    return C * p + ((c + p) & (C - 1));
-}
-
-// The pipe kernel's blocks are one wave: lanes see each other's LDS writes in
-// issue order (a wave's LDS operations execute in order), so a wave-scope
-// fence and a code-motion barrier order them.  A workgroup barrier's release
-// fence would also wait for the wave's in-flight global stores.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Chunk i of a lane's prefetch: the 16-byte piece of LDS slot s = 64 i + lane,
-// i.e. chunk ((s mod C) - p) mod C of polynomial p = s / C (the rotated layout
-// of tile_slot), read from HBM with the same coalesced addressing.  Slots past
-// the tile's last polynomial reload polynomial 0 (in bounds; never stored), so
-// every load is unconditional and the array stays in VGPRs.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4))); // promotable to VGPRs across loop iterations
-
-template <uint32_t C>
-__device__ __forceinline__ void tile_fetch(const uint32_t *__restrict__ g, uint32_t np, u32x4 (&r)[C], uint32_t lane)
-{
-#pragma unroll
-    for (uint32_t i = 0; i < C; i++) {
-        const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
-        const uint32_t pp = p < np ? p : 0;
-        r[i] = *reinterpret_cast<const u32x4 *>(g + (pp * C + c) * 4);
-    }
-}
-
 template <int LOG_N, int AR, int MODE>
 __global__ __launch_bounds__(64) void ntt_pipe_kernel(uint32_t *__restrict__ data, TabSet tabs, uint32_t polys,
                                                       uint32_t blocks_per_plan, uint32_t total)

@@ -81,4 +81,45 @@ __device__ __forceinline__ void ntt_regs(uint32_t (&a)[1u << LOG_N], const uint3
     }
 }
 
+// ---- tiles of 64 polynomials, shared by the one-wave register kernels --------
+
+template <uint32_t C>
+__device__ __forceinline__ uint32_t tile_slot(uint32_t p, uint32_t c)
+{
+    return C * p + ((c + p) & (C - 1));
+}
+
+// Chunk c (16 bytes) of polynomial p of a 64-polynomial tile lives at LDS
+// slot tile_slot(p, c): a lane reading its own polynomial's chunks, and a
+// wave writing 64 consecutive slots, are both bank-conflict free.
+
+// For blocks of one wave: lanes see each other's LDS writes in issue order (a
+// wave's LDS operations execute in order), so a wave-scope fence and a
+// code-motion barrier order them.  A workgroup barrier's release fence would
+// also wait for the wave's in-flight global stores.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Chunk i of a lane's prefetch: the 16-byte piece of LDS slot s = 64 i + lane,
+// i.e. chunk ((s mod C) - p) mod C of polynomial p = s / C (the rotated layout
+// of tile_slot), read from HBM with the same coalesced addressing.  Slots past
+// the tile's last polynomial reload polynomial 0 (in bounds; never stored), so
+// every load is unconditional and the array stays in VGPRs.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4))); // promotable to VGPRs across loop iterations
+
+template <uint32_t C>
+__device__ __forceinline__ void tile_fetch(const uint32_t *__restrict__ g, uint32_t np, u32x4 (&r)[C], uint32_t lane)
+{
+#pragma unroll
+    for (uint32_t i = 0; i < C; i++) {
+        const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+        const uint32_t pp = p < np ? p : 0;
+        r[i] = *reinterpret_cast<const u32x4 *>(g + (pp * C + c) * 4);
+    }
+}
+
 } // namespace lzx

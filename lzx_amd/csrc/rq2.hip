@@ -4,12 +4,13 @@
 // each, so a batch is [count][2][n] (the reference's to_bytes order).
 //
 // rq2_kernel (n = 4 .. 64): one 64-lane wave owns a tile of 32 elements, lane
-// 2e + h holding half h of element e. Both operands stream through LDS tiles
-// whose row l is exactly lane l's polynomial, so global loads and stores are
-// coalesced and the per-lane row reads are bank-conflict free (rows padded to
-// n + 1 words). Each lane transforms its halves in VGPRs (the shared register
-// NTT), combines slot-wise with its partner lane (lane ^ 1, one DPP move per
-// value) and inverts its result half:
+// 2e + h holding half h of element e, so row l of a 64-polynomial tile is
+// exactly lane l's polynomial. Each operand arrives as coalesced 16-byte
+// chunks (b's loads in flight while a is transformed) and is transposed
+// through one LDS tile in the rotated, bank-conflict-free slot layout of the
+// batched NTT (16 KiB at n = 64: two waves per SIMD). Each lane transforms its
+// halves in VGPRs (the shared register NTT), combines slot-wise with its
+// partner lane (lane ^ 1, one DPP move per value) and inverts its result half:
 //   mul      c0 = a0 b0 - a1 b1 (even lane), c1 = a0 b1 + a1 b0 (odd lane),
 //            two slot products per lane (in the NTT domain the Karatsuba
 //            saving is one slot product; the result is the same ring element);
@@ -61,46 +62,51 @@ template <int LOG_N, int AR, int MODE>
 __global__ __launch_bounds__(64) void rq2_kernel(const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
                                                  const uint32_t *__restrict__ tab, uint64_t barrett, size_t count)
 {
-    constexpr uint32_t N = 1u << LOG_N;
-    constexpr uint32_t S = N + 1;
-    __shared__ uint32_t ta[64 * S];
-    __shared__ uint32_t tb[MODE == RQ2_MUL ? 64 * S : 1];
+    constexpr uint32_t N = 1u << LOG_N, C = N / 4;
+    __shared__ __attribute__((aligned(16))) uint32_t T[64 * N]; // one tile: row l = lane l's polynomial
     const QConst Q{tab[0], tab[1], tab[5], tab[6]};
     const uint32_t q = Q.q;
     const uint32_t lane = threadIdx.x, odd = lane & 1;
     const size_t tiles = (count + RQ2_TILE - 1) / RQ2_TILE;
+    u32x4 r[C];
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const size_t e0 = t * RQ2_TILE;
         const uint32_t ne = count - e0 < RQ2_TILE ? (uint32_t)(count - e0) : RQ2_TILE;
-        const uint32_t words = 2 * ne * N; // rows 0 .. 2 ne - 1
-        const uint32_t *ga = a + e0 * 2 * N;
-        for (uint32_t i = lane; i < words; i += 64) ta[(i >> LOG_N) * S + (i & (N - 1))] = ga[i];
-        if (MODE == RQ2_MUL) {
-            const uint32_t *gb = b + e0 * 2 * N;
-            for (uint32_t i = lane; i < words; i += 64) tb[(i >> LOG_N) * S + (i & (N - 1))] = gb[i];
-        }
-        __syncthreads();
+        const uint32_t rows = 2 * ne;
+        tile_fetch<C>(a + e0 * 2 * N, rows, r, lane);
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
+        wave_sync();
         // every lane computes (rows past 2 ne hold stale values and are not stored),
         // so the partner exchange always sees both lanes of a pair
         uint32_t x[N];
-        const uint32_t *row = ta + lane * S;
 #pragma unroll
-        for (uint32_t c = 0; c < N; c++) x[c] = row[c];
+        for (uint32_t c4 = 0; c4 < C; c4++) {
+            const u32x4 v = *reinterpret_cast<const u32x4 *>(T + 4 * tile_slot<C>(lane, c4));
+            x[4 * c4] = v.x, x[4 * c4 + 1] = v.y, x[4 * c4 + 2] = v.z, x[4 * c4 + 3] = v.w;
+        }
+        if (MODE == RQ2_MUL) tile_fetch<C>(b + e0 * 2 * N, rows, r, lane); // in flight during a's transform
         ntt_regs<LOG_N, AR, NTT_FWD>(x, tab); // x[j]: evaluation brv(j), the order the inverse reads
         bool good = true;
         if (MODE == RQ2_MUL) {
-            uint32_t y[N];
-            const uint32_t *rb = tb + lane * S;
+            wave_sync(); // every lane has read its row of a: b goes through the same tile
 #pragma unroll
-            for (uint32_t c = 0; c < N; c++) y[c] = rb[c];
+            for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
+            wave_sync();
+            uint32_t y[N];
+#pragma unroll
+            for (uint32_t c4 = 0; c4 < C; c4++) {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(T + 4 * tile_slot<C>(lane, c4));
+                y[4 * c4] = v.x, y[4 * c4 + 1] = v.y, y[4 * c4 + 2] = v.z, y[4 * c4 + 3] = v.w;
+            }
             ntt_regs<LOG_N, AR, NTT_FWD>(y, tab);
 #pragma unroll
             for (uint32_t j = 0; j < N; j++) {
                 const uint32_t pa = rq2_swap(x[j]), pb = rq2_swap(y[j]);
                 // even: x y - pa pb = a0 b0 - a1 b1; odd: pa y + x pb = a0 b1 + a1 b0
                 const uint32_t p = rq2_mul(odd ? pa : x[j], y[j], q, barrett);
-                const uint32_t r = rq2_mul(odd ? x[j] : pa, pb, q, barrett);
-                x[j] = odd ? addq<AR>(p, r, Q) : subq<AR>(p, r, Q);
+                const uint32_t rr = rq2_mul(odd ? x[j] : pa, pb, q, barrett);
+                x[j] = odd ? addq<AR>(p, rr, Q) : subq<AR>(p, rr, Q);
             }
         } else if (MODE == RQ2_NORM) {
 #pragma unroll
@@ -109,13 +115,12 @@ __global__ __launch_bounds__(64) void rq2_kernel(const uint32_t *a, const uint32
                 x[j] = addq<AR>(s, rq2_swap(s), Q);
             }
         } else {
-            // pre[j] = N_0 ... N_{j-1}, kept in this lane's tile row (read already)
-            uint32_t *pre = ta + lane * S;
+            // pre[j] = N_0 ... N_{j-1}, kept in this lane's own row of the tile (read already)
             uint32_t acc = 1;
 #pragma unroll
             for (uint32_t j = 0; j < N; j++) {
                 const uint32_t s = rq2_mul(x[j], x[j], q, barrett);
-                pre[j] = acc;
+                T[4 * tile_slot<C>(lane, j / 4) + (j % 4)] = acc;
                 acc = rq2_mul(acc, addq<AR>(s, rq2_swap(s), Q), q, barrett);
             }
             good = acc != 0; // q is prime: the product vanishes iff a slot does
@@ -125,27 +130,33 @@ __global__ __launch_bounds__(64) void rq2_kernel(const uint32_t *a, const uint32
                 const uint32_t j = N - 1 - jj;
                 const uint32_t s = rq2_mul(x[j], x[j], q, barrett);
                 const uint32_t nj = addq<AR>(s, rq2_swap(s), Q);
-                const uint32_t ninv = rq2_mul(inv, pre[j], q, barrett);
+                const uint32_t ninv = rq2_mul(inv, T[4 * tile_slot<C>(lane, j / 4) + (j % 4)], q, barrett);
                 inv = rq2_mul(inv, nj, q, barrett);
                 const uint32_t v = odd ? (x[j] ? q - x[j] : 0) : x[j]; // conj: -c1 on the odd lane
                 x[j] = rq2_mul(v, ninv, q, barrett);
             }
         }
         ntt_regs<LOG_N, AR, NTT_INV>(x, tab);
-        // results back through the tile (every lane has read its rows already)
+        // results back through the tile: row l for the product and inverse, row l/2
+        // (the element) for the norm, written by the even lane
+        wave_sync();
         const uint32_t orow = MODE == RQ2_NORM ? lane >> 1 : lane;
-        const bool writes = lane < 2 * ne && (MODE != RQ2_NORM || !odd);
-        if (writes) {
-            uint32_t *w = ta + orow * S;
+        if (MODE != RQ2_NORM || !odd) {
 #pragma unroll
-            for (uint32_t c = 0; c < N; c++) w[c] = x[c];
+            for (uint32_t c4 = 0; c4 < C; c4++)
+                *reinterpret_cast<u32x4 *>(T + 4 * tile_slot<C>(orow, c4)) =
+                    u32x4{x[4 * c4], x[4 * c4 + 1], x[4 * c4 + 2], x[4 * c4 + 3]};
         }
-        if (MODE == RQ2_INV && lane < 2 * ne && !odd) ok[e0 + (lane >> 1)] = good ? 1 : 0;
-        __syncthreads();
-        const uint32_t owords = MODE == RQ2_NORM ? ne * N : words;
+        if (MODE == RQ2_INV && lane < rows && !odd) ok[e0 + (lane >> 1)] = good ? 1 : 0;
+        wave_sync();
+        const uint32_t orows = MODE == RQ2_NORM ? ne : rows;
         uint32_t *go = out + e0 * (MODE == RQ2_NORM ? N : 2 * N);
-        for (uint32_t i = lane; i < owords; i += 64) go[i] = ta[(i >> LOG_N) * S + (i & (N - 1))];
-        __syncthreads(); // the next tile overwrites ta
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) {
+            const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+            if (p < orows) *reinterpret_cast<u32x4 *>(go + (p * C + c) * 4) = *reinterpret_cast<const u32x4 *>(T + 4 * s);
+        }
+        wave_sync(); // the stores have read the tile before the next one is staged
     }
 }
 
@@ -296,7 +307,8 @@ static lzx_status rq2_run(int mode, const lzx_ntt_plan *plan, const uint32_t *a,
     if (!plan || !a || !out || (mode == RQ2_MUL && !b) || (mode == RQ2_INV && !ok)) return LZX_INVALID_ARGUMENT;
     if (count == 0) return LZX_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (plan->log_n >= 2 && plan->log_n <= 6) {
+    const bool aligned = (((uintptr_t)a | (uintptr_t)(b ? b : a) | (uintptr_t)out) & 15) == 0;
+    if (plan->log_n >= 2 && plan->log_n <= 6 && aligned) {
         if (plan->arith == ARITH_MONT)
             dispatch_fused<ARITH_MONT>(mode, a, b, out, ok, plan, count, s);
         else
