@@ -278,7 +278,7 @@ def bench_ring50(reps: int = 10):
     """The ~2^50 quadratic-slot ring (q = 2^50 - 2687): 2^16 polynomials of
     n = 128, u64 residues.  Forward transform in place (read + write 64 MiB) and
     the fused ring product (forward both, slotwise Karatsuba, inverse: read
-    2 x 64 MiB, write 64 MiB); HIP events on the launch stream."""
+    2 x 64 MiB, write 64 MiB)."""
     import torch
 
     from lzx_amd import device as D
@@ -290,24 +290,10 @@ def bench_ring50(reps: int = 10):
     b = D.to_device(rng.integers(0, R.Q_50, polys * n, dtype=np.uint64))
     out = D.empty(polys * n, np.uint64)
     T = R.QuadSlotTables(R.Modulus50.Q_50, 7)
-    stream = torch.cuda.current_stream()
     words = polys * n
 
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        ts = []
-        for _ in range(reps):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            fn()
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ts.append(e0.elapsed_time(e1) * 1e-3)
-        return statistics.median(ts)
-
-    t_fwd = timed(lambda: T.forward_batch(a))
-    t_mul = timed(lambda: T.mul_batch(a, b, out))
+    t_fwd = _timed_events(torch, lambda: T.forward_batch(a), reps)
+    t_mul = _timed_events(torch, lambda: T.mul_batch(a, b, out), reps)
     return {"workload": "quadratic-slot ring q=2^50-2687, 2^16 polys x n=128, u64",
             "forward_us": round(t_fwd * 1e6, 2), "forward_gbs": round(2 * 8 * words / t_fwd / 1e9, 1),
             "mul_us": round(t_mul * 1e6, 2), "mul_gbs": round(3 * 8 * words / t_mul / 1e9, 1),
@@ -315,19 +301,23 @@ def bench_ring50(reps: int = 10):
 
 
 def _timed_events(torch, fn, reps):
-    """Median seconds of fn over reps, HIP events on the launch stream (torch's current stream)."""
+    """Seconds per call of fn: reps back-to-back calls between two HIP events on
+    the launch stream (torch's current stream), after one warm-up call.  Calls
+    queue behind each other, so the host's launch latency is hidden as it is
+    in a pipeline of such calls; the median of three such runs."""
     stream = torch.cuda.current_stream()
     fn()
     torch.cuda.synchronize()
-    ts = []
-    for _ in range(reps):
+    runs = []
+    for _ in range(3):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        fn()
+        for _ in range(reps):
+            fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1) * 1e-3)
-    return statistics.median(ts)
+        runs.append(e0.elapsed_time(e1) * 1e-3 / reps)
+    return statistics.median(runs)
 
 
 def bench_extension(reps: int = 10):

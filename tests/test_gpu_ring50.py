@@ -50,6 +50,37 @@ def test_transforms_and_products_match_oracle(r50, log_n, polys):
     assert np.array_equal(D.to_host(ta, np.uint64), a)
 
 
+@pytest.mark.parametrize("log_n", range(1, 8))
+def test_extreme_residues_every_path(r50, log_n):
+    """Inputs drawn from the largest and middle residues (q-1, q-2, (q+-1)/2, 0, 1)
+    push the kernels' intermediate ranges to their bounds on every path."""
+    R, D = r50
+    n = 1 << log_n
+    polys = 300
+    rng = np.random.default_rng(50 + log_n)
+    pool = np.array([Q - 1, Q - 2, (Q - 1) // 2, (Q + 1) // 2, 0, 1], dtype=np.uint64)
+    a = pool[rng.integers(0, len(pool), polys * n)]
+    b = pool[rng.integers(0, 2, polys * n)]  # q-1 and q-2 only
+    a[:n] = Q - 1
+    b[:n] = Q - 1
+    T = R.QuadSlotTables(R.Modulus50.Q_50, log_n)
+    ta = D.to_device(a)
+    T.forward_batch(ta)
+    _, fa = O.quad_forward(Q, ADIC, GEN, log_n, a)
+    assert np.array_equal(D.to_host(ta, np.uint64), fa)
+    ti = D.to_device(a)  # a as slot-form input to the inverse
+    T.inverse_batch(ti)
+    _, back = O.quad_inverse(Q, ADIC, GEN, log_n, a)
+    assert np.array_equal(D.to_host(ti, np.uint64), back)
+    out = D.empty(polys * n, np.uint64)
+    T.slotwise_mul_batch(D.to_device(a), D.to_device(b), out)
+    _, sm = O.quad_slotwise_mul(Q, ADIC, GEN, log_n, a, b)
+    assert np.array_equal(D.to_host(out, np.uint64), sm)
+    T.mul_batch(D.to_device(a), D.to_device(b), out)
+    _, prod = O.quad_mul(Q, ADIC, GEN, log_n, a, b)
+    assert np.array_equal(D.to_host(out, np.uint64), prod)
+
+
 def test_edge_values_and_ring_identities(r50):
     R, D = r50
     ring = R.RingConfig50(7)
