@@ -300,6 +300,31 @@ def bench_ring50(reps: int = 10):
             "mul_polys_per_s": round(polys / t_mul, 1)}
 
 
+def bench_ntt_q32(reps: int = 10):
+    """The register NTT at the reference's default modulus q = 3*2^30 + 1
+    (above 2^31, so the kernels run exact FP64 butterflies): 2^18 polynomials
+    of d = 64, forward and inverse per call (134 MB each).  Every forward call
+    is matched by an inverse call, so the data must come back unchanged."""
+    import torch
+
+    from lzx_amd import device as D
+    from lzx_amd import ring as R
+
+    polys = 1 << 18
+    rng = np.random.default_rng(0x32)
+    host = rng.integers(0, R.Modulus32.Q_32.q, polys * N, dtype=np.uint32)
+    data = D.to_device(host)
+    T = R.NttTables(R.Modulus32.Q_32, LOG_N)
+    t_fwd = _timed_events(torch, lambda: T.forward_batch(data), reps)
+    t_inv = _timed_events(torch, lambda: T.inverse_batch(data), reps)
+    ok = bool(np.array_equal(D.to_host(data, np.uint32).reshape(-1), host))
+    bytes_pass = 2 * polys * N * 4
+    return {"workload": "batched negacyclic NTT, q=3221225473, 2^18 polys x d=64, u32",
+            "fwd_us": round(t_fwd * 1e6, 2), "fwd_gbs": round(bytes_pass / t_fwd / 1e9, 1),
+            "inv_us": round(t_inv * 1e6, 2), "inv_gbs": round(bytes_pass / t_inv / 1e9, 1),
+            "roundtrip_exact": ok}
+
+
 def _timed_events(torch, fn, reps):
     """Seconds per call of fn: reps back-to-back calls between two HIP events on
     the launch stream (torch's current stream), after one warm-up call.  Calls
@@ -467,6 +492,7 @@ def main():
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven commitment batch leg")
     ap.add_argument("--only-sumcheck", action="store_true", help="run only the 2^24 sum-check prove leg")
     ap.add_argument("--only-ring50", action="store_true", help="run only the 50-bit quadratic-slot ring leg")
+    ap.add_argument("--only-ntt-q32", action="store_true", help="run only the default-modulus (q > 2^31) NTT leg")
     ap.add_argument("--only-extension", action="store_true", help="run only the R_q[Y]/(Y^2+1) leg")
     ap.add_argument("--only-packing", action="store_true", help="run only the witness-packing leg")
     ap.add_argument("--no-autotune", action="store_true",
@@ -484,6 +510,9 @@ def main():
     D.require_gpu()
     if args.only_ring50:
         print(json.dumps({"ring50": bench_ring50()}), flush=True)
+        return
+    if args.only_ntt_q32:
+        print(json.dumps({"ntt_q32": bench_ntt_q32()}), flush=True)
         return
     if args.only_extension:
         print(json.dumps({"extension": bench_extension()}), flush=True)
@@ -534,7 +563,8 @@ def main():
     if not args.only_ntt:
         line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
                              "uneven_batch": bench_uneven_batch(), "ring50": bench_ring50(),
-                             "extension": bench_extension(), "packing": bench_packing()}
+                             "extension": bench_extension(), "packing": bench_packing(),
+                             "ntt_q32": bench_ntt_q32()}
         if not args.no_cpu:
             line["secondary"]["cpu_baseline"] = cpu_baseline_secondary()
     print(json.dumps(line), flush=True)

@@ -99,8 +99,11 @@ LZX_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t q, uint32_t qinv)
 // Arithmetic classes of a u32 prime, chosen once per plan:
 //   ARITH_SHOUP  q < 2^31: Shoup constant multiplication (3 multiplies) and
 //                canonical add/sub in 32 bits;
-//   ARITH_MONT   q > 2^31: Montgomery (4 multiplies), 33-bit add/sub.
-enum Arith { ARITH_SHOUP = 0, ARITH_MONT = 2 };
+//   ARITH_MONT   q > 2^31: Montgomery (4 multiplies), 33-bit add/sub;
+//   ARITH_F64    any q < 2^32, register transforms only: exact FP64 products
+//                on lazily reduced values (ntt_regs_f64); the register
+//                kernels use it in place of ARITH_MONT.
+enum Arith { ARITH_SHOUP = 0, ARITH_F64 = 1, ARITH_MONT = 2 };
 
 struct QConst {
     uint32_t q, qinv, c, k;

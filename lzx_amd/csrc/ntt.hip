@@ -48,13 +48,13 @@ __global__ __launch_bounds__(64) void ntt_reg_kernel(uint32_t *__restrict__ data
     const uint32_t nw = np * N;
     for (uint32_t i = tid; i < nw; i += 64) tile[(i / N) * S + (i % N)] = g[i];
     __syncthreads();
-    uint32_t a[N];
+    ntt_val_t<AR> a[N];
     uint32_t *row = tile + tid * S;
 #pragma unroll
     for (uint32_t c = 0; c < N; c++) a[MODE == NTT_FWD ? c : brv_bits(c, LOG_N)] = row[c];
-    ntt_regs<LOG_N, AR, MODE>(a, tab);
+    ntt_regs_any<LOG_N, AR, MODE>(a, tab);
 #pragma unroll
-    for (uint32_t c = 0; c < N; c++) row[c] = a[MODE == NTT_FWD ? brv_bits(c, LOG_N) : c];
+    for (uint32_t c = 0; c < N; c++) row[c] = ntt_out<AR>(a[MODE == NTT_FWD ? brv_bits(c, LOG_N) : c], tab, N);
     __syncthreads();
     for (uint32_t i = tid; i < nw; i += 64) g[i] = tile[(i / N) * S + (i % N)];
 }
@@ -91,23 +91,31 @@ __global__ __launch_bounds__(64) void ntt_pipe_kernel(uint32_t *__restrict__ dat
         np = min(64u, polys - p0);
         return data + ((size_t)plan * polys + p0) * N;
     };
+    // PF: the next tile's loads are in flight in VGPRs while this one is
+    // transformed.  The FP64 class holds a polynomial in 2n VGPRs, so at n = 64
+    // a prefetch buffer would leave one wave per SIMD; it loads each tile at
+    // the top of its iteration instead and relies on two resident waves.
+    constexpr bool PF = AR != ARITH_F64;
     uint32_t np, plan;
-    u32x4 nx[C]; // the next tile, in flight while the current one is transformed
-    tile_fetch<C>(locate(b, np, plan), np, nx, lane);
+    u32x4 nx[C]; // the tile being staged (PF: the next tile, in flight)
     // The compiler's waits before staging nx count the vector memory ops issued
     // after each load; on the steady path those include the previous tile's C
     // stores, so the stores stay in flight.  Every other way into the loop
     // header (this prologue, a partial tile) ends with nothing pending, so it
     // cannot lower those counts.
     constexpr int WAIT_VM0 = 0x0F70; // s_waitcnt vmcnt(0), lgkmcnt and expcnt free
-    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+    if (PF) {
+        tile_fetch<C>(locate(b, np, plan), np, nx, lane);
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+    }
     for (; b < b_end; b++) {
         uint32_t *g = locate(b, np, plan);
         const uint32_t *__restrict__ tab = tabs.p[plan];
+        if (!PF) tile_fetch<C>(g, np, nx, lane);
 #pragma unroll
         for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = nx[i];
         wave_sync();
-        uint32_t a[N];
+        ntt_val_t<AR> a[N];
 #pragma unroll
         for (uint32_t c4 = 0; c4 < C; c4++) {
             const uint4 v = *reinterpret_cast<const uint4 *>(T + 4 * tile_slot<C>(lane, c4));
@@ -115,18 +123,19 @@ __global__ __launch_bounds__(64) void ntt_pipe_kernel(uint32_t *__restrict__ dat
 #pragma unroll
             for (uint32_t e = 0; e < 4; e++) a[MODE == NTT_FWD ? 4 * c4 + e : brv_bits(4 * c4 + e, LOG_N)] = vv[e];
         }
-        if (b + 1 < b_end) {
+        if (PF && b + 1 < b_end) {
             uint32_t npn, pn;
             const uint32_t *gn = locate(b + 1, npn, pn);
             tile_fetch<C>(gn, npn, nx, lane);
         }
-        ntt_regs<LOG_N, AR, MODE>(a, tab);
+        ntt_regs_any<LOG_N, AR, MODE>(a, tab);
         wave_sync(); // every lane has read its row before the results overwrite the tile
 #pragma unroll
         for (uint32_t c4 = 0; c4 < C; c4++) {
             uint32_t vv[4];
 #pragma unroll
-            for (uint32_t e = 0; e < 4; e++) vv[e] = a[MODE == NTT_FWD ? brv_bits(4 * c4 + e, LOG_N) : 4 * c4 + e];
+            for (uint32_t e = 0; e < 4; e++)
+                vv[e] = ntt_out<AR>(a[MODE == NTT_FWD ? brv_bits(4 * c4 + e, LOG_N) : 4 * c4 + e], tab, N);
             *reinterpret_cast<uint4 *>(T + 4 * tile_slot<C>(lane, c4)) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
         }
         wave_sync();
@@ -371,8 +380,13 @@ static void dispatch_reg_n(uint32_t log_n, NttMode mode, bool dma, dim3 grid, ui
 static void dispatch_reg(int arith, uint32_t log_n, NttMode mode, bool dma, dim3 grid, uint32_t *data,
                          const TabSet &ts, uint32_t polys, uint32_t bpp, uint32_t total, hipStream_t stream)
 {
+    // q > 2^31 runs the register kernels in exact FP64 (8 operations of ~4.3
+    // clocks per butterfly against Montgomery's 4 multiplies of ~4.1 clocks and
+    // 33-bit add/sub): 16% faster per pass at n = 64.  Below 2^31 a Shoup
+    // butterfly (3 multiplies, 9 single-rate operations) costs as much as the
+    // FP64 one and needs half the registers.
     if (arith == ARITH_MONT)
-        dispatch_reg_n<ARITH_MONT>(log_n, mode, dma, grid, data, ts, polys, bpp, total, stream);
+        dispatch_reg_n<ARITH_F64>(log_n, mode, dma, grid, data, ts, polys, bpp, total, stream);
     else
         dispatch_reg_n<ARITH_SHOUP>(log_n, mode, dma, grid, data, ts, polys, bpp, total, stream);
 }
@@ -482,6 +496,19 @@ static void build_tables(lzx_ntt_plan *p)
         const uint32_t e = brv_bits(k, log_n);
         put(TabLayout::fwd_w(n), TabLayout::fwd_wp(n), k, host_pow(p->psi, e, q));
         put(TabLayout::inv_w(n), TabLayout::inv_wp(n), k, host_pow(p->psi_inv, e, q));
+    }
+    // FP64 constants: each w (< 2^32, exact) beside fl(w / q)
+    const double qd = (double)q;
+    auto dset = [&](uint32_t off, double v) { memcpy(&t[TabLayout::dbl(n) + 2 * off], &v, sizeof v); };
+    auto dput = [&](uint32_t off, uint64_t w) { dset(off, (double)w), dset(off + 1, (double)w / qd); };
+    dset(0, qd);
+    dset(1, 1.0 / qd);
+    dput(2, p->n_inv);
+    dput(4, (uint64_t)z1inv * p->n_inv % q);
+    for (uint32_t k = 1; k < n; k++) {
+        const uint32_t e = brv_bits(k, log_n);
+        dput(TabLayout::dbl_fwd(n) + 2 * k, host_pow(p->psi, e, q));
+        dput(TabLayout::dbl_inv(n) + 2 * k, host_pow(p->psi_inv, e, q));
     }
     uint64_t pw = 1;
     for (uint32_t i = 0; i < n; i++) {
@@ -672,9 +699,10 @@ extern "C" lzx_status lzx_ntt_inverse_host(const lzx_ntt_plan *plan, uint32_t *a
 extern "C" lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, const uint32_t *b_dev,
                                          uint32_t *out_dev, size_t len, void *stream)
 {
-    if (!plan || !a_dev || !b_dev || !out_dev) return LZX_INVALID_ARGUMENT;
+    if (!plan) return LZX_INVALID_ARGUMENT;
     if (len % plan->n) return LZX_RING_LENGTH_MISMATCH;
-    if (len == 0) return LZX_OK;
+    if (len == 0) return LZX_OK; // an empty batch may pass null pointers
+    if (!a_dev || !b_dev || !out_dev) return LZX_INVALID_ARGUMENT;
     hipStream_t s = (hipStream_t)stream;
     uint32_t *tmp = nullptr;
     LZX_HIP(scratch_alloc((void **)&tmp, len * sizeof(uint32_t), s));

@@ -34,7 +34,13 @@ struct TabLayout {
     static __host__ __device__ constexpr uint32_t psi_wp(uint32_t n) { return HDR + 5 * n; }
     static __host__ __device__ constexpr uint32_t omg_w(uint32_t n) { return HDR + 6 * n; }
     static __host__ __device__ constexpr uint32_t omg_wp(uint32_t n) { return HDR + 6 * n + (n > 1 ? n / 2 : 1); }
-    static __host__ __device__ constexpr uint32_t words(uint32_t n) { return HDR + 7 * n + 2; }
+    // FP64 constants (8-byte aligned; offsets in doubles from dbl(n)):
+    // [q, 1/q, n^-1, fl(n^-1/q), last-level w, fl(w/q), 0, 0], then a
+    // (w, fl(w/q)) pair per forward twiddle k < n, then one per inverse twiddle
+    static __host__ __device__ constexpr uint32_t dbl(uint32_t n) { return (HDR + 7 * n + 2 + 1) & ~1u; }
+    static __host__ __device__ constexpr uint32_t dbl_fwd(uint32_t) { return 8; }
+    static __host__ __device__ constexpr uint32_t dbl_inv(uint32_t n) { return 8 + 2 * n; }
+    static __host__ __device__ constexpr uint32_t words(uint32_t n) { return dbl(n) + 2 * (8 + 4 * n); }
 };
 
 constexpr uint32_t NTT_MAX_LOG_N = 13; // LDS tile limit of the generic kernel

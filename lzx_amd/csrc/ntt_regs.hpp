@@ -2,6 +2,8 @@
 // and the fused extension-ring kernels share); tables as laid out by TabLayout.
 #pragma once
 
+#include <type_traits>
+
 #include "ntt.hpp"
 
 namespace lzx {
@@ -78,6 +80,115 @@ __device__ __forceinline__ void ntt_regs(uint32_t (&a)[1u << LOG_N], const uint3
                 }
             }
         }
+    }
+}
+
+// ---- FP64 register transforms (any q < 2^32) ----------------------------------
+//
+// Residues are integer doubles.  For |a| < 2^40 and a constant 0 <= w < q,
+// a w = hi + lo exactly (hi = fl(a w), lo = fma(a, w, -hi)); t = rint(a fl(w/q))
+// is within 1/2 + 2^-11 of a w / q, so r = fma(-t, q, hi) + lo = a w - t q
+// exactly, |r| <= q/2 + 2^-11 q: six FP64 operations in place of Montgomery's
+// four 32-bit multiplies and its 33-bit corrections.  Sums and differences are
+// left unreduced: the forward adds at most (1/2 + 2^-11) q per level to |x|
+// (about 4q after six levels from input below q), the inverse at most doubles
+// it (32q before its last level, which multiplies both outputs).  Every value
+// stays below 2^38 (2^40 for any u32 input), so nothing rounds.  (No FMA contraction: each step is
+// written as the rounding it needs.)
+
+__device__ __forceinline__ double f64_mulc(double a, double w, double wq, double q)
+{
+#pragma clang fp contract(off)
+    const double hi = a * w, lo = __fma_rn(a, w, -hi);
+    const double t = __builtin_rint(a * wq);
+    return __fma_rn(-t, q, hi) + lo;
+}
+
+// x mod q in [0, q) for integer |x| < 2^50: floor(x fl(1/q)) is exact unless
+// x = k q, where it may give k - 1 and leave exactly q
+__device__ __forceinline__ uint32_t f64_canonical(double x, double q, double qinv)
+{
+#pragma clang fp contract(off)
+    const double r = __fma_rn(-__builtin_floor(x * qinv), q, x);
+    return (uint32_t)(r >= q ? 0.0 : r);
+}
+
+// the same transforms as ntt_regs, on doubles; the outputs are lazy (reduce
+// them with f64_canonical)
+template <int LOG_N, int MODE>
+__device__ __forceinline__ void ntt_regs_f64(double (&a)[1u << LOG_N], const uint32_t *__restrict__ table)
+{
+    constexpr uint32_t N = 1u << LOG_N;
+    const __attribute__((address_space(4))) double *dt =
+        (const __attribute__((address_space(4))) double *)(table + TabLayout::dbl(N));
+    const double q = dt[0];
+    if (MODE == NTT_FWD) {
+#pragma unroll
+        for (uint32_t len = N / 2; len >= 1; len >>= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                const double w = dt[TabLayout::dbl_fwd(N) + 2 * k], wq = dt[TabLayout::dbl_fwd(N) + 2 * k + 1];
+#pragma unroll
+                for (uint32_t j = s; j < s + len; j++) {
+                    const double t = f64_mulc(a[j + len], w, wq, q);
+                    const double u = a[j];
+                    a[j] = u + t;
+                    a[j + len] = u - t;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t len = 1; len <= N / 2; len <<= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                if (len == N / 2) {
+                    const double nw = dt[2], nwq = dt[3], lw = dt[4], lwq = dt[5];
+#pragma unroll
+                    for (uint32_t j = s; j < s + len; j++) {
+                        const double u = a[j], v = a[j + len];
+                        a[j] = f64_mulc(u + v, nw, nwq, q);
+                        a[j + len] = f64_mulc(u - v, lw, lwq, q);
+                    }
+                } else {
+                    const double w = dt[TabLayout::dbl_inv(N) + 2 * k], wq = dt[TabLayout::dbl_inv(N) + 2 * k + 1];
+#pragma unroll
+                    for (uint32_t j = s; j < s + len; j++) {
+                        const double u = a[j], v = a[j + len];
+                        a[j] = u + v;
+                        a[j + len] = f64_mulc(u - v, w, wq, q);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// value type of a register transform's working array
+template <int AR>
+using ntt_val_t = typename std::conditional<AR == ARITH_F64, double, uint32_t>::type;
+
+template <int LOG_N, int AR, int MODE>
+__device__ __forceinline__ void ntt_regs_any(ntt_val_t<AR> (&a)[1u << LOG_N], const uint32_t *__restrict__ table)
+{
+    if constexpr (AR == ARITH_F64)
+        ntt_regs_f64<LOG_N, MODE>(a, table);
+    else
+        ntt_regs<LOG_N, AR, MODE>(a, table);
+}
+
+// a working value -> canonical u32
+template <int AR>
+__device__ __forceinline__ uint32_t ntt_out(ntt_val_t<AR> v, const uint32_t *__restrict__ table, uint32_t n)
+{
+    if constexpr (AR == ARITH_F64) {
+        const __attribute__((address_space(4))) double *dt =
+            (const __attribute__((address_space(4))) double *)(table + TabLayout::dbl(n));
+        return f64_canonical(v, dt[0], dt[1]);
+    } else {
+        return v;
     }
 }
 

@@ -330,9 +330,10 @@ void launch(const lzx_quad_plan *p, const uint64_t *a, const uint64_t *b, uint64
 template <int MODE>
 lzx_status run(const lzx_quad_plan *p, const uint64_t *a, const uint64_t *b, uint64_t *out, size_t len, void *stream)
 {
-    if (!p || !a || !out || ((MODE == QS_SLOT || MODE == QS_MUL) && !b)) return LZX_INVALID_ARGUMENT;
+    if (!p) return LZX_INVALID_ARGUMENT;
     if (len % ((size_t)1 << p->log_n)) return LZX_MOD50_LENGTH_MISMATCH; // Mod50Error::LengthMismatch
-    if (len == 0) return LZX_OK;
+    if (len == 0) return LZX_OK; // an empty batch may pass null pointers
+    if (!a || !out || ((MODE == QS_SLOT || MODE == QS_MUL) && !b)) return LZX_INVALID_ARGUMENT;
     launch<MODE>(p, a, b, out, len, (hipStream_t)stream);
     LZX_LAUNCH_CHECK("quad_kernel");
     return LZX_OK;

@@ -304,8 +304,9 @@ static lzx_status rq2_composed(int mode, const uint32_t *a, const uint32_t *b, u
 static lzx_status rq2_run(int mode, const lzx_ntt_plan *plan, const uint32_t *a, const uint32_t *b, uint32_t *out,
                           uint8_t *ok, size_t count, void *stream)
 {
-    if (!plan || !a || !out || (mode == RQ2_MUL && !b) || (mode == RQ2_INV && !ok)) return LZX_INVALID_ARGUMENT;
-    if (count == 0) return LZX_OK;
+    if (!plan) return LZX_INVALID_ARGUMENT;
+    if (count == 0) return LZX_OK; // an empty batch may pass null pointers
+    if (!a || !out || (mode == RQ2_MUL && !b) || (mode == RQ2_INV && !ok)) return LZX_INVALID_ARGUMENT;
     hipStream_t s = (hipStream_t)stream;
     const bool aligned = (((uintptr_t)a | (uintptr_t)(b ? b : a) | (uintptr_t)out) & 15) == 0;
     if (plan->log_n >= 2 && plan->log_n <= 6 && aligned) {
@@ -322,8 +323,9 @@ static lzx_status rq2_run(int mode, const lzx_ntt_plan *plan, const uint32_t *a,
 static lzx_status rq2_linear(int op, const lzx_ntt_plan *plan, const uint32_t *a, const uint32_t *b, uint32_t *out,
                              size_t count, void *stream)
 {
-    if (!plan || !a || !out || (op <= RQ2_SUB && !b)) return LZX_INVALID_ARGUMENT;
-    if (count == 0) return LZX_OK;
+    if (!plan) return LZX_INVALID_ARGUMENT;
+    if (count == 0) return LZX_OK; // an empty batch may pass null pointers
+    if (!a || !out || (op <= RQ2_SUB && !b)) return LZX_INVALID_ARGUMENT;
     hipLaunchKernelGGL(rq2_linear_kernel, dim3(rq2_grid(count * 2 * plan->n, 256)), dim3(256), 0,
                        (hipStream_t)stream, op, a, b, out, plan->log_n, count, plan->q);
     LZX_LAUNCH_CHECK("rq2_linear_kernel");
