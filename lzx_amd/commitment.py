@@ -51,6 +51,7 @@ class AjtaiPublicKey:
     def __init__(self, params: AjtaiParams, handle, seed=None):
         self.params = params
         self._h = handle
+        self._destroy = lib().lzx_ajtai_key_destroy  # bound now: lib() may be gone at interpreter exit
         self.seed = seed
 
     @classmethod
@@ -76,9 +77,9 @@ class AjtaiPublicKey:
         return cls(params, h)
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().lzx_ajtai_key_destroy(h)
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
             self._h = None
 
     @property

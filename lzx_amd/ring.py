@@ -47,13 +47,14 @@ class NttTables:
         self.modulus = modulus
         self.log_n = log_n
         self._h = C.c_void_p()
+        self._destroy = lib().lzx_ntt_plan_destroy  # bound now: lib() may be gone at interpreter exit
         check(lib().lzx_ntt_plan_create(modulus.q, modulus.two_adicity, modulus.generator, log_n,
                                         C.byref(self._h)), "NttTables::new")
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().lzx_ntt_plan_destroy(h)
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
             self._h = None
 
     @property

@@ -68,6 +68,7 @@ class QuadSlotTables:
     def __init__(self, modulus: Modulus50, log_n: int):
         self.modulus, self.log_n = modulus, log_n
         self._h = C.c_void_p()
+        self._destroy = lib().lzx_quad_plan_destroy  # bound now: lib() may be gone at interpreter exit
         check(lib().lzx_quad_plan_create(modulus.q, modulus.two_adicity, modulus.generator, log_n,
                                          C.byref(self._h)), "QuadSlotTables::new")
         g = np.zeros(max(1, self.slots()), dtype=np.uint64)
@@ -77,9 +78,9 @@ class QuadSlotTables:
         self.leaf_gamma = g[: self.slots()]
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().lzx_quad_plan_destroy(h)
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
             self._h = None
 
     def n(self) -> int:

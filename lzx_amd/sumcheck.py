@@ -23,6 +23,7 @@ class Transcript:
     """Transcript::new(protocol, budget) (transcript.rs:46)."""
 
     def __init__(self, protocol: bytes = b"", query_budget: int = 1 << 20, _h=None):
+        self._destroy = lib().lzx_transcript_destroy  # bound now: lib() may be gone at interpreter exit
         if _h is not None:
             self._h = _h
             return
@@ -34,9 +35,9 @@ class Transcript:
         return cls(protocol, 1 << 20)
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().lzx_transcript_destroy(h)
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
             self._h = None
 
     @property
@@ -414,6 +415,7 @@ class SumcheckSession:
             raise ValueError("a session needs every factor table")
         self._vp = vp  # the caller's tables are read until the first fold
         ptrs, coeffs, off, ids = vp._ffi()
+        self._destroy = lib().lzx_sumcheck_session_destroy  # bound now: lib() may be gone at interpreter exit
         self._h = C.c_void_p()
         check(lib().lzx_sumcheck_session_create(vp.num_vars, len(vp.factors), ptrs, len(vp.terms), coeffs.ctypes.data,
                                                 off.ctypes.data, ids.ctypes.data, dev.stream_handle(stream),
@@ -422,9 +424,9 @@ class SumcheckSession:
         self.nfactors = len(vp.factors)
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().lzx_sumcheck_session_destroy(h)
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy:
+            destroy(h)
             self._h = None
 
     def round_eval(self) -> np.ndarray:
