@@ -104,6 +104,17 @@ __device__ __forceinline__ double f64_mulc(double a, double w, double wq, double
     return __fma_rn(-t, q, hi) + lo;
 }
 
+// a b mod q up to a multiple of q, for integers with |a b| < 2^100: the
+// quotient estimate rint(fl(a b) fl(1/q)) is within 1/2 + |a b| 1.5 2^-52 / q
+// of a b / q, so |result| <= q/2 + |a b| 1.5 2^-52 and neither step rounds
+__device__ __forceinline__ double f64_mul(double a, double b, double q, double qinv)
+{
+#pragma clang fp contract(off)
+    const double hi = a * b, lo = __fma_rn(a, b, -hi);
+    const double t = __builtin_rint(hi * qinv);
+    return __fma_rn(-t, q, hi) + lo;
+}
+
 // x mod q in [0, q) for integer |x| < 2^50: floor(x fl(1/q)) is exact unless
 // x = k q, where it may give k - 1 and leave exactly q
 __device__ __forceinline__ uint32_t f64_canonical(double x, double q, double qinv)

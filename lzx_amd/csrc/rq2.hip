@@ -19,7 +19,8 @@
 //            inverted by one exponentiation (prefix products, Montgomery's
 //            batch inversion); an element with a zero slot has no inverse:
 //            its flag is 0 and its output is zero.
-// Nothing but the inputs and the outputs touches HBM.
+// Nothing but the inputs and the outputs touches HBM.  For q > 2^31 the same
+// pass runs in exact FP64 (rq2_f64_kernel), in place of Montgomery.
 //
 // Other n (1, 2 and 128 .. 8192): the tuned batched NTT launches on scratch
 // copies, rq2_slot_kernel for the slot-wise step, and the inverse launch.
@@ -160,6 +161,132 @@ __global__ __launch_bounds__(64) void rq2_kernel(const uint32_t *a, const uint32
     }
 }
 
+// The fused pass for q > 2^31 in exact FP64 (the register NTT's ARITH_F64
+// class: ntt_regs_f64, f64_mul).  Same tiles, lanes and slot algebra as
+// rq2_kernel; values are integer doubles, reduced lazily.  For the product,
+// a's transform is parked as canonical u32 while b is staged and transformed,
+// so at most one polynomial is held as doubles.  The twiddle pointer is made
+// opaque per tile so the compiler reloads each transform's constants from the
+// scalar cache instead of keeping all of them in SGPRs across the loop.
+__device__ __forceinline__ double rq2_swap_d(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = rq2_swap((uint32_t)u), hi = rq2_swap((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+template <int LOG_N, int MODE>
+__global__ __launch_bounds__(64) void rq2_f64_kernel(const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
+                                                     const uint32_t *__restrict__ tab, size_t count)
+{
+    constexpr uint32_t N = 1u << LOG_N, C = N / 4;
+    __shared__ __attribute__((aligned(16))) uint32_t T[64 * N]; // one tile: row l = lane l's polynomial
+    const uint32_t lane = threadIdx.x, odd = lane & 1;
+    const size_t tiles = (count + RQ2_TILE - 1) / RQ2_TILE;
+    u32x4 r[C];
+    for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t *tb = tab;
+        asm volatile("" : "+s"(tb));
+        const __attribute__((address_space(4))) double *dt =
+            (const __attribute__((address_space(4))) double *)(tb + TabLayout::dbl(N));
+        const double q = dt[0], qinv = dt[1];
+        const size_t e0 = t * RQ2_TILE;
+        const uint32_t ne = count - e0 < RQ2_TILE ? (uint32_t)(count - e0) : RQ2_TILE;
+        const uint32_t rows = 2 * ne;
+        tile_fetch<C>(a + e0 * 2 * N, rows, r, lane);
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
+        wave_sync();
+        // every lane computes (rows past 2 ne hold stale values and are not stored),
+        // so the partner exchange always sees both lanes of a pair
+        double x[N];
+#pragma unroll
+        for (uint32_t c4 = 0; c4 < C; c4++) {
+            const u32x4 v = *reinterpret_cast<const u32x4 *>(T + 4 * tile_slot<C>(lane, c4));
+            x[4 * c4] = v.x, x[4 * c4 + 1] = v.y, x[4 * c4 + 2] = v.z, x[4 * c4 + 3] = v.w;
+        }
+        if (MODE == RQ2_MUL) tile_fetch<C>(b + e0 * 2 * N, rows, r, lane); // in flight during a's transform
+        ntt_regs_f64<LOG_N, NTT_FWD>(x, tb); // x[j]: evaluation brv(j), the order the inverse reads
+        bool good = true;
+        if (MODE == RQ2_MUL) {
+            uint32_t xa[N]; // a's transform, parked
+#pragma unroll
+            for (uint32_t j = 0; j < N; j++) xa[j] = f64_canonical(x[j], q, qinv);
+            wave_sync(); // every lane has read its row of a: b goes through the same tile
+#pragma unroll
+            for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
+            wave_sync();
+#pragma unroll
+            for (uint32_t c4 = 0; c4 < C; c4++) {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(T + 4 * tile_slot<C>(lane, c4));
+                x[4 * c4] = v.x, x[4 * c4 + 1] = v.y, x[4 * c4 + 2] = v.z, x[4 * c4 + 3] = v.w;
+            }
+            ntt_regs_f64<LOG_N, NTT_FWD>(x, tb); // b's transform
+#pragma unroll
+            for (uint32_t j = 0; j < N; j++) {
+                const double y = x[j], pb = rq2_swap_d(y);
+                const double xs = xa[j], pa = rq2_swap(xa[j]);
+                // even: x y - pa pb = a0 b0 - a1 b1; odd: pa y + x pb = a0 b1 + a1 b0
+                const double p = f64_mul(odd ? pa : xs, y, q, qinv);
+                const double rr = f64_mul(odd ? xs : pa, pb, q, qinv);
+                x[j] = odd ? p + rr : p - rr;
+            }
+        } else if (MODE == RQ2_NORM) {
+#pragma unroll
+            for (uint32_t j = 0; j < N; j++) {
+                const double s = f64_mul(x[j], x[j], q, qinv);
+                x[j] = s + rq2_swap_d(s);
+            }
+        } else {
+            // pre[j] = N_0 ... N_{j-1} (canonical), kept in this lane's own row of the tile (read already)
+            double acc = 1.0;
+#pragma unroll
+            for (uint32_t j = 0; j < N; j++) {
+                const double s = f64_mul(x[j], x[j], q, qinv);
+                T[4 * tile_slot<C>(lane, j / 4) + (j % 4)] = f64_canonical(acc, q, qinv);
+                acc = f64_mul(acc, s + rq2_swap_d(s), q, qinv);
+            }
+            const uint32_t prod = f64_canonical(acc, q, qinv);
+            good = prod != 0; // q is prime: the product vanishes iff a slot does
+            double inv = 1.0, base = prod;
+            for (uint32_t e = (uint32_t)q - 2; e; e >>= 1) { // Fermat: prod^(q-2)
+                if (e & 1) inv = f64_mul(inv, base, q, qinv);
+                base = f64_mul(base, base, q, qinv);
+            }
+#pragma unroll
+            for (uint32_t jj = 0; jj < N; jj++) {
+                const uint32_t j = N - 1 - jj;
+                const double s = f64_mul(x[j], x[j], q, qinv);
+                const double ninv = f64_mul(inv, (double)T[4 * tile_slot<C>(lane, j / 4) + (j % 4)], q, qinv);
+                inv = f64_mul(inv, s + rq2_swap_d(s), q, qinv);
+                x[j] = f64_mul(odd ? -x[j] : x[j], ninv, q, qinv); // conj: -c1 on the odd lane
+            }
+        }
+        ntt_regs_f64<LOG_N, NTT_INV>(x, tb);
+        // results back through the tile: row l for the product and inverse, row l/2
+        // (the element) for the norm, written by the even lane
+        wave_sync();
+        const uint32_t orow = MODE == RQ2_NORM ? lane >> 1 : lane;
+        if (MODE != RQ2_NORM || !odd) {
+#pragma unroll
+            for (uint32_t c4 = 0; c4 < C; c4++)
+                *reinterpret_cast<u32x4 *>(T + 4 * tile_slot<C>(orow, c4)) =
+                    u32x4{f64_canonical(x[4 * c4], q, qinv), f64_canonical(x[4 * c4 + 1], q, qinv),
+                          f64_canonical(x[4 * c4 + 2], q, qinv), f64_canonical(x[4 * c4 + 3], q, qinv)};
+        }
+        if (MODE == RQ2_INV && lane < rows && !odd) ok[e0 + (lane >> 1)] = good ? 1 : 0;
+        wave_sync();
+        const uint32_t orows = MODE == RQ2_NORM ? ne : rows;
+        uint32_t *go = out + e0 * (MODE == RQ2_NORM ? N : 2 * N);
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) {
+            const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
+            if (p < orows) *reinterpret_cast<u32x4 *>(go + (p * C + c) * 4) = *reinterpret_cast<const u32x4 *>(T + 4 * s);
+        }
+        wave_sync(); // the stores have read the tile before the next one is staged
+    }
+}
+
 // Slot-wise step of the composed path on NTT-domain batches: A, B are
 // [count][2][n]; MUL writes C = A B to out ([count][2][n]); NORM writes
 // A0^2 + A1^2 to out ([count][n]); INV_FLAG clears ok[e] when a slot of N
@@ -249,6 +376,34 @@ static void launch_fused(int mode, const uint32_t *a, const uint32_t *b, uint32_
                            plan->dev_tab, bar, count);
 }
 
+template <int LOG_N>
+static void launch_fused_f64(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
+                             const lzx_ntt_plan *plan, size_t count, hipStream_t s)
+{
+    const uint32_t blocks = (uint32_t)std::min<size_t>((count + RQ2_TILE - 1) / RQ2_TILE, RQ2_MAX_BLOCKS);
+    if (mode == RQ2_MUL)
+        hipLaunchKernelGGL((rq2_f64_kernel<LOG_N, RQ2_MUL>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
+                           plan->dev_tab, count);
+    else if (mode == RQ2_NORM)
+        hipLaunchKernelGGL((rq2_f64_kernel<LOG_N, RQ2_NORM>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
+                           plan->dev_tab, count);
+    else
+        hipLaunchKernelGGL((rq2_f64_kernel<LOG_N, RQ2_INV>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
+                           plan->dev_tab, count);
+}
+
+static void dispatch_fused_f64(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
+                               const lzx_ntt_plan *plan, size_t count, hipStream_t s)
+{
+    switch (plan->log_n) {
+    case 2: launch_fused_f64<2>(mode, a, b, out, ok, plan, count, s); break;
+    case 3: launch_fused_f64<3>(mode, a, b, out, ok, plan, count, s); break;
+    case 4: launch_fused_f64<4>(mode, a, b, out, ok, plan, count, s); break;
+    case 5: launch_fused_f64<5>(mode, a, b, out, ok, plan, count, s); break;
+    default: launch_fused_f64<6>(mode, a, b, out, ok, plan, count, s); break;
+    }
+}
+
 template <int AR>
 static void dispatch_fused(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
                            const lzx_ntt_plan *plan, size_t count, hipStream_t s)
@@ -310,8 +465,8 @@ static lzx_status rq2_run(int mode, const lzx_ntt_plan *plan, const uint32_t *a,
     hipStream_t s = (hipStream_t)stream;
     const bool aligned = (((uintptr_t)a | (uintptr_t)(b ? b : a) | (uintptr_t)out) & 15) == 0;
     if (plan->log_n >= 2 && plan->log_n <= 6 && aligned) {
-        if (plan->arith == ARITH_MONT)
-            dispatch_fused<ARITH_MONT>(mode, a, b, out, ok, plan, count, s);
+        if (plan->arith == ARITH_MONT) // q > 2^31: exact FP64 in place of Montgomery
+            dispatch_fused_f64(mode, a, b, out, ok, plan, count, s);
         else
             dispatch_fused<ARITH_SHOUP>(mode, a, b, out, ok, plan, count, s);
         LZX_LAUNCH_CHECK("rq2_kernel");
