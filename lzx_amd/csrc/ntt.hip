@@ -195,18 +195,18 @@ __global__ __launch_bounds__(64) void ntt_digits_kernel(const int8_t *__restrict
         }
     }
     __syncthreads();
-    uint32_t a[N];
+    ntt_val_t<AR> a[N];
 #pragma unroll
     for (uint32_t c4 = 0; c4 < C; c4++) {
         const uint4 v = *reinterpret_cast<const uint4 *>(T + 4 * tile_slot<C>(lane, c4));
         a[4 * c4] = v.x, a[4 * c4 + 1] = v.y, a[4 * c4 + 2] = v.z, a[4 * c4 + 3] = v.w;
     }
-    ntt_regs<LOG_N, AR, NTT_FWD>(a, tab);
+    ntt_regs_any<LOG_N, AR, NTT_FWD>(a, tab);
 #pragma unroll
     for (uint32_t c4 = 0; c4 < C; c4++) {
         uint32_t vv[4];
 #pragma unroll
-        for (uint32_t e = 0; e < 4; e++) vv[e] = a[brv_bits(4 * c4 + e, LOG_N)];
+        for (uint32_t e = 0; e < 4; e++) vv[e] = ntt_out<AR>(a[brv_bits(4 * c4 + e, LOG_N)], tab, N);
         *reinterpret_cast<uint4 *>(T + 4 * tile_slot<C>(lane, c4)) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
     }
     __syncthreads();
@@ -534,11 +534,11 @@ lzx_status ntt_forward_digits(const lzx_ntt_plan *plan, const int8_t *digits, ui
     const uint32_t np = (uint32_t)polys;
 #define LZX_DIGITS_LAUNCH(LN, AR)                                                                                   \
     hipLaunchKernelGGL((ntt_digits_kernel<LN, AR>), grid, dim3(64), 0, stream, digits, out, plan->dev_tab, np)
-    const bool mont = plan->arith == ARITH_MONT;
+    const bool big = plan->arith == ARITH_MONT; // q > 2^31: the FP64 register transform
     switch (plan->log_n) {
-    case 4: if (mont) LZX_DIGITS_LAUNCH(4, ARITH_MONT); else LZX_DIGITS_LAUNCH(4, ARITH_SHOUP); break;
-    case 5: if (mont) LZX_DIGITS_LAUNCH(5, ARITH_MONT); else LZX_DIGITS_LAUNCH(5, ARITH_SHOUP); break;
-    default: if (mont) LZX_DIGITS_LAUNCH(6, ARITH_MONT); else LZX_DIGITS_LAUNCH(6, ARITH_SHOUP); break;
+    case 4: if (big) LZX_DIGITS_LAUNCH(4, ARITH_F64); else LZX_DIGITS_LAUNCH(4, ARITH_SHOUP); break;
+    case 5: if (big) LZX_DIGITS_LAUNCH(5, ARITH_F64); else LZX_DIGITS_LAUNCH(5, ARITH_SHOUP); break;
+    default: if (big) LZX_DIGITS_LAUNCH(6, ARITH_F64); else LZX_DIGITS_LAUNCH(6, ARITH_SHOUP); break;
     }
 #undef LZX_DIGITS_LAUNCH
     LZX_LAUNCH_CHECK("ntt_digits_kernel");
