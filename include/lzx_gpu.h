@@ -22,6 +22,10 @@
  *   - Pointers named *_dev are device (HBM) pointers; *_host are host pointers.
  *     `stream` is a hipStream_t passed as void* (NULL = the null stream).
  *     Device entry points are stream-ordered and return before completion.
+ *   - A batched device entry point whose batch is empty (no elements in and
+ *     none out) returns LZX_OK without touching its data pointers, which may
+ *     then be NULL.  (Packing an empty input still writes one element, as the
+ *     reference's packers do.)
  *   - Residues are canonical: u32 in [0, q), Goldilocks u64 in [0, p).
  *   - Handles are immutable after creation and may be shared across host
  *     threads; calls on one handle from several threads only read it.
