@@ -95,6 +95,28 @@ def test_crt_multi_full_workload(ring):
     assert np.array_equal(D.to_host(t, np.uint32).reshape(data.shape), data)
 
 
+@pytest.mark.parametrize("mod", [O.Q_32, O.CRT_PRIMES[0]], ids=lambda m: str(m[0]))
+def test_lds_staged_register_kernel(ring, mod, monkeypatch):
+    """The LDS-staged register kernel (plans created with LZX_NTT_STAGE=lds):
+    the FP64 class for q > 2^31, Shoup below, every register size, a partial
+    last tile."""
+    from lzx_amd import device as D
+    q, adic, g = mod
+    monkeypatch.setenv("LZX_NTT_STAGE", "lds")
+    for log_n in (2, 3, 4, 5, 6):
+        rng = np.random.default_rng(77 + log_n)
+        polys = 77
+        a = _rng_poly(rng, q, polys << log_n)
+        tab = ring.NttTables(ring.Modulus32(q, adic, g), log_n)
+        ot = O.Ntt(q, adic, g, log_n)
+        _, want = ot.forward(a)
+        t = D.to_device(a)
+        tab.forward_batch(t)
+        assert np.array_equal(D.to_host(t, np.uint32), want), f"forward log_n={log_n}"
+        tab.inverse_batch(t)
+        assert np.array_equal(D.to_host(t, np.uint32), a), f"round trip log_n={log_n}"
+
+
 def test_host_signature_and_errors(ring):
     from lzx_amd._lib import LzxError
     q, adic, g = O.Q_32
