@@ -198,10 +198,18 @@ __device__ __forceinline__ uint64_t div_barrett(uint64_t x, uint64_t b, uint64_t
 // STAGE: a block's PK_BLOCK values stage their digits in LDS (dynamic, one
 // PK_BLOCK * nd * sizeof(DT) tile) and leave as coalesced 4-byte stores.
 // POW2: base = 2^m, so the quotient and remainder are a shift and a mask.
+//
+// POW2 with m nd <= 120 (off = o sum_{k<nd} 2^(m k), o = base/2 - 1, as two
+// words): the digits come from y = x + off without the borrow chain.  The
+// recurrence's digits lie in [-o, base - 1 - o], a complete residue system mod
+// base, so the representation of x with nd such digits plus a top part is
+// unique; y's plain base-2^m digits less o are such a representation, so
+// they are the recurrence's digits and y >> (m nd) is its final rem.
 template <typename DT, bool STAGE, bool POW2>
 __global__ __launch_bounds__(PK_BLOCK) void decompose_kernel(const uint64_t *__restrict__ x, size_t count, uint64_t base,
                                                              uint64_t m, uint32_t nd, DT *__restrict__ digits,
-                                                             uint32_t *flag)
+                                                             uint32_t *flag, uint64_t off_lo, uint64_t off_hi,
+                                                             uint32_t closed)
 {
     extern __shared__ uint8_t pk_stage[];
     const uint64_t half = base / 2;
@@ -209,7 +217,17 @@ __global__ __launch_bounds__(PK_BLOCK) void decompose_kernel(const uint64_t *__r
     for (size_t c = blockIdx.x; c < chunks; c += gridDim.x) {
         const size_t i = c * PK_BLOCK + threadIdx.x;
         DT *dst = STAGE ? (DT *)pk_stage + (size_t)threadIdx.x * nd : digits + i * nd;
-        if (i < count) {
+        if (i < count && POW2 && closed) {
+            const uint64_t xi = x[i];
+            uint64_t lo = xi + off_lo, hi = off_hi + (lo < xi ? 1 : 0);
+            const int64_t o = (int64_t)(half - 1);
+            for (uint32_t k = 0; k < nd; k++) {
+                dst[k] = (DT)((int64_t)(lo & (base - 1)) - o);
+                lo = (lo >> m) | (hi << (64 - m));
+                hi >>= m;
+            }
+            if (lo | hi) atomicOr(flag, 1u); // DecompositionError::ValueTooLarge
+        } else if (i < count) {
             uint64_t rem = x[i];
             for (uint32_t k = 0; k < nd; k++) {
                 uint64_t r, qt;
@@ -433,13 +451,21 @@ extern "C" lzx_status lzx_gadget_decompose(const uint64_t *x_dev, size_t count, 
     if (pow2)
         for (m = 0; (1ULL << m) != base; m++) {
         }
+    // the closed form for power-of-two bases (see decompose_kernel)
+    uint64_t off_lo = 0, off_hi = 0;
+    const uint32_t closed = pow2 && m * num_digits <= 120 ? 1u : 0u;
+    if (closed) {
+        unsigned __int128 off = 0;
+        for (uint32_t k = 0; k < num_digits; k++) off += (unsigned __int128)(base / 2 - 1) << (m * k);
+        off_lo = (uint64_t)off, off_hi = (uint64_t)(off >> 64);
+    }
     const size_t stage = (size_t)PK_BLOCK * num_digits * digit_bytes;
     const bool staged = stage <= 32768 && (uintptr_t)digits_dev % 4 == 0;
     const dim3 g(pk_grid(count));
     const size_t sh = staged ? stage : 0;
 #define PK_DECOMPOSE(DT, ST, P2)                                                                                    \
     hipLaunchKernelGGL((decompose_kernel<DT, ST, P2>), g, dim3(PK_BLOCK), sh, s, x_dev, count, base, m, num_digits,    \
-                       (DT *)digits_dev, f.dev)
+                       (DT *)digits_dev, f.dev, off_lo, off_hi, closed)
     if (digit_bytes == 1 && staged && pow2)
         PK_DECOMPOSE(int8_t, true, true);
     else if (digit_bytes == 1 && staged)

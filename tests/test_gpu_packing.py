@@ -88,7 +88,8 @@ def test_crt_packing_matches_oracle(pk):
     assert e.value.code == 50
 
 
-@pytest.mark.parametrize("base,nd,db", [(256, 9, 8), (16, 17, 1), (10, 21, 8), (2, 65, 1), (255, 9, 1), (4, 33, 8)])
+@pytest.mark.parametrize("base,nd,db", [(256, 9, 8), (16, 17, 1), (10, 21, 8), (2, 65, 1), (255, 9, 1), (4, 33, 8),
+                                        (2, 125, 1), (1 << 32, 3, 8)])
 def test_gadget_decomposition_matches_oracle(pk, base, nd, db):
     PK, D, R, _ = pk
     g = PK.GadgetDecomposition(base, nd)
