@@ -245,6 +245,25 @@ LZX_HD uint64_t gl_mul(uint64_t a, uint64_t b)
     return gl_reduce128(hi, lo);
 }
 
+// a b as any representative below 2^64 (gl_mul without its final reduction
+// below p), for products that only feed lazy sums or further products
+LZX_HD uint64_t gl_mul_lazy(uint64_t a, uint64_t b)
+{
+    uint64_t hi, lo;
+    mul128(a, b, hi, lo);
+    const uint32_t hh = (uint32_t)(hi >> 32), hl = (uint32_t)hi;
+    uint64_t t0 = lo - hh;
+    if (lo < hh) t0 -= GL_EPS;
+    uint64_t sh = (uint64_t)hl << 32;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(sh));
+#endif
+    const uint64_t t1 = sh - hl;
+    uint64_t r = t0 + t1;
+    if (r < t1) r += GL_EPS;
+    return r;
+}
+
 // Lazy sum: true value = acc + carries * 2^64.
 struct GlAcc {
     uint64_t acc;

@@ -79,26 +79,26 @@ __device__ __forceinline__ uint64_t eval_terms(const uint64_t (&v)[NF], const Sc
         for (int f = 0; f < NF; f++) {
             const uint32_t e = T.exps[t][f];
             for (uint32_t x = 0; x < e; x++) {
-                prod = seeded ? gl_mul(prod, v[f]) : v[f];
+                prod = seeded ? gl_mul_lazy(prod, v[f]) : v[f];
                 seeded = true;
             }
         }
         const uint64_t c = T.coeff[t];
         if (c == GL_P - 1) {
-            prod = gl_canon(prod); // a lone factor may be an unreduced point value
+            prod = gl_canon(prod); // a lone factor or a lazy product may be unreduced
             prod = prod == 0 ? 0 : GL_P - prod;
         } else if (c != 1) {
-            prod = gl_mul(prod, c);
+            prod = gl_mul_lazy(prod, c);
         }
         gl_acc_add(acc, prod);
     }
-    uint64_t sum = gl_acc_value(acc); // exact mod p; reduced by the products below or by the caller's sum
+    uint64_t sum = gl_acc_value(acc); // exact mod p, a representative below 2^64
     if (T.has_common) {
 #pragma unroll
         for (int f = 0; f < NF; f++)
-            for (uint32_t x = 0; x < T.common[f]; x++) sum = gl_mul(sum, v[f]);
+            for (uint32_t x = 0; x < T.common[f]; x++) sum = gl_mul_lazy(sum, v[f]);
     }
-    return sum;
+    return sum; // the caller's lazy sums reduce it
 }
 
 // Accumulate g(t) (t < T.npts) of one pair (lo, hi) into acc[]; with the
@@ -108,7 +108,7 @@ __device__ __forceinline__ void eval_pair(const uint64_t (&lo)[NF], const uint64
                                           uint64_t eq_w, GlAcc (&acc)[D + 1])
 {
     const bool split = T.eq_f >= 0;
-    auto put = [&](GlAcc &a, uint64_t v) { gl_acc_add(a, split ? gl_mul(v, eq_w) : v); };
+    auto put = [&](GlAcc &a, uint64_t v) { gl_acc_add(a, split ? gl_mul_lazy(v, eq_w) : v); };
     uint64_t diff[NF], cur[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
