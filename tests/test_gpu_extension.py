@@ -72,6 +72,37 @@ def test_rq2_inverse_flags_zero_norm_slots(ext, log_n):
     assert not D.to_host(inv, np.uint32).reshape(600, 2 * n)[okh == 0].any()
 
 
+@pytest.mark.parametrize("log_n", [2, 6])
+def test_rq2_inverse_flags_default_modulus(ext, log_n):
+    """q = 3*2^30 + 1 (the FP64 fused pass): elements whose norm N = c0^2 + c1^2
+    vanishes in one slot (c1's evaluation there is i times c0's, i^2 = -1),
+    zero elements and ordinary ones in one batch."""
+    E, D, R = ext
+    ring, ntt, q = _setup(R, "q32", log_n)
+    n = 1 << log_n
+    rng = np.random.default_rng(40 + log_n)
+    count = 100
+    a = rng.integers(0, q, (count, 2 * n), dtype=np.uint64).astype(np.uint32)
+    i4 = pow(5, (q - 1) // 4, q)  # 5 generates the group mod q, so this is a square root of -1
+    assert i4 * i4 % q == q - 1
+    for e in range(0, count, 7):  # one vanishing slot per chosen element
+        e0 = rng.integers(1, q, n, dtype=np.uint64)
+        e1 = rng.integers(1, q, n, dtype=np.uint64)
+        j = int(rng.integers(0, n))
+        e1[j] = e0[j] * i4 % q
+        a[e, :n] = ntt.inverse(e0.astype(np.uint32))[1]
+        a[e, n:] = ntt.inverse(e1.astype(np.uint32))[1]
+    a[3] = 0
+    ok, inv = E.inverse_batch(ring, D.to_device(a))
+    wok, winv = ntt.rq2_inverse(a)
+    okh = D.to_host(ok, np.uint8)
+    assert okh.tolist() == wok.tolist()
+    assert okh[0] == 0 and okh[3] == 0 and okh[7] == 0 and okh[1] == 1
+    assert np.array_equal(D.to_host(inv, np.uint32).reshape(count, 2 * n), winv)
+    norm = D.to_host(E.norm_batch(ring, D.to_device(a)), np.uint32).reshape(count, n)
+    assert np.array_equal(norm, ntt.rq2_norm(a))
+
+
 def test_rq2_linear_ops(ext):
     E, D, R = ext
     ring, _ntt, q = _setup(R, "q32", 6)
