@@ -124,56 +124,96 @@ __device__ __forceinline__ uint32_t f64_canonical(double x, double q, double qin
     return (uint32_t)(r >= q ? 0.0 : r);
 }
 
+// One level of length LEN over the blocks in [S0, S1) (every index a template
+// constant, so the array stays in VGPRs).
+typedef const __attribute__((address_space(4))) double *f64_tab_t;
+
+template <uint32_t N, uint32_t LEN, uint32_t S0, uint32_t S1>
+__device__ __forceinline__ void f64_fwd_level(double (&a)[N], f64_tab_t dt, double q)
+{
+#pragma unroll
+    for (uint32_t s = S0; s < S1; s += 2 * LEN) {
+        const uint32_t k = N / (2 * LEN) + s / (2 * LEN);
+        const double w = dt[TabLayout::dbl_fwd(N) + 2 * k], wq = dt[TabLayout::dbl_fwd(N) + 2 * k + 1];
+#pragma unroll
+        for (uint32_t j = s; j < s + LEN; j++) {
+            const double t = f64_mulc(a[j + LEN], w, wq, q);
+            const double u = a[j];
+            a[j] = u + t;
+            a[j + LEN] = u - t;
+        }
+    }
+}
+
+// levels LEN, LEN/2, ..., 1 over [S0, S1)
+template <uint32_t N, uint32_t LEN, uint32_t S0, uint32_t S1>
+__device__ __forceinline__ void f64_fwd_levels(double (&a)[N], f64_tab_t dt, double q)
+{
+    if constexpr (LEN >= 1) {
+        f64_fwd_level<N, LEN, S0, S1>(a, dt, q);
+        f64_fwd_levels<N, LEN / 2, S0, S1>(a, dt, q);
+    }
+}
+
+template <uint32_t N, uint32_t LEN, uint32_t S0, uint32_t S1>
+__device__ __forceinline__ void f64_inv_level(double (&a)[N], f64_tab_t dt, double q)
+{
+#pragma unroll
+    for (uint32_t s = S0; s < S1; s += 2 * LEN) {
+        const uint32_t k = N / (2 * LEN) + s / (2 * LEN);
+        if constexpr (LEN == N / 2) {
+            const double nw = dt[2], nwq = dt[3], lw = dt[4], lwq = dt[5];
+#pragma unroll
+            for (uint32_t j = s; j < s + LEN; j++) {
+                const double u = a[j], v = a[j + LEN];
+                a[j] = f64_mulc(u + v, nw, nwq, q);
+                a[j + LEN] = f64_mulc(u - v, lw, lwq, q);
+            }
+        } else {
+            const double w = dt[TabLayout::dbl_inv(N) + 2 * k], wq = dt[TabLayout::dbl_inv(N) + 2 * k + 1];
+#pragma unroll
+            for (uint32_t j = s; j < s + LEN; j++) {
+                const double u = a[j], v = a[j + LEN];
+                a[j] = u + v;
+                a[j + LEN] = f64_mulc(u - v, w, wq, q);
+            }
+        }
+    }
+}
+
+// levels LEN, 2 LEN, ..., TOP over [S0, S1)
+template <uint32_t N, uint32_t LEN, uint32_t TOP, uint32_t S0, uint32_t S1>
+__device__ __forceinline__ void f64_inv_levels(double (&a)[N], f64_tab_t dt, double q)
+{
+    if constexpr (LEN >= 1 && LEN <= TOP) {
+        f64_inv_level<N, LEN, S0, S1>(a, dt, q);
+        f64_inv_levels<N, 2 * LEN, TOP, S0, S1>(a, dt, q);
+    }
+}
+
 // the same transforms as ntt_regs, on doubles; the outputs are lazy (reduce
-// them with f64_canonical)
+// them with f64_canonical).  After the forward's first level (before the
+// inverse's last) the two halves are independent transforms: they run one
+// after the other, with the scheduler kept from interleaving them, so only
+// one half's working set is live beside the other half's values.
 template <int LOG_N, int MODE>
 __device__ __forceinline__ void ntt_regs_f64(double (&a)[1u << LOG_N], const uint32_t *__restrict__ table)
 {
     constexpr uint32_t N = 1u << LOG_N;
-    const __attribute__((address_space(4))) double *dt =
-        (const __attribute__((address_space(4))) double *)(table + TabLayout::dbl(N));
+    const f64_tab_t dt = (f64_tab_t)(table + TabLayout::dbl(N));
     const double q = dt[0];
     if (MODE == NTT_FWD) {
-#pragma unroll
-        for (uint32_t len = N / 2; len >= 1; len >>= 1) {
-#pragma unroll
-            for (uint32_t s = 0; s < N; s += 2 * len) {
-                const uint32_t k = N / (2 * len) + s / (2 * len);
-                const double w = dt[TabLayout::dbl_fwd(N) + 2 * k], wq = dt[TabLayout::dbl_fwd(N) + 2 * k + 1];
-#pragma unroll
-                for (uint32_t j = s; j < s + len; j++) {
-                    const double t = f64_mulc(a[j + len], w, wq, q);
-                    const double u = a[j];
-                    a[j] = u + t;
-                    a[j + len] = u - t;
-                }
-            }
-        }
+        f64_fwd_level<N, N / 2, 0, N>(a, dt, q);
+        if constexpr (N >= 8) __builtin_amdgcn_sched_barrier(0);
+        f64_fwd_levels<N, N / 4, 0, N / 2>(a, dt, q);
+        if constexpr (N >= 8) __builtin_amdgcn_sched_barrier(0);
+        f64_fwd_levels<N, N / 4, N / 2, N>(a, dt, q);
     } else {
-#pragma unroll
-        for (uint32_t len = 1; len <= N / 2; len <<= 1) {
-#pragma unroll
-            for (uint32_t s = 0; s < N; s += 2 * len) {
-                const uint32_t k = N / (2 * len) + s / (2 * len);
-                if (len == N / 2) {
-                    const double nw = dt[2], nwq = dt[3], lw = dt[4], lwq = dt[5];
-#pragma unroll
-                    for (uint32_t j = s; j < s + len; j++) {
-                        const double u = a[j], v = a[j + len];
-                        a[j] = f64_mulc(u + v, nw, nwq, q);
-                        a[j + len] = f64_mulc(u - v, lw, lwq, q);
-                    }
-                } else {
-                    const double w = dt[TabLayout::dbl_inv(N) + 2 * k], wq = dt[TabLayout::dbl_inv(N) + 2 * k + 1];
-#pragma unroll
-                    for (uint32_t j = s; j < s + len; j++) {
-                        const double u = a[j], v = a[j + len];
-                        a[j] = u + v;
-                        a[j + len] = f64_mulc(u - v, w, wq, q);
-                    }
-                }
-            }
-        }
+        f64_inv_levels<N, 1, N / 4, 0, N / 2>(a, dt, q);
+        if constexpr (N >= 8) __builtin_amdgcn_sched_barrier(0);
+        f64_inv_levels<N, 1, N / 4, N / 2, N>(a, dt, q);
+        if constexpr (N >= 8) __builtin_amdgcn_sched_barrier(0);
+        f64_inv_level<N, N / 2, 0, N>(a, dt, q);
     }
 }
 
