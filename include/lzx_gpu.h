@@ -144,7 +144,9 @@ lzx_status lzx_ring_mul_batch(const lzx_ntt_plan *plan, const uint32_t *a_dev, c
  * Quadratic extension ring R_q[Y]/(Y^2+1) (lattice-ring/src/extension.rs).
  * An element c0 + c1 Y is [c0 | c1], n canonical coefficients each; a batch
  * of count elements is [count][2][n] in device memory (the order of
- * Rq2::to_bytes). The plan names q and n. Outputs may alias inputs.
+ * Rq2::to_bytes). The plan names q and n. Outputs may alias or overlap
+ * inputs in any way (an in-place norm writes [count][n] over the first half
+ * of its input); overlapping calls stage their inputs in scratch first.
  * ------------------------------------------------------------------------- */
 
 /* Rq2::mul (extension.rs:115-124): out = a * b, count elements. */
@@ -244,11 +246,11 @@ lzx_status lzx_ajtai_key_matrix_ntt(const lzx_ajtai_key *key, uint32_t *Ahat_hos
 lzx_status lzx_ajtai_commit_batch(const lzx_ajtai_key *key, const uint32_t *s_dev, size_t s_len, size_t W,
                                   uint32_t *out_dev, void *stream);
 
-/* Batched commit of small-digit witnesses: digits_dev [W][m][n] int8 with
- * |digit| <= 127 (coefficient c represents c mod q, ring.rs:232-243
+/* Batched commit of small-digit witnesses: digits_dev [W][m][n] int8, any
+ * value in [-128, 127] (coefficient c represents c mod q, ring.rs:232-243
  * from_signed).  path selects the NTT-domain kernels, the int8 MFMA
- * coefficient-domain kernels, or AUTO (per-shape choice).  Results are
- * identical on every path. */
+ * coefficient-domain kernels (n = 64, q < 2^32 - 2^25), or AUTO (per-shape
+ * choice).  Results are identical on every path, for every int8 digit. */
 lzx_status lzx_ajtai_commit_digits_batch(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t s_len,
                                          size_t W, uint32_t *out_dev, int32_t path, void *stream);
 

@@ -25,8 +25,16 @@
 //     (rho, g) reads one 64-byte window of R; because S is Toeplitz, the four
 //     row-tile fragments are consecutive 16-byte slices of that window, each
 //     realigned with v_alignbyte;
-//   - int32 accumulation is exact for chunks of <= 2048 ring elements with
-//     |digit| <= 127 (|product| <= 127*128, 64 products per ring element);
+//   - every int8 digit is exact, -128 included.  -(-128) does not fit int8,
+//     so a digit vector is split as s = s1 + s2 with s1 = s where s != -128
+//     and -127 where s = -128, and s2 = -1 where s = -128, else 0; both
+//     halves negate safely.  A 16-element digit batch whose ballot finds a
+//     -128 byte gets a second window pass over R2 (the Toeplitz of s2) into
+//     the same accumulators; batches without one pay only the ballot;
+//   - int32 accumulation is exact for chunks of <= 2032 ring elements: the
+//     Toeplitz entries of s are at most 128 in magnitude, limbs too, so one
+//     ring element adds at most 64 * 128 * 128 = 2^20 per output, and every
+//     running sum (the s1 pass of an element included) stays below 2^31;
 //     chunk partials are summed in 64-bit and folded mod q per limb by
 //     ajtai_i8_reduce_kernel;
 //   - blocks that share a j-chunk are numbered congruent mod 8 so they land on
@@ -41,13 +49,20 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int I8_WAVES = 4;
 constexpr int I8_ITW = 2;          // witnesses per wave
-constexpr uint32_t I8_MAX_JCHUNK = 2048;
+constexpr uint32_t I8_MAX_JCHUNK = 2032; // 2032 * 2^20 < 2^31
 constexpr uint64_t I8_Q_LIMIT = (1ULL << 32) - (1ULL << 25);
 
+// Bytewise two's-complement negation (each byte of x must be != 0x80).
 __device__ __forceinline__ uint32_t neg_bytes(uint32_t x)
 {
     const uint32_t n = ~x;
     return ((n & 0x7f7f7f7fu) + 0x01010101u) ^ (n & 0x80808080u);
+}
+
+// 0x80 in each byte of x that equals 0x80 (-128), 0 elsewhere.
+__device__ __forceinline__ uint32_t min_bytes(uint32_t x)
+{
+    return x & ~((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) & 0x80808080u;
 }
 
 // Combines the four limb accumulators of one output held by a lane quad
@@ -72,6 +87,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 {
     constexpr int JB = 16; // ring elements per digit batch (one 1 KiB load per witness)
     __shared__ __attribute__((aligned(16))) uint32_t Rs[I8_WAVES][I8_ITW][JB][36];
+    __shared__ __attribute__((aligned(16))) uint32_t Rs2[I8_WAVES][I8_ITW][JB][36]; // s2 windows (digit -128)
     const uint32_t b = blockIdx.x;
     const uint32_t chunk = (b / (8 * groups)) * 8 + (b % 8);
     const uint32_t grp = (b / 8) % groups;
@@ -86,8 +102,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const uint32_t e = lane >> 2, kq = lane & 3; // digit batch: ring element e, 16-byte chunk kq
 
     // padding dwords 32..35 of every R slot stay zero
-    for (uint32_t idx = lane; idx < I8_ITW * JB; idx += 64)
+    for (uint32_t idx = lane; idx < I8_ITW * JB; idx += 64) {
         *reinterpret_cast<uint4 *>(&Rs[wave][idx / JB][idx % JB][32]) = make_uint4(0, 0, 0, 0);
+        *reinterpret_cast<uint4 *>(&Rs2[wave][idx / JB][idx % JB][32]) = make_uint4(0, 0, 0, 0);
+    }
 
     v4i acc[I8_ITW][4][NT];
 #pragma unroll
@@ -116,10 +134,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     };
 
     // One window of R per (witness, ring element): the four row-tile fragments.
-    auto compute = [&](uint32_t jj, const v4i (&bf)[NT]) {
+    auto compute = [&](uint32_t (&Rw)[I8_ITW][JB][36], uint32_t jj, const v4i (&bf)[NT]) {
 #pragma unroll
         for (int x = 0; x < I8_ITW; x++) {
-            const uint32_t *R = Rs[wave][x][jj];
+            const uint32_t *R = Rw[x][jj];
             uint32_t win[17];
 #pragma unroll
             for (int q4 = 0; q4 < 17; q4++) win[q4] = R[d0 + q4];
@@ -142,14 +160,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     load_digits(j0, dnext);
     v4i b0[NT], b1[NT];
     load_b(j0, b0);
+    bool split = false; // the current batch holds a -128 digit (wave-uniform)
     for (uint32_t j = j0; j < j1; j += 2) {
         const uint32_t jj = (j - j0) % JB;
         if (jj == 0) {
-            // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i])
+            // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i]),
+            // built from s1; R2 the same from s2 when the batch has a -128 digit
+            uint32_t mins = 0;
+#pragma unroll
+            for (int x = 0; x < I8_ITW; x++)
+                mins |= min_bytes(dnext[x].x) | min_bytes(dnext[x].y) | min_bytes(dnext[x].z) |
+                        min_bytes(dnext[x].w);
+            split = __ballot(mins != 0) != 0;
 #pragma unroll
             for (int x = 0; x < I8_ITW; x++) {
-                const uint4 d = dnext[x];
+                uint4 d = dnext[x];
                 uint32_t *R = Rs[wave][x][e];
+                if (split) {
+                    // s1 = s + 1 where s = -128 (0x80 -> 0x81, no carry); s2 = -1 there
+                    const uint32_t o0 = min_bytes(d.x) >> 7, o1 = min_bytes(d.y) >> 7, o2 = min_bytes(d.z) >> 7,
+                                   o3 = min_bytes(d.w) >> 7;
+                    d = make_uint4(d.x + o0, d.y + o1, d.z + o2, d.w + o3);
+                    const uint32_t t0 = __builtin_bswap32(o0), t1 = __builtin_bswap32(o1), t2 = __builtin_bswap32(o2),
+                                   t3 = __builtin_bswap32(o3);
+                    uint32_t *R2 = Rs2[wave][x][e];
+                    // s2 bytes are -1 (= 0x01 * 255), -s2 bytes are +1
+                    *reinterpret_cast<uint4 *>(R2 + 12 - 4 * kq) = make_uint4(t3 * 255u, t2 * 255u, t1 * 255u, t0 * 255u);
+                    *reinterpret_cast<uint4 *>(R2 + 28 - 4 * kq) = make_uint4(t3, t2, t1, t0);
+                }
                 const uint32_t r0 = __builtin_bswap32(d.x), r1 = __builtin_bswap32(d.y), r2 = __builtin_bswap32(d.z),
                                r3 = __builtin_bswap32(d.w);
                 *reinterpret_cast<uint4 *>(R + 12 - 4 * kq) = make_uint4(r3, r2, r1, r0);
@@ -159,10 +197,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             if (j + JB < j1) load_digits(j + JB, dnext);
         }
         if (j + 1 < j1) load_b(j + 1, b1);
-        compute(jj, b0);
+        compute(Rs[wave], jj, b0);
+        if (split) compute(Rs2[wave], jj, b0);
         if (j + 1 < j1) {
             if (j + 2 < j1) load_b(j + 2, b0);
-            compute(jj + 1, b1);
+            compute(Rs[wave], jj + 1, b1);
+            if (split) compute(Rs2[wave], jj + 1, b1);
         }
     }
     // D layout: lane (rho, g), register i -> row 4g + i of tile t, column 16u + rho = 4 * row_i + limb.

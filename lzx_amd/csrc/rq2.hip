@@ -464,7 +464,20 @@ static lzx_status rq2_run(int mode, const lzx_ntt_plan *plan, const uint32_t *a,
     if (!a || !out || (mode == RQ2_MUL && !b) || (mode == RQ2_INV && !ok)) return LZX_INVALID_ARGUMENT;
     hipStream_t s = (hipStream_t)stream;
     const bool aligned = (((uintptr_t)a | (uintptr_t)(b ? b : a) | (uintptr_t)out) & 15) == 0;
-    if (plan->log_n >= 2 && plan->log_n <= 6 && aligned) {
+    // The fused pass writes each output tile after reading the same tile of
+    // its inputs.  That is safe in place only when out is exactly an input of
+    // the same [count][2][n] layout; a norm (out is [count][n]) or a shifted
+    // overlap would overwrite other tiles' unread inputs, so those take the
+    // composed path, which copies the inputs to scratch first.
+    const size_t in_bytes = count * 2 * (size_t)plan->n * sizeof(uint32_t);
+    const size_t out_bytes = mode == RQ2_NORM ? in_bytes / 2 : in_bytes;
+    auto overlaps = [&](const uint32_t *x) {
+        const uintptr_t xo = (uintptr_t)x, oo = (uintptr_t)out;
+        return x && xo < oo + out_bytes && oo < xo + in_bytes;
+    };
+    auto unsafe = [&](const uint32_t *x) { return overlaps(x) && (mode == RQ2_NORM || x != out); };
+    const bool alias_safe = !unsafe(a) && !unsafe(mode == RQ2_MUL ? b : nullptr);
+    if (plan->log_n >= 2 && plan->log_n <= 6 && aligned && alias_safe) {
         if (plan->arith == ARITH_MONT) // q > 2^31: exact FP64 in place of Montgomery
             dispatch_fused_f64(mode, a, b, out, ok, plan, count, s);
         else

@@ -1096,6 +1096,12 @@ extern "C" lzx_status lzx_sumcheck_verify(const uint64_t *rounds, size_t num_rou
 {
     if (!transcript || !final_claim || (num_rounds && (!rounds || !point_out))) return LZX_INVALID_ARGUMENT;
     if (num_rounds != num_vars) return LZX_SUMCHECK_BAD_ROUND_SHAPE; // sumcheck.rs:332-337
+    // Goldilocks values are canonical in the reference (field.rs:9-40); a
+    // non-canonical word would give one proof two encodings with different
+    // challenges, so it is refused before anything is absorbed.
+    if (claim >= GL_P || (has_expected && expected_final >= GL_P)) return LZX_INVALID_ARGUMENT;
+    for (size_t i = 0; i < num_rounds * round_len; i++)
+        if (rounds[i] >= GL_P) return LZX_INVALID_ARGUMENT;
     uint64_t current = claim;
     for (size_t r = 0; r < num_rounds; r++) {
         const uint64_t *ev = rounds + r * round_len;

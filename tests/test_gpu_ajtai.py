@@ -146,6 +146,44 @@ def test_int8_extreme_digits_and_limbs(mods):
         assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, 64), want), f"path {path}"
 
 
+def test_int8_digit_minus128_exact(mods):
+    """Digit -128 is a valid int8 digit: every path, AUTO included, gives the
+    oracle's cached-NTT commit (ajtai.rs:118-153).  Batches mix elements that
+    are all -128, sparse -128 among +-127, and 16-element runs with no -128
+    at all (both sides of the kernel's per-batch split); m spans several
+    int32 fold-back chunks; A entries sit at the centred extremes."""
+    CM, D, R = mods
+    q = O.Q_32[0]
+    k, m, W = 4, 4100, 3
+    rng = np.random.default_rng(128)
+    ext = np.array([0, 1, (q - 1) // 2, (q + 1) // 2, q - 1, q - 2], dtype=np.uint64)
+    A = ext[rng.integers(0, ext.size, size=(k * m, 64))].astype(np.uint32)
+    A[: k * m // 2] = rng.integers(0, q, size=(k * m // 2, 64), dtype=np.uint64).astype(np.uint32)
+    pk = CM.AjtaiPublicKey.from_matrix(_params(mods, 6, k, m, 128), A)
+    digits = rng.choice(np.array([-128, 127, -127, 1, 0, -1], dtype=np.int8), size=(W, m, 64))
+    digits[:, 32:64] = rng.choice(np.array([127, -127, 5, 0], dtype=np.int8), size=(W, 32, 64))  # no -128
+    digits[0, 100:140] = -128
+    digits[1, 2031:2034] = -128  # across a chunk boundary
+    digits[2, 7] = 0
+    digits[2, 7, 63] = -128
+    ot = O.Ntt(*O.Q_32, 6)
+    Ah = np.stack([ot.forward(A[i])[1] for i in range(k * m)])
+    resid = np.mod(digits.astype(np.int64), q).astype(np.uint32)
+    want = np.stack([O.ajtai_commit(ot, k, m, Ah, resid[w])[1] for w in range(W)])
+    dd = D.to_device(digits)
+    for path in (CM.COMMIT_NTT, CM.COMMIT_INT8, CM.COMMIT_AUTO):
+        out = D.empty(W * k * 64, np.uint32)
+        pk.commit_digits_batch(dd, W, out, path=path)
+        assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, 64), want), f"path {path}"
+    # one witness of nothing but -128
+    full = np.full((1, m, 64), -128, np.int8)
+    res1 = np.mod(full[0].astype(np.int64), q).astype(np.uint32)
+    want1 = O.ajtai_commit(ot, k, m, Ah, res1)[1]
+    out = D.empty(k * 64, np.uint32)
+    pk.commit_digits_batch(D.to_device(full), 1, out, path=CM.COMMIT_INT8)
+    assert np.array_equal(D.to_host(out, np.uint32).reshape(k, 64), want1)
+
+
 def test_full_workload_int8_equals_ntt_path(mods):
     """d=64, k=16, m=2^16, W=64, A from seed [0x43;32], digits in [-8,8) with
     P(0)=17/32: the two device paths agree bit for bit; witness 0 is checked

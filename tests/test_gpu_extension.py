@@ -52,6 +52,35 @@ def test_rq2_mul_norm_inverse_match_oracle(ext, name, log_n, count):
     assert np.array_equal(D.to_host(da, np.uint32).reshape(count, 2 * n), ntt.rq2_mul(a, b))
 
 
+@pytest.mark.parametrize("name", ["q32", "q31"])
+@pytest.mark.parametrize("log_n,count", [(6, 1000), (4, 257), (8, 40)])
+def test_rq2_in_place_norm_and_overlaps(ext, name, log_n, count):
+    """The norm's [count][n] output written over its own [count][2][n] input
+    (many tiles, so another tile's unread input would be overwritten by a
+    fused in-place pass), in-place inverse, and a product whose output
+    overlaps its input shifted by one element."""
+    E, D, R = ext
+    ring, ntt, q = _setup(R, name, log_n)
+    n = 1 << log_n
+    rng = np.random.default_rng(log_n * 7 + count)
+    a = rng.integers(0, q, (count, 2 * n), dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, q, (count, 2 * n), dtype=np.uint64).astype(np.uint32)
+    da = D.to_device(a.reshape(-1))
+    E.norm_batch(ring, da, out=da)
+    assert np.array_equal(D.to_host(da, np.uint32)[: count * n].reshape(count, n), ntt.rq2_norm(a))
+    da = D.to_device(a)
+    ok, _ = E.inverse_batch(ring, da, out=da)
+    wok, winv = ntt.rq2_inverse(a)
+    assert D.to_host(ok, np.uint8).tolist() == wok.tolist()
+    assert np.array_equal(D.to_host(da, np.uint32).reshape(count, 2 * n), winv)
+    # out = a shifted by one element, over a buffer of count + 1 elements
+    buf = D.to_device(np.concatenate([a, np.zeros((1, 2 * n), np.uint32)]).reshape(-1))
+    db = D.to_device(b)
+    E.mul_batch(ring, buf[: count * 2 * n], db, out=buf[2 * n:])
+    got = D.to_host(buf, np.uint32).reshape(count + 1, 2 * n)[1:]
+    assert np.array_equal(got, ntt.rq2_mul(a, b))
+
+
 @pytest.mark.parametrize("log_n", [4, 6, 8])
 def test_rq2_inverse_flags_zero_norm_slots(ext, log_n):
     # over q = 12289 a norm slot vanishes often enough to see both outcomes in one batch

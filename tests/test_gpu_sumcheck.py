@@ -127,8 +127,24 @@ def test_errors_and_zero_polynomial(sc):
     with pytest.raises(LzxError) as e:
         S.verify(out.rounds[:-1], nv, 3, claim, S.Transcript(b"t"))
     assert e.value.code == 10
+    with pytest.raises(LzxError) as e:
+        S.verify(out.rounds, nv, 3, P, S.Transcript(b"t"))
+    assert e.value.code == 30
     z = S.prove(S.VirtualPolynomial(3), 0, S.Transcript(b"t"))
     assert z.rounds.shape == (3, 2) and not z.rounds.any()
+    S.verify(z.rounds, 3, 1, 0, S.Transcript(b"t"), 0)
+    # non-canonical Goldilocks words (p encodes the same element as 0) are
+    # refused before anything is absorbed: round values, claim, final claim
+    for i in range(z.rounds.size):
+        r2 = z.rounds.copy()
+        r2.reshape(-1)[i] = P
+        with pytest.raises(LzxError) as e:
+            S.verify(r2, 3, 1, 0, S.Transcript(b"t"), 0)
+        assert e.value.code == 30
+    for claim_, fin_ in [(P, 0), (0, P)]:
+        with pytest.raises(LzxError) as e:
+            S.verify(z.rounds, 3, 1, claim_, S.Transcript(b"t"), fin_)
+        assert e.value.code == 30
     bad = S.VirtualPolynomial(nv)
     bad.add_factor(D.to_device(tables[0]))
     bad.add_term(1, [])
