@@ -254,6 +254,22 @@ lzx_status lzx_ajtai_commit_batch(const lzx_ajtai_key *key, const uint32_t *s_de
 lzx_status lzx_ajtai_commit_digits_batch(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t s_len,
                                          size_t W, uint32_t *out_dev, int32_t path, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): lzx_ajtai_commit_digits_batch with path
+ * LZX_COMMIT_INT8, bit-identical, but allocating nothing.  Its chunk partial
+ * sums live in a fixed array of LZX_NOALLOC_SCRATCH_BYTES in the library's
+ * device code, one per device, held until the process ends.  Limit:
+ * ceil(m / 2032) * W * k * 256 bytes <= LZX_NOALLOC_SCRATCH_BYTES, else
+ * LZX_UNSUPPORTED (the reason is in lzx_last_device_error()); n = 64 and
+ * q < 2^32 - 2^25 as for the int8 path.  Every call on a device shares that
+ * device's array: calls that overlap on one device (other streams, other
+ * threads) race, so serialise them.  The array keeps the last call's
+ * contents; each call writes every word it later reads. */
+#define LZX_NOALLOC_SCRATCH_BYTES (16u << 20)
+lzx_status lzx_ajtai_commit_digits_int8_noalloc(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t s_len,
+                                                size_t W, uint32_t *out_dev, void *stream);
+
 /* Synchronous host forms: commit(&self, &[RingElement]) (ajtai.rs:118) and
  * verify_opening(&self, &AjtaiCommitment, &[RingElement]) (ajtai.rs:210):
  * norm check (NormExceeded) then recommit and compare (CommitmentMismatch). */

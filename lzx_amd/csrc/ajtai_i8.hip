@@ -40,6 +40,7 @@
 //   - blocks that share a j-chunk are numbered congruent mod 8 so they land on
 //     one XCD (round-robin dispatch) and share that XCD's L2 copy of B.
 #include <algorithm>
+#include <mutex>
 
 #include "ajtai.hpp"
 
@@ -283,63 +284,149 @@ lzx_status ajtai_build_limbs(lzx_ajtai_key *key)
     return LZX_OK;
 }
 
+// Chunking of one int8 commit: nchunks j-chunks of jchunk ring elements
+// (jchunk <= I8_MAX_JCHUNK), each writing one [W][k][64] u32 partial plane.
+struct I8Plan {
+    uint32_t m, k, npad, groups, nchunks, jchunk, padded;
+    size_t plane;
+};
+
+// max_chunks caps the chunk count (the partial buffer holds that many planes).
+static lzx_status i8_plan(const lzx_ajtai_key *key, size_t W, uint32_t max_chunks, I8Plan &p)
+{
+    p.m = (uint32_t)key->m;
+    p.k = (uint32_t)key->k;
+    p.npad = npad_of(key->k);
+    const uint32_t per_block = I8_WAVES * I8_ITW;
+    p.groups = (uint32_t)((W + per_block - 1) / per_block);
+    const uint32_t min_chunks = (p.m + I8_MAX_JCHUNK - 1) / I8_MAX_JCHUNK;
+    if (min_chunks > max_chunks) return LZX_UNSUPPORTED;
+    // ~2 blocks per CU in flight; every chunk within the int32 fold-back bound
+    uint32_t nchunks = std::max<uint32_t>(min_chunks, (1024 + p.groups - 1) / p.groups);
+    nchunks = std::min<uint32_t>(nchunks, (p.m + 15) / 16);
+    nchunks = std::min<uint32_t>(nchunks, max_chunks);
+    nchunks = std::max<uint32_t>(nchunks, min_chunks);
+    nchunks = std::max<uint32_t>(nchunks, 1);
+    p.jchunk = (p.m + nchunks - 1) / nchunks;
+    p.nchunks = (p.m + p.jchunk - 1) / p.jchunk;
+    p.padded = (p.nchunks + 7) / 8 * 8;
+    if ((uint64_t)p.padded * p.groups > 0x7FFFFFFFull) return LZX_UNSUPPORTED;
+    p.plane = W * (size_t)p.k * 64;
+    return LZX_OK;
+}
+
+// The MFMA kernels over every 64-column slice of the limb planes, then the
+// chunk reduction into out.  `partial` holds nchunks planes; every word the
+// reduction reads is written first by the same call.
+static lzx_status i8_launch(const lzx_ajtai_key *key, const I8Plan &p, const int8_t *digits_dev, size_t W,
+                            uint32_t *partial, uint32_t *out_dev, hipStream_t stream)
+{
+    const uint32_t q = key->plan->q;
+    const uint64_t barrett = key->plan->mod.barrett;
+    for (uint32_t col0 = 0; col0 < p.npad; col0 += 64) {
+        const uint32_t nt = std::min<uint32_t>(4, (p.npad - col0) / 16);
+        const dim3 grid(p.padded * p.groups);
+#define LZX_I8_LAUNCH(NT)                                                                                        \
+    hipLaunchKernelGGL(ajtai_i8_kernel<NT>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,      \
+                       (uint32_t)W, p.m, p.k, p.npad, col0, p.jchunk, p.nchunks, p.groups, q, barrett)
+        switch (nt) {
+        case 1: LZX_I8_LAUNCH(1); break;
+        case 2: LZX_I8_LAUNCH(2); break;
+        case 3: LZX_I8_LAUNCH(3); break;
+        default: LZX_I8_LAUNCH(4); break;
+        }
+#undef LZX_I8_LAUNCH
+        LZX_LAUNCH_CHECK("ajtai_i8_kernel");
+    }
+    const uint32_t blocks = (uint32_t)std::min<size_t>((p.plane + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(ajtai_i8_reduce_kernel, dim3(blocks), dim3(256), 0, stream, partial, out_dev, p.plane,
+                       p.nchunks, q, barrett);
+    LZX_LAUNCH_CHECK("ajtai_i8_reduce_kernel");
+    return LZX_OK;
+}
+
 lzx_status ajtai_commit_int8(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t W, uint32_t *out_dev,
                              hipStream_t stream)
 {
     if (W == 0) return LZX_OK;
     if (!ajtai_int8_supported(key) || !key->limbs_dev) return LZX_UNSUPPORTED;
     if ((uintptr_t)digits_dev % 16) return LZX_INVALID_ARGUMENT;
-    const uint32_t m = (uint32_t)key->m, k = (uint32_t)key->k, npad = npad_of(k);
-    const uint32_t q = key->plan->q;
-    const uint64_t barrett = key->plan->mod.barrett;
-    const uint32_t per_block = I8_WAVES * I8_ITW;
-    const uint32_t groups = (uint32_t)((W + per_block - 1) / per_block);
-    // ~2 blocks per CU in flight; every chunk within the int32 fold-back bound
-    uint32_t nchunks = std::max<uint32_t>((m + I8_MAX_JCHUNK - 1) / I8_MAX_JCHUNK, (1024 + groups - 1) / groups);
-    nchunks = std::min<uint32_t>(nchunks, (m + 15) / 16);
-    nchunks = std::max<uint32_t>(nchunks, 1);
-    const uint32_t jchunk = (m + nchunks - 1) / nchunks;
-    nchunks = (m + jchunk - 1) / jchunk;
-    const uint32_t padded = (nchunks + 7) / 8 * 8;
-    if ((uint64_t)padded * groups > 0x7FFFFFFFull) return LZX_UNSUPPORTED;
-    const size_t plane = W * (size_t)k * 64;
-
+    I8Plan p;
+    lzx_status st = i8_plan(key, W, UINT32_MAX, p);
+    if (st != LZX_OK) return st;
     uint32_t *partial = nullptr;
-    LZX_HIP(scratch_alloc((void **)&partial, (size_t)nchunks * plane * sizeof(uint32_t), stream));
-    lzx_status st = LZX_OK;
-    for (uint32_t col0 = 0; col0 < npad && st == LZX_OK; col0 += 64) {
-        const uint32_t nt = std::min<uint32_t>(4, (npad - col0) / 16);
-        const dim3 grid(padded * groups);
-        switch (nt) {
-        case 1:
-            hipLaunchKernelGGL(ajtai_i8_kernel<1>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
-            break;
-        case 2:
-            hipLaunchKernelGGL(ajtai_i8_kernel<2>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
-            break;
-        case 3:
-            hipLaunchKernelGGL(ajtai_i8_kernel<3>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
-            break;
-        default:
-            hipLaunchKernelGGL(ajtai_i8_kernel<4>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,
-                               (uint32_t)W, m, k, npad, col0, jchunk, nchunks, groups, q, barrett);
-            break;
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) st = device_fail(e, "ajtai_i8_kernel");
-    }
-    if (st == LZX_OK) {
-        const uint32_t blocks = (uint32_t)std::min<size_t>((plane + 255) / 256, 1 << 16);
-        hipLaunchKernelGGL(ajtai_i8_reduce_kernel, dim3(blocks), dim3(256), 0, stream, partial, out_dev, plane,
-                           nchunks, q, barrett);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) st = device_fail(e, "ajtai_i8_reduce_kernel");
-    }
+    LZX_HIP(scratch_alloc((void **)&partial, (size_t)p.nchunks * p.plane * sizeof(uint32_t), stream));
+    st = i8_launch(key, p, digits_dev, W, partial, out_dev, stream);
     (void)scratch_free(partial, stream);
     return st;
 }
 
+// ---- allocation-free variant --------------------------------------------
+//
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops): the chunk partials live in this fixed array
+// instead of pool scratch.  Each device the library runs on holds its own
+// copy at its own address until the process ends.  Host code never passes
+// the array's name to a kernel (in host code it is a stand-in, not a device
+// address); it uses the address hipGetSymbolAddress returns for the device.
+constexpr size_t NOALLOC_WORDS = LZX_NOALLOC_SCRATCH_BYTES / sizeof(uint32_t);
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+uint32_t *g_noalloc_ptr[NOALLOC_MAX_DEVICES]; // per-device address of `scratch`
+} // namespace
+
+// Address of `scratch` on device dev, looked up once per device with that
+// device current.
+static lzx_status noalloc_scratch(int dev, uint32_t **out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return LZX_UNSUPPORTED;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        int cur = 0;
+        LZX_HIP(hipGetDevice(&cur));
+        if (cur != dev) LZX_HIP(hipSetDevice(dev));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return device_fail(e, "hipGetSymbolAddress(scratch)");
+        g_noalloc_ptr[dev] = (uint32_t *)ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return LZX_OK;
+}
+
 } // namespace lzx
+
+using namespace lzx;
+
+extern "C" lzx_status lzx_ajtai_commit_digits_int8_noalloc(const lzx_ajtai_key *key, const int8_t *digits_dev,
+                                                          size_t s_len, size_t W, uint32_t *out_dev, void *stream)
+{
+    if (!key) return LZX_INVALID_ARGUMENT;
+    if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH;
+    if (W == 0) return LZX_OK;
+    if (!digits_dev || !out_dev || ((uintptr_t)digits_dev % 16)) return LZX_INVALID_ARGUMENT;
+    if (!ajtai_int8_supported(key) || !key->limbs_dev) return LZX_UNSUPPORTED;
+    // the partials of at least ceil(m / 2032) chunks must fit the fixed array
+    const size_t plane = W * key->k * 64;
+    if (plane > NOALLOC_WORDS) {
+        note_error("lzx_ajtai_commit_digits_int8_noalloc: W*k*64*4 bytes exceed LZX_NOALLOC_SCRATCH_BYTES");
+        return LZX_UNSUPPORTED;
+    }
+    I8Plan p;
+    if (i8_plan(key, W, (uint32_t)std::min<size_t>(NOALLOC_WORDS / plane, UINT32_MAX), p) != LZX_OK) {
+        note_error("lzx_ajtai_commit_digits_int8_noalloc: ceil(m/2032)*W*k*64*4 bytes exceed "
+                   "LZX_NOALLOC_SCRATCH_BYTES");
+        return LZX_UNSUPPORTED;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipDevice_t dev = 0;
+    LZX_HIP(hipStreamGetDevice(s, &dev));
+    uint32_t *partial = nullptr;
+    lzx_status st = noalloc_scratch(dev, &partial);
+    if (st != LZX_OK) return st;
+    return i8_launch(key, p, digits_dev, W, partial, out_dev, s);
+}

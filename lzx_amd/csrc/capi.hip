@@ -25,6 +25,8 @@ lzx_status device_fail(hipError_t e, const char *what)
     return LZX_DEVICE_ERROR;
 }
 
+void note_error(const char *what) { g_last_error = what; }
+
 // Stream-ordered scratch from a per-device pool that keeps freed memory for
 // reuse (release threshold = unlimited), so a call that needs the same
 // working set as the last one maps no new memory.  The pool holds the

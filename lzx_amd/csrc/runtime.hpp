@@ -13,6 +13,10 @@ namespace lzx {
 // status code.
 lzx_status device_fail(hipError_t e, const char *what);
 
+// Records a message for lzx_last_device_error() without a HIP error (a
+// refusal whose reason a caller may want to read).
+void note_error(const char *what);
+
 // Stream-ordered device scratch from the library's per-device pool (freed
 // memory stays cached in the pool for the next call).
 hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t s);
