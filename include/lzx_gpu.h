@@ -275,6 +275,27 @@ lzx_status lzx_ajtai_commit_digits_int8_noalloc(const lzx_ajtai_key *key, const 
  * norm check (NormExceeded) then recommit and compare (CommitmentMismatch). */
 lzx_status lzx_ajtai_commit_host(const lzx_ajtai_key *key, const uint32_t *s_host, size_t s_len,
                                  uint32_t *out_host);
+
+/* Replaces commit_naive_reference (ajtai.rs:158-180): every product A_ij * s_j
+ * as its own ring multiplication from the coefficient-domain A (NTT of both,
+ * pointwise, inverse NTT per product), summed over j in the coefficient
+ * domain.  Equal to commit on every input; kept as the differential path.
+ * Device form: s_dev [W][m][n], out_dev [W][k][n]; host form: one witness. */
+lzx_status lzx_ajtai_commit_naive_batch(const lzx_ajtai_key *key, const uint32_t *s_dev, size_t s_len, size_t W,
+                                        uint32_t *out_dev, void *stream);
+lzx_status lzx_ajtai_commit_naive_host(const lzx_ajtai_key *key, const uint32_t *s_host, size_t s_len,
+                                       uint32_t *out_host);
+
+/* Replaces sample_small_secret(&RingConfig, m, bound, seed) (ajtai.rs:307-348)
+ * on the device, one thread per ring element: m elements of n = 2^log_n
+ * coefficients uniform in [-bound, bound], written as residues mod q to
+ * out_dev [m][n] u32 and/or as int8 digits to digits_dev [m][n] (bound <= 127);
+ * either pointer may be NULL, not both.  LZX_INVALID_ARGUMENT for
+ * bound >= 2^31 (the reference's span 2*bound + 1 would exceed a u32 window
+ * and its rejection loop would never end); LZX_UNSUPPORTED for seed_len > 95
+ * (the framed XOF input must fit one SHAKE-256 block). */
+lzx_status lzx_ajtai_sample_small_secret(uint32_t q, uint32_t log_n, size_t m, uint32_t bound, const uint8_t *seed,
+                                         size_t seed_len, uint32_t *out_dev, int8_t *digits_dev, void *stream);
 lzx_status lzx_ajtai_verify_opening_host(const lzx_ajtai_key *key, const uint32_t *t_host, const uint32_t *s_host,
                                          size_t s_len);
 

@@ -66,6 +66,9 @@ PROTOTYPES = {
     "lzx_ajtai_commit_digits_batch": (st_t, [vp, vp, sz, sz, vp, i32, vp]),
     "lzx_ajtai_commit_digits_int8_noalloc": (st_t, [vp, vp, sz, sz, vp, vp]),
     "lzx_ajtai_commit_host": (st_t, [vp, vp, sz, vp]),
+    "lzx_ajtai_commit_naive_batch": (st_t, [vp, vp, sz, sz, vp, vp]),
+    "lzx_ajtai_commit_naive_host": (st_t, [vp, vp, sz, vp]),
+    "lzx_ajtai_sample_small_secret": (st_t, [u32, u32, sz, u32, u8p, sz, vp, vp, vp]),
     "lzx_ajtai_verify_opening_host": (st_t, [vp, vp, vp, sz]),
     # Witness packing
     "lzx_pack_field_elements_len": (sz, [sz, u32]),

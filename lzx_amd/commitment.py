@@ -113,6 +113,15 @@ class AjtaiPublicKey:
               "AjtaiPublicKey::commit")
         return AjtaiCommitment(out, self.params.ring)
 
+    def commit_naive_reference(self, s) -> AjtaiCommitment:
+        """commit_naive_reference(&[RingElement]) (ajtai.rs:158-180): one ring
+        product per (i, j), summed in the coefficient domain."""
+        arr = _witness_array(s, self.params.ring.n())
+        out = np.zeros((self.params.k, self.params.ring.n()), dtype=np.uint32)
+        check(lib().lzx_ajtai_commit_naive_host(self._h, arr.ctypes.data, arr.shape[0], out.ctypes.data),
+              "AjtaiPublicKey::commit_naive_reference")
+        return AjtaiCommitment(out, self.params.ring)
+
     def verify_opening(self, commitment: AjtaiCommitment, s) -> None:
         """verify_opening (ajtai.rs:210-230); raises LzxError on failure."""
         arr = _witness_array(s, self.params.ring.n())
@@ -132,6 +141,28 @@ class AjtaiPublicKey:
                                                   self.params.m if s_len is None else s_len, W, dev.ptr(out_dev),
                                                   path, dev.stream_handle(stream)),
               "lzx_ajtai_commit_digits_batch")
+
+
+    def commit_naive_batch(self, s_dev, W: int, out_dev, s_len=None, stream=None) -> None:
+        """commit_naive_reference over [W][m][n] residues in HBM."""
+        check(lib().lzx_ajtai_commit_naive_batch(self._h, dev.ptr(s_dev), self.params.m if s_len is None else s_len,
+                                                 W, dev.ptr(out_dev), dev.stream_handle(stream)),
+              "lzx_ajtai_commit_naive_batch")
+
+
+def sample_small_secret(ring: RingConfig, m: int, bound: int, seed: bytes, out=None, digits=None, stream=None):
+    """sample_small_secret(&RingConfig, m, bound, seed) (ajtai.rs:307-348) on
+    the device.  Writes residues [m][n] into `out` and/or int8 digits into
+    `digits` (device tensors; allocated when both are None, residues then)."""
+    n = ring.n()
+    if out is None and digits is None:
+        out = dev.empty((m, n), np.uint32)
+    mod: Modulus32 = ring.modulus
+    check(lib().lzx_ajtai_sample_small_secret(mod.q, ring.log_n, m, bound, seed, len(seed),
+                                              dev.ptr(out) if out is not None else None,
+                                              dev.ptr(digits) if digits is not None else None,
+                                              dev.stream_handle(stream)), "sample_small_secret")
+    return out if digits is None else (out, digits)
 
 
 def _witness_array(s, n):

@@ -256,35 +256,112 @@ __global__ __launch_bounds__(256) void derive_uniform_kernel(XofBlock base, uint
     }
 }
 
-// Host side of the block: absorb the fixed framing + salt with idx = 0, pad.
-static void build_xof_block(const uint8_t *domain, size_t dlen, const uint8_t *seed, size_t slen, XofBlock &blk,
-                            uint32_t &io)
+// Host side of the block: the framed XOF message
+//   "LZXX" || u32le |xdom| || xdom || u32le |msg| || msg,   msg = pre1 || pre2 || var
+// (transcript.rs:229-237) with the var_len variable bytes zero, padded as one
+// SHAKE-256 block; the kernel XORs its variable bytes in at offset io.
+// Returns false if the message does not fit one block.
+static bool build_xof_block(const char *xdom, const uint8_t *pre1, size_t len1, const uint8_t *pre2, size_t len2,
+                            size_t var_len, XofBlock &blk, uint32_t &io)
 {
+    const size_t xdlen = strlen(xdom);
+    if (12 + xdlen + len1 + len2 + var_len > 135) return false;
     uint8_t b[136] = {};
     size_t o = 0;
     auto put = [&](const void *p, size_t len) {
-        memcpy(b + o, p, len);
+        if (len) memcpy(b + o, p, len);
         o += len;
     };
     auto put32 = [&](uint32_t v) {
         const uint8_t x[4] = {(uint8_t)v, (uint8_t)(v >> 8), (uint8_t)(v >> 16), (uint8_t)(v >> 24)};
         put(x, 4);
     };
-    static const uint8_t TAG[] = {'L', 'Z', 'X', 'X'}, DOM[] = {'u', 'n', 'i', 'f', 'o', 'r', 'm'};
-    put(TAG, 4);
-    put32(7);
-    put(DOM, 7);
-    put32((uint32_t)(dlen + slen + 8));
-    put(domain, dlen);
-    put(seed, slen);
+    put("LZXX", 4);
+    put32((uint32_t)xdlen);
+    put(xdom, xdlen);
+    put32((uint32_t)(len1 + len2 + var_len));
+    put(pre1, len1);
+    put(pre2, len2);
     io = (uint32_t)o;
-    o += 8; // u64le(idx), XORed in per element (o + 1 <= 135: the callers' salts are 39 bytes)
+    o += var_len;
     b[o] ^= 0x1f;  // SHAKE-256 domain suffix
     b[135] ^= 0x80; // pad10*1
     for (int l = 0; l < 17; l++) {
         uint64_t v = 0;
         for (int i = 0; i < 8; i++) v |= (uint64_t)b[8 * l + i] << (8 * i);
         blk.lane[l] = v;
+    }
+    return true;
+}
+
+// XORs the 8-byte little-endian word v into the absorbed block at byte io.
+__device__ __forceinline__ void xor_u64_at(uint64_t (&st)[25], uint32_t io, uint64_t v)
+{
+    const uint32_t L = io >> 3, sh = 8 * (io & 7);
+#pragma unroll
+    for (int l = 0; l < 17; l++) {
+        if (l == (int)L) st[l] ^= v << sh;
+        if (l == (int)L + 1 && sh) st[l] ^= v >> (64 - sh);
+    }
+}
+
+// sample_small_secret (ajtai.rs:307-348), one thread per ring element i: for
+// ctr = 0, 1, ...: stream = XOF("ajtai-secret", seed || u64le(i) || u64le(ctr),
+// words_per_try * 4 bytes), 4-byte LE words from byte 0; a word v < limit =
+// floor((2^32 - 1) / span) * span gives the next coefficient (v mod span) -
+// bound, stored as its residue mod q (and as an int8 digit when asked).
+__global__ __launch_bounds__(256) void small_secret_kernel(XofBlock base, uint32_t io, uint32_t words_per_try,
+                                                           uint32_t bound, uint32_t span, uint32_t limit, uint32_t q,
+                                                           uint32_t n, uint64_t m, uint32_t *__restrict__ out,
+                                                           int8_t *__restrict__ digits)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t got = 0;
+        for (uint64_t ctr = 0; got < n; ctr++) {
+            uint64_t st[25];
+#pragma unroll
+            for (int l = 0; l < 25; l++) st[l] = l < 17 ? base.lane[l] : 0;
+            xor_u64_at(st, io, i);
+            xor_u64_at(st, io + 8, ctr);
+            for (uint32_t w0 = 0; w0 < words_per_try && got < n; w0 += 34) {
+                keccak_f1600(st);
+#pragma unroll
+                for (int w = 0; w < 34; w++) {
+                    const uint32_t v = (uint32_t)(st[w >> 1] >> (32 * (w & 1)));
+                    if (w0 + w < words_per_try && got < n && v < limit) {
+                        const int64_t c = (int64_t)(v % span) - (int64_t)bound;
+                        const int64_t r = c % (int64_t)q;
+                        if (out) out[i * n + got] = (uint32_t)(r < 0 ? r + q : r);
+                        if (digits) digits[i * n + got] = (int8_t)c;
+                        got++;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// commit_naive_reference (ajtai.rs:158-180): the k*m ring products A_ij * s_j
+// one by one (NTT of A_ij and of s_j, pointwise product, inverse NTT per
+// product), then a coefficient-domain sum over j.  P[i][j][t] = X[i][j][t] *
+// S[j][t] forms the products in the NTT domain; naive_sum_kernel sums the
+// inverse-transformed products per row.
+__global__ void naive_pointwise_kernel(uint32_t *__restrict__ P, const uint32_t *__restrict__ S, size_t mn,
+                                       size_t total, uint32_t q, uint64_t barrett)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+        P[e] = reduce_u64((uint64_t)P[e] * S[e % mn], q, barrett);
+}
+
+__global__ void naive_sum_kernel(const uint32_t *__restrict__ P, uint32_t *__restrict__ out, uint32_t k, uint32_t m,
+                                 uint32_t n, uint32_t q)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < (size_t)k * n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = e / n, t = e % n;
+        uint32_t acc = 0;
+        for (uint32_t j = 0; j < m; j++) acc = add_big(acc, P[(i * m + j) * n + t], q);
+        out[e] = acc;
     }
 }
 
@@ -355,7 +432,7 @@ extern "C" lzx_status lzx_ajtai_key_from_seed(uint32_t q, uint32_t two_adicity, 
     // ajtai.rs:91-96: element idx = i*m + j, derived on the device
     XofBlock blk;
     uint32_t io = 0;
-    build_xof_block(DOM, 7, seed, 32, blk, io);
+    build_xof_block("uniform", DOM, 7, seed, 32, 8, blk, io);
     hipError_t e = hipMalloc(&key->A_dev, total * n * sizeof(uint32_t));
     if (e == hipSuccess) {
         const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, 65536);
@@ -414,22 +491,91 @@ extern "C" lzx_status lzx_ajtai_commit_host(const lzx_ajtai_key *key, const uint
 {
     if (!key || !s_host || !out_host) return LZX_INVALID_ARGUMENT;
     if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH;
-    LZX_HIP(hipSetDevice(key->device));
-    const size_t sw = key->m * key->n, ow = key->k * key->n;
-    uint32_t *s = nullptr, *o = nullptr;
-    LZX_HIP(hipMalloc(&s, sw * sizeof(uint32_t)));
-    hipError_t e = hipMalloc(&o, ow * sizeof(uint32_t));
+    const size_t sb = key->m * key->n * sizeof(uint32_t), ob = key->k * key->n * sizeof(uint32_t);
+    return host_slice_call(key->device, s_host, sb, out_host, sb, ob, sb + ob, [&](void *d, hipStream_t s) {
+        return ajtai_commit_ntt(key, (const uint32_t *)d, nullptr, 1, (uint32_t *)((uint8_t *)d + sb), s);
+    });
+}
+
+// commit_naive_reference for W witnesses in HBM (see naive_pointwise_kernel).
+static lzx_status commit_naive(const lzx_ajtai_key *key, const uint32_t *s_dev, size_t W, uint32_t *out_dev,
+                               hipStream_t s)
+{
+    if (W == 0) return LZX_OK;
+    const size_t n = key->n, m = key->m, k = key->k, km = k * m, mn = m * n;
+    uint32_t *P = nullptr;
+    LZX_HIP(scratch_alloc((void **)&P, (km + m) * n * sizeof(uint32_t), s));
+    uint32_t *S = P + km * n;
+    const uint32_t q = key->plan->q;
     lzx_status st = LZX_OK;
-    if (e == hipSuccess) e = hipMemcpy(s, s_host, sw * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (e != hipSuccess) st = device_fail(e, "ajtai host commit staging");
-    if (st == LZX_OK) st = ajtai_commit_ntt(key, s, nullptr, 1, o, nullptr);
-    if (st == LZX_OK) {
-        e = hipMemcpy(out_host, o, ow * sizeof(uint32_t), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) st = device_fail(e, "ajtai host commit readback");
+    auto grid = [](size_t len) { return (uint32_t)std::min<size_t>((len + 255) / 256, 65536); };
+    for (size_t w = 0; w < W && st == LZX_OK; w++) {
+        hipError_t e = hipMemcpyAsync(P, key->A_dev, km * n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(S, s_dev + w * mn, mn * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) st = device_fail(e, "naive commit staging");
+        if (st == LZX_OK) st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, P, km + m, NTT_FWD, 0, s);
+        if (st == LZX_OK) {
+            hipLaunchKernelGGL(naive_pointwise_kernel, dim3(grid(km * n)), dim3(256), 0, s, P, S, mn, km * n, q,
+                               key->plan->mod.barrett);
+            e = hipGetLastError();
+            if (e != hipSuccess) st = device_fail(e, "naive_pointwise_kernel");
+        }
+        if (st == LZX_OK) st = ntt_launch((const lzx_ntt_plan *const *)&key->plan, 1, P, km, NTT_INV, 0, s);
+        if (st == LZX_OK) {
+            hipLaunchKernelGGL(naive_sum_kernel, dim3(grid(k * n)), dim3(256), 0, s, P, out_dev + w * k * n,
+                               (uint32_t)k, (uint32_t)m, (uint32_t)n, q);
+            e = hipGetLastError();
+            if (e != hipSuccess) st = device_fail(e, "naive_sum_kernel");
+        }
     }
-    (void)hipFree(s);
-    if (o) (void)hipFree(o);
+    (void)scratch_free(P, s);
     return st;
+}
+
+extern "C" lzx_status lzx_ajtai_commit_naive_batch(const lzx_ajtai_key *key, const uint32_t *s_dev, size_t s_len,
+                                                   size_t W, uint32_t *out_dev, void *stream)
+{
+    if (!key) return LZX_INVALID_ARGUMENT;
+    if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH; // ajtai.rs:159-164
+    if (W && (!s_dev || !out_dev)) return LZX_INVALID_ARGUMENT;
+    if (key->k * key->m > 0xFFFFFFFFull / key->n) return LZX_UNSUPPORTED;
+    return commit_naive(key, s_dev, W, out_dev, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_ajtai_commit_naive_host(const lzx_ajtai_key *key, const uint32_t *s_host, size_t s_len,
+                                                  uint32_t *out_host)
+{
+    if (!key || !s_host || !out_host) return LZX_INVALID_ARGUMENT;
+    if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH;
+    if (key->k * key->m > 0xFFFFFFFFull / key->n) return LZX_UNSUPPORTED;
+    const size_t sb = key->m * key->n * sizeof(uint32_t), ob = key->k * key->n * sizeof(uint32_t);
+    return host_slice_call(key->device, s_host, sb, out_host, sb, ob, sb + ob, [&](void *d, hipStream_t s) {
+        return commit_naive(key, (const uint32_t *)d, 1, (uint32_t *)((uint8_t *)d + sb), s);
+    });
+}
+
+extern "C" lzx_status lzx_ajtai_sample_small_secret(uint32_t q, uint32_t log_n, size_t m, uint32_t bound,
+                                                    const uint8_t *seed, size_t seed_len, uint32_t *out_dev,
+                                                    int8_t *digits_dev, void *stream)
+{
+    if ((seed_len && !seed) || q < 2) return LZX_INVALID_ARGUMENT;
+    if (m == 0) return LZX_OK;
+    if (!out_dev && !digits_dev) return LZX_INVALID_ARGUMENT;
+    // span = 2*bound + 1 must fit a u32 word (otherwise no word is ever accepted)
+    if (bound >= (1u << 31) || (digits_dev && bound > 127)) return LZX_INVALID_ARGUMENT;
+    if (log_n > NTT_MAX_LOG_N) return LZX_UNSUPPORTED;
+    XofBlock blk;
+    uint32_t io = 0;
+    if (!build_xof_block("ajtai-secret", seed, seed_len, nullptr, 0, 16, blk, io)) return LZX_UNSUPPORTED;
+    const uint32_t n = 1u << log_n;
+    const uint64_t span = 2ull * bound + 1;
+    const uint32_t limit = (uint32_t)((0xFFFFFFFFull / span) * span);
+    const uint32_t words = std::max<uint32_t>(16 * n, 64) / 4;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((m + 255) / 256, 65536);
+    hipLaunchKernelGGL(small_secret_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, blk, io, words, bound,
+                       (uint32_t)span, limit, q, n, (uint64_t)m, out_dev, digits_dev);
+    LZX_LAUNCH_CHECK("small_secret_kernel");
+    return LZX_OK;
 }
 
 extern "C" lzx_status lzx_ajtai_verify_opening_host(const lzx_ajtai_key *key, const uint32_t *t_host,

@@ -1,5 +1,6 @@
 // Status strings, HIP error capture and the library's scratch allocators for
 // the C ABI (include/lzx_gpu.h).
+#include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -90,6 +91,39 @@ void stage_release(HostStage &h)
     std::lock_guard<std::mutex> g(g_alloc_mu);
     g_stages.push_back(h);
     h = HostStage{};
+}
+
+lzx_status host_slice_call(int device, const void *in, size_t in_bytes, void *out, size_t out_off, size_t out_bytes,
+                           size_t dev_bytes, const std::function<lzx_status(void *, hipStream_t)> &run)
+{
+    LZX_HIP(hipSetDevice(device));
+    hipStream_t s = nullptr;
+    HostStage h;
+    const size_t stage_bytes = in_bytes > out_bytes ? in_bytes : out_bytes;
+    LZX_HIP(stage_acquire(h, stage_bytes ? stage_bytes : 1));
+    void *d = nullptr;
+    hipError_t e = scratch_alloc(&d, dev_bytes ? dev_bytes : 1, s);
+    if (e != hipSuccess) {
+        stage_release(h);
+        return device_fail(e, "host slice scratch");
+    }
+    lzx_status st = LZX_OK;
+    if (in_bytes) {
+        memcpy(h.p, in, in_bytes);
+        e = hipMemcpyAsync(d, h.p, in_bytes, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) st = device_fail(e, "host slice upload");
+    }
+    if (st == LZX_OK) st = run(d, s);
+    if (st == LZX_OK && out_bytes) {
+        e = hipMemcpyAsync(h.p, (uint8_t *)d + out_off, out_bytes, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) st = device_fail(e, "host slice readback");
+    }
+    (void)scratch_free(d, s);
+    e = hipStreamSynchronize(s);
+    if (st == LZX_OK && e != hipSuccess) st = device_fail(e, "host slice synchronize");
+    if (st == LZX_OK && out_bytes) memcpy(out, h.p, out_bytes);
+    stage_release(h);
+    return st;
 }
 
 } // namespace lzx

@@ -550,19 +550,10 @@ static lzx_status run_host(const lzx_ntt_plan *plan, uint32_t *a_host, size_t le
     if (!plan || (!a_host && len)) return LZX_INVALID_ARGUMENT;
     if (len % plan->n) return LZX_NTT_LENGTH_MISMATCH;
     if (len == 0) return LZX_OK;
-    LZX_HIP(hipSetDevice(plan->device));
-    uint32_t *d = nullptr;
-    LZX_HIP(hipMalloc(&d, len * sizeof(uint32_t)));
-    lzx_status st = LZX_OK;
-    hipError_t e = hipMemcpy(d, a_host, len * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (e != hipSuccess) st = device_fail(e, "hipMemcpy H2D");
-    if (st == LZX_OK) st = ntt_launch(&plan, 1, d, len / plan->n, mode, levels, nullptr);
-    if (st == LZX_OK) {
-        e = hipMemcpy(a_host, d, len * sizeof(uint32_t), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) st = device_fail(e, "hipMemcpy D2H");
-    }
-    (void)hipFree(d);
-    return st;
+    const size_t bytes = len * sizeof(uint32_t);
+    return host_slice_call(plan->device, a_host, bytes, a_host, 0, bytes, bytes, [&](void *d, hipStream_t s) {
+        return ntt_launch(&plan, 1, (uint32_t *)d, len / plan->n, mode, levels, s);
+    });
 }
 
 } // namespace lzx

@@ -73,7 +73,7 @@ enum NttMode { NTT_FWD = 0, NTT_INV = 1, NTT_PARTIAL = 2, NTT_COMPLETE = 3 };
 
 // Launch one transform mode over nplans plans sharing n and bigq; data is
 // laid out [plan][poly][n] with `polys` polynomials per plan.
-// Forward NTT of `polys` small-digit polynomials (i8, |digit| <= 127, 16-byte
+// Forward NTT of `polys` small-digit polynomials (i8, any value, 16-byte
 // aligned) into u32 NTT-domain output; 16 <= n <= 64, else LZX_UNSUPPORTED.
 lzx_status ntt_forward_digits(const lzx_ntt_plan *plan, const int8_t *digits, uint32_t *out, size_t polys,
                               hipStream_t stream);

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+
 #include "../../include/lzx_gpu.h"
 
 namespace lzx {
@@ -29,6 +31,15 @@ struct HostStage {
 };
 hipError_t stage_acquire(HostStage &h, size_t bytes);
 void stage_release(HostStage &h);
+
+// A synchronous call on host slices (the reference's &mut [u32] / &[RingElement]
+// signatures): in_bytes of `in` are staged through a pinned buffer into
+// dev_bytes of pool scratch on the null stream of `device`, run(dev, stream)
+// is enqueued, out_bytes from dev + out_off are read back into `out`, with one
+// synchronisation.  Nothing is allocated once the pool and the pinned reuse
+// list hold buffers of this size.
+lzx_status host_slice_call(int device, const void *in, size_t in_bytes, void *out, size_t out_off, size_t out_bytes,
+                           size_t dev_bytes, const std::function<lzx_status(void *, hipStream_t)> &run);
 
 } // namespace lzx
 

@@ -214,7 +214,31 @@ def vectors():
                     for x in range(1 << num_vars)) % P
         out["sumcheck"].append({"num_vars": num_vars, "factors": factors, "terms": terms, "challenges": chal,
                                 "claim": claim, "rounds": rounds, "factor_claims": claims})
+    out["c1"] = c1_vector()
     return out
+
+
+def c1_vector():
+    """The plain Ajtai workload: Q_32, d = 64, k = 8, m = 1024, one witness.
+    A[idx] = uniform_from_seed("ajtai-A", [0x42; 32], idx) by the counter
+    rule; witness coefficient c (row-major [j][c]) = (x_c >> 60) - 8 for the
+    splitmix64(0xC1) outputs x_0, x_1, ...; t_i = sum_j A_ij * s_j in
+    Z_q[X]/(X^64 + 1) by exact negacyclic schoolbook in int64 (|A s| terms
+    < 2^32 * 8, at most 2^16 of them per output: < 2^51)."""
+    import numpy as np
+    q, log_n, k, m = 3221225473, 6, 8, 1024
+    n = 1 << log_n
+    A = np.array([uniform_counter(q, log_n, b"ajtai-A", bytes([0x42]) * 32, idx) for idx in range(k * m)],
+                 dtype=np.int64).reshape(k, m, n)
+    rs = sm64(0xC1)
+    s = np.array([(next(rs) >> 60) - 8 for _ in range(m * n)], dtype=np.int64).reshape(m, n)
+    # negacyclic Toeplitz of s_j: T[j][r][c] = s_j[r - c] (c <= r), -s_j[r - c + n] (c > r)
+    r_idx, c_idx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    T = np.where(c_idx <= r_idx, s[:, (r_idx - c_idx) % n], -s[:, (r_idx - c_idx) % n])
+    t = np.einsum("jrc,ijc->ir", T, A)
+    assert np.abs(t).max() < 2**62
+    return {"q": q, "log_n": log_n, "k": k, "m": m, "key_seed_byte": 0x42, "witness_seed": 0xC1,
+            "witness": s.reshape(-1).tolist(), "commitment": np.mod(t, q).tolist()}
 
 
 def main():

@@ -219,6 +219,94 @@ def test_full_workload_int8_equals_ntt_path(mods):
     assert np.array_equal(got, ((a[1].astype(np.uint64) + a[2]) % q).astype(np.uint32))
 
 
+def test_c1_plain_commit_every_entry_point(mods):
+    """The plain Ajtai workload exactly as stated: Q_32, d = 64, k = 8,
+    m = 1024, A from seed [0x42; 32], one witness from splitmix64(0xC1) in
+    [-8, 8).  The reference-signature host commit, the naive host path, the
+    residue batch, the digit batch on every path and the allocation-free
+    entry point all equal the first-principles schoolbook vector."""
+    CM, D, R = mods
+    from lzx_amd import workloads as Wk
+    from lzx_amd._lib import lib
+    v = VEC["c1"]
+    q, k, m = v["q"], v["k"], v["m"]
+    want = np.array(v["commitment"], dtype=np.uint32)
+    pk = CM.AjtaiPublicKey.from_seed(_params(mods, 6, k, m, 8), bytes([v["key_seed_byte"]]) * 32)
+    digits = Wk.small_digits(v["witness_seed"], m * 64, zero_half=False).reshape(1, m, 64)
+    s = Wk.digits_to_residues(digits[0], q)
+    t = pk.commit(s)
+    assert np.array_equal(t.rows_array, want)
+    assert np.array_equal(pk.commit_naive_reference(s).rows_array, want)
+    pk.verify_opening(t, s)
+    out = D.empty(k * 64, np.uint32)
+    pk.commit_batch(D.to_device(s), 1, out)
+    assert np.array_equal(D.to_host(out, np.uint32).reshape(k, 64), want)
+    dd = D.to_device(digits)
+    for path in (CM.COMMIT_NTT, CM.COMMIT_INT8, CM.COMMIT_AUTO):
+        out = D.empty(k * 64, np.uint32)
+        pk.commit_digits_batch(dd, 1, out, path=path)
+        assert np.array_equal(D.to_host(out, np.uint32).reshape(k, 64), want), f"path {path}"
+    out = D.empty(k * 64, np.uint32)
+    assert lib().lzx_ajtai_commit_digits_int8_noalloc(pk.handle, D.ptr(dd), m, 1, D.ptr(out), D.stream_handle()) == 0
+    assert np.array_equal(D.to_host(out, np.uint32).reshape(k, 64), want)
+
+
+def test_commit_naive_equals_commit(mods):
+    """commit_naive_reference == commit (ajtai.rs:490-506's differential test)
+    on the reference's four shapes plus batched witnesses, through the device
+    naive path (coefficient-domain A, one ring product per (i, j))."""
+    CM, D, R = mods
+    q = O.Q_32[0]
+    for v in VEC["ajtai"]:
+        pk = CM.AjtaiPublicKey.from_seed(_params(mods, v["log_n"], v["k"], v["m"], v["norm_bound"]),
+                                         bytes([v["key_seed_byte"]]) * 32)
+        s = O.sample_small_secret(v["q"], v["log_n"], v["m"], v["density"], v["secret_seed"].encode())
+        assert pk.commit_naive_reference(s).rows_array.tolist() == v["commitment"]
+    for log_n, k, m, W in [(6, 4, 37, 3), (5, 3, 200, 2), (8, 2, 20, 2), (2, 5, 9, 4)]:
+        n = 1 << log_n
+        pk = CM.AjtaiPublicKey.from_seed(_params(mods, log_n, k, m, 8), bytes([log_n * 3 + k]) * 32)
+        rng = np.random.default_rng(m)
+        s = rng.integers(0, q, size=(W, m, n), dtype=np.uint64).astype(np.uint32)
+        s[:, 1::4] = 0
+        ds = D.to_device(s)
+        a = D.empty(W * k * n, np.uint32)
+        b = D.empty(W * k * n, np.uint32)
+        pk.commit_batch(ds, W, a)
+        pk.commit_naive_batch(ds, W, b)
+        assert np.array_equal(D.to_host(a, np.uint32), D.to_host(b, np.uint32)), (log_n, k, m, W)
+
+
+def test_sample_small_secret_matches_oracle(mods):
+    """sample_small_secret (ajtai.rs:307-348) on the device: the golden
+    vectors (hashlib), the oracle over several spans, seed lengths and ring
+    degrees, int8 digits beside residues, and the argument checks."""
+    CM, D, R = mods
+    from lzx_amd._lib import LzxError
+    for v in VEC["small_secret"]:
+        ring = R.RingConfig(R.Modulus32(v["q"], 30, 5), v["log_n"])
+        got = CM.sample_small_secret(ring, v["m"], v["bound"], v["seed"].encode())
+        assert D.to_host(got, np.uint32).tolist() == v["coeffs"]
+    cases = [(O.Q_32, 6, 300, 8, bytes(range(32))), (O.Q_32, 6, 5, 127, b""), (O.Q_32, 3, 50, 1, b"x" * 95),
+             (O.Q_12289, 5, 77, 3000, b"secret"), (O.Q_2013265921, 8, 9, (1 << 31) - 1, b"wide"),
+             (O.Q_32, 0, 10, 2, b"n1"), (O.Q_32, 10, 4, 100, b"big-n")]
+    for mod, log_n, m, bound, seed in cases:
+        ring = R.RingConfig(R.Modulus32(*mod), log_n)
+        want = O.sample_small_secret(mod[0], log_n, m, bound, seed)
+        got = CM.sample_small_secret(ring, m, bound, seed)
+        assert np.array_equal(D.to_host(got, np.uint32), want.reshape(m, -1)), (log_n, m, bound)
+        if bound <= 127:
+            res, dig = CM.sample_small_secret(ring, m, bound, seed, out=D.empty((m, 1 << log_n), np.uint32),
+                                              digits=D.empty((m, 1 << log_n), np.int8))
+            d = D.to_host(dig, np.int8).astype(np.int64)
+            assert np.array_equal(np.mod(d, mod[0]).astype(np.uint32), want.reshape(m, -1))
+            assert np.array_equal(D.to_host(res, np.uint32), want.reshape(m, -1))
+    ring = R.RingConfig(R.Modulus32.Q_32, 4)
+    for bound, seed, digits, code in [(1 << 31, b"s", False, 30), (200, b"s", True, 30), (3, b"s" * 96, False, 31)]:
+        with pytest.raises(LzxError) as e:
+            CM.sample_small_secret(ring, 2, bound, seed, digits=D.empty((2, 16), np.int8) if digits else None)
+        assert e.value.code == code
+
+
 def test_dimension_mismatch_batch(mods):
     CM, D, R = mods
     from lzx_amd._lib import LzxError

@@ -163,6 +163,25 @@ def test_ajtai_cached_and_naive_match_schoolbook_vectors():
         assert st == 0 and naive.tolist() == v["commitment"]
 
 
+def test_c1_plain_commit_matches_schoolbook_vector():
+    """The plain Ajtai workload (Q_32, d = 64, k = 8, m = 1024, key seed
+    [0x42; 32], witness from splitmix64(0xC1)): the oracle's cached-NTT commit
+    (ajtai.rs:118-153) and its naive path (:158-180) equal the first-principles
+    schoolbook vector, and the workload generator reproduces its witness."""
+    from lzx_amd import workloads as Wk
+    v = VEC["c1"]
+    q, log_n, k, m = v["q"], v["log_n"], v["k"], v["m"]
+    digits = Wk.small_digits(v["witness_seed"], m * 64, zero_half=False)
+    assert digits.tolist() == v["witness"]
+    A, Ah = O.ajtai_from_seed(O.Q_32, log_n, k, m, bytes([v["key_seed_byte"]]) * 32)
+    s = Wk.digits_to_residues(digits, q).reshape(m, 64)
+    t = O.Ntt(*O.Q_32, log_n)
+    st, fast = O.ajtai_commit(t, k, m, Ah, s)
+    assert st == 0 and fast.tolist() == v["commitment"]
+    st, naive = O.ajtai_commit_naive(t, k, m, A, s)
+    assert st == 0 and naive.tolist() == v["commitment"]
+
+
 def test_ajtai_linearity_and_zero_skip():
     """ajtai.rs:467-487 (linearity) and :547-570 (zero elements)."""
     q = O.Q_32[0]
