@@ -1195,6 +1195,75 @@ size_t or_commit_wide_check_at(uint64_t q, uint32_t n, size_t k, size_t m, size_
     return bad;
 }
 
+/* Checker of many commitments against one key, by evaluation at a root z of
+ * X^n + 1 (the map a(X) -> a(z) is a ring homomorphism Z_q[X]/(X^n+1) -> Z_q,
+ * so t_i = sum_j A_ij s_j implies t_i(z) = sum_j A_ij(z) s_j(z)).  The key's
+ * evaluations are computed once; each witness then costs m*n + k*m steps.
+ * Multiplication by the fixed z uses Shoup's precomputed quotient
+ * floor(z * 2^64 / q) (q < 2^62). */
+static inline uint64_t or_shoup_mul(uint64_t v, uint64_t z, uint64_t zs, uint64_t q)
+{
+    uint64_t hi = (uint64_t)(((u128)v * zs) >> 64);
+    uint64_t r = v * z - hi * q;
+    return r >= q ? r - q : r;
+}
+
+/* Az[i * m + j] = A_ij(z) mod q for i < k, j < m of a [k][lda][n] key. */
+int or_eval_key_at(uint64_t q, uint32_t n, size_t k, size_t m, size_t lda, const void *A, int elem_bytes, uint64_t z,
+                   uint64_t *Az)
+{
+    if (q < 2 || (q >> 62) || z >= q || lda < m || (elem_bytes != 4 && elem_bytes != 8)) return OR_BAD_ARGUMENT;
+    const uint64_t zs = (uint64_t)(((u128)z << 64) / q);
+    for (size_t i = 0; i < k; i++)
+        for (size_t j = 0; j < m; j++) {
+            const size_t off = (i * lda + j) * n;
+            uint64_t v = 0;
+            for (uint32_t c = n; c-- > 0;) {
+                uint64_t a = or_elem(A, elem_bytes, off + c);
+                if (a >= q) a %= q;
+                v = or_shoup_mul(v, z, zs, q) + a;
+                if (v >= q) v -= q;
+            }
+            Az[i * m + j] = v;
+        }
+    return OR_OK;
+}
+
+/* Number of rows i with t_i(z) != sum_{j<m} Az[i * ldz + j] * s_j(z), for one
+ * witness of int8 digits [m][n] and its commitment t [k][n] (u32 or u64). */
+size_t or_commit_check_at_evals(uint64_t q, uint32_t n, size_t k, size_t m, size_t ldz, const uint64_t *Az,
+                                const int8_t *digits, const void *t, int t_bytes, uint64_t z)
+{
+    const uint64_t zs = (uint64_t)(((u128)z << 64) / q);
+    uint64_t *sz = (uint64_t *)malloc(sizeof(uint64_t) * (m ? m : 1));
+    for (size_t j = 0; j < m; j++) {
+        uint64_t v = 0;
+        for (uint32_t c = n; c-- > 0;) {
+            const int64_t d = digits[j * n + c];
+            const uint64_t dr = d < 0 ? q - (uint64_t)(-d) % q : (uint64_t)d % q;
+            v = or_shoup_mul(v, z, zs, q) + dr;
+            if (v >= q) v -= q;
+        }
+        sz[j] = v;
+    }
+    size_t bad = 0;
+    for (size_t i = 0; i < k; i++) {
+        uint64_t lhs = 0;
+        for (uint32_t c = n; c-- > 0;) {
+            lhs = or_shoup_mul(lhs, z, zs, q) + or_elem(t, t_bytes, i * n + c) % q;
+            if (lhs >= q) lhs -= q;
+        }
+        u128 acc = 0;
+        for (size_t j = 0; j < m; j++) {
+            acc += (u128)Az[i * ldz + j] * sz[j]; /* < 2^124 per term */
+            if ((j & 7) == 7) acc %= q;
+        }
+        bad += (uint64_t)(acc % q) != lhs;
+    }
+    free(sz);
+    return bad;
+}
+
 /* ---- the ~2^50 quadratic-slot ring (crates/lattice-ring/src/modulus50.rs) ----
  * Status codes follow Mod50Error (:417-422): 40 LogNOutOfRange, 41
  * AdicityTooSmall, 42 GeneratorInvalid, 43 LengthMismatch. */

@@ -114,6 +114,8 @@ def lib():
         "or_powmod64": (u64, [u64, u64, u64]),
         "or_commit_wide_schoolbook": (C.c_int, [u64, u32, sz, sz, sz, vp, C.c_int, vp, _u64p]),
         "or_commit_wide_check_at": (sz, [u64, u32, sz, sz, sz, vp, C.c_int, vp, vp, C.c_int, u64]),
+        "or_eval_key_at": (C.c_int, [u64, u32, sz, sz, sz, vp, C.c_int, u64, vp]),
+        "or_commit_check_at_evals": (sz, [u64, u32, sz, sz, sz, vp, vp, vp, C.c_int, u64]),
         "or_quad_tables": (C.c_int, [u64, u32, u64, u32, C.POINTER(u64), vp, vp, vp]),
         "or_quad_forward": (C.c_int, [u64, u32, u64, u32, _u64p]),
         "or_quad_inverse": (C.c_int, [u64, u32, u64, u32, _u64p]),
@@ -374,6 +376,30 @@ def commit_wide_check_at(q, A, digits, t, z, m=None):
     assert t.shape == (k, n) and t.dtype in (np.uint32, np.uint64)
     return lib().or_commit_wide_check_at(q, n, k, m, lda, A.ctypes.data, A.itemsize, digits.ctypes.data,
                                          t.ctypes.data, t.itemsize, z)
+
+
+def eval_key_at(q, A, z, m=None):
+    """A_ij(z) mod q for a (k, lda, n) key (u32/u64), j < m: (k, m) uint64."""
+    A = np.ascontiguousarray(A)
+    assert A.dtype in (np.uint32, np.uint64) and A.ndim == 3
+    k, lda, n = A.shape
+    m = lda if m is None else m
+    out = np.zeros((k, m), dtype=np.uint64)
+    st = lib().or_eval_key_at(q, n, k, m, lda, A.ctypes.data, A.itemsize, z, out.ctypes.data)
+    assert st == 0, st
+    return out
+
+
+def commit_check_at_evals(q, Az, digits, t, z, m=None):
+    """Rows of t (k, n) with t_i(z) != sum_{j<m} Az[i, j] s_j(z), s = int8 digits (>= m, n)."""
+    Az = np.ascontiguousarray(Az, dtype=np.uint64)
+    digits = np.ascontiguousarray(digits, dtype=np.int8)
+    t = np.ascontiguousarray(t)
+    k, n = t.shape
+    m = digits.shape[0] if m is None else m
+    assert digits.shape[1] == n and Az.shape[0] == k and Az.shape[1] >= m and t.dtype in (np.uint32, np.uint64)
+    return lib().or_commit_check_at_evals(q, n, k, m, Az.shape[1], Az.ctypes.data, digits.ctypes.data,
+                                          t.ctypes.data, t.itemsize, z)
 
 
 # ---------------------------------------------------------------------------

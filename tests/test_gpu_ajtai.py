@@ -186,8 +186,10 @@ def test_int8_digit_minus128_exact(mods):
 
 def test_full_workload_int8_equals_ntt_path(mods):
     """d=64, k=16, m=2^16, W=64, A from seed [0x43;32], digits in [-8,8) with
-    P(0)=17/32: the two device paths agree bit for bit; witness 0 is checked
-    against the oracle; linearity holds across witnesses."""
+    P(0)=17/32: the two device paths agree bit for bit; witness 0 equals the
+    oracle's cached-NTT commit exactly, and all 64 witnesses pass the
+    oracle's evaluation check against the key; linearity holds across
+    witnesses."""
     CM, D, R = mods
     from lzx_amd import workloads as Wk
     q = O.Q_32[0]
@@ -211,6 +213,12 @@ def test_full_workload_int8_equals_ntt_path(mods):
     s0 = np.mod(digits[0].astype(np.int64), q).astype(np.uint32)
     _, want0 = O.ajtai_commit(ot, k, m, Ah, s0)
     assert np.array_equal(a[0], want0)
+    # every witness against the key by evaluation at a root z of X^64 + 1
+    psi = O.powmod64(O.Q_32[2], (q - 1) // 128, q)
+    z = O.powmod64(psi, 77, q)
+    Az = O.eval_key_at(q, A.reshape(k, m, 64), z)
+    bad = [w for w in range(W) if O.commit_check_at_evals(q, Az, digits[w], a[w], z) != 0]
+    assert not bad, bad
     # linearity: commit(s1 + s2) == commit(s1) + commit(s2) (digits stay small)
     s12 = (digits[1].astype(np.int16) + digits[2]).astype(np.int8)[None]
     o12 = D.empty(k * 64, np.uint32)

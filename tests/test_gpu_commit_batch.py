@@ -205,31 +205,40 @@ def test_error_paths(env):
     assert e.value.code == 30
 
 
-def test_full_uneven_batch_sampled(env):
-    """The 512-job batch at full size; sampled jobs checked by evaluation at two
-    roots of X^n + 1 (and the smallest exactly)."""
+def test_full_uneven_batch_all_jobs(env):
+    """The 512-job batch at full size.  Every job is checked by evaluation at
+    a root z of X^n + 1 against its class key's evaluations A_ij(z), computed
+    once per class; sampled jobs of every modulus at a second root; the
+    smallest job of each class exactly by schoolbook."""
     CB, D, W = env
     wl = CB.uneven_batch_workload()
     b = wl["batch"]
     b.run()
     b.check()
     spec = wl["spec"]
-    picks = set()
-    for kind in range(3):
-        idx = [i for i, (m, ln, (q, g)) in enumerate(spec) if CB.class_of(q, ln) // 3 == kind]
-        idx.sort(key=lambda i: spec[i][0] << spec[i][1])
-        picks.update(idx[:2])
-        picks.add(idx[len(idx) // 3])
-    smallest = min(picks, key=lambda i: spec[i][0] << spec[i][1])
-    for i in sorted(picks):
-        m, log_n, (q, g) = spec[i]
+    by_class = {}
+    for i, (m, log_n, (q, g)) in enumerate(spec):
+        by_class.setdefault(CB.class_of(q, log_n), []).append(i)
+    checked = 0
+    for c, idx in sorted(by_class.items()):
+        m0, log_n, (q, g) = spec[idx[0]]
         n = 1 << log_n
-        key, d, _, out = wl["jobs"][i]
-        A = wl["A"][CB.class_of(q, log_n)][:, :m, :].contiguous()
-        A = D.to_host(A, key.dtype)
-        dig = d.cpu().numpy()
-        t = D.to_host(out, key.dtype).reshape(16, n)
-        for z in _roots(q, g, n):
-            assert O.commit_wide_check_at(q, A, dig, t, z) == 0, (i, m, n, q)
-        if i == smallest:
-            assert np.array_equal(t.astype(np.uint64), O.commit_wide_schoolbook(q, A, dig)), i
+        key = wl["jobs"][idx[0]][0]
+        A = D.to_host(wl["A"][c], key.dtype)  # (16, class m, n)
+        z1, z2 = _roots(q, g, n)
+        Az1 = O.eval_key_at(q, A, z1)
+        idx.sort(key=lambda i: spec[i][0])
+        for pos, i in enumerate(idx):
+            m = spec[i][0]
+            _, d, _, out = wl["jobs"][i]
+            dig = d.cpu().numpy()
+            t = D.to_host(out, key.dtype).reshape(16, n)
+            assert O.commit_check_at_evals(q, Az1, dig, t, z1, m=m) == 0, (i, m, n, q)
+            if pos in (0, len(idx) // 2, len(idx) - 1):
+                assert O.commit_wide_check_at(q, A, dig, t, z2, m=m) == 0, (i, m, n, q)
+            if pos == 0:
+                want = O.commit_wide_schoolbook(q, np.ascontiguousarray(A[:, :m]), dig)
+                assert np.array_equal(t.astype(np.uint64), want), i
+            checked += 1
+        del A
+    assert checked == 512

@@ -416,6 +416,35 @@ def test_wide_evaluation_check_accepts_and_rejects():
     assert t32.max() < 2013265921
 
 
+def test_key_evaluation_checker():
+    """eval_key_at / commit_check_at_evals (the full-size checkers): the key
+    evaluations equal Python big-integer Horner values; every row of a
+    correct commitment passes, at Q_32 (u32 key) and the 50-bit prime (u64
+    key), with a prefix m of a wider key; a single planted coefficient error
+    in one row, or a digit flipped in the witness, is caught."""
+    rng = np.random.default_rng(17)
+    for q, g, n, k, lda, m in [(3221225473, 5, 64, 3, 40, 33), (Q_50, G_50, 128, 2, 20, 20),
+                               (2013265921, 31, 256, 4, 9, 5)]:
+        dt = np.uint32 if q < 2**32 else np.uint64
+        A = rng.integers(0, q, size=(k, lda, n), dtype=np.uint64).astype(dt)
+        d = rng.integers(-128, 128, size=(m, n)).astype(np.int8)
+        d[3] = 0
+        t = O.commit_wide_schoolbook(q, A, d, m=m)
+        psi = pow(g, (q - 1) // (2 * n), q)
+        for z in (psi, pow(psi, 2 * n - 1, q)):
+            Az = O.eval_key_at(q, A, z, m=m)
+            for i, j in [(0, 0), (k - 1, m - 1), (1, 7 % m)]:
+                assert int(Az[i, j]) == sum(int(A[i, j, c]) * pow(z, c, q) for c in range(n)) % q
+            assert O.commit_check_at_evals(q, Az, d, t, z) == 0
+            assert O.commit_check_at_evals(q, Az, d, t.astype(np.uint64), z) == 0
+            bad = t.copy()
+            bad[1, n - 1] = (int(bad[1, n - 1]) + q - 1) % q
+            assert O.commit_check_at_evals(q, Az, d, bad, z) == 1
+            d2 = d.copy()
+            d2[m - 1, 0] ^= 1
+            assert O.commit_check_at_evals(q, Az, d2, t, z) == k
+
+
 def test_uneven_batch_job_rule():
     from lzx_amd import workloads as W
     jobs = W.uneven_batch_jobs()
