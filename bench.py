@@ -66,18 +66,23 @@ def bench_ntt(steps: int, warmup: int):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     total_ms = start.elapsed_time(end)
-    # per-launch durations for the roofline: the same launches, each between its own events
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
-           torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-    for e0, e1, e2 in ev:
+    # per-launch durations for the roofline: K forward launches back to back
+    # between one event pair, then K inverse launches likewise (each launch is
+    # a full transform of the buffer, so K inverses restore the input)
+    fwd_ms, inv_ms = [], []
+    for _ in range(3):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         e0.record(stream)
-        R.forward_multi(tabs, data, POLYS, stream)
+        for _ in range(steps):
+            R.forward_multi(tabs, data, POLYS, stream)
         e1.record(stream)
-        R.inverse_multi(tabs, data, POLYS, stream)
+        for _ in range(steps):
+            R.inverse_multi(tabs, data, POLYS, stream)
         e2.record(stream)
-    torch.cuda.synchronize()
-    fwd_ms = statistics.mean(a.elapsed_time(b) for a, b, _ in ev)
-    inv_ms = statistics.mean(b.elapsed_time(c) for _, b, c in ev)
+        torch.cuda.synchronize()
+        fwd_ms.append(e0.elapsed_time(e1) / steps)
+        inv_ms.append(e1.elapsed_time(e2) / steps)
+    fwd_ms, inv_ms = statistics.median(fwd_ms), statistics.median(inv_ms)
     # round trip must restore the input exactly after every step
     ok = bool(np.array_equal(D.to_host(data, np.uint32).reshape(host.shape), host))
     bytes_pass = 2 * len(moduli) * POLYS * N * 4
@@ -94,32 +99,69 @@ def bench_ntt(steps: int, warmup: int):
     }
 
 
+def host_cpu():
+    """Model name, logical CPUs, and the OpenMP thread count the all-cores
+    columns use (OMP_NUM_THREADS when set: the box's CPU share)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+    return {"model": model, "nproc": nproc, "threads": max(1, min(threads, nproc))}
+
+
+def _median_s(fn, reps, warmup=1):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
 def cpu_baseline_ntt(polys: int = POLYS, reps: int = 5):
-    """The oracle's scalar forward+inverse (restating ntt.rs:118-216) on one
-    core over the same 4 x polys x 64 input; median of reps (harness.rs:24-41)."""
+    """The oracle's scalar forward+inverse (restating ntt.rs:118-216, which has
+    no SIMD) over the same 4 x polys x 64 input: one core, median of reps after
+    a warmup (harness.rs:24-41), plus a labelled all-cores column (OpenMP over
+    polynomials)."""
     from lzx_amd import ring as R
     from lzx_amd import workloads as W
     from oracle import oracle as O
 
+    cpu = host_cpu()
     host = W.ntt_batch_input(R.CRT_PRIMES, polys, N)
     ots = [O.Ntt(m.q, m.two_adicity, m.generator, LOG_N) for m in R.CRT_PRIMES]
     L = O.lib()
-    times = []
-    for r in range(reps + 1):
-        buf = host.copy()
-        t0 = time.perf_counter()
+    buf = host.copy()
+
+    def one_core():
         for i, ot in enumerate(ots):
-            row = np.ascontiguousarray(buf[i].reshape(-1))
+            row = buf[i].reshape(-1)
             L.or_ntt_forward_batch(ot.h, row, polys)
             L.or_ntt_inverse_batch(ot.h, row, polys)
-        dt = time.perf_counter() - t0
-        if r:
-            times.append(dt)
-    med = statistics.median(times)
+
+    def all_cores():
+        for i, ot in enumerate(ots):
+            O.ntt_roundtrip_par(ot, buf[i].reshape(-1), polys, cpu["threads"])
+
+    med = _median_s(one_core, reps)
+    med_all = _median_s(all_cores, reps)
+    ok = bool(np.array_equal(buf, host))
     bytes_step = 2 * 2 * len(R.CRT_PRIMES) * polys * N * 4
     return {"value": bytes_step / med / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"oracle scalar forward+inverse, 4 primes x {polys} polys x d=64, median of {reps} reps "
-                      f"({med:.3f} s each)"}
+            "sample": f"the full workload: oracle scalar forward+inverse, 4 primes x {polys} polys x d=64, "
+                      f"median of {reps} reps ({med:.3f} s each), -O3 -march=x86-64-v2",
+            "cpu": cpu["model"], "nproc": cpu["nproc"], "roundtrip_exact": ok,
+            "all_cores": {"value": bytes_step / med_all / 1e9, "cores": cpu["threads"],
+                          "how": "OpenMP over polynomials (the reference is single-threaded)"}}
 
 
 def _median_gpu_ms(torch, fn, reps, warmup=1):
@@ -446,30 +488,127 @@ def bench_uneven_batch(reps: int = 5, autotune: bool = True):
     return res
 
 
-def cpu_baseline_secondary():
-    """Bounded one-core oracle samples for the secondary workloads."""
+def bench_c1(reps: int = 50):
+    """The plain Ajtai workload: Q_32, d = 64, k = 8, m = 1024, A from seed
+    [0x42; 32], one witness from splitmix64(0xC1) in [-8, 8) (8,192
+    ring-MACs).  Times the drop-in host call (lzx_ajtai_commit_host: the
+    witness crosses PCIe in and the commitment back, one synchronisation) and
+    the device-resident digit commit on each path."""
+    import torch
+
+    from lzx_amd import commitment as CM
+    from lzx_amd import device as D
+    from lzx_amd import ring as R
+    from lzx_amd import workloads as W
+
+    k, m = 8, 1024
+    pk = CM.AjtaiPublicKey.from_seed(CM.AjtaiParams(R.RingConfig(R.Modulus32.Q_32, 6), k, m, 8), bytes([0x42]) * 32)
+    digits = W.small_digits(0xC1, m * 64, zero_half=False).reshape(1, m, 64)
+    s = W.digits_to_residues(digits[0], R.Modulus32.Q_32.q)
+    t_host = _median_s(lambda: pk.commit(s), reps, warmup=3)
+    want = pk.commit(s).rows_array
+    dd = D.to_device(digits)
+    res = {"workload": "plain Ajtai commit, q=3221225473, d=64, k=8, m=1024, W=1, seed [0x42;32], "
+                       "witness splitmix64(0xC1) in [-8,8)",
+           "ring_macs": k * m,
+           "host_call": {"entry": "lzx_ajtai_commit_host (AjtaiPublicKey::commit signature, PCIe in and out)",
+                         "us": round(t_host * 1e6, 2), "ring_macs_per_s": k * m / t_host}}
+    for name, path in (("ntt_path", CM.COMMIT_NTT), ("int8_path", CM.COMMIT_INT8), ("auto", CM.COMMIT_AUTO)):
+        out = D.empty(k * 64, np.uint32)
+        t = _timed_events(torch, lambda: pk.commit_digits_batch(dd, 1, out, path=path), 20)
+        same = bool(np.array_equal(D.to_host(out, np.uint32).reshape(k, 64), want))
+        res[name] = {"us": round(t * 1e6, 2), "ring_macs_per_s": k * m / t, "equals_host_call": same}
+    return res, (pk, s, want)
+
+
+def cpu_baseline_secondary(c1=None, reps: int = 5):
+    """One-core oracle baselines of the secondary workloads (the reference is
+    single-threaded), each with a labelled all-cores column, the CPU model and
+    thread count recorded:
+      - the plain Ajtai workload at full size (median of reps);
+      - the large Ajtai workload by its exact scaling rule: one witness of the
+        full key (k = 16, m = 2^16, A^ derived on the GPU and checked equal to
+        the oracle's at sampled entries) is timed; a W-witness commit is W
+        such independent commits (ajtai.rs:118-153 has no cross-witness
+        work), so 64 x the sample is the workload's time;
+      - the 2^24 sum-check prove at full size: the reference's AVX-512F
+        kernels (its default where the CPU has avx512f) and its scalar
+        LZX_NO_SIMD=1 mode, one core each, plus all cores."""
+    from lzx_amd import commitment as CM
+    from lzx_amd import device as D
+    from lzx_amd import ring as R
+    from lzx_amd import sumcheck as S
     from lzx_amd import workloads as W
     from oracle import oracle as O
 
-    res = {}
-    k, m = 16, 1 << 12
-    _, Ah = O.ajtai_from_seed(O.Q_32, 6, k, m, bytes([0x43]) * 32, want_A=False)
-    s = np.mod(W.small_digits(0xC3, m * 64).reshape(m, 64).astype(np.int64), O.Q_32[0]).astype(np.uint32)
+    cpu = host_cpu()
+    th = cpu["threads"]
+    res = {"cpu": cpu["model"], "nproc": cpu["nproc"], "all_cores_threads": th,
+           "avx512f": O.cpu_has_avx512f(), "build": "gcc -O3 -march=x86-64-v2 (+ avx512f kernels per function, "
+                                                    "selected at run time by CPUID)"}
     ot = O.Ntt(*O.Q_32, 6)
-    t0 = time.perf_counter()
-    O.ajtai_commit(ot, k, m, Ah, s)
-    dt = time.perf_counter() - t0
-    res["ajtai"] = {"ring_macs_per_s": k * m / dt, "cores": 1, "kind": "port",
-                    "sample": f"oracle cached-NTT commit (ajtai.rs:118-153), k=16, m=2^12, W=1: {dt:.3f} s"}
-    nv = 18
-    facs = [W.uniform_gl(s_, 1 << nv) for s_ in (0xC4E, 0xC4A, 0xC4B, 0xC4C)]
+    # plain Ajtai (C1) at full size
+    if c1 is not None:
+        pk, s, want = c1
+        Ah = pk.matrix_ntt()
+        t1 = _median_s(lambda: O.ajtai_commit(ot, 8, 1024, Ah, s), reps)
+        ta = _median_s(lambda: O.ajtai_commit_par(ot, 8, 1024, Ah, s, th), reps)
+        ok = bool(np.array_equal(O.ajtai_commit(ot, 8, 1024, Ah, s)[1], want))
+        res["c1_plain_ajtai"] = {"ring_macs_per_s": 8192 / t1, "us": round(t1 * 1e6, 1), "cores": 1, "kind": "port",
+                                 "sample": f"full workload, oracle cached-NTT commit (ajtai.rs:118-153), median of {reps}",
+                                 "equals_gpu": ok,
+                                 "all_cores": {"ring_macs_per_s": 8192 / ta, "us": round(ta * 1e6, 1), "cores": th}}
+    # large Ajtai (C3): one witness of the full key, times 64
+    k, m = 16, 1 << 16
+    pk = CM.AjtaiPublicKey.from_seed(CM.AjtaiParams(R.RingConfig(R.Modulus32.Q_32, 6), k, m, 8), bytes([0x43]) * 32)
+    Ah = pk.matrix_ntt()
+    A = pk.matrix()
+    for idx in (0, 777777, k * m - 1):
+        assert np.array_equal(ot.forward(A[idx])[1], Ah[idx]), "GPU-derived key differs from the oracle"
+    del A, pk
+    s0 = W.digits_to_residues(W.small_digits(0xC3, m * 64).reshape(m, 64), O.Q_32[0])
+    t1 = _median_s(lambda: O.ajtai_commit(ot, k, m, Ah, s0), 3, warmup=0)
+    ta = _median_s(lambda: O.ajtai_commit_par(ot, k, m, Ah, s0, th), 3, warmup=0)
+    res["c3_ajtai"] = {"ring_macs_per_s": k * m / t1, "s_per_witness": round(t1, 3),
+                       "s_per_c3_commit": round(64 * t1, 2), "cores": 1, "kind": "port",
+                       "sample": "witness 0 of the full key (k=16, m=2^16), median of 3; C3 = 64 independent witnesses",
+                       "all_cores": {"ring_macs_per_s": k * m / ta, "s_per_c3_commit": round(64 * ta, 2),
+                                     "cores": th}}
+    del Ah
+    # sum-check (C4) at full size
+    nv = 24
+    r = W.uniform_gl(0xC4E, nv)
+    facs = [D.to_host(S.eq_table(r), np.uint64)] + [W.uniform_gl(x, 1 << nv) for x in (0xC4A, 0xC4B, 0xC4C)]
     terms = [(1, [0, 1, 2]), (O.GL_P - 1, [0, 3])]
-    claim = O.vp_sum_over_hypercube(nv, facs, terms)
-    t0 = time.perf_counter()
-    O.sumcheck_prove(nv, facs, terms, claim, O.Transcript(b"lzx-mi355x-sumcheck"))
-    dt = time.perf_counter() - t0
-    res["sumcheck"] = {"ms_per_prove_2^18": round(dt * 1e3, 2), "cores": 1, "kind": "port",
-                       "sample": "oracle scalar sumcheck::prove (sumcheck.rs:62-162), 2^18 tables, same polynomial"}
+    vp = S.VirtualPolynomial(nv)
+    for f in facs:
+        vp.add_factor(D.to_device(f))
+    for c, ids in terms:
+        vp.add_term(c, ids)
+    claim = vp.sum_over_hypercube()
+    gpu = S.prove(vp, claim, S.Transcript(b"lzx-mi355x-sumcheck"))
+    del vp
+    out = {}
+
+    def run(simd, threads):
+        st, o = O.sumcheck_prove(nv, facs, terms, claim, O.Transcript(b"lzx-mi355x-sumcheck"), simd=simd,
+                                 threads=threads)
+        assert st == 0, st
+        out["same"] = bool(np.array_equal(o["rounds"], gpu.rounds) and o["final_claim"] == gpu.final_claim)
+
+    sc = {"workload": "2^24 tables, eq*(a*b-c), degree 3, transcript challenges (full size)", "kind": "port"}
+    if O.cpu_has_avx512f():
+        t = _median_s(lambda: run(1, 1), 3, warmup=0)
+        sc["avx512_1core"] = {"ms_per_prove": round(t * 1e3, 1), "cores": 1, "same_proof_as_gpu": out["same"]}
+        t = _median_s(lambda: run(1, th), 3, warmup=0)
+        sc["avx512_all_cores"] = {"ms_per_prove": round(t * 1e3, 1), "cores": th, "same_proof_as_gpu": out["same"]}
+    else:
+        sc["avx512_1core"] = None
+        sc["avx512_note"] = "this CPU reports no avx512f; the reference would run its scalar path"
+    t = _median_s(lambda: run(0, 1), 1, warmup=0)
+    sc["scalar_1core"] = {"ms_per_prove": round(t * 1e3, 1), "cores": 1, "same_proof_as_gpu": out["same"],
+                          "note": "the reference's LZX_NO_SIMD=1 mode; one run"}
+    res["c4_sumcheck"] = sc
     return res
 
 
@@ -491,6 +630,8 @@ def main():
     ap.add_argument("--only-ntt", action="store_true", help="skip the secondary workloads (profiling runs)")
     ap.add_argument("--only-batch", action="store_true", help="run only the uneven commitment batch leg")
     ap.add_argument("--only-sumcheck", action="store_true", help="run only the 2^24 sum-check prove leg")
+    ap.add_argument("--only-c1", action="store_true", help="run only the plain Ajtai (k=8, m=1024) leg")
+    ap.add_argument("--only-cpu", action="store_true", help="run only the CPU baseline legs")
     ap.add_argument("--only-ring50", action="store_true", help="run only the 50-bit quadratic-slot ring leg")
     ap.add_argument("--only-ntt-q32", action="store_true", help="run only the default-modulus (q > 2^31) NTT leg")
     ap.add_argument("--only-extension", action="store_true", help="run only the R_q[Y]/(Y^2+1) leg")
@@ -519,6 +660,13 @@ def main():
         return
     if args.only_packing:
         print(json.dumps({"packing": bench_packing()}), flush=True)
+        return
+    if args.only_c1:
+        print(json.dumps({"c1_plain_ajtai": bench_c1()[0]}), flush=True)
+        return
+    if args.only_cpu:
+        print(json.dumps({"cpu_baseline": cpu_baseline_ntt(), "secondary": cpu_baseline_secondary(bench_c1()[1])}),
+              flush=True)
         return
     if args.only_sumcheck:
         print(json.dumps({"sumcheck_prove": bench_sumcheck(reps=5)}), flush=True)
@@ -561,12 +709,13 @@ def main():
                                            "how": "device-to-device copy of 128 MiB (read + write)",
                                            "frac": round(r["fwd_gbs"] / copy_gbs, 4)}
     if not args.only_ntt:
-        line["secondary"] = {"ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
+        c1, c1_inputs = bench_c1()
+        line["secondary"] = {"c1_plain_ajtai": c1, "ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
                              "uneven_batch": bench_uneven_batch(), "ring50": bench_ring50(),
                              "extension": bench_extension(), "packing": bench_packing(),
                              "ntt_q32": bench_ntt_q32()}
         if not args.no_cpu:
-            line["secondary"]["cpu_baseline"] = cpu_baseline_secondary()
+            line["secondary"]["cpu_baseline"] = cpu_baseline_secondary(c1_inputs)
     print(json.dumps(line), flush=True)
 
 

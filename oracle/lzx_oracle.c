@@ -26,33 +26,13 @@
 #include <stdlib.h>
 #include <string.h>
 
-typedef unsigned __int128 u128;
+#include "lzx_oracle.h"
 
-enum {
-    OR_OK = 0,
-    OR_NTT_LENGTH_MISMATCH = 1,
-    OR_NTT_ADICITY_TOO_SMALL = 2,
-    OR_AJTAI_DIMENSION_MISMATCH = 4,
-    OR_AJTAI_NORM_EXCEEDED = 5,
-    OR_AJTAI_COMMITMENT_MISMATCH = 6,
-    OR_SUMCHECK_BAD_ROUND_SHAPE = 10,
-    OR_SUMCHECK_ROUND_CHECK_FAILED = 11,
-    OR_SUMCHECK_FINAL_CHECK_FAILED = 12,
-    OR_SUMCHECK_CLAIM_MISMATCH = 13,
-    OR_SUMCHECK_VIRTUAL_POLY = 14,
-    OR_TRANSCRIPT_REJECTION_BUDGET = 20,
-    OR_TRANSCRIPT_MESSAGE_TOO_LARGE = 21,
-    OR_BAD_ARGUMENT = 30,
-};
 
 /* ------------------------------------------------------------------------ */
 /* Modulus32: lattice-ring/src/modulus.rs:22-162                             */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
-    uint32_t q, two_adicity, generator;
-    uint64_t barrett; /* floor((2^64-1)/q), modulus.rs:15-17 */
-} or_mod32;
 
 or_mod32 or_mod32_new(uint32_t q, uint32_t two_adicity, uint32_t generator)
 {
@@ -135,16 +115,6 @@ uint32_t or_root_of_unity(const or_mod32 *m, uint32_t k)
 /* NttTables: lattice-ring/src/ntt.rs:16-342                                 */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
-    or_mod32 mod;
-    uint32_t log_n;
-    uint32_t n;
-    uint32_t psi, n_inv;
-    uint32_t *pow_psi;       /* [n] */
-    uint32_t *pow_psi_inv;   /* [n] */
-    uint32_t *pow_omega;     /* [max(n/2,1)] */
-    uint32_t *pow_omega_inv; /* [max(n/2,1)] */
-} or_ntt;
 
 void or_ntt_free(or_ntt *t)
 {
@@ -443,12 +413,6 @@ void or_keccak_f1600(uint64_t st[25])
     }
 }
 
-typedef struct {
-    uint64_t st[25];
-    size_t pos;
-    size_t rate;
-    uint8_t suffix;
-} or_sponge;
 
 static void sponge_init(or_sponge *s, size_t rate, uint8_t suffix)
 {
@@ -760,11 +724,6 @@ uint64_t or_gl_inverse(uint64_t a) { return a == 0 ? 0 : or_gl_pow(a, GL_P - 2);
 /* Transcript: lattice-core/src/transcript.rs:36-238                        */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
-    or_sponge sponge;
-    uint64_t query_count;
-    uint64_t query_budget;
-} or_transcript;
 
 /* transcript.rs:46-59 */
 or_transcript *or_transcript_new(const uint8_t *protocol, size_t plen, uint64_t budget)

@@ -111,6 +111,11 @@ def lib():
         "or_sumcheck_prove": (C.c_int, [sz, sz, C.POINTER(C.c_void_p), sz, _u64p, _u32p, _u32p, u64, vp,
                                         _u64p, _u64p, C.POINTER(u64), _u64p, C.POINTER(sz)]),
         "or_sumcheck_verify": (C.c_int, [_u64p, sz, sz, sz, sz, u64, vp, C.c_int, u64, _u64p, C.POINTER(u64)]),
+        "or_sc_prove_ref": (C.c_int, [sz, sz, C.POINTER(C.c_void_p), sz, _u64p, _u32p, _u32p, u64, vp,
+                                      _u64p, _u64p, C.POINTER(u64), _u64p, C.POINTER(sz), C.c_int, C.c_int]),
+        "or_cpu_has_avx512f": (C.c_int, []),
+        "or_ntt_roundtrip_par": (None, [vp, _u32p, sz, C.c_int]),
+        "or_ajtai_commit_par": (C.c_int, [vp, sz, sz, _u32p, _u32p, sz, _u32p, C.c_int]),
         "or_powmod64": (u64, [u64, u64, u64]),
         "or_commit_wide_schoolbook": (C.c_int, [u64, u32, sz, sz, sz, vp, C.c_int, vp, _u64p]),
         "or_commit_wide_check_at": (sz, [u64, u32, sz, sz, sz, vp, C.c_int, vp, vp, C.c_int, u64]),
@@ -331,6 +336,24 @@ def ajtai_commit(ntt: Ntt, k, m, Ahat, s):
     return st, out
 
 
+def cpu_has_avx512f() -> bool:
+    return bool(lib().or_cpu_has_avx512f())
+
+
+def ntt_roundtrip_par(ntt: Ntt, data, polys, threads):
+    """Forward then inverse of `polys` polynomials in place over `threads` threads."""
+    lib().or_ntt_roundtrip_par(ntt.h, data, polys, threads)
+
+
+def ajtai_commit_par(ntt: Ntt, k, m, Ahat, s, threads):
+    """ajtai.rs:118-153 over `threads` OpenMP threads; same values as ajtai_commit."""
+    s = np.ascontiguousarray(s, dtype=np.uint32).reshape(-1)
+    out = np.zeros((k, ntt.n), dtype=np.uint32)
+    st = lib().or_ajtai_commit_par(ntt.h, k, m, np.ascontiguousarray(Ahat, dtype=np.uint32).reshape(-1), s,
+                                   s.size // ntt.n, out.reshape(-1), threads)
+    return st, out
+
+
 def ajtai_commit_naive(ntt: Ntt, k, m, A, s):
     """commit_naive_reference (ajtai.rs:158-180)."""
     s = np.ascontiguousarray(s, dtype=np.uint32)
@@ -486,8 +509,11 @@ def vp_sum_over_hypercube(num_vars, factors, terms):
     return lib().or_vp_sum_over_hypercube(num_vars, len(facs), ptrs, len(terms), coeffs, off, ids)
 
 
-def sumcheck_prove(num_vars, factors, terms, claim, transcript: Transcript):
-    """sumcheck::prove (sumcheck.rs:62-162).  Returns (status, dict)."""
+def sumcheck_prove(num_vars, factors, terms, claim, transcript: Transcript, simd=None, threads=1):
+    """sumcheck::prove (sumcheck.rs:62-162).  Returns (status, dict).
+    simd=None runs the plain restatement; simd=0/1 the reference's
+    single-binding round structure in its scalar (LZX_NO_SIMD=1) or AVX-512
+    form (the CPU baseline), over `threads` OpenMP threads."""
     coeffs, off, ids = _terms_csr(terms)
     facs = [np.ascontiguousarray(f, dtype=np.uint64) for f in factors]
     ptrs = (C.c_void_p * max(len(facs), 1))(*[f.ctypes.data for f in facs])
@@ -497,8 +523,12 @@ def sumcheck_prove(num_vars, factors, terms, claim, transcript: Transcript):
     fclaims = np.zeros(max(len(facs), 1), dtype=np.uint64)
     fin = C.c_uint64(0)
     dout = C.c_size_t(0)
-    st = lib().or_sumcheck_prove(num_vars, len(facs), ptrs, len(terms), coeffs, off, ids, claim, transcript.h,
-                                 rounds, chals, C.byref(fin), fclaims, C.byref(dout))
+    if simd is None:
+        st = lib().or_sumcheck_prove(num_vars, len(facs), ptrs, len(terms), coeffs, off, ids, claim, transcript.h,
+                                     rounds, chals, C.byref(fin), fclaims, C.byref(dout))
+    else:
+        st = lib().or_sc_prove_ref(num_vars, len(facs), ptrs, len(terms), coeffs, off, ids, claim, transcript.h,
+                                   rounds, chals, C.byref(fin), fclaims, C.byref(dout), int(simd), int(threads))
     w = dout.value + 1
     return st, {
         "rounds": rounds[: num_vars * w].reshape(num_vars, w),

@@ -161,6 +161,66 @@ def test_eq_table_matches_oracle(sc):
         assert np.array_equal(D.to_host(S.eq_table(pt), np.uint64), O.eq_table(pt))
 
 
+def _device_prove(S, D, nv, protocol, prior, tables, terms, budget=1 << 20):
+    """A device prove (its transcript runs in the close and tail kernels)
+    after `prior` host appends; returns (output, transcript afterwards)."""
+    t = S.Transcript(protocol, budget)
+    for label, msg in prior:
+        t.append_message(label, msg)
+    vp = _vp(S, D, nv, tables, terms)
+    return S.prove(vp, O.vp_sum_over_hypercube(nv, tables, terms), t), t
+
+
+def test_reference_transcript_properties_through_the_device_sponge(sc):
+    """transcript.rs:244-316's five properties, on the sponge the device runs
+    between rounds: 14 variables use both the per-round close kernel (halves
+    above 2048) and the one-block tail, and the device hands its state back to
+    the host transcript, which continues it."""
+    S, D = sc
+    from lzx_amd._lib import LzxError
+    nv = 14
+    rng = np.random.default_rng(244)
+    tables = [rng.integers(0, P, 1 << nv, dtype=np.uint64) for _ in range(2)]
+    terms = [(1, [0, 1]), (9, [1])]
+    base, t_base = _device_prove(S, D, nv, b"test-proto", [(b"a", b"hello")], tables, terms)
+    # deterministic_and_label_sensitive: the same history gives the same proof;
+    # another label or protocol changes every challenge
+    again, _ = _device_prove(S, D, nv, b"test-proto", [(b"a", b"hello")], tables, terms)
+    assert again.rounds.tolist() == base.rounds.tolist() and again.challenges.tolist() == base.challenges.tolist()
+    for proto, prior in [(b"test-proto", [(b"other", b"hello")]), (b"other-proto", [(b"a", b"hello")])]:
+        o, _ = _device_prove(S, D, nv, proto, prior, tables, terms)
+        assert o.rounds[0].tolist() == base.rounds[0].tolist()  # round 0 precedes any challenge
+        assert all(x != y for x, y in zip(o.challenges.tolist(), base.challenges.tolist()))
+    # label_concatenation_unambiguous: ("ab", "c") and ("a", "bc") frame differently
+    ab, _ = _device_prove(S, D, nv, b"amb", [(b"ab", b"c")], tables, terms)
+    a_bc, _ = _device_prove(S, D, nv, b"amb", [(b"a", b"bc")], tables, terms)
+    assert all(x != y for x, y in zip(ab.challenges.tolist(), a_bc.challenges.tolist()))
+    # resumable_squeeze_keeps_isolation: after the device rounds the host keeps
+    # squeezing the same sponge; a fresh replay reproduces the sequence, and
+    # so does the oracle's transcript after its own prove
+    seq = [t_base.challenge_field(lbl) for lbl in (b"r1", b"r2", b"r3")]
+    assert len(set(seq)) == 3
+    _, t_again = _device_prove(S, D, nv, b"test-proto", [(b"a", b"hello")], tables, terms)
+    assert [t_again.challenge_field(lbl) for lbl in (b"r1", b"r2", b"r3")] == seq
+    ot = O.Transcript(b"test-proto")
+    ot.append_message(b"a", b"hello")
+    O.sumcheck_prove(nv, tables, terms, O.vp_sum_over_hypercube(nv, tables, terms), ot)
+    assert [ot.challenge_field(lbl)[1] for lbl in (b"r1", b"r2", b"r3")] == seq
+    # query_budget_enforced: one round's challenge past the budget fails on the device
+    with pytest.raises(LzxError) as e:
+        _device_prove(S, D, nv, b"budget", [], tables, terms, budget=nv - 1)
+    assert e.value.code == 20
+    _, t_full = _device_prove(S, D, nv, b"budget", [], tables, terms, budget=nv)
+    with pytest.raises(LzxError) as e:
+        t_full.challenge_field(b"d")
+    assert e.value.code == 20
+    # many_challenges_distinct_and_canonical
+    ch = base.challenges.tolist() + ab.challenges.tolist()
+    assert len(set(ch)) == len(ch) and max(ch) < P
+    many = S.Transcript(b"many").challenge_fields(b"vec", 1000).tolist()
+    assert len(many) == 1000 and len(set(many)) == 1000 and max(many) < P
+
+
 def test_full_workload_matches_oracle(sc):
     """2^24-entry tables, VP eq(r,x)*(a*b - c), degree 3, transcript
     challenges: every round message, challenge and factor claim bit-exact

@@ -416,6 +416,44 @@ def test_wide_evaluation_check_accepts_and_rejects():
     assert t32.max() < 2013265921
 
 
+def test_cpu_baseline_variants_equal_the_restatement():
+    """The timed CPU baselines compute the same values as the plain
+    restatement: the reference's single-binding round structure with Sum8 in
+    scalar form, in AVX-512F form where the CPU has it, and over 1 or 3
+    threads (sizes that leave 8-lane tails and ranges shorter than a chunk);
+    the parallel NTT round trip and Ajtai commit likewise."""
+    rng = np.random.default_rng(99)
+    for nv, nf, terms in [(1, 2, [(3, [0, 1])]), (3, 3, [(3, [0, 1]), (5, [1, 2, 0]), (11, [2])]),
+                          (9, 4, [(1, [0, 1, 2]), (O.GL_P - 1, [0, 3])]),
+                          (7, 3, [(7, [0, 0, 1]), (O.GL_P - 2, [2, 1]), (1, [2])])]:
+        facs = [rng.integers(0, O.GL_P, 1 << nv, dtype=np.uint64) for _ in range(nf)]
+        edge = np.array([0, 1, O.GL_P - 1, O.GL_P - 2, 0xFFFFFFFF], dtype=np.uint64)
+        facs[0][: min(5, 1 << nv)] = edge[: min(5, 1 << nv)]
+        claim = O.vp_sum_over_hypercube(nv, facs, terms)
+        st, want = O.sumcheck_prove(nv, facs, terms, claim, O.Transcript(b"b"))
+        assert st == 0
+        modes = [(0, 1), (0, 3)] + ([(1, 1), (1, 3)] if O.cpu_has_avx512f() else [])
+        for simd, threads in modes:
+            st, got = O.sumcheck_prove(nv, facs, terms, claim, O.Transcript(b"b"), simd=simd, threads=threads)
+            assert st == 0, (simd, threads)
+            for key in ("rounds", "challenges", "factor_claims"):
+                assert np.array_equal(got[key], want[key]), (nv, simd, threads, key)
+            assert got["final_claim"] == want["final_claim"]
+    if not O.cpu_has_avx512f():
+        st, _ = O.sumcheck_prove(3, facs[:1], [(1, [0])], 0, O.Transcript(b"b"), simd=1)
+        assert st == 30
+    ot = O.Ntt(*O.Q_32, 6)
+    data = rng.integers(0, O.Q_32[0], 37 * 64, dtype=np.uint64).astype(np.uint32)
+    buf = data.copy()
+    O.ntt_roundtrip_par(ot, buf, 37, 3)
+    assert np.array_equal(buf, data)
+    k, m = 3, 21
+    _, Ah = O.ajtai_from_seed(O.Q_32, 6, k, m, bytes([4]) * 32, want_A=False)
+    sres = np.mod(rng.integers(-8, 8, size=(m, 64)), O.Q_32[0]).astype(np.uint32)
+    sres[2] = 0
+    assert np.array_equal(O.ajtai_commit_par(ot, k, m, Ah, sres, 3)[1], O.ajtai_commit(ot, k, m, Ah, sres)[1])
+
+
 def test_key_evaluation_checker():
     """eval_key_at / commit_check_at_evals (the full-size checkers): the key
     evaluations equal Python big-integer Horner values; every row of a
