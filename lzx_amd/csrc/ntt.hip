@@ -158,6 +158,161 @@ __global__ __launch_bounds__(64) void ntt_pipe_kernel(uint32_t *__restrict__ dat
     }
 }
 
+// Persistent register kernel of the u32 classes below 2^31 (the default for
+// n in [4, 64] at those primes): the lazily reduced butterflies of
+// ntt_regs_lazy, Harvey's [0, 4q) range where the plan's flag (q < 2^30)
+// allows it.  A wave runs its tiles in one loop per class and switches loops
+// where its tile range crosses into a plan of the other class, so a CRT batch
+// whose primes straddle 2^30 is still one launch (each loop keeps its own
+// registers: one loop holding both bodies spilled).  The tile goes through
+// LDS in padded rows of n + 4 words: polynomial p's chunk c sits at word
+// (n + 4) p + 4 c.  The coalesced staging writes (16 lanes per polynomial row)
+// and the per-lane row reads both touch distinct banks in every 16-lane group,
+// and every global and LDS address is a lane constant plus an immediate, so a
+// full tile spends no VALU on addressing.  The next tile's loads are in flight
+// in VGPRs while the current one is transformed.  The last tile of a plan may
+// be partial: its loads clamp to polynomial 0 and its stores are masked.
+typedef __attribute__((address_space(1))) u32x4 g_u32x4;
+typedef __attribute__((address_space(1))) uint8_t g_u8;
+
+template <uint32_t N>
+struct LazyTile {
+    static constexpr uint32_t C = N / 4, ROW = N + 4, PPI = 64 / C, GROUPS = (C + 3) / 4;
+    uint32_t *data;
+    const TabSet *tabs;
+    uint32_t polys, blocks_per_plan;
+    uint32_t lane, stage, row;
+    uint32_t voff[GROUPS]; // lane byte offsets of each group of four 1 KiB loads
+
+    // a tile's base, its polynomial count and plan (wave-uniform: readfirstlane
+    // keeps them in SGPRs, since the division runs on the VALU)
+    __device__ __forceinline__ g_u8 *locate(uint32_t bb, uint32_t &np, uint32_t &plan) const
+    {
+        plan = (uint32_t)__builtin_amdgcn_readfirstlane(bb / blocks_per_plan);
+        const uint32_t p0 = (bb - plan * blocks_per_plan) * 64;
+        np = min(64u, polys - p0);
+        const uint64_t g = (uint64_t)(data + ((size_t)plan * polys + p0) * N);
+        // (readfirstlane returns a signed int: zero-extend it, never sign-extend)
+        const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)g);
+        const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(g >> 32));
+        return (g_u8 *)((hi << 32) | lo);
+    }
+    __device__ __forceinline__ g_u32x4 *full(g_u8 *g, uint32_t i) const
+    {
+        return (g_u32x4 *)(g + voff[i / 4] + 1024 * (i % 4));
+    }
+    __device__ __forceinline__ g_u32x4 *part(g_u8 *g, uint32_t i, uint32_t p) const
+    {
+        return (g_u32x4 *)(g + 16 * (C * p + lane % C));
+    }
+    __device__ __forceinline__ void fetch(g_u8 *g, uint32_t np, u32x4 (&r)[C]) const
+    {
+        if (np == 64) {
+#pragma unroll
+            for (uint32_t i = 0; i < C; i++) r[i] = *full(g, i);
+        } else {
+#pragma unroll
+            for (uint32_t i = 0; i < C; i++) {
+                const uint32_t p = PPI * i + lane / C;
+                r[i] = *part(g, i, p < np ? p : 0);
+            }
+        }
+    }
+};
+
+constexpr int WAIT_VM0 = 0x0F70; // s_waitcnt vmcnt(0), lgkmcnt and expcnt free
+
+// Tiles [b, b_end) of the wave while their plan's class is HV; nx holds tile
+// b's chunks on entry (and the next tile's on return).
+template <int LOG_N, int MODE, bool HV>
+__device__ __forceinline__ void lazy_run(const LazyTile<1u << LOG_N> &L, uint32_t *__restrict__ T, uint32_t &b,
+                                         uint32_t b_end, u32x4 (&nx)[(1u << LOG_N) / 4])
+{
+    constexpr uint32_t N = 1u << LOG_N, C = N / 4, ROW = N + 4, PPI = 64 / C;
+    for (; b < b_end; b++) {
+        uint32_t np, plan;
+        g_u8 *g = L.locate(b, np, plan);
+        const uint32_t *__restrict__ tab = L.tabs->p[plan];
+        if ((((const __attribute__((address_space(4))) uint32_t *)tab)[12] != 0) != HV) return;
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + L.stage + ROW * PPI * i) = nx[i];
+        wave_sync();
+        uint32_t a[N];
+#pragma unroll
+        for (uint32_t c4 = 0; c4 < C; c4++) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(T + L.row + 4 * c4);
+            const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (uint32_t e = 0; e < 4; e++) a[MODE == NTT_FWD ? 4 * c4 + e : brv_bits(4 * c4 + e, LOG_N)] = vv[e];
+        }
+        if (b + 1 < b_end) {
+            uint32_t npn, pn;
+            g_u8 *gn = L.locate(b + 1, npn, pn);
+            L.fetch(gn, npn, nx);
+        }
+        ntt_regs_lazy<LOG_N, HV, MODE>(a, tab);
+        wave_sync(); // every lane has read its row before the results overwrite the tile
+#pragma unroll
+        for (uint32_t c4 = 0; c4 < C; c4++) {
+            uint32_t vv[4];
+#pragma unroll
+            for (uint32_t e = 0; e < 4; e++) vv[e] = a[MODE == NTT_FWD ? brv_bits(4 * c4 + e, LOG_N) : 4 * c4 + e];
+            *reinterpret_cast<uint4 *>(T + L.row + 4 * c4) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+        }
+        wave_sync();
+        if (np == 64) {
+#pragma unroll
+            for (uint32_t i = 0; i < C; i++)
+                *L.full(g, i) = *reinterpret_cast<const u32x4 *>(T + L.stage + ROW * PPI * i);
+        } else {
+#pragma unroll
+            for (uint32_t i = 0; i < C; i++) {
+                const uint32_t p = PPI * i + L.lane / C;
+                if (p < np) *L.part(g, i, p) = *reinterpret_cast<const u32x4 *>(T + L.stage + ROW * PPI * i);
+            }
+            __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+        }
+        wave_sync(); // the stores have read the tile before the next one is staged
+    }
+}
+
+template <int LOG_N, int MODE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void ntt_lazy_kernel(
+    uint32_t *__restrict__ data, TabSet tabs, uint32_t polys, uint32_t blocks_per_plan, uint32_t total)
+{
+    constexpr uint32_t N = 1u << LOG_N, C = N / 4;
+    __shared__ __attribute__((aligned(16))) uint32_t T[64 * (N + 4)];
+    LazyTile<N> L;
+    L.data = data;
+    L.tabs = &tabs;
+    L.polys = polys;
+    L.blocks_per_plan = blocks_per_plan;
+    L.lane = threadIdx.x;
+    L.stage = (N + 4) * (L.lane / C) + 4 * (L.lane % C); // load i's chunk: LDS word stage + ROW PPI i
+    L.row = (N + 4) * L.lane;                           // this lane's polynomial
+#pragma unroll
+    for (uint32_t k = 0; k < LazyTile<N>::GROUPS; k++) {
+        // hidden from the optimiser, which would otherwise fold the group offset
+        // into a per-lane 64-bit address and an immediate too large to encode
+        L.voff[k] = 16 * L.lane + 4096 * k;
+        asm volatile("" : "+v"(L.voff[k]));
+    }
+    const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+    uint32_t b = blockIdx.x * per;
+    const uint32_t b_end = min(total, b + per);
+    if (b >= b_end) return;
+    u32x4 nx[C];
+    {
+        uint32_t np, plan;
+        L.fetch(L.locate(b, np, plan), np, nx);
+    }
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+    while (b < b_end) {
+        lazy_run<LOG_N, MODE, false>(L, T, b, b_end, nx);
+        lazy_run<LOG_N, MODE, true>(L, T, b, b_end, nx);
+    }
+}
+
 // Forward NTT of small-digit polynomials (i8 in, u32 NTT-domain out): the
 // witness transform of the digit-input Ajtai commit with the digit -> residue
 // conversion fused into the load (ring.rs:232-243 from_signed).  One
@@ -333,7 +488,9 @@ static uint32_t pipe_blocks_per_cu()
     static const uint32_t nb = [] {
         int v = 0;
         const hipError_t e =
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, ntt_pipe_kernel<LOG_N, AR, MODE>, 64, 0);
+            AR == ARITH_SHOUP
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, ntt_lazy_kernel<LOG_N, MODE>, 64, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, ntt_pipe_kernel<LOG_N, AR, MODE>, 64, 0);
         return (uint32_t)(e == hipSuccess && v > 0 ? v : 4);
     }();
     return nb;
@@ -345,7 +502,16 @@ template <int LOG_N, int AR>
 static void launch_reg(NttMode mode, bool dma, dim3 grid, uint32_t *data, const TabSet &ts, uint32_t polys,
                        uint32_t bpp, uint32_t total, hipStream_t stream)
 {
-    if (dma) {
+    if (dma && AR == ARITH_SHOUP) {
+        const uint32_t cus = grid.x;
+        if (mode == NTT_FWD) {
+            const dim3 g(std::min<uint32_t>(total, cus * pipe_blocks_per_cu<LOG_N, AR, NTT_FWD>()));
+            hipLaunchKernelGGL((ntt_lazy_kernel<LOG_N, NTT_FWD>), g, dim3(64), 0, stream, data, ts, polys, bpp, total);
+        } else {
+            const dim3 g(std::min<uint32_t>(total, cus * pipe_blocks_per_cu<LOG_N, AR, NTT_INV>()));
+            hipLaunchKernelGGL((ntt_lazy_kernel<LOG_N, NTT_INV>), g, dim3(64), 0, stream, data, ts, polys, bpp, total);
+        }
+    } else if (dma) {
         const uint32_t cus = grid.x;
         if (mode == NTT_FWD) {
             const dim3 g(std::min<uint32_t>(total, cus * pipe_blocks_per_cu<LOG_N, AR, NTT_FWD>()));
@@ -489,6 +655,7 @@ static void build_tables(lzx_ntt_plan *p)
     t[5] = p->prime_c;
     t[6] = p->prime_k;
     t[7] = (uint32_t)p->arith;
+    t[12] = q < (1u << 30) ? 1 : 0; // Harvey's lazy range [0, 4q) fits 32 bits
     put(8, 9, 0, p->n_inv);
     const uint32_t z1inv = host_pow(p->psi_inv, brv_bits(1, log_n), q);
     put(10, 11, 0, (uint32_t)((uint64_t)z1inv * p->n_inv % q));

@@ -17,7 +17,8 @@ namespace lzx {
 // evaluations on store.  The ref* tables restate NttTables' pow_psi and
 // pow_omega (ntt.rs:60-79) for the incomplete transforms, which must follow
 // the reference's own level structure.
-//   [0] q  [1] qinv  [2] n  [3] log_n  [4] bigq
+//   [0] q  [1] qinv  [2] n  [3] log_n  [4] bigq  [5] c  [6] k (q = c 2^k + 1)
+//   [7] arithmetic class  [12] 1 if q < 2^30 (Harvey's lazy range fits 32 bits)
 //   [8] n^{-1} (w, w')   [10] z_1^{-1} n^{-1} (w, w')
 //   fwd zeta  z_k = psi^brv(k), k in [1, n):   FWD_W + k, FWD_WP + k
 //   inv zeta  z_k^{-1}:                         INV_W + k, INV_WP + k

@@ -83,6 +83,99 @@ __device__ __forceinline__ void ntt_regs(uint32_t (&a)[1u << LOG_N], const uint3
     }
 }
 
+// ---- lazily reduced u32 transforms (q < 2^31) ---------------------------------
+//
+// Shoup's product a w - mulhi(a, w') q lies in [0, 2q) for every 32-bit a, so
+// values need not be canonical between levels.  Two classes:
+//   HV = false, q < 2^31: values in [0, 2q) (2q < 2^32).  A CT butterfly
+//     reduces its X input and the product to [0, q) (one subtract and one
+//     unsigned min each) and emits X + T and X - T + q, both in [0, 2q): 3
+//     multiplies + 8 ALU operations against 3 + 9 canonical;
+//   HV = true, q < 2^30 (4q < 2^32, Harvey's range): values in [0, 4q); X is
+//     reduced to [0, 2q), the product is used as it comes in [0, 2q), and the
+//     outputs X + T and X - T + 2q are in [0, 4q): 3 multiplies + 6.
+// The inverse (GS) keeps its inputs in [0, 2q): for HV it forms U + V in
+// [0, 4q) and reduces it once, for !HV it reduces both inputs to [0, q).
+// The first level's inputs are canonical and skip their reductions; every
+// output leaves canonical.
+
+// x in [0, 2c) -> x mod c: the unsigned min picks x - c exactly when x >= c
+__device__ __forceinline__ uint32_t red_once(uint32_t x, uint32_t c)
+{
+    const uint32_t t = x - c;
+    return t < x ? t : x;
+}
+
+// a w mod q up to one multiple of q: in [0, 2q) for every 32-bit a (Shoup)
+__device__ __forceinline__ uint32_t shoup_lazy(uint32_t a, uint32_t w, uint32_t wp, uint32_t q)
+{
+    return a * w - __umulhi(a, wp) * q;
+}
+
+template <int LOG_N, bool HV, int MODE>
+__device__ __forceinline__ void ntt_regs_lazy(uint32_t (&a)[1u << LOG_N], const uint32_t *__restrict__ table)
+{
+    const __attribute__((address_space(4))) uint32_t *tab = (const __attribute__((address_space(4))) uint32_t *)table;
+    constexpr uint32_t N = 1u << LOG_N;
+    const uint32_t q = tab[0], q2 = 2 * q;
+    if (MODE == NTT_FWD) {
+#pragma unroll
+        for (uint32_t len = N / 2; len >= 1; len >>= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                const uint32_t w = tab[TabLayout::fwd_w(N) + k];
+                const uint32_t wp = tab[TabLayout::fwd_wp(N) + k];
+#pragma unroll
+                for (uint32_t j = s; j < s + len; j++) {
+                    const bool first = len == N / 2;
+                    if (HV) {
+                        const uint32_t x = first ? a[j] : red_once(a[j], q2);
+                        const uint32_t t = shoup_lazy(a[j + len], w, wp, q);
+                        a[j] = x + t;
+                        a[j + len] = x - t + q2;
+                    } else {
+                        const uint32_t x = first ? a[j] : red_once(a[j], q);
+                        const uint32_t t = red_once(shoup_lazy(a[j + len], w, wp, q), q);
+                        a[j] = x + t;
+                        a[j + len] = x - t + q;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < N; i++) a[i] = HV ? red_once(red_once(a[i], q2), q) : red_once(a[i], q);
+    } else {
+#pragma unroll
+        for (uint32_t len = 1; len <= N / 2; len <<= 1) {
+#pragma unroll
+            for (uint32_t s = 0; s < N; s += 2 * len) {
+                const uint32_t k = N / (2 * len) + s / (2 * len);
+                const bool first = len == 1, last = len == N / 2;
+                const uint32_t w = last ? tab[8] : tab[TabLayout::inv_w(N) + k];
+                const uint32_t wp = last ? tab[9] : tab[TabLayout::inv_wp(N) + k];
+                const uint32_t lw = tab[10], lwp = tab[11];
+#pragma unroll
+                for (uint32_t j = s; j < s + len; j++) {
+                    uint32_t u = a[j], v = a[j + len];
+                    if (!HV && !first) {
+                        u = red_once(u, q);
+                        v = red_once(v, q);
+                    }
+                    const uint32_t d = HV ? u - v + q2 : u - v + q;
+                    if (last) {
+                        a[j] = red_once(shoup_lazy(u + v, w, wp, q), q);
+                        a[j + len] = red_once(shoup_lazy(d, lw, lwp, q), q);
+                    } else {
+                        a[j] = HV ? red_once(u + v, q2) : u + v;
+                        a[j + len] = shoup_lazy(d, w, wp, q);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // ---- FP64 register transforms (any q < 2^32) ----------------------------------
 //
 // Residues are integer doubles.  For |a| < 2^40 and a constant 0 <= w < q,
