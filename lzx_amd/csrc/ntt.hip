@@ -260,6 +260,13 @@ __device__ __forceinline__ void lazy_run(const LazyTile<1u << LOG_N> &L, uint32_
             *reinterpret_cast<uint4 *>(T + L.row + 4 * c4) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
         }
         wave_sync();
+        // The next tile's loads were issued before the butterflies and have
+        // landed by now.  Waiting for them here, before this tile's stores are
+        // issued, leaves only stores in flight at the loop head, so staging the
+        // next tile does not wait for them: loads and stores share one counter
+        // and may complete out of order with respect to each other, so a wait
+        // for loads issued before stores also waits for the stores.
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
         if (np == 64) {
 #pragma unroll
             for (uint32_t i = 0; i < C; i++)
@@ -276,9 +283,14 @@ __device__ __forceinline__ void lazy_run(const LazyTile<1u << LOG_N> &L, uint32_
     }
 }
 
+// skew: the second half of the grid (the second resident wave on each SIMD
+// under round-robin dispatch) starts skew x 1024 cycles late, so the first
+// tiles of the first half arrive in half the time and their butterflies
+// overlap the second half's loads instead of every wave waiting on one burst.
 template <int LOG_N, int MODE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void ntt_lazy_kernel(
-    uint32_t *__restrict__ data, TabSet tabs, uint32_t polys, uint32_t blocks_per_plan, uint32_t total)
+    uint32_t *__restrict__ data, TabSet tabs, uint32_t polys, uint32_t blocks_per_plan, uint32_t total,
+    uint32_t skew)
 {
     constexpr uint32_t N = 1u << LOG_N, C = N / 4;
     __shared__ __attribute__((aligned(16))) uint32_t T[64 * (N + 4)];
@@ -301,6 +313,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     uint32_t b = blockIdx.x * per;
     const uint32_t b_end = min(total, b + per);
     if (b >= b_end) return;
+    if (blockIdx.x >= gridDim.x / 2)
+        for (uint32_t z = 0; z < skew; z++) __builtin_amdgcn_s_sleep(16);
     u32x4 nx[C];
     {
         uint32_t np, plan;
@@ -496,6 +510,18 @@ static uint32_t pipe_blocks_per_cu()
     return nb;
 }
 
+// Start skew of the lazy kernel's second half (units of 1024 cycles): 6 by
+// measurement on the 4-prime x 2^16 x n = 64 batch (forward 29.5 -> 29.0 us,
+// inverse 31.4 -> 29.0 us; 8 and above lose again); LZX_NTT_SKEW overrides.
+static uint32_t ntt_skew()
+{
+    static const uint32_t v = [] {
+        const char *e = getenv("LZX_NTT_SKEW");
+        return e ? (uint32_t)strtoul(e, nullptr, 10) : 6u;
+    }();
+    return v;
+}
+
 // grid: for the persistent pipe kernel, the number of CUs (the launch takes
 // as many waves as stay resident); otherwise one block per tile.
 template <int LOG_N, int AR>
@@ -506,10 +532,12 @@ static void launch_reg(NttMode mode, bool dma, dim3 grid, uint32_t *data, const 
         const uint32_t cus = grid.x;
         if (mode == NTT_FWD) {
             const dim3 g(std::min<uint32_t>(total, cus * pipe_blocks_per_cu<LOG_N, AR, NTT_FWD>()));
-            hipLaunchKernelGGL((ntt_lazy_kernel<LOG_N, NTT_FWD>), g, dim3(64), 0, stream, data, ts, polys, bpp, total);
+            hipLaunchKernelGGL((ntt_lazy_kernel<LOG_N, NTT_FWD>), g, dim3(64), 0, stream, data, ts, polys, bpp, total,
+                               ntt_skew());
         } else {
             const dim3 g(std::min<uint32_t>(total, cus * pipe_blocks_per_cu<LOG_N, AR, NTT_INV>()));
-            hipLaunchKernelGGL((ntt_lazy_kernel<LOG_N, NTT_INV>), g, dim3(64), 0, stream, data, ts, polys, bpp, total);
+            hipLaunchKernelGGL((ntt_lazy_kernel<LOG_N, NTT_INV>), g, dim3(64), 0, stream, data, ts, polys, bpp, total,
+                               ntt_skew());
         }
     } else if (dma) {
         const uint32_t cus = grid.x;
