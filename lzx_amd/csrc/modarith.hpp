@@ -294,4 +294,84 @@ LZX_HD uint64_t gl_acc_value(const GlAcc &s)
 
 LZX_HD uint64_t gl_acc_finish(const GlAcc &s) { return gl_canon(gl_acc_value(s)); }
 
+// A representative below 2^64 of a + b for any a, b below 2^64: a wrapped
+// sum is worth 2^64 = eps more; adding eps wraps again only when the wrapped
+// sum is at least 2^64 - eps, and then the result is below eps.
+LZX_HD uint64_t gl_add_lazy(uint64_t a, uint64_t b)
+{
+    uint64_t s = a + b;
+    if (s < b) {
+        s += GL_EPS;
+        if (s < GL_EPS) s += GL_EPS;
+    }
+    return s;
+}
+
+// p - x for any representative x below 2^64, as a representative below 2^64.
+LZX_HD uint64_t gl_neg_lazy(uint64_t x)
+{
+    x = gl_canon(x);
+    return GL_P - x; // p for x = 0, a representative of 0
+}
+
+// Lazy sum of 64-bit values and full 128-bit products: the true value is
+// top * 2^128 + hi * 2^64 + lo.  Products are added without reducing them; the
+// sum is reduced once at the end, with 2^128 = (2^32 - 1)^2 = -2^32 (mod p).
+// A product's high word is at most 2^64 - 2^33 + 1, so top counts the high
+// word's wraps: at most one per added product.
+struct GlAcc2 {
+    uint64_t lo, hi;
+    uint32_t top;
+};
+
+LZX_HD void gl_acc2_add(GlAcc2 &s, uint64_t v)
+{
+    s.lo += v;
+    s.hi += s.lo < v; // below 2^64 for fewer than 2^64 additions
+}
+
+LZX_HD void gl_acc2_add_mul(GlAcc2 &s, uint64_t a, uint64_t b)
+{
+    uint64_t hi, lo;
+    mul128(a, b, hi, lo);
+    s.lo += lo;
+    hi += s.lo < lo; // <= 2^64 - 2^33 + 2
+    s.hi += hi;
+    s.top += s.hi < hi;
+}
+
+LZX_HD uint64_t gl_acc2_finish(const GlAcc2 &s)
+{
+    // top * 2^32 <= 2^64 - 2^32 < p is canonical
+    return gl_sub(gl_reduce128(s.hi, s.lo), (uint64_t)s.top << 32);
+}
+
+// a b + c as 128 bits for any a, b, c below 2^64.  On the device c's halves
+// ride in the 32x32 multiply-adds: a0 b0 + c0, a0 b1 + carry and
+// a1 b0 + t0 + c1 each stay below 2^64.
+LZX_HD void mul128_add(uint64_t a, uint64_t b, uint64_t c, uint64_t &hi, uint64_t &lo)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0 + (uint32_t)c;
+    const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);
+    const uint64_t u = (uint64_t)a1 * b0 + ((uint64_t)(uint32_t)t + (c >> 32));
+    lo = (uint32_t)p00 | (u << 32);
+    hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);
+#else
+    const unsigned __int128 pr = (unsigned __int128)a * b + c;
+    hi = (uint64_t)(pr >> 64);
+    lo = (uint64_t)pr;
+#endif
+}
+
+// a + (b - a) r for canonical a, b, r (bind_first_half_in_place's fold,
+// field_simd.rs:297-313), canonical: the 128-bit a + (b - a) r is reduced once.
+LZX_HD uint64_t gl_fold(uint64_t a, uint64_t b, uint64_t r)
+{
+    uint64_t hi, lo;
+    mul128_add(gl_sub_c(b, a), r, a, hi, lo);
+    return gl_reduce128(hi, lo);
+}
+
 } // namespace lzx

@@ -19,8 +19,17 @@
 // round's transcript step; once a fold's half drops to SC_TAIL_HALF, a single
 // sc_tail_kernel block runs every remaining round.  All rounds are enqueued
 // at once and the host synchronizes once.
+//
+// Arithmetic: each point's last product (by the last common factor or by
+// E(p)) is added to a 128-bit lazy sum unreduced, and a fold is one 128-bit
+// multiply-add reduced once.  The virtual polynomial's term structure is
+// interpreted at run time (ScGenericShape), or, for the shapes the reference's
+// provers build most (ScShape), fixed at compile time so each point is
+// straight-line code.  Consecutive large kernels visit their pairs in
+// opposite orders, so each starts on the entries its predecessor touched last.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "modarith.hpp"
@@ -101,30 +110,154 @@ __device__ __forceinline__ uint64_t eval_terms(const uint64_t (&v)[NF], const Sc
     return sum; // the caller's lazy sums reduce it
 }
 
+// A term structure fixed at compile time, for the shapes the reference's
+// provers build most: at most 4 factors and 2 terms, each factor at most
+// once per term beyond the common part.  Term t multiplies the factors of bit
+// mask M[t] by its coefficient, of class C[t]: 1 for +1, 2 for -1, 0 for
+// T.coeff[t]; CM is the set of factors every term shares (T.common).  With
+// the structure known the compiler lays out each point's products as
+// straight-line code; ScGenericShape interprets T at run time instead.
+struct ScGenericShape {
+};
+
+template <uint32_t NT_, uint32_t CM_, uint32_t M0_, uint32_t C0_, uint32_t M1_, uint32_t C1_, bool SPLIT_>
+struct ScShape {
+    static constexpr uint32_t NT = NT_, CM = CM_;
+    static constexpr bool SPLIT = SPLIT_; // the split eq factor (T.eq_f >= 0, never in a mask)
+    static constexpr uint32_t mask(int t) { return t == 0 ? M0_ : M1_; }
+    static constexpr uint32_t cls(int t) { return t == 0 ? C0_ : C1_; }
+    static constexpr uint32_t used() { return CM | M0_ | (NT > 1 ? M1_ : 0); }
+    // A one-factor term with coefficient -1 whose factor appears nowhere
+    // else is negated in the pair itself: -(lo + t (hi - lo)) = (p - lo) + t (lo - hi).
+    static constexpr uint32_t neg_mask()
+    {
+        uint32_t n = 0;
+        for (int t = 0; t < (int)NT; t++) {
+            const uint32_t m = mask(t), others = CM | (t == 0 ? (NT > 1 ? M1_ : 0) : M0_);
+            if (cls(t) == 2 && m && (m & (m - 1)) == 0 && !(m & others)) n |= m;
+        }
+        return n;
+    }
+};
+
+template <class SH>
+constexpr bool sc_generic() { return std::is_same<SH, ScGenericShape>::value; }
+
+// Factor f's table is read: it is in the shape, or with a generic shape one
+// of the T.nf factors other than the split eq factor.
+template <class SH>
+__device__ __forceinline__ bool factor_live(int f, const ScTerms &T)
+{
+    if constexpr (sc_generic<SH>()) return (uint32_t)f < T.nf && f != T.eq_f;
+    else return (SH::used() >> f) & 1;
+}
+
+// One point of a compile-time shape: the term sum, times the common factors,
+// times E(p) with the split eq factor, added to acc without reducing the last
+// product.
+template <class SH, int NF>
+__device__ __forceinline__ void shape_point(const uint64_t (&v)[NF], const ScTerms &T, uint64_t eq_w, GlAcc2 &acc)
+{
+    constexpr bool split = SH::SPLIT;
+    constexpr uint32_t NEG = SH::neg_mask();
+    // s_one: the term sum is the constant 1 (a pure product of common factors)
+    constexpr bool s_one = SH::NT == 1 && SH::mask(0) == 0 && SH::cls(0) == 1;
+    uint64_t s = 0;
+    if constexpr (!s_one) {
+#pragma unroll
+        for (int t = 0; t < (int)SH::NT; t++) {
+            const uint32_t m = SH::mask(t), c = SH::cls(t);
+            uint64_t x = 0;
+            bool seeded = false;
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+                if ((m >> f) & 1) {
+                    x = seeded ? gl_mul_lazy(x, v[f]) : v[f];
+                    seeded = true;
+                }
+            if (!seeded) x = c == 1 ? 1 : (c == 2 ? GL_P - 1 : T.coeff[t]);
+            else if (c == 2 && !(m & NEG)) x = gl_neg_lazy(x);
+            else if (c == 0) x = gl_mul_lazy(x, T.coeff[t]);
+            s = t == 0 ? x : gl_add_lazy(s, x);
+        }
+    }
+    // the common factors; the last product goes to the accumulator unreduced
+    constexpr uint32_t CM = SH::CM;
+    constexpr int LASTC = CM ? 31 - __builtin_clz(CM) : -1;
+    bool one = s_one;
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+        if (((CM >> f) & 1) && (f != LASTC || split)) {
+            s = one ? v[f] : gl_mul_lazy(s, v[f]);
+            one = false;
+        }
+    if constexpr (split) {
+        if (one) gl_acc2_add(acc, eq_w);
+        else gl_acc2_add_mul(acc, s, eq_w);
+    } else if constexpr (LASTC >= 0) {
+        if (one) gl_acc2_add(acc, v[LASTC]);
+        else gl_acc2_add_mul(acc, s, v[LASTC]);
+    } else {
+        gl_acc2_add(acc, s);
+    }
+}
+
 // Accumulate g(t) (t < T.npts) of one pair (lo, hi) into acc[]; with the
 // split eq factor every point's term sum is weighted by eq_w = E(p).
-template <int NF, int D>
+template <int NF, int D, class SH>
 __device__ __forceinline__ void eval_pair(const uint64_t (&lo)[NF], const uint64_t (&hi)[NF], const ScTerms &T,
-                                          uint64_t eq_w, GlAcc (&acc)[D + 1])
+                                          uint64_t eq_w, GlAcc2 (&acc)[D + 1])
 {
     const bool split = T.eq_f >= 0;
-    auto put = [&](GlAcc &a, uint64_t v) { gl_acc_add(a, split ? gl_mul_lazy(v, eq_w) : v); };
     uint64_t diff[NF], cur[NF];
+    if constexpr (sc_generic<SH>()) {
+        auto put = [&](GlAcc2 &a, uint64_t v) {
+            if (split) gl_acc2_add_mul(a, v, eq_w);
+            else gl_acc2_add(a, v);
+        };
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-        diff[f] = gl_sub_c(hi[f], lo[f]);
-        cur[f] = lo[f];
-    }
-    put(acc[0], eval_terms<NF>(cur, T));
-    if (T.npts > 1) put(acc[1], eval_terms<NF>(hi, T));
+        for (int f = 0; f < NF; f++) {
+            diff[f] = gl_sub_c(hi[f], lo[f]);
+            cur[f] = lo[f];
+        }
+        put(acc[0], eval_terms<NF>(cur, T));
+        if (T.npts > 1) put(acc[1], eval_terms<NF>(hi, T));
 #pragma unroll
-    for (int f = 0; f < NF; f++) cur[f] = hi[f];
+        for (int f = 0; f < NF; f++) cur[f] = hi[f];
 #pragma unroll
-    for (int t = 2; t <= D; t++) {
-        if ((uint32_t)t >= T.npts) break;
+        for (int t = 2; t <= D; t++) {
+            if ((uint32_t)t >= T.npts) break;
 #pragma unroll
-        for (int f = 0; f < NF; f++) cur[f] = gl_add_nc(cur[f], diff[f]); // lo + t*(hi-lo), unreduced
-        put(acc[t], eval_terms<NF>(cur, T));
+            for (int f = 0; f < NF; f++) cur[f] = gl_add_nc(cur[f], diff[f]); // lo + t*(hi-lo), unreduced
+            put(acc[t], eval_terms<NF>(cur, T));
+        }
+    } else {
+        // points 0 .. D: degree D = |CM| + the largest term, D + 1 points, or D
+        // with the split eq factor (its point D is extrapolated)
+        constexpr uint32_t USED = SH::used(), NEG = SH::neg_mask();
+        constexpr int NPTS = SH::SPLIT ? D : D + 1;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            cur[f] = diff[f] = 0;
+            if ((USED >> f) & 1) {
+                cur[f] = (NEG >> f) & 1 ? GL_P - lo[f] : lo[f];
+                diff[f] = (NEG >> f) & 1 ? gl_sub_c(lo[f], hi[f]) : gl_sub_c(hi[f], lo[f]);
+            }
+        }
+        shape_point<SH, NF>(cur, T, eq_w, acc[0]);
+        if constexpr (NPTS > 1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+                if ((USED >> f) & 1) cur[f] = (NEG >> f) & 1 ? GL_P - hi[f] : hi[f];
+            shape_point<SH, NF>(cur, T, eq_w, acc[1]);
+        }
+#pragma unroll
+        for (int t = 2; t < NPTS; t++) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+                if ((USED >> f) & 1) cur[f] = gl_add_nc(cur[f], diff[f]);
+            shape_point<SH, NF>(cur, T, eq_w, acc[t]);
+        }
     }
 }
 
@@ -151,16 +284,19 @@ struct ScShared {
     uint32_t status;
 };
 
+__device__ __forceinline__ uint64_t acc_finish(const GlAcc &a) { return gl_acc_finish(a); }
+__device__ __forceinline__ uint64_t acc_finish(const GlAcc2 &a) { return gl_acc2_finish(a); }
+
 // Canonical block totals of the per-thread lazy sums acc[0..dd] into out[].
 // The loop runs to the compile-time D so acc[] stays in registers.
-template <int D, int WAVES>
-__device__ __forceinline__ void block_totals(GlAcc (&acc)[D + 1], uint32_t dd, ScShared &sh, uint64_t *out)
+template <int D, int WAVES, class ACC>
+__device__ __forceinline__ void block_totals(ACC (&acc)[D + 1], uint32_t dd, ScShared &sh, uint64_t *out)
 {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int t = 0; t <= D; t++)
         if ((uint32_t)t <= dd) {
-            const uint64_t v = wave_sum(gl_acc_finish(acc[t]));
+            const uint64_t v = wave_sum(acc_finish(acc[t]));
             if (lane == 0) sh.wsum[wave][t] = v;
         }
     __syncthreads();
@@ -175,7 +311,7 @@ __device__ __forceinline__ void block_totals(GlAcc (&acc)[D + 1], uint32_t dd, S
 
 // Thread 0 publishes the block's totals for the close kernel.
 template <int D>
-__device__ __forceinline__ void publish_totals(GlAcc (&acc)[D + 1], uint32_t dd, ScShared &sh,
+__device__ __forceinline__ void publish_totals(GlAcc2 (&acc)[D + 1], uint32_t dd, ScShared &sh,
                                                uint64_t *__restrict__ partial)
 {
     block_totals<D, SC_BLOCK / 64>(acc, dd, sh, sh.ev);
@@ -445,75 +581,86 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_close_kernel(const uint64_t *__re
 }
 
 // Round 0: evaluate g_0 (or S_0 with the split eq factor) over the caller's tables.
-template <int NF, int D>
+// Index of the i-th pair a grid visits: ascending, or descending (rev) so a
+// kernel starts on the entries its predecessor touched last, which the
+// device's last-level cache may still hold.
+__device__ __forceinline__ uint64_t sc_pair_index(uint64_t i, uint64_t count, uint32_t rev)
+{
+    return rev ? count - 1 - i : i;
+}
+
+template <int NF, int D, class SH>
 __global__ __launch_bounds__(SC_BLOCK) void sc_eval_kernel(ScTables tabs, ScTerms T, uint64_t half, ScEqArgs E,
-                                                           uint64_t *__restrict__ partial)
+                                                           uint64_t *__restrict__ partial, uint32_t rev)
 {
     __shared__ ScShared sh;
-    GlAcc acc[D + 1];
+    GlAcc2 acc[D + 1];
 #pragma unroll
-    for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
-    for (uint64_t p = (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x; p < half; p += (uint64_t)gridDim.x * SC_BLOCK) {
+    for (int t = 0; t <= D; t++) acc[t] = GlAcc2{0, 0, 0};
+    for (uint64_t i = (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x; i < half; i += (uint64_t)gridDim.x * SC_BLOCK) {
+        const uint64_t p = sc_pair_index(i, half, rev);
         uint64_t lo[NF], hi[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            const bool live = (uint32_t)f < T.nf && f != T.eq_f;
+            const bool live = factor_live<SH>(f, T);
             lo[f] = live ? tabs.src[f][p] : 0;
             hi[f] = live ? tabs.src[f][p + half] : 0;
         }
-        eval_pair<NF, D>(lo, hi, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
+        eval_pair<NF, D, SH>(lo, hi, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
     }
     publish_totals<D>(acc, T.npts - 1, sh, partial);
 }
 
 // Fold pairs p < quarter of tables of length 2*half with challenge r (in place
 // from the second round on) and accumulate the next round's values (E: the
-// next round's split eq tables).
-template <int NF, int D>
+// next round's split eq tables).  A thread reads and writes only its own
+// pairs' entries, so the visiting order is free.
+template <int NF, int D, class SH>
 __device__ __forceinline__ void fold_eval_range(const ScTables &tabs, const ScTerms &T, uint64_t half, uint64_t r,
-                                                const ScEqArgs &E, uint64_t p0, uint64_t stride,
-                                                GlAcc (&acc)[D + 1])
+                                                const ScEqArgs &E, uint64_t i0, uint64_t stride, uint32_t rev,
+                                                GlAcc2 (&acc)[D + 1])
 {
     const uint64_t quarter = half >> 1;
-    for (uint64_t p = p0; p < quarter; p += stride) {
+    for (uint64_t i = i0; i < quarter; i += stride) {
+        const uint64_t p = sc_pair_index(i, quarter, rev);
         uint64_t n0[NF], n1[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            if ((uint32_t)f < T.nf && f != T.eq_f) {
+            if (factor_live<SH>(f, T)) {
                 const uint64_t *s = tabs.src[f];
                 const uint64_t a0 = s[p], b0 = s[p + half], a1 = s[p + quarter], b1 = s[p + half + quarter];
-                n0[f] = gl_add(a0, gl_mul(gl_sub_c(b0, a0), r));
-                n1[f] = gl_add(a1, gl_mul(gl_sub_c(b1, a1), r));
+                n0[f] = gl_fold(a0, b0, r);
+                n1[f] = gl_fold(a1, b1, r);
                 tabs.dst[f][p] = n0[f];
                 tabs.dst[f][p + quarter] = n1[f];
             } else {
                 n0[f] = n1[f] = 0;
             }
         }
-        eval_pair<NF, D>(n0, n1, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
+        eval_pair<NF, D, SH>(n0, n1, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
     }
 }
 
 // Fold with challenge rnd (half >= 2) and evaluate round rnd+1.
-template <int NF, int D>
+template <int NF, int D, class SH>
 __global__ __launch_bounds__(SC_BLOCK) void sc_fold_eval_kernel(ScTables tabs, ScTerms T, uint64_t half,
                                                                 const uint64_t *__restrict__ rp, ScEqArgs E,
-                                                                uint64_t *__restrict__ partial)
+                                                                uint64_t *__restrict__ partial, uint32_t rev)
 {
     __shared__ ScShared sh;
     const uint64_t r = *rp; // written on the device by the close kernel
-    GlAcc acc[D + 1];
+    GlAcc2 acc[D + 1];
 #pragma unroll
-    for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
-    fold_eval_range<NF, D>(tabs, T, half, r, E, (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x,
-                           (uint64_t)gridDim.x * SC_BLOCK, acc);
+    for (int t = 0; t <= D; t++) acc[t] = GlAcc2{0, 0, 0};
+    fold_eval_range<NF, D, SH>(tabs, T, half, r, E, (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x,
+                               (uint64_t)gridDim.x * SC_BLOCK, rev, acc);
     publish_totals<D>(acc, T.npts - 1, sh, partial);
 }
 
 // Rounds r0 .. num_vars-1 in one block: fold with the challenge of round r,
 // sum, close round r+1; the last fold leaves one entry per table (the factor
 // claims, sumcheck.rs:139-143).  tabs.src is the input of round r0's fold.
-template <int NF, int D>
+template <int NF, int D, class SH>
 __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, ScTerms T, uint64_t half0, uint32_t r0,
                                                                 uint32_t num_vars, ScRoundArgs A)
 {
@@ -553,13 +700,13 @@ __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, S
             }
             break;
         }
-        GlAcc acc[D + 1];
+        GlAcc2 acc[D + 1];
 #pragma unroll
-        for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
+        for (int t = 0; t <= D; t++) acc[t] = GlAcc2{0, 0, 0};
         const uint32_t k = r - r0; // round r+1's slot in the preloaded tables
         ScEqArgs E{};
         if (A.eq_c) E = sh.eqr[k];
-        fold_eval_range<NF, D>(tabs, T, half, ch, E, threadIdx.x, SC_TAIL_BLOCK, acc);
+        fold_eval_range<NF, D, SH>(tabs, T, half, ch, E, threadIdx.x, SC_TAIL_BLOCK, 0, acc);
 #pragma unroll
         for (int f = 0; f < NF; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
         block_totals<D, SC_MAX_WAVES>(acc, T.npts - 1, sh, sh.ev); // its barriers also order the folded tables
@@ -692,26 +839,28 @@ static lzx_status build_problem(size_t num_vars, size_t nfactors, const uint64_t
     return LZX_OK;
 }
 
+// eval / fold launchers; rev = 1 visits the pairs in descending order
 struct ScKernels {
-    void (*eval)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const ScEqArgs &, uint64_t *);
+    void (*eval)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const ScEqArgs &, uint64_t *,
+                 uint32_t);
     void (*fold)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const uint64_t *, const ScEqArgs &,
-                 uint64_t *);
+                 uint64_t *, uint32_t);
     void (*close)(hipStream_t, const uint64_t *, uint32_t, const ScRoundArgs &, uint32_t);
     void (*tail)(hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint32_t, uint32_t, const ScRoundArgs &);
 };
 
-template <int NF, int D>
+template <int NF, int D, class SH>
 static void launch_eval(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half,
-                        const ScEqArgs &E, uint64_t *part)
+                        const ScEqArgs &E, uint64_t *part, uint32_t rev)
 {
-    hipLaunchKernelGGL((sc_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, E, part);
+    hipLaunchKernelGGL((sc_eval_kernel<NF, D, SH>), g, dim3(SC_BLOCK), 0, s, tb, T, half, E, part, rev);
 }
 
-template <int NF, int D>
+template <int NF, int D, class SH>
 static void launch_fold(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half,
-                        const uint64_t *rp, const ScEqArgs &E, uint64_t *part)
+                        const uint64_t *rp, const ScEqArgs &E, uint64_t *part, uint32_t rev)
 {
-    hipLaunchKernelGGL((sc_fold_eval_kernel<NF, D>), g, dim3(SC_BLOCK), 0, s, tb, T, half, rp, E, part);
+    hipLaunchKernelGGL((sc_fold_eval_kernel<NF, D, SH>), g, dim3(SC_BLOCK), 0, s, tb, T, half, rp, E, part, rev);
 }
 
 template <int D>
@@ -720,21 +869,76 @@ static void launch_close(hipStream_t s, const uint64_t *part, uint32_t nblocks, 
     hipLaunchKernelGGL((sc_close_kernel<D>), dim3(1), dim3(SC_BLOCK), 0, s, part, nblocks, A, rnd);
 }
 
-template <int NF, int D>
+template <int NF, int D, class SH>
 static void launch_tail(hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half0, uint32_t r0,
                         uint32_t num_vars, const ScRoundArgs &A)
 {
-    hipLaunchKernelGGL((sc_tail_kernel<NF, D>), dim3(1), dim3(SC_TAIL_BLOCK), 0, s, tb, T, half0, r0, num_vars, A);
+    hipLaunchKernelGGL((sc_tail_kernel<NF, D, SH>), dim3(1), dim3(SC_TAIL_BLOCK), 0, s, tb, T, half0, r0, num_vars, A);
 }
 
-template <int NF, int D>
+template <int NF, int D, class SH = ScGenericShape>
 static ScKernels kernels_for()
 {
-    return ScKernels{launch_eval<NF, D>, launch_fold<NF, D>, launch_close<D>, launch_tail<NF, D>};
+    return ScKernels{launch_eval<NF, D, SH>, launch_fold<NF, D, SH>, launch_close<D>, launch_tail<NF, D, SH>};
 }
 
-static ScKernels pick_kernels(size_t nf, size_t d)
+// The compile-time shape P.T has, when it has one of the registered ones.
+struct ScShapeKey {
+    uint32_t nt, cm, m0, c0, m1, c1, split;
+    // a registered coefficient class 0 reads T.coeff, so it takes any coefficient
+    bool runs(const ScShapeKey &reg) const
+    {
+        auto cls_ok = [](uint32_t have, uint32_t want) { return want == 0 || have == want; };
+        return nt == reg.nt && cm == reg.cm && m0 == reg.m0 && cls_ok(c0, reg.c0) &&
+               (nt < 2 || (m1 == reg.m1 && cls_ok(c1, reg.c1))) && split == reg.split;
+    }
+};
+
+static bool shape_key(const ScProblem &P, ScShapeKey &k)
 {
+    if (P.nf > 4 || P.nterms < 1 || P.nterms > 2) return false;
+    uint32_t m[2] = {0, 0}, c[2] = {0, 0}, cm = 0;
+    for (size_t f = 0; f < P.nf; f++) {
+        if (P.T.common[f] > 1) return false;
+        if (P.T.common[f]) cm |= 1u << f;
+    }
+    for (size_t t = 0; t < P.nterms; t++) {
+        for (size_t f = 0; f < P.nf; f++) {
+            if (P.T.exps[t][f] > 1) return false;
+            if (P.T.exps[t][f]) m[t] |= 1u << f;
+        }
+        c[t] = P.T.coeff[t] == 1 ? 1 : (P.T.coeff[t] == GL_P - 1 ? 2 : 0);
+    }
+    k = ScShapeKey{(uint32_t)P.nterms, cm, m[0], c[0], m[1], c[1], P.T.eq_f >= 0};
+    return true;
+}
+
+template <uint32_t NT, uint32_t CM, uint32_t M0, uint32_t C0, uint32_t M1, uint32_t C1, bool SPLIT>
+static constexpr ScShapeKey key_of(ScShape<NT, CM, M0, C0, M1, C1, SPLIT>)
+{
+    return ScShapeKey{NT, CM, M0, C0, M1, C1, SPLIT};
+}
+
+// Registered shapes (factor indices in the caller's order):
+//   eq * (a*b - c) with eq a table (eq, a, b, c)              sumcheck over R1CS rows
+//   (a*b - c) with the split eq factor at index 0             prove_fast_with_eq of the same
+//   f * e                                                    pcs.rs:95-98, salsa_response.rs:106-109
+//   rho_0 f_0 e_0 + rho_1 f_1 e_1                             pcs.rs:225-231, salsa_binding.rs:182-188
+using ShEqABC = ScShape<2, 0x1, 0x6, 1, 0x8, 2, false>;
+using ShSplitABC = ScShape<2, 0x0, 0x6, 1, 0x8, 2, true>;
+using ShProd2 = ScShape<1, 0x3, 0x0, 1, 0, 0, false>;
+using ShBatch2 = ScShape<2, 0x0, 0x3, 0, 0xC, 0, false>;
+
+static ScKernels pick_kernels(const ScProblem &P)
+{
+    const size_t nf = P.nf, d = P.d;
+    ScShapeKey k;
+    if (!getenv("LZX_SC_GENERIC") && shape_key(P, k)) {
+        if (k.runs(key_of(ShEqABC{})) && nf == 4) return kernels_for<4, 3, ShEqABC>();
+        if (k.runs(key_of(ShSplitABC{})) && nf == 4) return kernels_for<4, 3, ShSplitABC>();
+        if (k.runs(key_of(ShProd2{})) && nf == 2) return kernels_for<2, 2, ShProd2>();
+        if (k.runs(key_of(ShBatch2{})) && nf == 4) return kernels_for<4, 2, ShBatch2>();
+    }
     if (nf <= 2 && d <= 2) return kernels_for<2, 2>();
     if (nf <= 4 && d <= 3) return kernels_for<4, 3>();
     return kernels_for<SC_MAXF, SC_MAXD>();
@@ -773,7 +977,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
 {
     const size_t m = P.num_vars, d = P.d, width = d + 1, nf = P.nf;
     const uint64_t N = 1ULL << m;
-    const ScKernels K = pick_kernels(nf, d);
+    const ScKernels K = pick_kernels(P);
 
     ScRoundArgs A{};
     A.width = (uint32_t)width;
@@ -899,7 +1103,8 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     };
     if (st == LZX_OK && m > 0) {
         const uint32_t g0 = sc_grid(N / 2);
-        K.eval(dim3(g0), stream, tabs, P.T, N / 2, eq_args(0), partial);
+        // consecutive kernels visit their pairs in opposite orders
+        K.eval(dim3(g0), stream, tabs, P.T, N / 2, eq_args(0), partial, 0);
         check("sc_eval_kernel");
         K.close(stream, partial, g0, A, 0);
         check("sc_close_kernel");
@@ -907,7 +1112,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
         for (; st == LZX_OK && (N >> (r + 1)) > SC_TAIL_HALF; r++) { // large rounds: a grid per fold
             const uint64_t half = N >> (r + 1);
             const uint32_t g = sc_grid(half / 2);
-            K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, eq_args(r + 1), partial);
+            K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, eq_args(r + 1), partial, r % 2 == 0);
             check("sc_fold_eval_kernel");
             K.close(stream, partial, g, A, (uint32_t)r + 1);
             check("sc_close_kernel");
@@ -1172,7 +1377,7 @@ extern "C" lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors
     lzx_status st = build_problem(num_vars, nfactors, factors_dev, nterms, coeffs, term_off, term_ids, P);
     if (st != LZX_OK) return st;
     hipStream_t s = (hipStream_t)stream;
-    const ScKernels K = pick_kernels(P.nf, P.d);
+    const ScKernels K = pick_kernels(P);
     ScTables tabs{};
     for (size_t f = 0; f < P.nf; f++) tabs.src[f] = factors_dev[f];
     if (num_vars == 0) {
@@ -1188,7 +1393,7 @@ extern "C" lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors
     uint64_t *partial = nullptr, *dev_out = nullptr;
     LZX_HIP(scratch_alloc((void **)&partial, ((size_t)g + 1) * width * sizeof(uint64_t), s));
     dev_out = partial + (size_t)g * width;
-    K.eval(dim3(g), s, tabs, P.T, half, ScEqArgs{}, partial);
+    K.eval(dim3(g), s, tabs, P.T, half, ScEqArgs{}, partial, 0);
     LZX_LAUNCH_CHECK("sc_eval_kernel");
     hipLaunchKernelGGL(sc_reduce_kernel, dim3(1), dim3(SC_BLOCK), 0, s, partial, g, (uint32_t)width, dev_out);
     LZX_LAUNCH_CHECK("sc_reduce_kernel");
@@ -1243,9 +1448,9 @@ static lzx_status split_eq_sum(size_t num_vars, const uint64_t *poly_dev, const 
     hipError_t e = hipMemcpyAsync(dar, arena.data(), arena.size() * 8, hipMemcpyHostToDevice, s);
     ScTables tabs{};
     tabs.src[0] = poly_dev;
-    const ScKernels K = pick_kernels(2, 2);
+    const ScKernels K = pick_kernels(P);
     if (e == hipSuccess) {
-        K.eval(dim3(g), s, tabs, P.T, half, ScEqArgs{dar + hi_off, dar + lo_off, (uint32_t)lb, 0}, partial);
+        K.eval(dim3(g), s, tabs, P.T, half, ScEqArgs{dar + hi_off, dar + lo_off, (uint32_t)lb, 0}, partial, 0);
         hipLaunchKernelGGL(sc_reduce_kernel, dim3(1), dim3(SC_BLOCK), 0, s, partial, g, 2u, dev_out);
         e = hipGetLastError();
     }
@@ -1572,7 +1777,7 @@ extern "C" lzx_status lzx_sumcheck_session_create(size_t num_vars, size_t nfacto
         delete S;
         return st;
     }
-    S->K = pick_kernels(S->P.nf, S->P.d);
+    S->K = pick_kernels(S->P);
     S->width = S->P.d + 1;
     const size_t nf = S->P.nf, N = (size_t)1 << num_vars, halfN = num_vars ? N / 2 : 1;
     const size_t tab_bytes = std::max<size_t>(1, nf) * halfN * 8, part_bytes = (size_t)SC_MAX_BLOCKS * S->width * 8;
@@ -1594,7 +1799,7 @@ extern "C" lzx_status lzx_sumcheck_session_create(size_t num_vars, size_t nfacto
     S->half = num_vars ? N / 2 : 0;
     if (num_vars) { // round 0's evaluations over the caller's tables
         const uint32_t g = sc_grid(S->half);
-        S->K.eval(dim3(g), S->stream, S->tabs, S->P.T, S->half, ScEqArgs{}, S->partial);
+        S->K.eval(dim3(g), S->stream, S->tabs, S->P.T, S->half, ScEqArgs{}, S->partial, 0);
         hipError_t le = hipGetLastError();
         if (le == hipSuccess) st = session_reduce(S, g);
         else st = device_fail(le, "sc_eval_kernel");
@@ -1646,7 +1851,7 @@ extern "C" lzx_status lzx_sumcheck_session_fold(lzx_sumcheck_session *S, uint64_
         S->half = 0;
     } else {
         const uint32_t g = sc_grid(S->half / 2);
-        S->K.fold(dim3(g), S->stream, S->tabs, S->P.T, S->half, S->rdev, ScEqArgs{}, S->partial);
+        S->K.fold(dim3(g), S->stream, S->tabs, S->P.T, S->half, S->rdev, ScEqArgs{}, S->partial, 0);
         LZX_LAUNCH_CHECK("sc_fold_eval_kernel");
         for (size_t f = 0; f < S->P.nf; f++) S->tabs.src[f] = S->tabs.dst[f]; // later rounds fold in place
         S->half >>= 1;

@@ -78,6 +78,13 @@ SHAPES = [
     (9, 2, [(1, [0, 1])]),                                   # zerocheck shape
     (11, 5, [(7, [0, 0, 1]), (P - 2, [2, 3, 4, 1]), (1, [4])]),  # repeated factor, generic kernel
     (12, 1, [(1, [0])]),
+    # the compile-time shapes of the kernels, with fold rounds (num_vars >= 13)
+    (14, 4, [(1, [0, 1, 2]), (P - 1, [0, 3])]),             # eq*(a*b - c)
+    (14, 2, [(1, [0, 1])]),                                   # f*e
+    (3, 4, [(5, [0, 1]), (P - 9, [2, 3])]),                  # rho_0 f_0 e_0 + rho_1 f_1 e_1
+    (15, 4, [(12345678901, [0, 1]), (1, [3, 2])]),
+    (13, 4, [(1, [1, 2, 0]), (P - 1, [3, 0])]),              # eq*(a*b - c), factors in another order
+    (13, 4, [(1, [0, 1, 2]), (P - 1, [0, 2])]),              # -c's factor also in the other term: generic
 ]
 
 
