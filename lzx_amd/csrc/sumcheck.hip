@@ -261,10 +261,57 @@ __device__ __forceinline__ void eval_pair(const uint64_t (&lo)[NF], const uint64
     }
 }
 
-__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
+// ---- cross-lane sums on DPP (no LDS round trips) --------------------------
+// update_dpp with old = 0: a lane whose source is outside its row, or whose
+// row is masked off, reads 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = gl_add(v, __shfl_xor(v, o));
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+// One step of a lazy 64-bit sum with a carry count (true value x + c 2^64).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_lazy_step(uint64_t &x, uint32_t &c)
+{
+    const uint64_t o = (uint64_t)dpp_u32<CTRL, ROWS>((uint32_t)(x >> 32)) << 32 | dpp_u32<CTRL, ROWS>((uint32_t)x);
+    const uint32_t oc = dpp_u32<CTRL, ROWS>(c);
+    x += o;
+    c += oc + (x < o);
+}
+
+// Sum of v over the wave's 64 lanes, canonical in lane 63: an inclusive scan
+// within each 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row
+// 1 and row 2's into row 3 (row_bcast:15), then rows 0-1 into rows 2-3
+// (row_bcast:31).
+__device__ __forceinline__ uint64_t wave_sum_last(uint64_t v)
+{
+    uint32_t c = 0;
+    dpp_lazy_step<0x111, 0xF>(v, c);
+    dpp_lazy_step<0x112, 0xF>(v, c);
+    dpp_lazy_step<0x114, 0xF>(v, c);
+    dpp_lazy_step<0x118, 0xF>(v, c);
+    dpp_lazy_step<0x142, 0xA>(v, c);
+    dpp_lazy_step<0x143, 0xC>(v, c);
+    return gl_acc_finish(GlAcc{v, c});
+}
+
+// Canonical sum of the canonical values on each aligned group of G lanes
+// (G = 4, 8, 16), in every lane of the group: quad_perm [1,0,3,2] and
+// [2,3,0,1], then half-row and row mirrors (each lane then meets the other
+// half's sum).
+template <int G>
+__device__ __forceinline__ uint64_t group_sum(uint64_t v)
+{
+    auto step = [&](auto ctrl) {
+        constexpr int C = decltype(ctrl)::value;
+        const uint64_t o = (uint64_t)dpp_u32<C, 0xF>((uint32_t)(v >> 32)) << 32 | dpp_u32<C, 0xF>((uint32_t)v);
+        v = gl_add(v, o);
+    };
+    step(std::integral_constant<int, 0xB1>{});
+    step(std::integral_constant<int, 0x4E>{});
+    if constexpr (G >= 8) step(std::integral_constant<int, 0x141>{});
+    if constexpr (G >= 16) step(std::integral_constant<int, 0x140>{});
     return v;
 }
 
@@ -272,6 +319,8 @@ constexpr int SC_TAIL_BLOCK = 512;
 constexpr int SC_MAX_WAVES = SC_TAIL_BLOCK / 64;
 // The rounds whose fold has half <= SC_TAIL_HALF run in one single-block kernel.
 constexpr uint64_t SC_TAIL_HALF = 2048;
+// Folds with at most this many blocks close the previous round themselves.
+constexpr uint32_t SC_FUSE_BLOCKS = 256;
 
 struct ScShared {
     uint64_t wsum[SC_MAX_WAVES][SC_MAXD + 1];
@@ -296,8 +345,8 @@ __device__ __forceinline__ void block_totals(ACC (&acc)[D + 1], uint32_t dd, ScS
 #pragma unroll
     for (int t = 0; t <= D; t++)
         if ((uint32_t)t <= dd) {
-            const uint64_t v = wave_sum(acc_finish(acc[t]));
-            if (lane == 0) sh.wsum[wave][t] = v;
+            const uint64_t v = wave_sum_last(acc_finish(acc[t]));
+            if (lane == 63) sh.wsum[wave][t] = v;
         }
     __syncthreads();
     if (threadIdx.x <= dd) {
@@ -362,6 +411,7 @@ struct ScRoundArgs {
     uint32_t width;        // d + 1 values per round message
     uint32_t pw;           // values per block total: d + 1, or d with the split eq factor
     ScLagrange L;
+    uint8_t frame[52];     // the round message's framing: 22 bytes before the values, 30 after
 };
 
 // The transcript and running claim as wave 0 holds them: lane l < 50 keeps
@@ -370,11 +420,14 @@ struct TxRegs {
     uint32_t a;
     uint32_t pos, status;
     uint64_t qc, budget, current, eq_scale;
+    uint64_t lw, lx; // Lagrange weights of node `lane` (A.L.w, A.L.x), loaded up front
 };
 
-__device__ __forceinline__ TxRegs tx_load(const ScCtl *ctl, uint32_t lane)
+__device__ __forceinline__ TxRegs tx_load(const ScRoundArgs &A, const ScCtl *ctl, uint32_t lane)
 {
     TxRegs s;
+    s.lw = lane < A.width ? A.L.w[lane] : 0;
+    s.lx = lane < A.width - 1 ? A.L.x[lane] : 0;
     const uint64_t word = lane < 50 ? ctl->tx.st[lane >> 1] : 0;
     s.a = (uint32_t)(bit_unshuffle64(word) >> (32 * (lane & 1)));
     s.pos = ctl->tx.pos;
@@ -393,6 +446,7 @@ __device__ __forceinline__ void tx_store(ScCtl *ctl, const TxRegs &s, uint32_t l
     if (lane == 0) {
         ctl->tx.pos = s.pos;
         ctl->tx.query_count = s.qc;
+        ctl->tx.query_budget = s.budget;
         ctl->current = s.current;
         ctl->eq_scale = s.eq_scale;
         ctl->status = s.status;
@@ -402,26 +456,28 @@ __device__ __forceinline__ void tx_store(ScCtl *ctl, const TxRegs &s, uint32_t l
 // The round's transcript bytes in one buffer: append_field_slice framing
 // u32le|label| "sumcheck-round" u32le(8 (d+1)), the d+1 values (u64 LE, written
 // per round), then challenge_fields' framing "CHAL" u32le|label|
-// "sumcheck-challenge" u32le(1) (transcript.rs:70-120).
-__device__ __forceinline__ uint32_t tx_frame_byte(uint32_t i, uint32_t width)
+// "sumcheck-challenge" u32le(1) (transcript.rs:70-120).  The host lays out the
+// 22 + 30 framing bytes in A.frame (sc_frame_bytes).
+static void sc_frame_bytes(uint32_t width, uint8_t (&fr)[52])
 {
-    const uint32_t vend = 22 + 8 * width;
-    if (i < 4) return i == 0 ? 14 : 0;
-    if (i < 18) return (uint8_t)"sumcheck-round"[i - 4];
-    if (i < 22) return ((8 * width) >> (8 * (i - 18))) & 0xFF;
-    if (i < vend) return 0;
-    i -= vend;
-    if (i < 4) return (uint8_t)"CHAL"[i];
-    if (i < 8) return i == 4 ? 18 : 0;
-    if (i < 26) return (uint8_t)"sumcheck-challenge"[i - 8];
-    if (i < 30) return i == 26 ? 1 : 0;
-    return 0;
+    const char *l1 = "sumcheck-round", *l2 = "sumcheck-challenge";
+    uint32_t k = 0;
+    auto u32 = [&](uint32_t v) {
+        for (int i = 0; i < 4; i++) fr[k++] = (uint8_t)(v >> (8 * i));
+    };
+    u32(14);
+    for (int i = 0; i < 14; i++) fr[k++] = (uint8_t)l1[i];
+    u32(8 * width);
+    fr[k++] = 'C', fr[k++] = 'H', fr[k++] = 'A', fr[k++] = 'L';
+    u32(18);
+    for (int i = 0; i < 18; i++) fr[k++] = (uint8_t)l2[i];
+    u32(1);
 }
 
 // Wave 0: the constant framing bytes of sh.msg.
-__device__ __forceinline__ void tx_frame_init(ScShared &sh, uint32_t width, uint32_t lane)
+__device__ __forceinline__ void tx_frame_init(ScShared &sh, const ScRoundArgs &A, uint32_t lane)
 {
-    for (uint32_t i = lane; i < sizeof(sh.msg); i += 64) sh.msg[i] = (uint8_t)tx_frame_byte(i, width);
+    if (lane < 52) sh.msg[lane < 22 ? lane : 8 * A.width + lane] = A.frame[lane];
 }
 
 // XOR len bytes of msg into the sponge at its position (rate 136 = 17 words),
@@ -430,12 +486,15 @@ __device__ __forceinline__ void tx_absorb(TxRegs &s, const uint8_t *msg, uint32_
 {
     for (uint32_t off = 0; off < len;) {
         const uint32_t take = min(136u - s.pos, len - off);
-        if (lane < 34) {
+        if (lane < 34) { // sponge word lane / 2 takes bytes [p0, p1) of the rate
+            const int32_t w0 = 8 * (int32_t)(lane >> 1), p0 = (int32_t)s.pos, p1 = p0 + (int32_t)take;
+            const int32_t k = (int32_t)off + w0 - p0; // message index of the word's byte 0
             uint64_t xw = 0;
 #pragma unroll
-            for (uint32_t b = 0; b < 8; b++) {
-                const uint32_t g = 8 * (lane >> 1) + b;
-                if (g >= s.pos && g < s.pos + take) xw |= (uint64_t)msg[off + g - s.pos] << (8 * b);
+            for (int b = 0; b < 8; b++) {
+                const bool in = w0 + b >= p0 && w0 + b < p1;
+                const uint32_t byte = msg[in ? k + b : 0];
+                xw |= (uint64_t)(in ? byte : 0) << (8 * b);
             }
             s.a ^= (uint32_t)(bit_unshuffle64(xw) >> (32 * (lane & 1)));
         }
@@ -452,7 +511,7 @@ __device__ __forceinline__ void tx_absorb(TxRegs &s, const uint8_t *msg, uint32_
 // claim).  sh.msg holds the framing bytes (tx_frame_init).
 template <int D>
 __device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t rnd, uint64_t given, uint64_t omw,
-                         uint64_t tw)
+                         uint64_t tw, bool write)
 {
     const uint32_t lane = threadIdx.x & 63, width = A.width;
     if (s.status) return; // an earlier round failed
@@ -463,9 +522,8 @@ __device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t
         // (degree < d) and scale, g(t) = c_t S(t)
         const uint32_t d = width - 1;
         uint64_t sv = lane < d ? sh.ev[lane] : 0;
-        uint64_t part = lane < d ? gl_mul(sv, A.L.x[lane]) : 0;
-#pragma unroll
-        for (int o = 1; o < LANES; o <<= 1) part = gl_add(part, __shfl_xor(part, o));
+        uint64_t part = lane < d ? gl_mul(sv, s.lx) : 0;
+        part = group_sum<LANES>(part);
         const uint64_t sd = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(part >> 32)) << 32) |
                             (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)part);
         if (lane == d) sv = sd;
@@ -476,10 +534,10 @@ __device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
     }
-    if (lane < width) A.rounds[(size_t)rnd * width + lane] = sh.ev[lane];
+    if (write && lane < width) A.rounds[(size_t)rnd * width + lane] = sh.ev[lane];
     // g(ch) = sum_i ev_i w_i prod_{j != i} (ch - j) (sumcheck.rs:286-306): node i on lane i;
     // ev_i w_i does not depend on the challenge
-    const uint64_t evw = lane < width ? gl_mul(sh.ev[lane], A.L.w[lane]) : 0;
+    const uint64_t evw = lane < width ? gl_mul(sh.ev[lane], s.lw) : 0;
     uint64_t ch = given;
     if (!A.given) {
         for (uint32_t i = lane; i < 8 * width; i += 64) sh.msg[22 + i] = (uint8_t)(sh.ev[i >> 3] >> (8 * (i & 7)));
@@ -516,7 +574,7 @@ __device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t
             return;
         }
     }
-    if (lane == 0) A.chal[rnd] = ch;
+    if (write && lane == 0) A.chal[rnd] = ch;
     sh.chal = ch;
     if (A.eq_c) s.eq_scale = gl_mul(s.eq_scale, gl_add(omw, gl_mul(tw, ch))); // eq(w_j, r_j)
     if (gl_add(sh.ev[0], sh.ev[1]) != s.current) {
@@ -536,10 +594,58 @@ __device__ void tx_round(TxRegs &s, ScShared &sh, const ScRoundArgs &A, uint32_t
 #pragma unroll
         for (int k = 0; k < n; k++) x[k] = gl_mul(x[k], x[k + n]);
     uint64_t term = lane < width ? x[0] : 0;
-#pragma unroll
-    for (int o = 1; o < LEAVES; o <<= 1) term = gl_add(term, __shfl_xor(term, o)); // nodes sit on lanes < LEAVES
+    term = group_sum<LEAVES>(term); // nodes sit on lanes < LEAVES
     s.current = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(term >> 32)) << 32) |
                 (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)term); // lane 0's group holds the sum
+}
+
+// The transcript state after round j lives in A.ctl[j & 1] (the initial state
+// in A.ctl[1]), and round j's block totals in the partial buffer j & 1, so a
+// kernel that closes one round and publishes the next never overwrites what
+// its own blocks still read.
+//
+// Close round rnd in this block from the nblocks block totals at part: sum
+// them, then wave 0 runs the transcript step.  With write set the block
+// stores the round's values, challenge and state; every block that calls it
+// ends with the same challenge (returned) and status (sh.status).
+template <int D, int BLOCK>
+__device__ __forceinline__ uint64_t close_in_block(const uint64_t *__restrict__ part, uint32_t nblocks,
+                                                   const ScRoundArgs &A, uint32_t rnd, ScShared &sh, TxRegs &s,
+                                                   bool write)
+{
+    const uint32_t pw = A.pw;
+    uint64_t given = 0, omw = 0, tw = 0;
+    if (threadIdx.x < 64) { // issued before the partial loads: one round trip for all
+        s = tx_load(A, A.ctl + ((rnd + 1) & 1), threadIdx.x);
+        if (A.given) given = A.given[rnd];
+        if (A.eq_c) omw = A.eq_c[2 * rnd], tw = A.eq_c[2 * rnd + 1];
+        tx_frame_init(sh, A, threadIdx.x);
+    }
+    GlAcc acc[D + 1];
+#pragma unroll
+    for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
+    constexpr int U = 4; // loads in flight per thread and pass
+    for (uint32_t b0 = threadIdx.x; b0 < nblocks; b0 += U * BLOCK) {
+        uint64_t v[U][D + 1];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t b = b0 + u * BLOCK;
+#pragma unroll
+            for (int t = 0; t <= D; t++) v[u][t] = b < nblocks && (uint32_t)t < pw ? part[(size_t)b * pw + t] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int t = 0; t <= D; t++) gl_acc_add(acc[t], v[u][t]);
+    }
+    block_totals<D, BLOCK / 64>(acc, pw - 1, sh, sh.ev);
+    if (threadIdx.x < 64) {
+        tx_round<D>(s, sh, A, rnd, given, omw, tw, write);
+        if (write) tx_store(A.ctl + (rnd & 1), s, threadIdx.x);
+        if (threadIdx.x == 0) sh.status = s.status;
+    }
+    __syncthreads();
+    return sh.chal;
 }
 
 // One block: sum the eval / fold kernel's block totals and close round rnd.
@@ -548,36 +654,8 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_close_kernel(const uint64_t *__re
                                                             ScRoundArgs A, uint32_t rnd)
 {
     __shared__ ScShared sh;
-    const uint32_t pw = A.pw;
     TxRegs s{};
-    uint64_t given = 0, omw = 0, tw = 0;
-    if (threadIdx.x < 64) { // issued before the partial loads: one round trip for all
-        s = tx_load(A.ctl, threadIdx.x);
-        if (A.given) given = A.given[rnd];
-        if (A.eq_c) omw = A.eq_c[2 * rnd], tw = A.eq_c[2 * rnd + 1];
-        tx_frame_init(sh, A.width, threadIdx.x);
-    }
-    GlAcc acc[D + 1];
-#pragma unroll
-    for (int t = 0; t <= D; t++) acc[t] = GlAcc{0, 0};
-    constexpr int U = SC_MAX_BLOCKS / SC_BLOCK; // every load issued before the first add
-    for (uint32_t b0 = threadIdx.x; b0 < nblocks; b0 += U * SC_BLOCK) {
-        uint64_t v[U][D + 1];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint32_t b = b0 + u * SC_BLOCK;
-#pragma unroll
-            for (int t = 0; t <= D; t++) v[u][t] = b < nblocks && (uint32_t)t < pw ? partial[(size_t)b * pw + t] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int t = 0; t <= D; t++) gl_acc_add(acc[t], v[u][t]);
-    }
-    block_totals<D, SC_BLOCK / 64>(acc, pw - 1, sh, sh.ev);
-    if (threadIdx.x >= 64) return;
-    tx_round<D>(s, sh, A, rnd, given, omw, tw);
-    tx_store(A.ctl, s, threadIdx.x);
+    close_in_block<D, SC_BLOCK>(partial, nblocks, A, rnd, sh, s, true);
 }
 
 // Round 0: evaluate g_0 (or S_0 with the split eq factor) over the caller's tables.
@@ -615,6 +693,46 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_eval_kernel(ScTables tabs, ScTerm
 // from the second round on) and accumulate the next round's values (E: the
 // next round's split eq tables).  A thread reads and writes only its own
 // pairs' entries, so the visiting order is free.
+template <int NF, class SH>
+__device__ __forceinline__ void fold_load(const ScTables &tabs, const ScTerms &T, uint64_t half, uint64_t p,
+                                          uint64_t (&v)[NF][4])
+{
+    const uint64_t quarter = half >> 1;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        if (factor_live<SH>(f, T)) {
+            const uint64_t *s = tabs.src[f];
+            v[f][0] = s[p];
+            v[f][1] = s[p + half];
+            v[f][2] = s[p + quarter];
+            v[f][3] = s[p + half + quarter];
+        } else {
+            v[f][0] = v[f][1] = v[f][2] = v[f][3] = 0;
+        }
+    }
+}
+
+template <int NF, int D, class SH>
+__device__ __forceinline__ void fold_eval_pair(const ScTables &tabs, const ScTerms &T, uint64_t half, uint64_t r,
+                                               const ScEqArgs &E, uint64_t p, const uint64_t (&v)[NF][4],
+                                               GlAcc2 (&acc)[D + 1])
+{
+    const uint64_t quarter = half >> 1;
+    uint64_t n0[NF], n1[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        if (factor_live<SH>(f, T)) {
+            n0[f] = gl_fold(v[f][0], v[f][1], r);
+            n1[f] = gl_fold(v[f][2], v[f][3], r);
+            tabs.dst[f][p] = n0[f];
+            tabs.dst[f][p + quarter] = n1[f];
+        } else {
+            n0[f] = n1[f] = 0;
+        }
+    }
+    eval_pair<NF, D, SH>(n0, n1, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
+}
+
 template <int NF, int D, class SH>
 __device__ __forceinline__ void fold_eval_range(const ScTables &tabs, const ScTerms &T, uint64_t half, uint64_t r,
                                                 const ScEqArgs &E, uint64_t i0, uint64_t stride, uint32_t rev,
@@ -623,21 +741,9 @@ __device__ __forceinline__ void fold_eval_range(const ScTables &tabs, const ScTe
     const uint64_t quarter = half >> 1;
     for (uint64_t i = i0; i < quarter; i += stride) {
         const uint64_t p = sc_pair_index(i, quarter, rev);
-        uint64_t n0[NF], n1[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            if (factor_live<SH>(f, T)) {
-                const uint64_t *s = tabs.src[f];
-                const uint64_t a0 = s[p], b0 = s[p + half], a1 = s[p + quarter], b1 = s[p + half + quarter];
-                n0[f] = gl_fold(a0, b0, r);
-                n1[f] = gl_fold(a1, b1, r);
-                tabs.dst[f][p] = n0[f];
-                tabs.dst[f][p + quarter] = n1[f];
-            } else {
-                n0[f] = n1[f] = 0;
-            }
-        }
-        eval_pair<NF, D, SH>(n0, n1, T, T.eq_f >= 0 ? eq_split_at(E, p) : 1, acc);
+        uint64_t v[NF][4];
+        fold_load<NF, SH>(tabs, T, half, p, v);
+        fold_eval_pair<NF, D, SH>(tabs, T, half, r, E, p, v, acc);
     }
 }
 
@@ -657,24 +763,48 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_fold_eval_kernel(ScTables tabs, S
     publish_totals<D>(acc, T.npts - 1, sh, partial);
 }
 
-// Rounds r0 .. num_vars-1 in one block: fold with the challenge of round r,
-// sum, close round r+1; the last fold leaves one entry per table (the factor
-// claims, sumcheck.rs:139-143).  tabs.src is the input of round r0's fold.
+// Close round rnd from the previous kernel's nprev block totals (every block
+// redundantly, block 0 writing), then fold with its challenge and evaluate
+// round rnd+1.  For small grids: each thread's first pairs are loaded before
+// the close, so their latency hides behind the transcript step.
+template <int NF, int D, class SH>
+__global__ __launch_bounds__(SC_BLOCK) void sc_fold_close_kernel(ScTables tabs, ScTerms T, uint64_t half, ScEqArgs E,
+                                                                 const uint64_t *__restrict__ part_in, uint32_t nprev,
+                                                                 uint64_t *__restrict__ part_out, ScRoundArgs A,
+                                                                 uint32_t rnd, uint32_t rev)
+{
+    __shared__ ScShared sh;
+    const uint64_t quarter = half >> 1, stride = (uint64_t)gridDim.x * SC_BLOCK;
+    uint64_t i = (uint64_t)blockIdx.x * SC_BLOCK + threadIdx.x;
+    uint64_t v[NF][4];
+    if (i < quarter) fold_load<NF, SH>(tabs, T, half, sc_pair_index(i, quarter, rev), v);
+    TxRegs s{};
+    const uint64_t r = close_in_block<D, SC_BLOCK>(part_in, nprev, A, rnd, sh, s, blockIdx.x == 0);
+    GlAcc2 acc[D + 1];
+#pragma unroll
+    for (int t = 0; t <= D; t++) acc[t] = GlAcc2{0, 0, 0};
+    for (bool first = true; i < quarter; i += stride, first = false) {
+        const uint64_t p = sc_pair_index(i, quarter, rev);
+        if (!first) fold_load<NF, SH>(tabs, T, half, p, v);
+        fold_eval_pair<NF, D, SH>(tabs, T, half, r, E, p, v, acc);
+    }
+    publish_totals<D>(acc, T.npts - 1, sh, part_out);
+}
+
+// Rounds r0 .. num_vars-1 in one block: close round r0 from the previous
+// kernel's nprev block totals, then round after round fold with the latest
+// challenge, sum, close the next round; the last fold leaves one entry per
+// table (the factor claims, sumcheck.rs:139-143).  tabs.src is the input of
+// round r0's fold.  The state after the last round goes to A.ctl[(num_vars - 1) & 1].
 template <int NF, int D, class SH>
 __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, ScTerms T, uint64_t half0, uint32_t r0,
-                                                                uint32_t num_vars, ScRoundArgs A)
+                                                                uint32_t num_vars, ScRoundArgs A,
+                                                                const uint64_t *__restrict__ part, uint32_t nprev)
 {
     __shared__ ScShared sh;
     const uint32_t lane = threadIdx.x & 63;
-    TxRegs s{};
     if (threadIdx.x < 64) {
-        s = tx_load(A.ctl, lane);
-        tx_frame_init(sh, A.width, lane);
-        if (lane == 0) {
-            sh.chal = A.chal[r0];
-            sh.status = s.status;
-        }
-        for (uint32_t i = lane; r0 + 1 + i < num_vars && i < 64; i += 64) {
+        for (uint32_t i = lane; r0 + 1 + i < num_vars && i < 64; i += 64) { // slot i: round r0 + 1 + i
             if (A.given) sh.given[i] = A.given[r0 + 1 + i];
             if (A.eq_c) {
                 sh.eqc[i][0] = A.eq_c[2 * (r0 + 1 + i)];
@@ -683,7 +813,8 @@ __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, S
             }
         }
     }
-    __syncthreads();
+    TxRegs s{};
+    close_in_block<D, SC_TAIL_BLOCK>(part, nprev, A, r0, sh, s, true); // its barriers order sh's slots
     uint64_t half = half0;
     for (uint32_t r = r0; r < num_vars && !sh.status; r++, half >>= 1) {
         const uint64_t ch = sh.chal;
@@ -712,12 +843,12 @@ __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, S
         block_totals<D, SC_MAX_WAVES>(acc, T.npts - 1, sh, sh.ev); // its barriers also order the folded tables
         if (threadIdx.x < 64) {
             tx_round<D>(s, sh, A, r + 1, A.given ? sh.given[k] : 0, A.eq_c ? sh.eqc[k][0] : 0,
-                        A.eq_c ? sh.eqc[k][1] : 0);
+                        A.eq_c ? sh.eqc[k][1] : 0, true);
             if (lane == 0) sh.status = s.status;
         }
         __syncthreads();
     }
-    if (threadIdx.x < 64) tx_store(A.ctl, s, lane);
+    if (threadIdx.x < 64 && num_vars > r0 + 1) tx_store(A.ctl + ((num_vars - 1) & 1), s, lane);
 }
 
 // Sums the block partials into the d+1 values of a round (sum_over_hypercube).
@@ -846,7 +977,10 @@ struct ScKernels {
     void (*fold)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const uint64_t *, const ScEqArgs &,
                  uint64_t *, uint32_t);
     void (*close)(hipStream_t, const uint64_t *, uint32_t, const ScRoundArgs &, uint32_t);
-    void (*tail)(hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint32_t, uint32_t, const ScRoundArgs &);
+    void (*fold_close)(dim3, hipStream_t, const ScTables &, const ScTerms &, uint64_t, const ScEqArgs &,
+                       const uint64_t *, uint32_t, uint64_t *, const ScRoundArgs &, uint32_t, uint32_t);
+    void (*tail)(hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint32_t, uint32_t, const ScRoundArgs &,
+                 const uint64_t *, uint32_t);
 };
 
 template <int NF, int D, class SH>
@@ -870,16 +1004,27 @@ static void launch_close(hipStream_t s, const uint64_t *part, uint32_t nblocks, 
 }
 
 template <int NF, int D, class SH>
-static void launch_tail(hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half0, uint32_t r0,
-                        uint32_t num_vars, const ScRoundArgs &A)
+static void launch_fold_close(dim3 g, hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half,
+                              const ScEqArgs &E, const uint64_t *part_in, uint32_t nprev, uint64_t *part_out,
+                              const ScRoundArgs &A, uint32_t rnd, uint32_t rev)
 {
-    hipLaunchKernelGGL((sc_tail_kernel<NF, D, SH>), dim3(1), dim3(SC_TAIL_BLOCK), 0, s, tb, T, half0, r0, num_vars, A);
+    hipLaunchKernelGGL((sc_fold_close_kernel<NF, D, SH>), g, dim3(SC_BLOCK), 0, s, tb, T, half, E, part_in, nprev,
+                       part_out, A, rnd, rev);
+}
+
+template <int NF, int D, class SH>
+static void launch_tail(hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half0, uint32_t r0,
+                        uint32_t num_vars, const ScRoundArgs &A, const uint64_t *part, uint32_t nprev)
+{
+    hipLaunchKernelGGL((sc_tail_kernel<NF, D, SH>), dim3(1), dim3(SC_TAIL_BLOCK), 0, s, tb, T, half0, r0, num_vars,
+                       A, part, nprev);
 }
 
 template <int NF, int D, class SH = ScGenericShape>
 static ScKernels kernels_for()
 {
-    return ScKernels{launch_eval<NF, D, SH>, launch_fold<NF, D, SH>, launch_close<D>, launch_tail<NF, D, SH>};
+    return ScKernels{launch_eval<NF, D, SH>, launch_fold<NF, D, SH>, launch_close<D>, launch_fold_close<NF, D, SH>,
+                     launch_tail<NF, D, SH>};
 }
 
 // The compile-time shape P.T has, when it has one of the registered ones.
@@ -981,6 +1126,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
 
     ScRoundArgs A{};
     A.width = (uint32_t)width;
+    sc_frame_bytes(A.width, A.frame);
     for (size_t i = 0; i < width; i++) { // Lagrange weights for nodes 0..d
         uint64_t den = 1;
         for (size_t j = 0; j < width; j++)
@@ -1042,8 +1188,8 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     // eq_c[m][2] | eq_rounds[m] | eq arena (setup only)
     const size_t mm = std::max<size_t>(1, m), nff = std::max<size_t>(1, nf);
     const uint64_t halfN = m ? N / 2 : 1;
-    const size_t tab_bytes = nff * halfN * sizeof(uint64_t), part_bytes = SC_MAX_BLOCKS * width * sizeof(uint64_t);
-    const size_t o_rounds = sizeof(ScCtl), o_chal = o_rounds + mm * width * 8, o_given = o_chal + mm * 8,
+    const size_t tab_bytes = nff * halfN * sizeof(uint64_t), part_bytes = 2 * SC_MAX_BLOCKS * width * sizeof(uint64_t);
+    const size_t o_rounds = 2 * sizeof(ScCtl), o_chal = o_rounds + mm * width * 8, o_given = o_chal + mm * 8,
                  o_claims = o_given + mm * 8, area_rb = o_claims + nff * 8;
     const size_t o_eqc = (area_rb + 15) & ~(size_t)15, o_eqr = o_eqc + eqc.size() * 8,
                  o_arena = o_eqr + (split ? m * sizeof(ScEqArgs) : 0), area = o_arena + arena.size() * 8;
@@ -1057,7 +1203,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     uint8_t *h = (uint8_t *)hs.p, *dctl = dev + tab_bytes + part_bytes;
     if (st == LZX_OK) {
         memset(h, 0, area);
-        ScCtl &c = *(ScCtl *)h;
+        ScCtl &c = ((ScCtl *)h)[1]; // the state before round 0
         if (tr) {
             for (int i = 0; i < 25; i++) c.tx.st[i] = tr->sponge.st[i];
             c.tx.pos = tr->sponge.pos;
@@ -1102,24 +1248,31 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
         if (le != hipSuccess && st == LZX_OK) st = device_fail(le, what);
     };
     if (st == LZX_OK && m > 0) {
-        const uint32_t g0 = sc_grid(N / 2);
-        // consecutive kernels visit their pairs in opposite orders
-        K.eval(dim3(g0), stream, tabs, P.T, N / 2, eq_args(0), partial, 0);
+        // round j's block totals go to part[j & 1]; consecutive large kernels
+        // visit their pairs in opposite orders
+        uint64_t *part[2] = {partial, partial + (size_t)SC_MAX_BLOCKS * width};
+        uint32_t gprev = sc_grid(N / 2);
+        K.eval(dim3(gprev), stream, tabs, P.T, N / 2, eq_args(0), part[0], 0);
         check("sc_eval_kernel");
-        K.close(stream, partial, g0, A, 0);
-        check("sc_close_kernel");
         size_t r = 0;
-        for (; st == LZX_OK && (N >> (r + 1)) > SC_TAIL_HALF; r++) { // large rounds: a grid per fold
+        for (; st == LZX_OK && (N >> (r + 1)) > SC_TAIL_HALF; r++) { // a grid per fold
             const uint64_t half = N >> (r + 1);
             const uint32_t g = sc_grid(half / 2);
-            K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, eq_args(r + 1), partial, r % 2 == 0);
-            check("sc_fold_eval_kernel");
-            K.close(stream, partial, g, A, (uint32_t)r + 1);
-            check("sc_close_kernel");
+            if (g > SC_FUSE_BLOCKS) { // a one-block kernel closes round r, then the fold reads its challenge
+                K.close(stream, part[r & 1], gprev, A, (uint32_t)r);
+                check("sc_close_kernel");
+                K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, eq_args(r + 1), part[(r + 1) & 1], r % 2 == 0);
+                check("sc_fold_eval_kernel");
+            } else { // every block closes round r itself
+                K.fold_close(dim3(g), stream, tabs, P.T, half, eq_args(r + 1), part[r & 1], gprev, part[(r + 1) & 1],
+                             A, (uint32_t)r, r % 2 == 0);
+                check("sc_fold_close_kernel");
+            }
+            gprev = g;
             for (size_t f = 0; f < nf; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
         }
         if (st == LZX_OK) {
-            K.tail(stream, tabs, P.T, N >> (r + 1), (uint32_t)r, (uint32_t)m, A);
+            K.tail(stream, tabs, P.T, N >> (r + 1), (uint32_t)r, (uint32_t)m, A, part[r & 1], gprev);
             check("sc_tail_kernel");
         }
     }
@@ -1136,7 +1289,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
         enqueued = false;
     }
     if (st == LZX_OK) {
-        const ScCtl &c = *(const ScCtl *)h;
+        const ScCtl &c = ((const ScCtl *)h)[m ? (m - 1) & 1 : 1]; // the state after the last round
         if (tr) { // the caller's transcript advances exactly as the reference's would
             for (int i = 0; i < 25; i++) tr->sponge.st[i] = c.tx.st[i];
             tr->sponge.pos = c.tx.pos;
