@@ -29,6 +29,8 @@
 // opposite orders, so each starts on the entries its predecessor touched last.
 #include <algorithm>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1114,6 +1116,137 @@ static void host_eq_append(const uint64_t *w, size_t a, size_t b, std::vector<ui
     out.insert(out.end(), t.begin(), t.end());
 }
 
+// ---- whole-prove graphs -------------------------------------------------------
+// A prove's launches depend only on its sizes, term structure, factor tables
+// and workspace.  From the second call with the same ones, the whole
+// sequence (upload, round kernels, readback) is captured once into a hipGraph
+// on a workspace and pinned stage owned by a cache entry; later calls fill the
+// stage and launch the graph.  At most SC_GRAPH_MAX keys are remembered and
+// SC_GRAPH_BYTES of workspace held, least recently used entries going first.
+// A call holds its entry (pinned against eviction, its lock taken) from the
+// stage fill to the readback, so callers that share an entry take turns.
+// LZX_SC_GRAPH=0 turns the cache off.
+struct ScGraphKey {
+    std::vector<uint8_t> b;
+    template <class T> ScGraphKey &add(const T &v)
+    {
+        const uint8_t *p = (const uint8_t *)&v;
+        b.insert(b.end(), p, p + sizeof(T));
+        return *this;
+    }
+};
+
+struct ScGraph {
+    std::vector<uint8_t> key;
+    uint32_t calls = 0, pins = 0;
+    uint64_t last = 0;
+    uint8_t *dev = nullptr;
+    void *stage = nullptr;
+    size_t bytes = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::mutex mu;
+    void release_memory()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        if (dev) (void)hipFree(dev);
+        if (stage) (void)hipHostFree(stage);
+        exec = nullptr, graph = nullptr, cap = nullptr, dev = nullptr, stage = nullptr, bytes = 0;
+    }
+    ~ScGraph() { release_memory(); }
+};
+
+constexpr size_t SC_GRAPH_MAX = 8;
+constexpr size_t SC_GRAPH_BYTES = 8ull << 30;
+
+static bool sc_graph_enabled()
+{
+    const char *e = getenv("LZX_SC_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+static std::mutex g_sc_graph_mu;
+static std::vector<std::unique_ptr<ScGraph>> g_sc_graphs;
+static uint64_t g_sc_graph_clock = 0;
+
+// Drops the least recently used entry that no call holds (its memory too);
+// false when every entry is held.  g_sc_graph_mu is held.
+static bool sc_graph_evict_one()
+{
+    size_t best = SIZE_MAX;
+    for (size_t i = 0; i < g_sc_graphs.size(); i++)
+        if (g_sc_graphs[i]->pins == 0 && (best == SIZE_MAX || g_sc_graphs[i]->last < g_sc_graphs[best]->last))
+            best = i;
+    if (best == SIZE_MAX) return false;
+    g_sc_graphs.erase(g_sc_graphs.begin() + best);
+    return true;
+}
+
+// Releases a held entry at the end of a call.
+struct ScGraphHold {
+    ScGraph *g = nullptr;
+    ~ScGraphHold()
+    {
+        if (!g) return;
+        g->mu.unlock();
+        std::lock_guard<std::mutex> lk(g_sc_graph_mu);
+        g->pins--;
+    }
+};
+
+// The entry for key, held, with its workspace of dev_bytes and stage of
+// area bytes allocated; none (hold.g == nullptr, run directly) for a key's
+// first call or when the cache cannot make room.
+static lzx_status sc_graph_get(const ScGraphKey &key, size_t dev_bytes, size_t area, ScGraphHold &hold)
+{
+    std::unique_lock<std::mutex> lk(g_sc_graph_mu);
+    ScGraph *g = nullptr;
+    for (auto &e : g_sc_graphs)
+        if (e->key == key.b) g = e.get();
+    if (!g) {
+        if (g_sc_graphs.size() >= SC_GRAPH_MAX && !sc_graph_evict_one()) return LZX_OK;
+        auto e = std::make_unique<ScGraph>();
+        e->key = key.b;
+        e->calls = 1;
+        e->last = ++g_sc_graph_clock;
+        g_sc_graphs.push_back(std::move(e));
+        return LZX_OK;
+    }
+    g->last = ++g_sc_graph_clock;
+    g->calls++;
+    if (!g->dev) {
+        size_t held = 0;
+        for (auto &e : g_sc_graphs) held += e->bytes;
+        while (held + dev_bytes > SC_GRAPH_BYTES) {
+            size_t victim = SIZE_MAX; // drop the memory of the least recently used idle entry
+            for (size_t i = 0; i < g_sc_graphs.size(); i++) {
+                ScGraph *v = g_sc_graphs[i].get();
+                if (v != g && v->pins == 0 && v->dev && (victim == SIZE_MAX || v->last < g_sc_graphs[victim]->last))
+                    victim = i;
+            }
+            if (victim == SIZE_MAX) return LZX_OK;
+            held -= g_sc_graphs[victim]->bytes;
+            g_sc_graphs[victim]->release_memory();
+        }
+        if (hipMalloc((void **)&g->dev, dev_bytes) != hipSuccess ||
+            hipHostMalloc(&g->stage, area, hipHostMallocDefault) != hipSuccess ||
+            hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError(); // out of memory here: the call runs directly
+            g->release_memory();
+            return LZX_OK;
+        }
+        g->bytes = dev_bytes;
+    }
+    g->pins++;
+    lk.unlock();
+    g->mu.lock();
+    hold.g = g;
+    return LZX_OK;
+}
+
 // eq_w: the split eq factor's point (P.T.eq_f >= 0), else nullptr.
 static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors, uint64_t claim, lzx_transcript *tr,
                              const uint64_t *given, const uint64_t *eq_w, uint64_t *rounds_out,
@@ -1124,9 +1257,9 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
     const uint64_t N = 1ULL << m;
     const ScKernels K = pick_kernels(P);
 
-    ScRoundArgs A{};
-    A.width = (uint32_t)width;
-    sc_frame_bytes(A.width, A.frame);
+    ScRoundArgs A0{}; // the pointer fields are set per workspace (enqueue)
+    A0.width = (uint32_t)width;
+    sc_frame_bytes(A0.width, A0.frame);
     for (size_t i = 0; i < width; i++) { // Lagrange weights for nodes 0..d
         uint64_t den = 1;
         for (size_t j = 0; j < width; j++)
@@ -1137,10 +1270,10 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
             base = gl_mul(base, base);
             e >>= 1;
         }
-        A.L.w[i] = inv;
+        A0.L.w[i] = inv;
     }
     const bool split = P.T.eq_f >= 0;
-    A.pw = split ? (uint32_t)d : (uint32_t)width;
+    A0.pw = split ? (uint32_t)d : (uint32_t)width;
     // split eq: per-round (1 - w_j, 2 w_j - 1), round j's E tables (pairs over
     // variables j+1..m-1: hi over all but the last lb, lo over the last lb)
     // and S(d) = sum_{i<d} x_i S(i) for the degree-(d-1) sum S
@@ -1160,7 +1293,7 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
                 base = gl_mul(base, base);
                 e >>= 1;
             }
-            A.L.x[i] = gl_mul(num, inv);
+            A0.L.x[i] = gl_mul(num, inv);
         }
         const size_t LB = m / 2;
         arena.push_back(1); // the empty hi table
@@ -1193,15 +1326,12 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
                  o_claims = o_given + mm * 8, area_rb = o_claims + nff * 8;
     const size_t o_eqc = (area_rb + 15) & ~(size_t)15, o_eqr = o_eqc + eqc.size() * 8,
                  o_arena = o_eqr + (split ? m * sizeof(ScEqArgs) : 0), area = o_arena + arena.size() * 8;
-    uint8_t *dev = nullptr;
-    HostStage hs;
-    bool enqueued = false;
-    lzx_status st = LZX_OK;
-    hipError_t e = scratch_alloc((void **)&dev, tab_bytes + part_bytes + area, stream);
-    if (e == hipSuccess) e = stage_acquire(hs, area);
-    if (e != hipSuccess) st = device_fail(e, "sumcheck scratch");
-    uint8_t *h = (uint8_t *)hs.p, *dctl = dev + tab_bytes + part_bytes;
-    if (st == LZX_OK) {
+    const size_t dev_bytes = tab_bytes + part_bytes + area;
+
+    // The staged control area for a workspace at dev: initial state (in
+    // A.ctl[1]), given challenges, split eq data.
+    auto fill_stage = [&](uint8_t *h, uint8_t *dev) {
+        uint8_t *dctl = dev + tab_bytes + part_bytes;
         memset(h, 0, area);
         ScCtl &c = ((ScCtl *)h)[1]; // the state before round 0
         if (tr) {
@@ -1220,75 +1350,69 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
             for (size_t j = 0; j < m; j++) er[j] = ScEqArgs{ar + hi_off[j], ar + lo_off[j], (uint32_t)lbits[j], 0};
             memcpy(h + o_arena, arena.data(), arena.size() * 8);
         }
-        e = hipMemcpyAsync(dctl, h, area, hipMemcpyHostToDevice, stream);
-        enqueued = true;
-        if (e != hipSuccess) st = device_fail(e, "sumcheck setup");
-    }
-    uint64_t *partial = (uint64_t *)(dev + tab_bytes);
-    A.ctl = (ScCtl *)dctl;
-    A.rounds = (uint64_t *)(dctl + o_rounds);
-    A.chal = (uint64_t *)(dctl + o_chal);
-    A.given = given ? (const uint64_t *)(dctl + o_given) : nullptr;
-    A.claims = (uint64_t *)(dctl + o_claims);
-    A.eq_c = split ? (const uint64_t *)(dctl + o_eqc) : nullptr;
-    A.eq_rounds = split ? (const ScEqArgs *)(dctl + o_eqr) : nullptr;
-    auto eq_args = [&](size_t j) {
-        if (!split) return ScEqArgs{};
-        const uint64_t *ar = (const uint64_t *)(dctl + o_arena);
-        return ScEqArgs{ar + hi_off[j], ar + lo_off[j], (uint32_t)lbits[j], 0};
     };
-
-    ScTables tabs{};
-    for (size_t f = 0; f < nf; f++) {
-        tabs.src[f] = factors[f];
-        tabs.dst[f] = (uint64_t *)dev + f * halfN;
-    }
-    auto check = [&](const char *what) {
-        const hipError_t le = hipGetLastError();
-        if (le != hipSuccess && st == LZX_OK) st = device_fail(le, what);
-    };
-    if (st == LZX_OK && m > 0) {
-        // round j's block totals go to part[j & 1]; consecutive large kernels
-        // visit their pairs in opposite orders
-        uint64_t *part[2] = {partial, partial + (size_t)SC_MAX_BLOCKS * width};
-        uint32_t gprev = sc_grid(N / 2);
-        K.eval(dim3(gprev), stream, tabs, P.T, N / 2, eq_args(0), part[0], 0);
-        check("sc_eval_kernel");
-        size_t r = 0;
-        for (; st == LZX_OK && (N >> (r + 1)) > SC_TAIL_HALF; r++) { // a grid per fold
-            const uint64_t half = N >> (r + 1);
-            const uint32_t g = sc_grid(half / 2);
-            if (g > SC_FUSE_BLOCKS) { // a one-block kernel closes round r, then the fold reads its challenge
-                K.close(stream, part[r & 1], gprev, A, (uint32_t)r);
-                check("sc_close_kernel");
-                K.fold(dim3(g), stream, tabs, P.T, half, A.chal + r, eq_args(r + 1), part[(r + 1) & 1], r % 2 == 0);
-                check("sc_fold_eval_kernel");
-            } else { // every block closes round r itself
-                K.fold_close(dim3(g), stream, tabs, P.T, half, eq_args(r + 1), part[r & 1], gprev, part[(r + 1) & 1],
-                             A, (uint32_t)r, r % 2 == 0);
-                check("sc_fold_close_kernel");
+    // Every step of a prove on the workspace at dev with stage h: the upload,
+    // the round kernels, the readback.  Nothing in it depends on a call's
+    // values (only on the sizes, tables and pointers), so it can be captured.
+    auto enqueue = [&](uint8_t *dev, uint8_t *h, hipStream_t s) -> lzx_status {
+        uint8_t *dctl = dev + tab_bytes + part_bytes;
+        LZX_HIP(hipMemcpyAsync(dctl, h, area, hipMemcpyHostToDevice, s));
+        uint64_t *partial = (uint64_t *)(dev + tab_bytes);
+        ScRoundArgs A = A0;
+        A.ctl = (ScCtl *)dctl;
+        A.rounds = (uint64_t *)(dctl + o_rounds);
+        A.chal = (uint64_t *)(dctl + o_chal);
+        A.given = given ? (const uint64_t *)(dctl + o_given) : nullptr;
+        A.claims = (uint64_t *)(dctl + o_claims);
+        A.eq_c = split ? (const uint64_t *)(dctl + o_eqc) : nullptr;
+        A.eq_rounds = split ? (const ScEqArgs *)(dctl + o_eqr) : nullptr;
+        auto eq_args = [&](size_t j) {
+            if (!split) return ScEqArgs{};
+            const uint64_t *ar = (const uint64_t *)(dctl + o_arena);
+            return ScEqArgs{ar + hi_off[j], ar + lo_off[j], (uint32_t)lbits[j], 0};
+        };
+        ScTables tabs{};
+        for (size_t f = 0; f < nf; f++) {
+            tabs.src[f] = factors[f];
+            tabs.dst[f] = (uint64_t *)dev + f * halfN;
+        }
+        if (m > 0) {
+            // round j's block totals go to part[j & 1]; consecutive large
+            // kernels visit their pairs in opposite orders
+            uint64_t *part[2] = {partial, partial + (size_t)SC_MAX_BLOCKS * width};
+            uint32_t gprev = sc_grid(N / 2);
+            K.eval(dim3(gprev), s, tabs, P.T, N / 2, eq_args(0), part[0], 0);
+            LZX_LAUNCH_CHECK("sc_eval_kernel");
+            size_t r = 0;
+            for (; (N >> (r + 1)) > SC_TAIL_HALF; r++) { // a grid per fold
+                const uint64_t half = N >> (r + 1);
+                const uint32_t g = sc_grid(half / 2);
+                if (g > SC_FUSE_BLOCKS) { // a one-block kernel closes round r, then the fold reads its challenge
+                    K.close(s, part[r & 1], gprev, A, (uint32_t)r);
+                    LZX_LAUNCH_CHECK("sc_close_kernel");
+                    K.fold(dim3(g), s, tabs, P.T, half, A.chal + r, eq_args(r + 1), part[(r + 1) & 1], r % 2 == 0);
+                    LZX_LAUNCH_CHECK("sc_fold_eval_kernel");
+                } else { // every block closes round r itself
+                    K.fold_close(dim3(g), s, tabs, P.T, half, eq_args(r + 1), part[r & 1], gprev,
+                                 part[(r + 1) & 1], A, (uint32_t)r, r % 2 == 0);
+                    LZX_LAUNCH_CHECK("sc_fold_close_kernel");
+                }
+                gprev = g;
+                for (size_t f = 0; f < nf; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
             }
-            gprev = g;
-            for (size_t f = 0; f < nf; f++) tabs.src[f] = tabs.dst[f]; // later rounds fold in place
+            K.tail(s, tabs, P.T, N >> (r + 1), (uint32_t)r, (uint32_t)m, A, part[r & 1], gprev);
+            LZX_LAUNCH_CHECK("sc_tail_kernel");
         }
-        if (st == LZX_OK) {
-            K.tail(stream, tabs, P.T, N >> (r + 1), (uint32_t)r, (uint32_t)m, A, part[r & 1], gprev);
-            check("sc_tail_kernel");
+        // zero rounds: the claims are the tables' single entries (sumcheck.rs:139-143)
+        for (size_t f = 0; m == 0 && f < nf; f++) {
+            if ((int)f == P.T.eq_f) continue; // eq over no variables is 1 (the scale)
+            LZX_HIP(hipMemcpyAsync(dctl + o_claims + 8 * f, factors[f], sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
         }
-    }
-    // zero rounds: the claims are the tables' single entries (sumcheck.rs:139-143)
-    for (size_t f = 0; st == LZX_OK && m == 0 && f < nf; f++) {
-        if ((int)f == P.T.eq_f) continue; // eq over no variables is 1 (the scale)
-        e = hipMemcpyAsync(dctl + o_claims + 8 * f, factors[f], sizeof(uint64_t), hipMemcpyDeviceToDevice, stream);
-        if (e != hipSuccess) st = device_fail(e, "factor claim copy");
-    }
-    if (st == LZX_OK) { // one readback of the control area
-        e = hipMemcpyAsync(h, dctl, area_rb, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) st = device_fail(e, "sumcheck readback");
-        enqueued = false;
-    }
-    if (st == LZX_OK) {
+        LZX_HIP(hipMemcpyAsync(h, dctl, area_rb, hipMemcpyDeviceToHost, s)); // one readback of the control area
+        return LZX_OK;
+    };
+    // The proof from the read-back control area at h.
+    auto finish = [&](const uint8_t *h) -> lzx_status {
         const ScCtl &c = ((const ScCtl *)h)[m ? (m - 1) & 1 : 1]; // the state after the last round
         if (tr) { // the caller's transcript advances exactly as the reference's would
             for (int i = 0; i < 25; i++) tr->sponge.st[i] = c.tx.st[i];
@@ -1297,15 +1421,62 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
         }
         memcpy(rounds_out, h + o_rounds, m * width * sizeof(uint64_t));
         memcpy(challenges_out, h + o_chal, m * sizeof(uint64_t));
-        if (c.status) {
-            st = (lzx_status)c.status;
-        } else {
-            memcpy(factor_claims_out, h + o_claims, nf * sizeof(uint64_t));
-            if (split) factor_claims_out[P.T.eq_f] = c.eq_scale; // eq(w, r): the bound eq factor
-            const uint64_t fin = host_eval_terms(P, factor_claims_out); // sumcheck.rs:144-154
-            *final_claim = fin;
-            if (fin != c.current) st = LZX_SUMCHECK_FINAL_CHECK_FAILED;
+        if (c.status) return (lzx_status)c.status;
+        memcpy(factor_claims_out, h + o_claims, nf * sizeof(uint64_t));
+        if (split) factor_claims_out[P.T.eq_f] = c.eq_scale; // eq(w, r): the bound eq factor
+        const uint64_t fin = host_eval_terms(P, factor_claims_out); // sumcheck.rs:144-154
+        *final_claim = fin;
+        return fin != c.current ? LZX_SUMCHECK_FINAL_CHECK_FAILED : LZX_OK;
+    };
+
+    if (m > 0 && sc_graph_enabled()) {
+        // the key: everything the captured launches bake in
+        ScGraphKey key;
+        int dev_id = 0;
+        LZX_HIP(hipGetDevice(&dev_id));
+        key.add(dev_id).add(m).add(nf).add(d).add(given != nullptr).add(tr != nullptr).add(P.T);
+        for (size_t f = 0; f < nf; f++) key.add(factors[f]);
+        ScGraphHold hold;
+        lzx_status gs = sc_graph_get(key, dev_bytes, area, hold);
+        if (gs != LZX_OK) return gs;
+        if (ScGraph *g = hold.g) { // held until the proof is read
+            if (!g->exec) {
+                if (g->graph) (void)hipGraphDestroy(g->graph); // an earlier instantiation failed
+                g->graph = nullptr;
+                LZX_HIP(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+                lzx_status cs = enqueue(g->dev, (uint8_t *)g->stage, g->cap);
+                hipGraph_t graph = nullptr;
+                const hipError_t ce = hipStreamEndCapture(g->cap, &graph);
+                if (cs != LZX_OK || ce != hipSuccess) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return cs != LZX_OK ? cs : device_fail(ce, "sumcheck graph capture");
+                }
+                g->graph = graph;
+                LZX_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+            }
+            fill_stage((uint8_t *)g->stage, g->dev);
+            LZX_HIP(hipGraphLaunch(g->exec, stream));
+            LZX_HIP(hipStreamSynchronize(stream));
+            return finish((const uint8_t *)g->stage);
         }
+    }
+
+    uint8_t *dev = nullptr;
+    HostStage hs;
+    lzx_status st = LZX_OK;
+    hipError_t e = scratch_alloc((void **)&dev, dev_bytes, stream);
+    if (e == hipSuccess) e = stage_acquire(hs, area);
+    if (e != hipSuccess) st = device_fail(e, "sumcheck scratch");
+    bool enqueued = false;
+    if (st == LZX_OK) {
+        fill_stage((uint8_t *)hs.p, dev);
+        st = enqueue(dev, (uint8_t *)hs.p, stream);
+        enqueued = true;
+    }
+    if (st == LZX_OK) {
+        e = hipStreamSynchronize(stream);
+        enqueued = false;
+        st = e == hipSuccess ? finish((const uint8_t *)hs.p) : device_fail(e, "sumcheck readback");
     }
     if (enqueued) (void)hipStreamSynchronize(stream); // the staging buffer may still be read
     if (dev) (void)scratch_free(dev, stream);

@@ -466,3 +466,33 @@ def test_session_errors(sc):
     with pytest.raises(LzxError) as e:  # sumcheck.rs:120-123
         S.prove_with_session(_vp(S, D, 3, tables, [(1, [0, 1])]), tampered, S.Transcript(b"x"))
     assert e.value.code == 13
+
+
+def test_prove_graph_replays_match_oracle(sc):
+    """From the second prove with the same tables the launches replay from a
+    captured graph; every replay, interleaved with other proves and host
+    reads, gives the oracle's proof."""
+    S, D = sc
+    cases = []
+    for nv, terms in ((15, [(1, [0, 1, 2]), (P - 1, [0, 3])]), (9, [(3, [0, 1]), (5, [1, 2, 0]), (11, [2])])):
+        nf = 1 + max(max(ids) for _, ids in terms)
+        rng = np.random.default_rng(7000 + nv)
+        tables = [rng.integers(0, P, 1 << nv, dtype=np.uint64) for _ in range(nf)]
+        claim = O.vp_sum_over_hypercube(nv, tables, terms)
+        st, want = O.sumcheck_prove(nv, tables, terms, claim, O.Transcript(b"sc-graph"))
+        assert st == 0
+        cases.append((_vp(S, D, nv, tables, terms), claim, want, tables))
+    for rep in range(4):
+        for vp, claim, want, tables in cases:
+            out = S.prove(vp, claim, S.Transcript(b"sc-graph"))
+            assert out.rounds.tolist() == want["rounds"].tolist(), rep
+            assert out.challenges.tolist() == want["challenges"].tolist(), rep
+            assert out.factor_claims.tolist() == want["factor_claims"].tolist(), rep
+            assert out.final_claim == want["final_claim"], rep
+            assert np.array_equal(D.to_host(vp.factors[0], np.uint64), tables[0])  # inputs untouched
+    # a wrong claim on a replayed shape is still refused
+    from lzx_amd._lib import LzxError
+    vp, claim, _, _ = cases[0]
+    with pytest.raises(LzxError) as e:
+        S.prove(vp, (claim + 1) % P, S.Transcript(b"sc-graph"))
+    assert e.value.code == 13  # SumcheckError::ClaimMismatch
