@@ -264,7 +264,14 @@ def _int8_achievable(torch, tops):
 def bench_sumcheck(reps: int = 3):
     """Sum-check prove of eq(r,x)*(a(x)*b(x) - c(x)) over 2^24-entry Goldilocks
     tables with the Fiat-Shamir transcript (run on the device; one host sync).
-    Algorithmic bytes with 4 materialised tables: sum_j 4*8*(2^(24-j) + 2^(23-j))."""
+    Algorithmic bytes with 4 materialised tables: sum_j 4*8*(2^(24-j) + 2^(23-j)).
+    The default path replays a captured graph (from the second call on); the
+    direct path enqueues kernel by kernel.  The round-0 evaluation and the first
+    fold are timed on their own (each call includes a small block-total
+    reduction and a readback, so the fractions are lower bounds) against the
+    HBM peak and against the measured Goldilocks product rate."""
+    import os
+
     import torch
 
     from lzx_amd import device as D
@@ -283,13 +290,39 @@ def bench_sumcheck(reps: int = 3):
     claim = vp.sum_over_hypercube()
     S.prove(vp, claim, S.Transcript(b"lzx-mi355x-sumcheck"))
     torch.cuda.synchronize()
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        out = S.prove(vp, claim, S.Transcript(b"lzx-mi355x-sumcheck"))
-        times.append(time.perf_counter() - t0)
-    sec = statistics.median(times)
+    prove = lambda: S.prove(vp, claim, S.Transcript(b"lzx-mi355x-sumcheck"))
+    sec = _median_s(prove, max(reps, 5), warmup=1)
+    out = prove()
+    os.environ["LZX_SC_GRAPH"] = "0"
+    try:
+        sec_direct = _median_s(prove, max(reps, 5), warmup=1)
+    finally:
+        del os.environ["LZX_SC_GRAPH"]
     alg_bytes = sum(4 * 8 * ((1 << (nv - j)) + (1 << (nv - 1 - j))) for j in range(nv))
+    # round 0 alone (sc_eval_kernel: 4 tables read, 8 products per pair) and
+    # the first fold (sc_fold_eval_kernel: 4 tables read, half written, 8 fold
+    # products + 8 point products per pair of the next round)
+    t_eval = _median_s(vp.sum_over_hypercube, 5, warmup=1)
+    sess_t = []
+    for _ in range(3):
+        sess = S.SumcheckSession(vp)
+        sess.round_eval()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sess.fold(12345)
+        sess_t.append(time.perf_counter() - t0)
+        del sess
+    t_fold = statistics.median(sess_t)
+    rate = S.gl_mul_rate(8)
+    eval_bytes, fold_bytes = 4 * 8 * N, 4 * 8 * N + 4 * 8 * (N // 2)
+    eval_prods, fold_prods = 8 * (N // 2), 8 * (N // 2) + 8 * (N // 4)
+
+    def rl(b, prods, t):
+        return {"bound": "hbm", "achieved": round(b / t / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(b / t / 1e9 / HBM_PEAK_GBS, 3), "traffic": None,
+                "valu": {"goldilocks_products_per_s": round(prods / t / 1e12, 3),
+                         "measured_rate": round(rate / 1e12, 3), "unit": "T/s", "frac": round(prods / t / rate, 3)}}
+
     # fastprover::prove_fast_with_eq on the same polynomial: eq(r, .) is never
     # materialised (3 streamed tables); the proof is the same, byte for byte
     r = W.uniform_gl(0xC4E, nv)
@@ -298,18 +331,17 @@ def bench_sumcheck(reps: int = 3):
     for f in vp.factors[1:]:
         vq.add_factor(f)
     vq.terms = list(vp.terms)
-    fast = S.prove_fast_with_eq(vq, claim, S.Transcript(b"lzx-mi355x-sumcheck"), None, [0], r)
+    fast_call = lambda: S.prove_fast_with_eq(vq, claim, S.Transcript(b"lzx-mi355x-sumcheck"), None, [0], r)
+    fast = fast_call()
     torch.cuda.synchronize()
-    times_eq = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        fast = S.prove_fast_with_eq(vq, claim, S.Transcript(b"lzx-mi355x-sumcheck"), None, [0], r)
-        times_eq.append(time.perf_counter() - t0)
-    sec_eq = statistics.median(times_eq)
+    sec_eq = _median_s(fast_call, max(reps, 5), warmup=1)
     alg_eq = sum(3 * 8 * ((1 << (nv - j)) + (1 << (nv - 1 - j))) for j in range(nv))
     return {"workload": "sum-check prove, 2^24-entry tables, eq*(a*b-c), degree 3, transcript challenges",
-            "ms_per_prove": round(sec * 1e3, 3), "alg_bytes": alg_bytes, "gbs": round(alg_bytes / sec / 1e9, 1),
-            "final_claim": int(out.final_claim),
+            "ms_per_prove": round(sec * 1e3, 3), "path": "captured graph (replayed)",
+            "ms_per_prove_direct": round(sec_direct * 1e3, 3), "alg_bytes": alg_bytes,
+            "gbs": round(alg_bytes / sec / 1e9, 1), "final_claim": int(out.final_claim),
+            "kernels": {"round0_eval": dict(us=round(t_eval * 1e6, 1), **rl(eval_bytes, eval_prods, t_eval)),
+                        "first_fold_eval": dict(us=round(t_fold * 1e6, 1), **rl(fold_bytes, fold_prods, t_fold))},
             "split_eq": {"entry": "prove_fast_with_eq (eq never materialised)", "ms_per_prove": round(sec_eq * 1e3, 3),
                          "alg_bytes": alg_eq, "gbs": round(alg_eq / sec_eq / 1e9, 1),
                          "same_proof": bool(fast.rounds.tolist() == out.rounds.tolist()

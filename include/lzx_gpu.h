@@ -529,6 +529,13 @@ lzx_status lzx_vp_sum_over_hypercube(size_t num_vars, size_t nfactors, const uin
                                      size_t nterms, const uint64_t *coeffs, const uint32_t *term_off,
                                      const uint32_t *term_ids, uint64_t *sum_out, void *stream);
 
+/* Measurement aid with no counterpart in the reference: the device's
+ * Goldilocks product rate (the sum-check kernels' VALU ceiling).  Runs
+ * independent chains of 64-bit modular products (lazily reduced, as the
+ * kernels' inner products are) on every CU at waves_per_simd (4 or 8) waves
+ * per SIMD for about 5 ms and returns products per second. */
+lzx_status lzx_gl_mul_rate(uint32_t waves_per_simd, double *products_per_s);
+
 /* ---- witness packing feeding the commitment ----------------------------------
  * Device buffers throughout; ring elements are u32 [elems][n] residues. */
 

@@ -123,6 +123,7 @@ PROTOTYPES = {
     "lzx_sumcheck_verify_batch": (st_t, [vp, sz, sz, sz, sz, vp, sz, vp, i32, u64, vp]),
     "lzx_sumcheck_verify_batch_star": (st_t, [vp, sz, sz, vp, sz, sz, vp, sz, vp, sz, vp, i32, u64, vp]),
     "lzx_vp_sum_over_hypercube": (st_t, [sz, sz, C.POINTER(vp), sz, vp, vp, vp, C.POINTER(u64), vp]),
+    "lzx_gl_mul_rate": (st_t, [u32, C.POINTER(C.c_double)]),
 }
 
 

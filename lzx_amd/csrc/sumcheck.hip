@@ -797,13 +797,18 @@ __global__ __launch_bounds__(SC_BLOCK) void sc_fold_close_kernel(ScTables tabs, 
 // kernel's nprev block totals, then round after round fold with the latest
 // challenge, sum, close the next round; the last fold leaves one entry per
 // table (the factor claims, sumcheck.rs:139-143).  tabs.src is the input of
-// round r0's fold.  The state after the last round goes to A.ctl[(num_vars - 1) & 1].
+// round r0's fold; its output, half0 entries per factor, and every later
+// fold stay in the block's LDS (the dynamic allocation of sc_tail_lds bytes).
+// The state after the last round goes to A.ctl[(num_vars - 1) & 1].
 template <int NF, int D, class SH>
 __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, ScTerms T, uint64_t half0, uint32_t r0,
                                                                 uint32_t num_vars, ScRoundArgs A,
                                                                 const uint64_t *__restrict__ part, uint32_t nprev)
 {
     __shared__ ScShared sh;
+    extern __shared__ uint64_t tail_tabs[];
+#pragma unroll
+    for (int f = 0; f < NF; f++) tabs.dst[f] = tail_tabs + (size_t)f * half0;
     const uint32_t lane = threadIdx.x & 63;
     if (threadIdx.x < 64) {
         for (uint32_t i = lane; r0 + 1 + i < num_vars && i < 64; i += 64) { // slot i: round r0 + 1 + i
@@ -826,9 +831,7 @@ __global__ __launch_bounds__(SC_TAIL_BLOCK) void sc_tail_kernel(ScTables tabs, S
                 for (int f = 0; f < NF; f++)
                     if ((uint32_t)f < T.nf && f != T.eq_f) {
                         const uint64_t a = tabs.src[f][0], b = tabs.src[f][1];
-                        const uint64_t v = gl_add(a, gl_mul(gl_sub(b, a), ch));
-                        tabs.dst[f][0] = v;
-                        A.claims[f] = v;
+                        A.claims[f] = gl_add(a, gl_mul(gl_sub(b, a), ch));
                     }
             }
             break;
@@ -1014,12 +1017,24 @@ static void launch_fold_close(dim3 g, hipStream_t s, const ScTables &tb, const S
                        part_out, A, rnd, rev);
 }
 
+// LDS for the tail's tables: half0 entries of each of the NF factor slots
+// (half0 <= SC_TAIL_HALF, so at most 128 KiB beside ScShared).
+template <int NF>
+static size_t sc_tail_lds(uint64_t half0)
+{
+    return (size_t)NF * half0 * sizeof(uint64_t);
+}
+
 template <int NF, int D, class SH>
 static void launch_tail(hipStream_t s, const ScTables &tb, const ScTerms &T, uint64_t half0, uint32_t r0,
                         uint32_t num_vars, const ScRoundArgs &A, const uint64_t *part, uint32_t nprev)
 {
-    hipLaunchKernelGGL((sc_tail_kernel<NF, D, SH>), dim3(1), dim3(SC_TAIL_BLOCK), 0, s, tb, T, half0, r0, num_vars,
-                       A, part, nprev);
+    static const hipError_t attr = hipFuncSetAttribute((const void *)sc_tail_kernel<NF, D, SH>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)sc_tail_lds<NF>(SC_TAIL_HALF));
+    (void)attr; // a failure shows as the launch's error
+    hipLaunchKernelGGL((sc_tail_kernel<NF, D, SH>), dim3(1), dim3(SC_TAIL_BLOCK), sc_tail_lds<NF>(half0), s, tb, T,
+                       half0, r0, num_vars, A, part, nprev);
 }
 
 template <int NF, int D, class SH = ScGenericShape>
@@ -2030,6 +2045,61 @@ extern "C" lzx_status lzx_sumcheck_verify_batch_star(const uint64_t *rounds, siz
     uint64_t fin = 0;
     return lzx_sumcheck_verify(rounds, num_rounds, round_len, max_vars, max_degree, combined, transcript,
                                has_expected, expected_final, point_out, &fin);
+}
+
+// ---- Goldilocks product rate (measurement aid) -------------------------------------
+namespace lzx {
+
+constexpr int GLR_CHAINS = 8;
+
+__global__ __launch_bounds__(1024) void gl_mul_rate_kernel(uint64_t *__restrict__ out, uint32_t iters, uint64_t seed)
+{
+    uint64_t x[GLR_CHAINS];
+    const uint64_t y = seed * 0x9E3779B97F4A7C15ULL + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < GLR_CHAINS; i++) x[i] = y + (uint64_t)i * 0x12345ULL;
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int i = 0; i < GLR_CHAINS; i++) x[i] = gl_mul_lazy(x[i], y);
+    }
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < GLR_CHAINS; i++) acc ^= x[i];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+} // namespace lzx
+
+extern "C" lzx_status lzx_gl_mul_rate(uint32_t waves_per_simd, double *products_per_s)
+{
+    if (!products_per_s || (waves_per_simd != 4 && waves_per_simd != 8)) return LZX_INVALID_ARGUMENT;
+    int dev = 0, cus = 0;
+    LZX_HIP(hipGetDevice(&dev));
+    LZX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const uint32_t threads = 1024, blocks = (uint32_t)cus * (256 * waves_per_simd / threads), iters = 4096;
+    uint64_t *out = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipMalloc((void **)&out, (size_t)blocks * threads * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    float best = 0;
+    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) { // the first run warms the clocks
+        hipLaunchKernelGGL(gl_mul_rate_kernel, dim3(blocks), dim3(threads), 0, 0, out, iters, 7);
+        e = hipEventRecord(a, 0);
+        if (e == hipSuccess) hipLaunchKernelGGL(gl_mul_rate_kernel, dim3(blocks), dim3(threads), 0, 0, out, iters, 7);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(b, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e == hipSuccess && rep > 0 && (best == 0 || ms < best)) best = ms;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return device_fail(e, "gl_mul_rate");
+    *products_per_s = (double)blocks * threads * iters * GLR_CHAINS / (best * 1e-3);
+    return LZX_OK;
 }
 
 // ---- round-level session ---------------------------------------------------------

@@ -150,6 +150,14 @@ def _prove(vp: VirtualPolynomial, claim: int, transcript, challenges, stream):
                           fclaims[: len(vp.factors) if vp.terms else 0])
 
 
+def gl_mul_rate(waves_per_simd: int = 8) -> float:
+    """Goldilocks products per second the device sustains (lzx_gl_mul_rate):
+    the VALU ceiling the sum-check kernels are measured against."""
+    out = C.c_double()
+    check(lib().lzx_gl_mul_rate(waves_per_simd, C.byref(out)), "gl_mul_rate")
+    return out.value
+
+
 def prove(vp: VirtualPolynomial, claim: int, transcript: Transcript, stream=None) -> SumcheckOutput:
     """sumcheck::prove(&vp, claim, &mut transcript) (sumcheck.rs:62-162)."""
     return _prove(vp, claim, transcript, None, stream)

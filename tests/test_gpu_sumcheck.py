@@ -496,3 +496,12 @@ def test_prove_graph_replays_match_oracle(sc):
     with pytest.raises(LzxError) as e:
         S.prove(vp, (claim + 1) % P, S.Transcript(b"sc-graph"))
     assert e.value.code == 13  # SumcheckError::ClaimMismatch
+
+
+def test_gl_mul_rate_probe(sc):
+    S, _ = sc
+    from lzx_amd._lib import LzxError
+    rate = S.gl_mul_rate(8)
+    assert 1e10 < rate < 1e14  # products per second on one device
+    with pytest.raises(LzxError):
+        S.gl_mul_rate(3)
