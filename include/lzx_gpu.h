@@ -381,6 +381,9 @@ uint64_t lzx_transcript_query_count(const lzx_transcript *t);
  * nfactors device tables of 2^num_vars canonical u64 (variable 0 = most
  * significant index bit, mle.rs:1-7) and terms in CSR form: term t has
  * coefficient coeffs[t] and factor ids term_ids[term_off[t] .. term_off[t+1]).
+ * Device limits (virtual_poly.rs:48-54 has none): at most 16 factors, 16
+ * terms and degree 15 (the longest term); a polynomial beyond them is refused
+ * with LZX_UNSUPPORTED before anything is enqueued.
  * ------------------------------------------------------------------------- */
 
 /* sumcheck::prove(&vp, claim, &mut transcript) (sumcheck.rs:62-162).  The
@@ -488,7 +491,7 @@ typedef struct {
  * Outputs: rhos_out[k], rounds_out[num_vars][degree+1], challenges_out[num_vars],
  * factor_claims_out[sum_i nfactors_i] in global factor order (claim 0's first).
  * Factor tables passed more than once (the same device pointer) are folded
- * once. At most 8 distinct tables and 16 terms in all (LZX_UNSUPPORTED). */
+ * once. At most 16 distinct tables, 16 terms and degree 15 in all (LZX_UNSUPPORTED). */
 lzx_status lzx_sumcheck_prove_batch(const lzx_batch_claim *claims, size_t nclaims, lzx_transcript *transcript,
                                     uint64_t *rhos_out, uint64_t *rounds_out, uint64_t *challenges_out,
                                     uint64_t *final_claim, uint64_t *factor_claims_out, size_t *degree_out,
@@ -499,7 +502,7 @@ lzx_status lzx_sumcheck_prove_batch(const lzx_batch_claim *claims, size_t nclaim
  * N - r variables (f'[j 2^(N-r) + s] = f[j]; its claim scales by 2^(N-r)),
  * rho_i = challenge_fields("batch-star-rho", k). N rounds; *num_vars_out = N.
  * The per-claim point is challenges_out[0 .. num_vars_i). Padded tables are
- * built in device scratch and count towards the 8 distinct tables. */
+ * built in device scratch and count towards the 16 distinct tables. */
 lzx_status lzx_sumcheck_prove_batch_star(const lzx_batch_claim *claims, size_t nclaims, lzx_transcript *transcript,
                                          uint64_t *rhos_out, uint64_t *rounds_out, uint64_t *challenges_out,
                                          uint64_t *final_claim, uint64_t *factor_claims_out, size_t *num_vars_out,

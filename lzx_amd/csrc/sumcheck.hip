@@ -16,7 +16,7 @@
 // caller's tables (left untouched) and writes the folded tables to scratch;
 // later rounds fold in place in scratch.  Each large kernel publishes
 // per-block totals and a one-block sc_close_kernel sums them and runs the
-// round's transcript step; once a fold's half drops to SC_TAIL_HALF, a single
+// round's transcript step; once a fold's half drops to the kernel set's tail threshold, a single
 // sc_tail_kernel block runs every remaining round.  All rounds are enqueued
 // at once and the host synchronizes once.
 //
@@ -41,9 +41,9 @@
 
 namespace lzx {
 
-constexpr int SC_MAXF = 8;
+constexpr int SC_MAXF = 16; // factors of a virtual polynomial (the widest kernel set)
 constexpr int SC_MAXT = 16;
-constexpr int SC_MAXD = 8;
+constexpr int SC_MAXD = 15; // degree: a round message of up to 16 values
 constexpr int SC_BLOCK = 256;
 constexpr uint32_t SC_MAX_BLOCKS = 2048;
 
@@ -319,8 +319,14 @@ __device__ __forceinline__ uint64_t group_sum(uint64_t v)
 
 constexpr int SC_TAIL_BLOCK = 512;
 constexpr int SC_MAX_WAVES = SC_TAIL_BLOCK / 64;
-// The rounds whose fold has half <= SC_TAIL_HALF run in one single-block kernel.
+// The rounds whose fold has half <= sc_tail_half<NF>() run in one
+// single-block kernel, its tables in LDS (NF * half * 8 <= 128 KiB).
 constexpr uint64_t SC_TAIL_HALF = 2048;
+template <int NF>
+constexpr uint64_t sc_tail_half()
+{
+    return NF <= 8 ? SC_TAIL_HALF : SC_TAIL_HALF * 8 / NF;
+}
 // Folds with at most this many blocks close the previous round themselves.
 constexpr uint32_t SC_FUSE_BLOCKS = 256;
 
@@ -331,7 +337,7 @@ struct ScShared {
     uint64_t given[64];       // caller-supplied challenges (tail kernel)
     uint64_t eqc[64][2];      // split eq constants per round (tail kernel)
     ScEqArgs eqr[64];         // split eq tables per round (tail kernel)
-    uint8_t msg[128]; // 22 + 8 (d+1) + 30 <= 124 transcript bytes per round
+    uint8_t msg[192]; // 22 + 8 (d+1) + 30 <= 180 transcript bytes per round
     uint32_t status;
 };
 
@@ -986,6 +992,7 @@ struct ScKernels {
                        const uint64_t *, uint32_t, uint64_t *, const ScRoundArgs &, uint32_t, uint32_t);
     void (*tail)(hipStream_t, const ScTables &, const ScTerms &, uint64_t, uint32_t, uint32_t, const ScRoundArgs &,
                  const uint64_t *, uint32_t);
+    uint64_t tail_half; // rounds whose fold has half <= tail_half run in the tail kernel
 };
 
 template <int NF, int D, class SH>
@@ -1018,7 +1025,7 @@ static void launch_fold_close(dim3 g, hipStream_t s, const ScTables &tb, const S
 }
 
 // LDS for the tail's tables: half0 entries of each of the NF factor slots
-// (half0 <= SC_TAIL_HALF, so at most 128 KiB beside ScShared).
+// (half0 <= sc_tail_half<NF>(), so at most 128 KiB beside ScShared).
 template <int NF>
 static size_t sc_tail_lds(uint64_t half0)
 {
@@ -1031,17 +1038,23 @@ static void launch_tail(hipStream_t s, const ScTables &tb, const ScTerms &T, uin
 {
     static const hipError_t attr = hipFuncSetAttribute((const void *)sc_tail_kernel<NF, D, SH>,
                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)sc_tail_lds<NF>(SC_TAIL_HALF));
+                                                       (int)sc_tail_lds<NF>(sc_tail_half<NF>()));
     (void)attr; // a failure shows as the launch's error
     hipLaunchKernelGGL((sc_tail_kernel<NF, D, SH>), dim3(1), dim3(SC_TAIL_BLOCK), sc_tail_lds<NF>(half0), s, tb, T,
                        half0, r0, num_vars, A, part, nprev);
 }
 
+// fold_close is null for the widest set: its registers are full without the
+// close's, so its rounds close in a kernel of their own.
 template <int NF, int D, class SH = ScGenericShape>
 static ScKernels kernels_for()
 {
-    return ScKernels{launch_eval<NF, D, SH>, launch_fold<NF, D, SH>, launch_close<D>, launch_fold_close<NF, D, SH>,
-                     launch_tail<NF, D, SH>};
+    if constexpr (NF <= 8)
+        return ScKernels{launch_eval<NF, D, SH>, launch_fold<NF, D, SH>, launch_close<D>,
+                         launch_fold_close<NF, D, SH>, launch_tail<NF, D, SH>, sc_tail_half<NF>()};
+    else
+        return ScKernels{launch_eval<NF, D, SH>, launch_fold<NF, D, SH>, launch_close<D>, nullptr,
+                         launch_tail<NF, D, SH>, sc_tail_half<NF>()};
 }
 
 // The compile-time shape P.T has, when it has one of the registered ones.
@@ -1103,6 +1116,7 @@ static ScKernels pick_kernels(const ScProblem &P)
     }
     if (nf <= 2 && d <= 2) return kernels_for<2, 2>();
     if (nf <= 4 && d <= 3) return kernels_for<4, 3>();
+    if (nf <= 8 && d <= 8) return kernels_for<8, 8>();
     return kernels_for<SC_MAXF, SC_MAXD>();
 }
 
@@ -1399,10 +1413,10 @@ static lzx_status prove_impl(const ScProblem &P, const uint64_t *const *factors,
             K.eval(dim3(gprev), s, tabs, P.T, N / 2, eq_args(0), part[0], 0);
             LZX_LAUNCH_CHECK("sc_eval_kernel");
             size_t r = 0;
-            for (; (N >> (r + 1)) > SC_TAIL_HALF; r++) { // a grid per fold
+            for (; (N >> (r + 1)) > K.tail_half; r++) { // a grid per fold
                 const uint64_t half = N >> (r + 1);
                 const uint32_t g = sc_grid(half / 2);
-                if (g > SC_FUSE_BLOCKS) { // a one-block kernel closes round r, then the fold reads its challenge
+                if (g > SC_FUSE_BLOCKS || !K.fold_close) { // a one-block kernel closes round r, then the fold
                     K.close(s, part[r & 1], gprev, A, (uint32_t)r);
                     LZX_LAUNCH_CHECK("sc_close_kernel");
                     K.fold(dim3(g), s, tabs, P.T, half, A.chal + r, eq_args(r + 1), part[(r + 1) & 1], r % 2 == 0);
@@ -1912,20 +1926,15 @@ static lzx_status prove_batch_impl(const lzx_batch_claim *claims, size_t k, bool
         for (size_t i = 0; i < k; i++)
             if (claims[i].num_vars != claims[0].num_vars) return LZX_BATCH_VARIABLE_COUNT_MISMATCH; // batch.rs:40-46
     if ((N && (!rounds_out || !challenges_out)) || (total_f && !factor_claims_out)) return LZX_INVALID_ARGUMENT;
-    lzx_status st = draw_rhos(tr, star ? "batch-star-rho" : "batch-rho", k, rhos_out);
-    if (st != LZX_OK) return st;
 
+    // the combined problem's shape first, so a refused batch leaves the transcript untouched
     std::vector<std::pair<const uint64_t *, uint32_t>> tabs; // distinct (table, lift)
     std::vector<uint32_t> gmap(total_f);                      // global factor -> distinct table
-    std::vector<uint64_t> coeffs;
     std::vector<uint32_t> off(1, 0), ids;
-    uint64_t combined = 0;
-    size_t g = 0;
+    size_t g = 0, nterms_all = 0, degree = 0;
     for (size_t i = 0; i < k; i++) {
         const lzx_batch_claim &c = claims[i];
         const uint32_t lift = (uint32_t)(N - c.num_vars);
-        const uint64_t rho = rhos_out[i];
-        combined = gl_add(combined, gl_mul(gl_mul(rho, gl_pow2(lift)), c.claimed_sum));
         const size_t base = g;
         for (size_t f = 0; f < c.nfactors; f++) {
             if (!c.factors_dev[f]) return LZX_INVALID_ARGUMENT;
@@ -1937,15 +1946,27 @@ static lzx_status prove_batch_impl(const lzx_batch_claim *claims, size_t k, bool
         for (size_t t = 0; t < c.nterms; t++) {
             if (c.term_off[t + 1] < c.term_off[t] || c.coeffs[t] >= GL_P) return LZX_INVALID_ARGUMENT;
             if (c.term_off[t + 1] == c.term_off[t]) return LZX_SUMCHECK_VIRTUAL_POLY; // add_term: EmptyProduct
-            coeffs.push_back(gl_mul(rho, c.coeffs[t]));
             for (uint32_t x = c.term_off[t]; x < c.term_off[t + 1]; x++) {
                 if (c.term_ids[x] >= c.nfactors) return LZX_SUMCHECK_VIRTUAL_POLY;
                 ids.push_back(gmap[base + c.term_ids[x]]);
             }
             off.push_back((uint32_t)ids.size());
+            degree = std::max<size_t>(degree, c.term_off[t + 1] - c.term_off[t]);
+            nterms_all++;
         }
     }
-    if (tabs.size() > (size_t)SC_MAXF || coeffs.size() > (size_t)SC_MAXT) return LZX_UNSUPPORTED;
+    if (tabs.size() > (size_t)SC_MAXF || nterms_all > (size_t)SC_MAXT || degree > (size_t)SC_MAXD)
+        return LZX_UNSUPPORTED;
+    lzx_status st = draw_rhos(tr, star ? "batch-star-rho" : "batch-rho", k, rhos_out);
+    if (st != LZX_OK) return st;
+    std::vector<uint64_t> coeffs;
+    uint64_t combined = 0;
+    for (size_t i = 0; i < k; i++) {
+        const lzx_batch_claim &c = claims[i];
+        const uint64_t rho = rhos_out[i];
+        combined = gl_add(combined, gl_mul(gl_mul(rho, gl_pow2((uint32_t)(N - c.num_vars))), c.claimed_sum));
+        for (size_t t = 0; t < c.nterms; t++) coeffs.push_back(gl_mul(rho, c.coeffs[t]));
+    }
     if (num_vars_out) *num_vars_out = N;
 
     // lifted tables: 2^N entries each in scratch

@@ -61,7 +61,7 @@ def test_prove_batch_matches_oracle(sc, nv, k):
     assert pt.tolist() == out.challenges.tolist()
 
 
-@pytest.mark.parametrize("nvs", [(3, 5, 5, 4), (0, 2), (7, 12, 9), (18, 14, 20)])
+@pytest.mark.parametrize("nvs", [(3, 5, 5, 4), (0, 2), (7, 12, 9), (18, 14, 20), (6, 13, 9, 11, 14, 8)])
 def test_prove_batch_star_matches_oracle(sc, nvs):
     S, D = sc
     rng = np.random.default_rng(sum(nvs))
@@ -137,8 +137,8 @@ def test_batch_errors(sc):
     with pytest.raises(LzxError) as e:
         S.prove_batch_star([], S.Transcript(b"t"))
     assert e.value.code == 17
-    with pytest.raises(LzxError) as e:  # 10 distinct tables exceed the device prover's 8
-        S.prove_batch(_device_claims(S, D, [_spec(3, rng) for _ in range(5)]), S.Transcript(b"t"))
+    with pytest.raises(LzxError) as e:  # 18 distinct tables exceed the device prover's 16
+        S.prove_batch(_device_claims(S, D, [_spec(3, rng) for _ in range(9)]), S.Transcript(b"t"))
     assert e.value.code == 31
     specs = [_spec(4, rng) for _ in range(2)]
     out, _ = S.prove_batch(_device_claims(S, D, specs), S.Transcript(b"lzx-batch-test"))

@@ -505,3 +505,46 @@ def test_gl_mul_rate_probe(sc):
     assert 1e10 < rate < 1e14  # products per second on one device
     with pytest.raises(LzxError):
         S.gl_mul_rate(3)
+
+
+# crates/lattice-memory/src/onehot_check.rs:143-159: the raf-evaluation
+# sum-check is one term eq_j * w * ra_1 * ... * ra_d (2 + d factors, degree 2 + d)
+@pytest.mark.parametrize("nv,d", [(6, 7), (14, 7), (13, 13), (11, 10)])
+def test_onehot_raf_shape_matches_oracle(sc, nv, d):
+    S, D = sc
+    rng = np.random.default_rng(31 * nv + d)
+    nf = 2 + d
+    tables = [rng.integers(0, P, 1 << nv, dtype=np.uint64) for _ in range(nf)]
+    tables[0] = O.eq_table(rng.integers(0, P, nv, dtype=np.uint64))
+    terms = [(1, list(range(nf)))]
+    claim = O.vp_sum_over_hypercube(nv, tables, terms)
+    vp = _vp(S, D, nv, tables, terms)
+    assert vp.sum_over_hypercube() == claim
+    out = S.prove(vp, claim, S.Transcript(b"onehot-raf"))
+    st, want = O.sumcheck_prove(nv, tables, terms, claim, O.Transcript(b"onehot-raf"))
+    assert st == 0
+    assert out.rounds.tolist() == want["rounds"].tolist()
+    assert out.challenges.tolist() == want["challenges"].tolist()
+    assert out.factor_claims.tolist() == want["factor_claims"].tolist()
+    assert out.final_claim == want["final_claim"]
+
+
+def test_virtual_polynomial_beyond_the_device_limits_is_refused(sc):
+    S, D = sc
+    from lzx_amd._lib import LzxError
+    nv = 3
+    rng = np.random.default_rng(5)
+    for nf, terms in ((17, [(1, list(range(17)))]), (2, [(1, [0] * 9 + [1] * 7)])):  # 17 factors; degree 16
+        tables = [rng.integers(0, P, 1 << nv, dtype=np.uint64) for _ in range(nf)]
+        vp = _vp(S, D, nv, tables, terms)
+        t = S.Transcript(b"too-wide")
+        with pytest.raises(LzxError) as e:
+            S.prove(vp, 0, t)
+        assert e.value.code == 31  # LZX_UNSUPPORTED
+        # refused before anything ran: the caller's transcript is untouched (a CPU fallback continues from it)
+        assert t.challenge_field(b"after") == S.Transcript(b"too-wide").challenge_field(b"after")
+        tb = S.Transcript(b"too-wide")
+        with pytest.raises(LzxError) as e:
+            S.prove_batch([S.BatchClaim(vp, 0), S.BatchClaim(vp, 0)], tb)
+        assert e.value.code == 31
+        assert tb.challenge_field(b"after") == S.Transcript(b"too-wide").challenge_field(b"after")
