@@ -243,6 +243,9 @@ def bench_ajtai(reps: int = 5):
     ms_ntt = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o1, path=CM.COMMIT_NTT), reps)
     ms_i8 = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o2, path=CM.COMMIT_INT8), reps)
     same = bool(torch.equal(o1, o2))
+    o3 = D.empty(nw * k * 64, np.uint32)
+    ms_auto = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o3, path=CM.COMMIT_AUTO), reps)
+    auto_choice = {CM.COMMIT_NTT: "ntt_path", CM.COMMIT_INT8: "int8_path"}.get(pk.commit_path_choice(nw), "not measured")
     rmacs = k * m * nw
     i8_ops = 2.0 * rmacs * 64 * 64 * 4
     return {"workload": "Ajtai commit d=64 k=16 m=2^16 W=64, q=3221225473, digits in [-8,8)",
@@ -252,6 +255,8 @@ def bench_ajtai(reps: int = 5):
                           "int8_tops": round(i8_ops / (ms_i8 * 1e-3) / 1e12, 1),
                           "int8_mfma_frac_of_5000_tops": round(i8_ops / (ms_i8 * 1e-3) / 5e15, 4)},
             "int8_over_ntt": round(ms_ntt / ms_i8, 3), "paths_bit_identical": same,
+            "auto": {"ms": round(ms_auto, 4), "choice": auto_choice, "equals_paths": bool(torch.equal(o1, o3)),
+                     "faster_path": "int8_path" if ms_i8 <= ms_ntt else "ntt_path"},
             "int8_achievable": _int8_achievable(torch, i8_ops / (ms_i8 * 1e-3) / 1e12)}
 
 
@@ -550,6 +555,9 @@ def bench_c1(reps: int = 50):
         t = _timed_events(torch, lambda: pk.commit_digits_batch(dd, 1, out, path=path), 20)
         same = bool(np.array_equal(D.to_host(out, np.uint32).reshape(k, 64), want))
         res[name] = {"us": round(t * 1e6, 2), "ring_macs_per_s": k * m / t, "equals_host_call": same}
+    names = {CM.COMMIT_NTT: "ntt_path", CM.COMMIT_INT8: "int8_path", CM.COMMIT_AUTO: "not measured"}
+    res["auto"]["choice"] = names[pk.commit_path_choice(1)]  # measured on AUTO's first call of the shape
+    res["auto"]["faster_path"] = min(("ntt_path", "int8_path"), key=lambda p: res[p]["us"])
     return res, (pk, s, want)
 
 

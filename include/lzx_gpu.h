@@ -249,10 +249,20 @@ lzx_status lzx_ajtai_commit_batch(const lzx_ajtai_key *key, const uint32_t *s_de
 /* Batched commit of small-digit witnesses: digits_dev [W][m][n] int8, any
  * value in [-128, 127] (coefficient c represents c mod q, ring.rs:232-243
  * from_signed).  path selects the NTT-domain kernels, the int8 MFMA
- * coefficient-domain kernels (n = 64, q < 2^32 - 2^25), or AUTO (per-shape
- * choice).  Results are identical on every path, for every int8 digit. */
+ * coefficient-domain kernels (n = 64, q < 2^32 - 2^25), or AUTO: the
+ * faster of the two for the shape (device, q, n, k, m, W), measured on the
+ * first AUTO call of that shape (which runs each path twice on the stream)
+ * and kept for the process and in LZX_AUTOTUNE_CACHE when set; keys without
+ * int8 support take the NTT path.  Results are identical on every path, for
+ * every int8 digit. */
 lzx_status lzx_ajtai_commit_digits_batch(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t s_len,
                                          size_t W, uint32_t *out_dev, int32_t path, void *stream);
+
+/* The path AUTO takes for W witnesses on this key: LZX_COMMIT_NTT or
+ * LZX_COMMIT_INT8 once measured (or kept from LZX_AUTOTUNE_CACHE), else
+ * LZX_COMMIT_AUTO (the next AUTO call measures).  No counterpart in the
+ * reference; for callers that report or pre-tune their shapes. */
+lzx_status lzx_ajtai_commit_path_choice(const lzx_ajtai_key *key, size_t W, int32_t *path_out);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -64,6 +64,7 @@ PROTOTYPES = {
     "lzx_ajtai_key_matrix_ntt": (st_t, [vp, vp]),
     "lzx_ajtai_commit_batch": (st_t, [vp, vp, sz, sz, vp, vp]),
     "lzx_ajtai_commit_digits_batch": (st_t, [vp, vp, sz, sz, vp, i32, vp]),
+    "lzx_ajtai_commit_path_choice": (st_t, [vp, sz, C.POINTER(i32)]),
     "lzx_ajtai_commit_digits_int8_noalloc": (st_t, [vp, vp, sz, sz, vp, vp]),
     "lzx_ajtai_commit_host": (st_t, [vp, vp, sz, vp]),
     "lzx_ajtai_commit_naive_batch": (st_t, [vp, vp, sz, sz, vp, vp]),

@@ -142,6 +142,12 @@ class AjtaiPublicKey:
                                                   path, dev.stream_handle(stream)),
               "lzx_ajtai_commit_digits_batch")
 
+    def commit_path_choice(self, W: int) -> int:
+        """The path COMMIT_AUTO takes for W witnesses: COMMIT_NTT or COMMIT_INT8
+        once measured, COMMIT_AUTO before the first AUTO call of the shape."""
+        out = C.c_int32()
+        check(lib().lzx_ajtai_commit_path_choice(self._h, W, C.byref(out)), "lzx_ajtai_commit_path_choice")
+        return int(out.value)
 
     def commit_naive_batch(self, s_dev, W: int, out_dev, s_len=None, stream=None) -> None:
         """commit_naive_reference over [W][m][n] residues in HBM."""

@@ -17,6 +17,8 @@
 // All-zero witness elements contribute zero, which is exactly the
 // reference's zero-skip (:130-137).
 #include <algorithm>
+#include <cstdio>
+#include <string>
 #include <vector>
 
 #include "ajtai.hpp"
@@ -472,6 +474,66 @@ extern "C" lzx_status lzx_ajtai_commit_batch(const lzx_ajtai_key *key, const uin
     return ajtai_commit_ntt(key, s_dev, nullptr, W, out_dev, (hipStream_t)stream);
 }
 
+// ---- LZX_COMMIT_AUTO: the faster path per shape, by measurement ---------
+//
+// Both digit paths give the same commitment.  The first AUTO call for a
+// (device, q, n, k, m, W) times each path on the caller's stream (after one
+// untimed run of each) and keeps the faster in the tuning cache (persisted
+// through LZX_AUTOTUNE_CACHE); later calls run the kept path.  A key without
+// int8 support always takes the NTT path; under stream capture, with no
+// kept choice, AUTO takes int8 without measuring.
+namespace {
+std::string path_key(const lzx_ajtai_key *key, size_t W)
+{
+    char buf[128];
+    snprintf(buf, sizeof buf, "ajtai_commit_path dev=%d q=%u n=%zu k=%zu m=%zu W=%zu", key->device, key->plan->q,
+             key->n, key->k, key->m, W);
+    return buf;
+}
+
+lzx_status run_path(const lzx_ajtai_key *key, int32_t path, const int8_t *digits_dev, size_t W, uint32_t *out_dev,
+                    hipStream_t s)
+{
+    return path == LZX_COMMIT_INT8 ? ajtai_commit_int8(key, digits_dev, W, out_dev, s)
+                                   : ajtai_commit_ntt(key, nullptr, digits_dev, W, out_dev, s);
+}
+
+lzx_status commit_auto(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t W, uint32_t *out_dev,
+                       hipStream_t s)
+{
+    if (!ajtai_int8_supported(key)) return run_path(key, LZX_COMMIT_NTT, digits_dev, W, out_dev, s);
+    const std::string pk = path_key(key, W);
+    uint32_t kept = 0;
+    if (autotune_get(pk, kept) && (kept == LZX_COMMIT_NTT || kept == LZX_COMMIT_INT8))
+        return run_path(key, (int32_t)kept, digits_dev, W, out_dev, s);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LZX_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return run_path(key, LZX_COMMIT_INT8, digits_dev, W, out_dev, s);
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    lzx_status st = LZX_OK;
+    for (int i = 0; i < 3 && st == LZX_OK; i++)
+        if (hipEventCreate(&ev[i]) != hipSuccess) st = device_fail(hipGetLastError(), "commit path events");
+    float t_i8 = 0, t_ntt = 0;
+    if (st == LZX_OK) st = run_path(key, LZX_COMMIT_INT8, digits_dev, W, out_dev, s); // untimed: first-call costs
+    if (st == LZX_OK) st = run_path(key, LZX_COMMIT_NTT, digits_dev, W, out_dev, s);
+    if (st == LZX_OK && hipEventRecord(ev[0], s) != hipSuccess) st = device_fail(hipGetLastError(), "event");
+    if (st == LZX_OK) st = run_path(key, LZX_COMMIT_INT8, digits_dev, W, out_dev, s);
+    if (st == LZX_OK && hipEventRecord(ev[1], s) != hipSuccess) st = device_fail(hipGetLastError(), "event");
+    if (st == LZX_OK) st = run_path(key, LZX_COMMIT_NTT, digits_dev, W, out_dev, s); // out: the commitment
+    if (st == LZX_OK && hipEventRecord(ev[2], s) != hipSuccess) st = device_fail(hipGetLastError(), "event");
+    if (st == LZX_OK) {
+        hipError_t e = hipEventSynchronize(ev[2]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t_i8, ev[0], ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t_ntt, ev[1], ev[2]);
+        if (e != hipSuccess) st = device_fail(e, "commit path timing");
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (st == LZX_OK) autotune_put(pk, t_i8 <= t_ntt ? LZX_COMMIT_INT8 : LZX_COMMIT_NTT);
+    return st;
+}
+} // namespace
+
 extern "C" lzx_status lzx_ajtai_commit_digits_batch(const lzx_ajtai_key *key, const int8_t *digits_dev, size_t s_len,
                                                     size_t W, uint32_t *out_dev, int32_t path, void *stream)
 {
@@ -479,11 +541,21 @@ extern "C" lzx_status lzx_ajtai_commit_digits_batch(const lzx_ajtai_key *key, co
     if (s_len != key->m) return LZX_AJTAI_DIMENSION_MISMATCH;
     if (W && (!digits_dev || !out_dev)) return LZX_INVALID_ARGUMENT;
     if (path < LZX_COMMIT_AUTO || path > LZX_COMMIT_INT8) return LZX_INVALID_ARGUMENT;
-    const bool int8_ok = ajtai_int8_supported(key);
-    if (path == LZX_COMMIT_INT8 && !int8_ok) return LZX_UNSUPPORTED;
-    if (path == LZX_COMMIT_INT8 || (path == LZX_COMMIT_AUTO && int8_ok))
-        return ajtai_commit_int8(key, digits_dev, W, out_dev, (hipStream_t)stream);
-    return ajtai_commit_ntt(key, nullptr, digits_dev, W, out_dev, (hipStream_t)stream);
+    if (path == LZX_COMMIT_INT8 && !ajtai_int8_supported(key)) return LZX_UNSUPPORTED;
+    if (W == 0) return LZX_OK;
+    if (path == LZX_COMMIT_AUTO) return commit_auto(key, digits_dev, W, out_dev, (hipStream_t)stream);
+    return run_path(key, path, digits_dev, W, out_dev, (hipStream_t)stream);
+}
+
+extern "C" lzx_status lzx_ajtai_commit_path_choice(const lzx_ajtai_key *key, size_t W, int32_t *path_out)
+{
+    if (!key || !path_out) return LZX_INVALID_ARGUMENT;
+    uint32_t kept = 0;
+    if (!ajtai_int8_supported(key)) *path_out = LZX_COMMIT_NTT;
+    else if (autotune_get(path_key(key, W), kept) && (kept == LZX_COMMIT_NTT || kept == LZX_COMMIT_INT8))
+        *path_out = (int32_t)kept;
+    else *path_out = LZX_COMMIT_AUTO;
+    return LZX_OK;
 }
 
 extern "C" lzx_status lzx_ajtai_commit_host(const lzx_ajtai_key *key, const uint32_t *s_host, size_t s_len,

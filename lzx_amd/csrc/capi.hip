@@ -1,6 +1,9 @@
 // Status strings, HIP error capture and the library's scratch allocators for
 // the C ABI (include/lzx_gpu.h).
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -66,6 +69,54 @@ hipError_t scratch_free(void *p, hipStream_t s) { return p ? hipFreeAsync(p, s) 
 
 // Pinned host staging buffers, reused across calls and threads: a call takes
 // an idle buffer of sufficient size (or allocates one) and hands it back.
+// ---- tuning cache: process-wide, optionally backed by a text file ----------
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, uint32_t> g_tune;
+bool g_tune_loaded = false;
+
+void tune_load_locked()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *path = getenv("LZX_AUTOTUNE_CACHE");
+    if (!path || !*path) return;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t tab = line.rfind('\t');
+        if (tab == std::string::npos) continue;
+        g_tune[line.substr(0, tab)] = (uint32_t)strtoul(line.c_str() + tab + 1, nullptr, 10);
+    }
+}
+} // namespace
+
+bool autotune_get(const std::string &key, uint32_t &val)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    val = it->second;
+    return true;
+}
+
+void autotune_put(const std::string &key, uint32_t val)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load_locked();
+    g_tune[key] = val;
+    const char *path = getenv("LZX_AUTOTUNE_CACHE");
+    if (!path || !*path) return;
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::trunc);
+        for (const auto &kv : g_tune) f << kv.first << '\t' << kv.second << '\n';
+    }
+    std::rename(tmp.c_str(), path);
+}
+
 hipError_t stage_acquire(HostStage &h, size_t bytes)
 {
     {

@@ -526,12 +526,6 @@ void tw_pair(uint32_t kind, uint64_t w, uint64_t q, uint64_t &a, uint64_t &b)
 
 uint32_t class_of(uint32_t kind, uint32_t log_n) { return kind * 3 + log_n - CB_MIN_LOG_N; }
 
-// ---- autotune cache: process-wide, optionally backed by a text file ----
-
-std::mutex g_cache_mu;
-std::map<std::string, uint32_t> g_cache;
-bool g_cache_loaded = false;
-
 std::string cache_key(uint32_t kind, uint32_t log_n, size_t k)
 {
     char buf[96];
@@ -539,46 +533,12 @@ std::string cache_key(uint32_t kind, uint32_t log_n, size_t k)
     return buf;
 }
 
-void cache_load_locked()
-{
-    if (g_cache_loaded) return;
-    g_cache_loaded = true;
-    const char *path = getenv("LZX_AUTOTUNE_CACHE");
-    if (!path || !*path) return;
-    std::ifstream f(path);
-    std::string line;
-    while (std::getline(f, line)) {
-        const size_t tab = line.rfind('\t');
-        if (tab == std::string::npos) continue;
-        const unsigned long v = strtoul(line.c_str() + tab + 1, nullptr, 10);
-        if (v >= 1 && v <= CB_MAX_CHUNK) g_cache[line.substr(0, tab)] = (uint32_t)v;
-    }
-}
-
 bool cache_get(const std::string &key, uint32_t &val)
 {
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    cache_load_locked();
-    auto it = g_cache.find(key);
-    if (it == g_cache.end()) return false;
-    val = it->second;
-    return true;
+    return autotune_get(key, val) && val >= 1 && val <= CB_MAX_CHUNK;
 }
 
-void cache_put(const std::string &key, uint32_t val)
-{
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    cache_load_locked();
-    g_cache[key] = val;
-    const char *path = getenv("LZX_AUTOTUNE_CACHE");
-    if (!path || !*path) return;
-    const std::string tmp = std::string(path) + ".tmp";
-    {
-        std::ofstream f(tmp, std::ios::trunc);
-        for (const auto &kv : g_cache) f << kv.first << '\t' << kv.second << '\n';
-    }
-    std::rename(tmp.c_str(), path);
-}
+void cache_put(const std::string &key, uint32_t val) { autotune_put(key, val); }
 
 uint32_t env_chunk()
 {

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <functional>
+#include <string>
 
 #include "../../include/lzx_gpu.h"
 
@@ -23,6 +24,12 @@ void note_error(const char *what);
 // memory stays cached in the pool for the next call).
 hipError_t scratch_alloc(void **p, size_t bytes, hipStream_t s);
 hipError_t scratch_free(void *p, hipStream_t s);
+
+// Tuned choices (a u32 per text key) shared by every module: kept for the
+// process and, when LZX_AUTOTUNE_CACHE names a file, loaded from and saved to
+// it (one "key<TAB>value" line each).
+bool autotune_get(const std::string &key, uint32_t &val);
+void autotune_put(const std::string &key, uint32_t val);
 
 // A pinned host buffer from the library's reuse list.
 struct HostStage {

@@ -323,3 +323,24 @@ def test_dimension_mismatch_batch(mods):
     with pytest.raises(LzxError) as e:
         pk.commit_digits_batch(D.to_device(np.zeros((1, 7, 64), np.int8)), 1, out, s_len=7)
     assert e.value.code == 4
+
+
+def test_auto_path_is_measured_and_kept(mods):
+    """LZX_COMMIT_AUTO times both digit paths on its first call of a shape,
+    keeps the faster, and gives the same commitment as either path."""
+    CM, D, R = mods
+    k, m, W = 3, 96, 2
+    pk = CM.AjtaiPublicKey.from_seed(CM.AjtaiParams(R.RingConfig(R.Modulus32.Q_32, 6), k, m, 8), bytes([0x5A]) * 32)
+    digits = D.to_device(np.random.default_rng(77).integers(-128, 128, size=(W, m, 64)).astype(np.int8))
+    assert pk.commit_path_choice(W) == CM.COMMIT_AUTO  # nothing measured for this shape yet
+    outs = []
+    for path in (CM.COMMIT_AUTO, CM.COMMIT_NTT, CM.COMMIT_INT8, CM.COMMIT_AUTO):
+        o = D.empty(W * k * 64, np.uint32)
+        pk.commit_digits_batch(digits, W, o, path=path)
+        outs.append(D.to_host(o, np.uint32))
+    assert all(np.array_equal(outs[0], x) for x in outs[1:])
+    assert pk.commit_path_choice(W) in (CM.COMMIT_NTT, CM.COMMIT_INT8)
+    assert pk.commit_path_choice(W + 1) == CM.COMMIT_AUTO  # kept per W
+    # a key without the int8 path always takes the NTT one
+    pk32 = CM.AjtaiPublicKey.from_seed(CM.AjtaiParams(R.RingConfig(R.Modulus32.Q_32, 5), 2, 8, 8), bytes([1]) * 32)
+    assert pk32.commit_path_choice(1) == CM.COMMIT_NTT
