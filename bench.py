@@ -26,9 +26,10 @@ sys.path.insert(0, ROOT)
 METRIC = "Ajtai ring-MACs/s (int8-MFMA vs NTT path, bit-exact) + batched NTT GB/s"
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per forward launch measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (each scaled by the reading for a 64 MiB copy kernel): 65,834 KiB
-# read + 65,536 KiB written; the rocprofv3 summary is kept under profiles/.
-NTT_FWD_TRAFFIC_BYTES = (65812 + 65536) * 1024
+# WRITE_SIZE (each scaled by the reading for a 64 MiB copy kernel), per kernel
+# class of the plans; the rocprofv3 summaries are kept under profiles/.
+NTT_FWD_KERNEL = {"mfma": "ntt_mfma_kernel<NTT_FWD>", "register": "ntt_lazy_kernel<6,NTT_FWD>"}
+NTT_FWD_TRAFFIC_BYTES = {"mfma": None, "register": (65812 + 65536) * 1024}
 POLYS = 1 << 16
 LOG_N = 6
 N = 1 << LOG_N
@@ -87,6 +88,7 @@ def bench_ntt(steps: int, warmup: int):
     ok = bool(np.array_equal(D.to_host(data, np.uint32).reshape(host.shape), host))
     bytes_pass = 2 * len(moduli) * POLYS * N * 4
     return {
+        "path": R.batch_path(tabs, POLYS),
         "ms_per_step": total_ms / steps,
         "wall_s": wall,
         "fwd_ms": fwd_ms,
@@ -404,6 +406,41 @@ def bench_ntt_q32(reps: int = 10):
             "roundtrip_exact": ok}
 
 
+def bench_ntt_large(reps: int = 10):
+    """The headline transform on a 4x larger batch (4 CRT primes x 2^18
+    polynomials of d = 64, 537 MB per pass): the per-launch ramp-up and drain
+    weigh 4x less, so this is the kernels' streaming rate.  Both kernel
+    classes are timed on the same data, each pinned at plan creation
+    (LZX_NTT_KERNEL): the int8-MFMA transform and the register butterfly
+    kernel."""
+    import torch
+
+    from lzx_amd import device as D
+    from lzx_amd import ring as R
+    from lzx_amd import workloads as W
+
+    polys = 1 << 18
+    host = W.ntt_batch_input(R.CRT_PRIMES, polys, N)
+    data = D.to_device(host)
+    bytes_pass = 2 * len(R.CRT_PRIMES) * polys * N * 4
+    out = {"workload": "batched negacyclic NTT, 4 CRT primes x 2^18 polys x d=64, u32"}
+    prev = os.environ.get("LZX_NTT_KERNEL")
+    for name in ("mfma", "register"):
+        os.environ["LZX_NTT_KERNEL"] = name  # read at plan creation
+        tabs = [R.NttTables(m, LOG_N) for m in R.CRT_PRIMES]
+        t_fwd = _timed_events(torch, lambda: R.forward_multi(tabs, data, polys), reps)
+        t_inv = _timed_events(torch, lambda: R.inverse_multi(tabs, data, polys), reps)
+        out[name] = {"path": R.batch_path(tabs, polys), "fwd_us": round(t_fwd * 1e6, 2),
+                     "fwd_gbs": round(bytes_pass / t_fwd / 1e9, 1), "fwd_hbm_frac": round(bytes_pass / t_fwd / 8e12, 4),
+                     "inv_us": round(t_inv * 1e6, 2), "inv_gbs": round(bytes_pass / t_inv / 1e9, 1)}
+    if prev is None:
+        os.environ.pop("LZX_NTT_KERNEL", None)
+    else:
+        os.environ["LZX_NTT_KERNEL"] = prev
+    out["roundtrip_exact"] = bool(np.array_equal(D.to_host(data, np.uint32).reshape(host.shape), host))
+    return out
+
+
 def _timed_events(torch, fn, reps):
     """Seconds per call of fn: reps back-to-back calls between two HIP events on
     the launch stream (torch's current stream), after one warm-up call.  Calls
@@ -674,6 +711,8 @@ def main():
     ap.add_argument("--only-cpu", action="store_true", help="run only the CPU baseline legs")
     ap.add_argument("--only-ring50", action="store_true", help="run only the 50-bit quadratic-slot ring leg")
     ap.add_argument("--only-ntt-q32", action="store_true", help="run only the default-modulus (q > 2^31) NTT leg")
+    ap.add_argument("--only-ntt-large", action="store_true",
+                    help="run only the 4 x 2^18-polynomial NTT leg (both kernel classes)")
     ap.add_argument("--only-extension", action="store_true", help="run only the R_q[Y]/(Y^2+1) leg")
     ap.add_argument("--only-packing", action="store_true", help="run only the witness-packing leg")
     ap.add_argument("--no-autotune", action="store_true",
@@ -694,6 +733,9 @@ def main():
         return
     if args.only_ntt_q32:
         print(json.dumps({"ntt_q32": bench_ntt_q32()}), flush=True)
+        return
+    if args.only_ntt_large:
+        print(json.dumps({"ntt_large": bench_ntt_large()}), flush=True)
         return
     if args.only_extension:
         print(json.dumps({"extension": bench_extension()}), flush=True)
@@ -736,9 +778,9 @@ def main():
         "config": {"workload": "batched negacyclic NTT forward+inverse, 4 CRT primes x 2^16 polys x d=64, u32",
                    "polys_per_prime": POLYS, "degree": N, "primes": [m.q for m in __import__(
                        "lzx_amd.ring", fromlist=["CRT_PRIMES"]).CRT_PRIMES], "roundtrip_exact": r["roundtrip_exact"]},
-        "roofline": {"bound": "hbm", "kernel": "ntt_lazy_kernel<6,NTT_FWD>",
+        "roofline": {"bound": "hbm", "kernel": NTT_FWD_KERNEL[r["path"]],
                      "achieved": round(r["fwd_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["fwd_gbs"] / HBM_PEAK_GBS, 4), "traffic": NTT_FWD_TRAFFIC_BYTES,
+                     "frac": round(r["fwd_gbs"] / HBM_PEAK_GBS, 4), "traffic": NTT_FWD_TRAFFIC_BYTES[r["path"]],
                      "alg_bytes": r["bytes_per_pass"],
                      "fwd_us": round(r["fwd_ms"] * 1e3, 3), "inv_us": round(r["inv_ms"] * 1e3, 3),
                      "inv_gbs": round(r["inv_gbs"], 2)},
@@ -753,7 +795,7 @@ def main():
         line["secondary"] = {"c1_plain_ajtai": c1, "ajtai_commit": bench_ajtai(), "sumcheck_prove": bench_sumcheck(),
                              "uneven_batch": bench_uneven_batch(), "ring50": bench_ring50(),
                              "extension": bench_extension(), "packing": bench_packing(),
-                             "ntt_q32": bench_ntt_q32()}
+                             "ntt_q32": bench_ntt_q32(), "ntt_large": bench_ntt_large()}
         if not args.no_cpu:
             line["secondary"]["cpu_baseline"] = cpu_baseline_secondary(c1_inputs)
     print(json.dumps(line), flush=True)

@@ -107,6 +107,29 @@ void lzx_ntt_plan_destroy(lzx_ntt_plan *plan);
 /* psi and n^{-1} of the plan (NttTables fields, ntt.rs:52-58). */
 lzx_status lzx_ntt_plan_constants(const lzx_ntt_plan *plan, uint32_t *psi, uint32_t *n_inv);
 
+/* The kernel class that a batched forward (inverse != 0: inverse) transform
+ * of polys_per_plan polynomials per plan runs on 16-byte-aligned data, for
+ * plans as passed to lzx_ntt_forward_multi.  No counterpart in the reference
+ * (its NttTables has one scalar path); for tests and benchmarks that report
+ * the kernel they measure.
+ *   LZX_NTT_PATH_MFMA: the int8 matrix-core transform (n = 64 and
+ *     2^20 < q < 2^31 - 2^16 for every plan; unavailable to plans created
+ *     with LZX_NTT_MFMA=0);
+ *   LZX_NTT_PATH_REGISTER: the register butterfly kernels (n <= 64; exact
+ *     FP64 butterflies for q > 2^31);
+ *   LZX_NTT_PATH_LDS: the LDS kernel (n > 64);
+ *   LZX_NTT_PATH_AUTO: MFMA-capable and not yet measured.  The first such
+ *     call for a (device, moduli, polys_per_plan, direction) times the MFMA
+ *     and register classes on a scratch copy of its data and keeps the
+ *     faster in the tuning cache (LZX_AUTOTUNE_CACHE); plans created with
+ *     LZX_NTT_KERNEL=mfma or =register always run that class. */
+#define LZX_NTT_PATH_REGISTER 0u
+#define LZX_NTT_PATH_MFMA 1u
+#define LZX_NTT_PATH_LDS 2u
+#define LZX_NTT_PATH_AUTO 3u
+lzx_status lzx_ntt_batch_path(const lzx_ntt_plan *const *plans, size_t nplans, size_t polys_per_plan,
+                              uint32_t inverse, uint32_t *path);
+
 /* In-place batched transforms of len/n polynomials laid out [poly][n] in
  * HBM.  Replace NttTables::forward/forward_fast (ntt.rs:153,118) and
  * inverse/inverse_fast (ntt.rs:220,187) applied to each polynomial.

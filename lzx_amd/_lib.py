@@ -27,6 +27,7 @@ PROTOTYPES = {
     "lzx_ntt_plan_create": (st_t, [u32, u32, u32, u32, C.POINTER(vp)]),
     "lzx_ntt_plan_destroy": (None, [vp]),
     "lzx_ntt_plan_constants": (st_t, [vp, C.POINTER(u32), C.POINTER(u32)]),
+    "lzx_ntt_batch_path": (st_t, [C.POINTER(vp), sz, sz, u32, C.POINTER(u32)]),
     "lzx_ntt_forward_batch": (st_t, [vp, vp, sz, vp]),
     "lzx_ntt_inverse_batch": (st_t, [vp, vp, sz, vp]),
     "lzx_ntt_forward_partial_batch": (st_t, [vp, vp, sz, u32, vp]),

@@ -23,8 +23,10 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "ntt.hpp"
+#include "ntt_mfma.hpp"
 #include "ntt_regs.hpp"
 
 namespace lzx {
@@ -620,6 +622,104 @@ static void dispatch_lds(int arith, NttMode mode, dim3 grid, size_t shmem, uint3
         dispatch_lds_mode<ARITH_SHOUP>(mode, grid, shmem, data, ts, polys, bpp, log_n, ppb, levels, stream);
 }
 
+// ---- the kernel class of an MFMA-capable launch, per shape -------------
+//
+// Both classes give the same transform.  A launch whose plans all hold MFMA
+// tables runs the class their creation asked for (LZX_NTT_KERNEL), or, for
+// the default AUTO, the class kept in the tuning cache for (device, moduli,
+// polynomials per plan, direction).  With none kept yet, the call times both
+// classes on a pool scratch copy of its data (the transforms are in place),
+// one untimed launch each and then three timed, keeps the faster, and runs
+// it on the caller's data.  Under stream capture it takes MFMA unmeasured.
+static std::string ntt_path_key(const lzx_ntt_plan *const *plans, size_t cnt, size_t polys, NttMode mode)
+{
+    std::string k = "ntt_path dev=" + std::to_string(plans[0]->device) + " n=" + std::to_string(plans[0]->n) +
+                    " polys=" + std::to_string(polys) + (mode == NTT_FWD ? " fwd q=" : " inv q=");
+    for (size_t c = 0; c < cnt; c++) k += std::to_string(plans[c]->q) + (c + 1 < cnt ? "," : "");
+    return k;
+}
+
+static void launch_register(const lzx_ntt_plan *const *plans, size_t cnt, const TabSet &ts, uint32_t *chunk,
+                            size_t polys, NttMode mode, hipStream_t stream)
+{
+    const uint32_t bpp = (uint32_t)((polys + 63) / 64);
+    const uint32_t total = (uint32_t)(bpp * cnt);
+    const bool dma = plans[0]->stage_dma;
+    const dim3 grid(dma ? (uint32_t)plans[0]->cus : total);
+    dispatch_reg(plans[0]->arith, plans[0]->log_n, mode, dma, grid, chunk, ts, (uint32_t)polys, bpp, total, stream);
+}
+
+static lzx_status choose_class(const lzx_ntt_plan *const *plans, size_t cnt, const TabSet &ts, uint32_t *chunk,
+                               size_t polys, NttMode mode, hipStream_t stream, uint32_t &path)
+{
+    if (plans[0]->kernel_pref != LZX_NTT_PATH_AUTO) {
+        path = plans[0]->kernel_pref;
+        return LZX_OK;
+    }
+    const std::string key = ntt_path_key(plans, cnt, polys, mode);
+    uint32_t kept = 0;
+    if (autotune_get(key, kept) && (kept == LZX_NTT_PATH_MFMA || kept == LZX_NTT_PATH_REGISTER)) {
+        path = kept;
+        return LZX_OK;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    LZX_HIP(hipStreamIsCapturing(stream, &cs));
+    path = LZX_NTT_PATH_MFMA;
+    if (cs != hipStreamCaptureStatusNone) return LZX_OK;
+    const size_t bytes = cnt * polys * plans[0]->n * sizeof(uint32_t);
+    uint32_t *tmp = nullptr;
+    LZX_HIP(scratch_alloc((void **)&tmp, bytes, stream));
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    lzx_status st = LZX_OK;
+    for (int k = 0; k < 3 && st == LZX_OK; k++)
+        if (hipEventCreate(&ev[k]) != hipSuccess) st = device_fail(hipGetLastError(), "ntt path events");
+    if (st == LZX_OK && hipMemcpyAsync(tmp, chunk, bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
+        st = device_fail(hipGetLastError(), "ntt path copy");
+    auto run = [&](uint32_t cls, int reps) {
+        for (int r = 0; r < reps; r++) {
+            if (cls == LZX_NTT_PATH_MFMA)
+                ntt_mfma_launch(plans, cnt, tmp, (uint32_t)polys, mode, stream);
+            else
+                launch_register(plans, cnt, ts, tmp, polys, mode, stream);
+        }
+    };
+    float t_mf = 0, t_reg = 0;
+    if (st == LZX_OK) {
+        run(LZX_NTT_PATH_MFMA, 1); // untimed: first-launch costs
+        run(LZX_NTT_PATH_REGISTER, 1);
+        if (hipEventRecord(ev[0], stream) != hipSuccess) st = device_fail(hipGetLastError(), "event");
+    }
+    if (st == LZX_OK) {
+        run(LZX_NTT_PATH_MFMA, 3);
+        if (hipEventRecord(ev[1], stream) != hipSuccess) st = device_fail(hipGetLastError(), "event");
+    }
+    if (st == LZX_OK) {
+        run(LZX_NTT_PATH_REGISTER, 3);
+        if (hipEventRecord(ev[2], stream) != hipSuccess) st = device_fail(hipGetLastError(), "event");
+    }
+    if (st == LZX_OK) {
+        hipError_t e = hipEventSynchronize(ev[2]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t_mf, ev[0], ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t_reg, ev[1], ev[2]);
+        if (e != hipSuccess) st = device_fail(e, "ntt path timing");
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)scratch_free(tmp, stream);
+    if (st != LZX_OK) return st;
+    path = t_mf <= t_reg ? LZX_NTT_PATH_MFMA : LZX_NTT_PATH_REGISTER;
+    autotune_put(key, path);
+    return LZX_OK;
+}
+
+static bool mfma_capable(const lzx_ntt_plan *const *plans, size_t cnt, NttMode mode, bool aligned)
+{
+    if (!(mode == NTT_FWD || mode == NTT_INV) || !aligned || plans[0]->log_n != 6) return false;
+    for (size_t c = 0; c < cnt; c++)
+        if (!plans[c]->mfma_dev) return false;
+    return true;
+}
+
 lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data, size_t polys, NttMode mode,
                       uint32_t levels, hipStream_t stream)
 {
@@ -641,12 +741,15 @@ lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t 
         }
         uint32_t *chunk = data + i * polys * n;
         const bool use_reg = (mode == NTT_FWD || mode == NTT_INV) && log_n >= 2 && log_n <= 6 && aligned;
-        if (use_reg) {
-            const uint32_t bpp = (uint32_t)((polys + 63) / 64);
-            const uint32_t total = (uint32_t)(bpp * cnt);
-            const bool dma = plans[i]->stage_dma;
-            const dim3 grid(dma ? (uint32_t)plans[i]->cus : total);
-            dispatch_reg(arith, log_n, mode, dma, grid, chunk, ts, (uint32_t)polys, bpp, total, stream);
+        uint32_t path = LZX_NTT_PATH_REGISTER;
+        if (mfma_capable(plans + i, cnt, mode, aligned)) {
+            const lzx_status st = choose_class(plans + i, cnt, ts, chunk, polys, mode, stream, path);
+            if (st != LZX_OK) return st;
+        }
+        if (path == LZX_NTT_PATH_MFMA) {
+            ntt_mfma_launch(plans + i, cnt, chunk, (uint32_t)polys, mode, stream);
+        } else if (use_reg) {
+            launch_register(plans + i, cnt, ts, chunk, polys, mode, stream);
         } else {
             const uint32_t ppb = std::max<uint32_t>(1u, 2048u >> log_n);
             const uint32_t bpp = (uint32_t)((polys + ppb - 1) / ppb);
@@ -657,6 +760,21 @@ lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t 
         LZX_LAUNCH_CHECK("ntt kernel launch");
         i += cnt;
     }
+    return LZX_OK;
+}
+
+lzx_status ntt_batch_path(const lzx_ntt_plan *const *plans, size_t nplans, size_t polys, NttMode mode, uint32_t &path)
+{
+    // the first launch's plans (a call covers up to MAX_PLANS_PER_LAUNCH per launch)
+    const size_t cnt = std::min<size_t>(MAX_PLANS_PER_LAUNCH, nplans);
+    path = plans[0]->log_n <= 6 ? LZX_NTT_PATH_REGISTER : LZX_NTT_PATH_LDS;
+    if (!mfma_capable(plans, cnt, mode, true)) return LZX_OK;
+    if (plans[0]->kernel_pref != LZX_NTT_PATH_AUTO) {
+        path = plans[0]->kernel_pref;
+        return LZX_OK;
+    }
+    uint32_t kept = 0;
+    path = autotune_get(ntt_path_key(plans, cnt, polys, mode), kept) ? kept : LZX_NTT_PATH_AUTO;
     return LZX_OK;
 }
 
@@ -794,9 +912,25 @@ extern "C" lzx_status lzx_ntt_plan_create(uint32_t q, uint32_t two_adicity, uint
     if (e == hipSuccess) e = hipMalloc(&p->dev_tab, p->host_tab.size() * sizeof(uint32_t));
     if (e == hipSuccess)
         e = hipMemcpy(p->dev_tab, p->host_tab.data(), p->host_tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    p->mfma_dev = nullptr;
+    // LZX_NTT_MFMA=0: no MFMA tables (the butterfly kernels only);
+    // LZX_NTT_KERNEL=mfma|register: that class for every MFMA-capable launch
+    const char *mf = getenv("LZX_NTT_MFMA");
+    const char *kp = getenv("LZX_NTT_KERNEL");
+    p->kernel_pref = kp && strcmp(kp, "mfma") == 0       ? LZX_NTT_PATH_MFMA
+                     : kp && strcmp(kp, "register") == 0 ? LZX_NTT_PATH_REGISTER
+                                                         : LZX_NTT_PATH_AUTO;
+    if (e == hipSuccess && ntt_mfma_supported(p) && !(mf && strcmp(mf, "0") == 0)) {
+        const std::vector<uint8_t> mt = ntt_mfma_tables(p);
+        if (!mt.empty()) {
+            e = hipMalloc(&p->mfma_dev, mt.size());
+            if (e == hipSuccess) e = hipMemcpy(p->mfma_dev, mt.data(), mt.size(), hipMemcpyHostToDevice);
+        }
+    }
     if (e != hipSuccess) {
         lzx_status st = device_fail(e, "ntt plan upload");
         if (p->dev_tab) (void)hipFree(p->dev_tab);
+        if (p->mfma_dev) (void)hipFree(p->mfma_dev);
         delete p;
         return st;
     }
@@ -808,7 +942,17 @@ extern "C" void lzx_ntt_plan_destroy(lzx_ntt_plan *plan)
 {
     if (!plan) return;
     if (plan->dev_tab) (void)hipFree(plan->dev_tab);
+    if (plan->mfma_dev) (void)hipFree(plan->mfma_dev);
     delete plan;
+}
+
+extern "C" lzx_status lzx_ntt_batch_path(const lzx_ntt_plan *const *plans, size_t nplans, size_t polys_per_plan,
+                                         uint32_t inverse, uint32_t *path)
+{
+    if (!plans || !path || nplans == 0) return LZX_INVALID_ARGUMENT;
+    for (size_t i = 0; i < nplans; i++)
+        if (!plans[i] || plans[i]->n != plans[0]->n) return LZX_INVALID_ARGUMENT;
+    return ntt_batch_path(plans, nplans, polys_per_plan, inverse ? NTT_INV : NTT_FWD, *path);
 }
 
 extern "C" lzx_status lzx_ntt_plan_constants(const lzx_ntt_plan *plan, uint32_t *psi, uint32_t *n_inv)

@@ -65,6 +65,8 @@ struct lzx_ntt_plan {
     lzx::Mod32 mod;
     std::vector<uint32_t> host_tab; // same words as dev_tab
     uint32_t *dev_tab;
+    uint8_t *mfma_dev;    // int8-MFMA limb matrices (n = 64, 2^20 < q < 2^31 - 2^16), else null
+    uint32_t kernel_pref; // LZX_NTT_PATH_AUTO, or the class an MFMA-capable launch must run
     int device;
 };
 
@@ -80,5 +82,9 @@ lzx_status ntt_forward_digits(const lzx_ntt_plan *plan, const int8_t *digits, ui
                               hipStream_t stream);
 lzx_status ntt_launch(const lzx_ntt_plan *const *plans, size_t nplans, uint32_t *data, size_t polys, NttMode mode,
                       uint32_t levels, hipStream_t stream);
+// The kernel class a batched forward/inverse of these plans runs (or
+// LZX_NTT_PATH_AUTO while an MFMA-capable shape has not been measured).
+lzx_status ntt_batch_path(const lzx_ntt_plan *const *plans, size_t nplans, size_t polys, NttMode mode,
+                          uint32_t &path);
 
 } // namespace lzx

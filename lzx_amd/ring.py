@@ -69,6 +69,11 @@ class NttTables:
         check(lib().lzx_ntt_plan_constants(self._h, C.byref(psi), C.byref(ninv)))
         return psi.value, ninv.value
 
+    def path(self, polys: int = 1, inverse: bool = False) -> str:
+        """The kernel class of this plan's batched transform of `polys`
+        polynomials (see batch_path)."""
+        return batch_path([self], polys, inverse)
+
     # -- exact reference signatures: in place on a host &mut [u32] ---------
     def _host(self, fn, a: np.ndarray, what: str):
         if a.dtype != np.uint32 or not a.flags.c_contiguous:
@@ -123,6 +128,19 @@ class NttTables:
 def _plan_array(tables):
     arr = (C.c_void_p * len(tables))(*[t.handle.value for t in tables])
     return arr
+
+
+PATHS = {0: "register", 1: "mfma", 2: "lds", 3: "auto"}
+
+
+def batch_path(tables, polys_per_plan: int, inverse: bool = False) -> str:
+    """The kernel class a batched transform of these plans runs: "mfma" (the
+    int8 matrix-core transform), "register" (butterflies in VGPRs), "lds"
+    (n > 64), or "auto" (MFMA-capable, not yet measured for this shape: the
+    first call times both classes and keeps the faster)."""
+    v = C.c_uint32()
+    check(lib().lzx_ntt_batch_path(_plan_array(tables), len(tables), polys_per_plan, int(inverse), C.byref(v)))
+    return PATHS[v.value]
 
 
 def forward_multi(tables, data, polys_per_plan: int, stream=None) -> None:

@@ -95,6 +95,130 @@ def test_crt_multi_full_workload(ring):
     assert np.array_equal(D.to_host(t, np.uint32).reshape(data.shape), data)
 
 
+# the int8-MFMA transform's modulus range is (2^20, 2^31 - 2^16): its smallest
+# and largest NTT-friendly primes (q = 1 mod 128, primitive roots 7 and 5)
+MFMA_EDGE = [(1049089, 9, 7), (2147415553, 9, 5)]
+
+
+def _patterns(rng, q, polys):
+    n = polys * 64
+    yield "random", rng.integers(0, q, size=n, dtype=np.uint64).astype(np.uint32)
+    yield "zeros", np.zeros(n, dtype=np.uint32)
+    yield "q-1", np.full(n, q - 1, dtype=np.uint32)
+    alt = np.zeros(n, dtype=np.uint32)
+    alt[1::2] = q - 1
+    yield "alternating", alt
+    yield "small", rng.integers(0, 256, size=n, dtype=np.uint64).astype(np.uint32)
+
+
+@pytest.mark.parametrize("mod", [*O.CRT_PRIMES, *MFMA_EDGE], ids=lambda m: str(m[0]))
+def test_mfma_transform_edges(ring, mod, monkeypatch):
+    """The int8-MFMA transform (n = 64): extreme residues and partial 32-
+    polynomial tiles, forward and inverse bit-exact vs the oracle, at the CRT
+    primes and at both ends of the modulus range."""
+    from lzx_amd import device as D
+    q, adic, g = mod
+    monkeypatch.setenv("LZX_NTT_KERNEL", "mfma")
+    tab = ring.NttTables(ring.Modulus32(q, adic, g), 6)
+    assert tab.path() == "mfma" and tab.path(inverse=True) == "mfma"
+    ot = O.Ntt(q, adic, g, 6)
+    rng = np.random.default_rng(q % 1000)
+    for polys in (1, 31, 32, 33, 65, 1000):
+        for name, a in _patterns(rng, q, polys):
+            _, want = ot.forward(a)
+            t = D.to_device(a)
+            tab.forward_batch(t)
+            assert np.array_equal(D.to_host(t, np.uint32), want), f"forward {name} polys={polys}"
+            _, want_inv = ot.inverse(a)
+            t = D.to_device(a)
+            tab.inverse_batch(t)
+            assert np.array_equal(D.to_host(t, np.uint32), want_inv), f"inverse {name} polys={polys}"
+
+
+def test_mfma_multi_plan_ragged(ring, monkeypatch):
+    """One launch over the 4 CRT primes with 1000 polynomials each (plan
+    boundaries inside blocks, a partial tile per plan): forward and inverse vs
+    the oracle."""
+    from lzx_amd import device as D
+    polys = 1000
+    rng = np.random.default_rng(5)
+    monkeypatch.setenv("LZX_NTT_KERNEL", "mfma")
+    tabs = [ring.NttTables(m, 6) for m in ring.CRT_PRIMES]
+    assert ring.batch_path(tabs, polys) == "mfma"
+    data = np.stack([rng.integers(0, m.q, size=polys * 64, dtype=np.uint64).astype(np.uint32) for m in ring.CRT_PRIMES])
+    t = D.to_device(data)
+    ring.forward_multi(tabs, t, polys)
+    got = D.to_host(t, np.uint32).reshape(data.shape)
+    for i, m in enumerate(ring.CRT_PRIMES):
+        _, want = O.Ntt(m.q, m.two_adicity, m.generator, 6).forward(data[i])
+        assert np.array_equal(got[i], want), f"forward prime {m.q}"
+    t = D.to_device(data)
+    ring.inverse_multi(tabs, t, polys)
+    got = D.to_host(t, np.uint32).reshape(data.shape)
+    for i, m in enumerate(ring.CRT_PRIMES):
+        _, want = O.Ntt(m.q, m.two_adicity, m.generator, 6).inverse(data[i])
+        assert np.array_equal(got[i], want), f"inverse prime {m.q}"
+
+
+@pytest.mark.parametrize("env", [("LZX_NTT_MFMA", "0"), ("LZX_NTT_KERNEL", "register")], ids=lambda e: e[0])
+@pytest.mark.parametrize("mod", [O.CRT_PRIMES[0], O.CRT_PRIMES[3]], ids=lambda m: str(m[0]))
+def test_butterfly_kernel_when_mfma_disabled(ring, mod, env, monkeypatch):
+    """Plans created without MFMA tables (LZX_NTT_MFMA=0) or pinned to the
+    register class (LZX_NTT_KERNEL=register) run the register butterfly
+    kernel (both lazy Shoup classes: q above and below 2^30), bit-exact vs
+    the oracle."""
+    from lzx_amd import device as D
+    q, adic, g = mod
+    monkeypatch.setenv(*env)
+    tab = ring.NttTables(ring.Modulus32(q, adic, g), 6)
+    assert tab.path() == "register"
+    ot = O.Ntt(q, adic, g, 6)
+    rng = np.random.default_rng(9)
+    for polys in (1, 63, 64, 1000):
+        for name, a in _patterns(rng, q, polys):
+            _, want = ot.forward(a)
+            t = D.to_device(a)
+            tab.forward_batch(t)
+            assert np.array_equal(D.to_host(t, np.uint32), want), f"forward {name} polys={polys}"
+            tab.inverse_batch(t)
+            assert np.array_equal(D.to_host(t, np.uint32), a), f"round trip {name} polys={polys}"
+
+
+def test_ntt_path_selection(ring):
+    """Kernel class per plan: MFMA-capable (so "auto" until measured) for
+    n = 64 inside its modulus range, the register kernels otherwise (q above
+    2^31, small q, n < 64), LDS above 64."""
+    assert ring.NttTables(ring.CRT_PRIMES[0], 6).path(polys=12345) == "auto"
+    assert ring.NttTables(ring.Modulus32(*O.Q_32), 6).path() == "register"
+    assert ring.NttTables(ring.Modulus32(*O.Q_12289), 6).path() == "register"
+    assert ring.NttTables(ring.CRT_PRIMES[0], 5).path() == "register"
+    assert ring.NttTables(ring.CRT_PRIMES[0], 7).path() == "lds"
+
+
+def test_auto_kernel_choice_measures_on_a_copy(ring):
+    """The first AUTO call of a shape times both classes on a scratch copy:
+    the caller's data is transformed exactly once (forward vs the oracle,
+    then inverse back), the choice is kept per direction, and later calls
+    reuse it."""
+    from lzx_amd import device as D
+    q, adic, g = O.CRT_PRIMES[1]
+    polys = 777
+    tab = ring.NttTables(ring.Modulus32(q, adic, g), 6)
+    assert tab.path(polys) == "auto" and tab.path(polys, inverse=True) == "auto"
+    rng = np.random.default_rng(77)
+    a = rng.integers(0, q, size=polys * 64, dtype=np.uint64).astype(np.uint32)
+    _, want = O.Ntt(q, adic, g, 6).forward(a)
+    t = D.to_device(a)
+    tab.forward_batch(t)
+    assert np.array_equal(D.to_host(t, np.uint32), want)
+    assert tab.path(polys) in ("mfma", "register") and tab.path(polys, inverse=True) == "auto"
+    tab.inverse_batch(t)
+    assert np.array_equal(D.to_host(t, np.uint32), a)
+    assert tab.path(polys, inverse=True) in ("mfma", "register")
+    tab.forward_batch(t)
+    assert np.array_equal(D.to_host(t, np.uint32), want)
+
+
 @pytest.mark.parametrize("mod", [O.Q_32, O.CRT_PRIMES[0]], ids=lambda m: str(m[0]))
 def test_lds_staged_register_kernel(ring, mod, monkeypatch):
     """The LDS-staged register kernel (plans created with LZX_NTT_STAGE=lds):
