@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE (each scaled by the reading for a 64 MiB copy kernel), per kernel
 # class of the plans; the rocprofv3 summaries are kept under profiles/.
 NTT_FWD_KERNEL = {"mfma": "ntt_mfma_kernel<NTT_FWD>", "register": "ntt_lazy_kernel<6,NTT_FWD>"}
-NTT_FWD_TRAFFIC_BYTES = {"mfma": None, "register": (65812 + 65536) * 1024}
+NTT_FWD_TRAFFIC_BYTES = {"mfma": (66670 + 65536) * 1024, "register": (65812 + 65536) * 1024}
 POLYS = 1 << 16
 LOG_N = 6
 N = 1 << LOG_N
