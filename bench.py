@@ -711,6 +711,7 @@ def main():
     ap.add_argument("--only-cpu", action="store_true", help="run only the CPU baseline legs")
     ap.add_argument("--only-ring50", action="store_true", help="run only the 50-bit quadratic-slot ring leg")
     ap.add_argument("--only-ntt-q32", action="store_true", help="run only the default-modulus (q > 2^31) NTT leg")
+    ap.add_argument("--only-ajtai", action="store_true", help="run only the large Ajtai commit leg (both paths)")
     ap.add_argument("--only-ntt-large", action="store_true",
                     help="run only the 4 x 2^18-polynomial NTT leg (both kernel classes)")
     ap.add_argument("--only-extension", action="store_true", help="run only the R_q[Y]/(Y^2+1) leg")
@@ -733,6 +734,9 @@ def main():
         return
     if args.only_ntt_q32:
         print(json.dumps({"ntt_q32": bench_ntt_q32()}), flush=True)
+        return
+    if args.only_ajtai:
+        print(json.dumps({"ajtai_commit": bench_ajtai()}), flush=True)
         return
     if args.only_ntt_large:
         print(json.dumps({"ntt_large": bench_ntt_large()}), flush=True)

@@ -142,27 +142,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             uint32_t win[17];
 #pragma unroll
             for (int q4 = 0; q4 < 17; q4++) win[q4] = R[d0 + q4];
+            // all four row-tile fragments first: each has its own registers, so
+            // realigning the next window never waits for an MFMA's operand reads
+            v4i af[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                v4i af;
+            for (int t = 0; t < 4; t++)
 #pragma unroll
                 for (int q4 = 0; q4 < 4; q4++)
-                    af[q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
+                    af[t][q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
+#pragma unroll
+            for (int t = 0; t < 4; t++)
 #pragma unroll
                 for (int u = 0; u < NT; u++)
-                    acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[u], acc[x][t][u], 0, 0, 0);
-            }
+                    acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[t], bf[u], acc[x][t][u], 0, 0, 0);
         }
     };
 
-    // B of ring element j lives in register set (j - j0) & 1 and is loaded one
-    // ring element ahead; digits are loaded one batch ahead.
+    // B of ring element j lives in register set (j - j0) % 4 and is loaded four
+    // ring elements ahead (an L2 round trip outlasts one ring element's
+    // MFMAs); digits are loaded one batch ahead.
     uint4 dnext[I8_ITW];
     load_digits(j0, dnext);
-    v4i b0[NT], b1[NT];
-    load_b(j0, b0);
+    v4i bs[4][NT];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++)
+        if (j0 + s4 < j1) load_b(j0 + s4, bs[s4]);
     bool split = false; // the current batch holds a -128 digit (wave-uniform)
-    for (uint32_t j = j0; j < j1; j += 2) {
+    for (uint32_t j = j0; j < j1; j += 4) {
         const uint32_t jj = (j - j0) % JB;
         if (jj == 0) {
             // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i]),
@@ -197,13 +203,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
             if (j + JB < j1) load_digits(j + JB, dnext);
         }
-        if (j + 1 < j1) load_b(j + 1, b1);
-        compute(Rs[wave], jj, b0);
-        if (split) compute(Rs2[wave], jj, b0);
-        if (j + 1 < j1) {
-            if (j + 2 < j1) load_b(j + 2, b0);
-            compute(Rs[wave], jj + 1, b1);
-            if (split) compute(Rs2[wave], jj + 1, b1);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++) {
+            if (j + s4 < j1) {
+                compute(Rs[wave], jj + s4, bs[s4]);
+                if (split) compute(Rs2[wave], jj + s4, bs[s4]);
+                if (j + s4 + 4 < j1) load_b(j + s4 + 4, bs[s4]);
+            }
         }
     }
     // D layout: lane (rho, g), register i -> row 4g + i of tile t, column 16u + rho = 4 * row_i + limb.
