@@ -628,9 +628,10 @@ static void dispatch_lds(int arith, NttMode mode, dim3 grid, size_t shmem, uint3
 // tables runs the class their creation asked for (LZX_NTT_KERNEL), or, for
 // the default AUTO, the class kept in the tuning cache for (device, moduli,
 // polynomials per plan, direction).  With none kept yet, the call times both
-// classes on a pool scratch copy of its data (the transforms are in place),
-// one untimed launch each and then three timed, keeps the faster, and runs
-// it on the caller's data.  Under stream capture it takes MFMA unmeasured.
+// classes on a pool scratch copy of its data (the transforms are in place):
+// one untimed launch each, then five launches of each class alternating, and
+// keeps the class with the faster best launch; then it runs that class on the
+// caller's data.  Under stream capture it takes MFMA unmeasured.
 static std::string ntt_path_key(const lzx_ntt_plan *const *plans, size_t cnt, size_t polys, NttMode mode)
 {
     std::string k = "ntt_path dev=" + std::to_string(plans[0]->device) + " n=" + std::to_string(plans[0]->n) +
@@ -669,38 +670,40 @@ static lzx_status choose_class(const lzx_ntt_plan *const *plans, size_t cnt, con
     const size_t bytes = cnt * polys * plans[0]->n * sizeof(uint32_t);
     uint32_t *tmp = nullptr;
     LZX_HIP(scratch_alloc((void **)&tmp, bytes, stream));
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // the two classes alternate launch by launch, each between its own events,
+    // and each class's fastest launch counts: clocks that are still ramping or
+    // a neighbour's burst then bias neither class
+    constexpr int PAIRS = 5, NEV = 2 * PAIRS + 1;
+    hipEvent_t ev[NEV] = {};
     lzx_status st = LZX_OK;
-    for (int k = 0; k < 3 && st == LZX_OK; k++)
+    for (int k = 0; k < NEV && st == LZX_OK; k++)
         if (hipEventCreate(&ev[k]) != hipSuccess) st = device_fail(hipGetLastError(), "ntt path events");
     if (st == LZX_OK && hipMemcpyAsync(tmp, chunk, bytes, hipMemcpyDeviceToDevice, stream) != hipSuccess)
         st = device_fail(hipGetLastError(), "ntt path copy");
-    auto run = [&](uint32_t cls, int reps) {
-        for (int r = 0; r < reps; r++) {
-            if (cls == LZX_NTT_PATH_MFMA)
-                ntt_mfma_launch(plans, cnt, tmp, (uint32_t)polys, mode, stream);
-            else
-                launch_register(plans, cnt, ts, tmp, polys, mode, stream);
-        }
+    auto run = [&](uint32_t cls) {
+        if (cls == LZX_NTT_PATH_MFMA)
+            ntt_mfma_launch(plans, cnt, tmp, (uint32_t)polys, mode, stream);
+        else
+            launch_register(plans, cnt, ts, tmp, polys, mode, stream);
     };
-    float t_mf = 0, t_reg = 0;
+    float t_mf = 1e30f, t_reg = 1e30f;
     if (st == LZX_OK) {
-        run(LZX_NTT_PATH_MFMA, 1); // untimed: first-launch costs
-        run(LZX_NTT_PATH_REGISTER, 1);
+        run(LZX_NTT_PATH_MFMA); // untimed: first-launch costs
+        run(LZX_NTT_PATH_REGISTER);
         if (hipEventRecord(ev[0], stream) != hipSuccess) st = device_fail(hipGetLastError(), "event");
     }
-    if (st == LZX_OK) {
-        run(LZX_NTT_PATH_MFMA, 3);
-        if (hipEventRecord(ev[1], stream) != hipSuccess) st = device_fail(hipGetLastError(), "event");
+    for (int k = 0; k < 2 * PAIRS && st == LZX_OK; k++) {
+        run(k % 2 == 0 ? LZX_NTT_PATH_MFMA : LZX_NTT_PATH_REGISTER);
+        if (hipEventRecord(ev[k + 1], stream) != hipSuccess) st = device_fail(hipGetLastError(), "event");
     }
     if (st == LZX_OK) {
-        run(LZX_NTT_PATH_REGISTER, 3);
-        if (hipEventRecord(ev[2], stream) != hipSuccess) st = device_fail(hipGetLastError(), "event");
-    }
-    if (st == LZX_OK) {
-        hipError_t e = hipEventSynchronize(ev[2]);
-        if (e == hipSuccess) e = hipEventElapsedTime(&t_mf, ev[0], ev[1]);
-        if (e == hipSuccess) e = hipEventElapsedTime(&t_reg, ev[1], ev[2]);
+        hipError_t e = hipEventSynchronize(ev[NEV - 1]);
+        for (int k = 0; k < 2 * PAIRS && e == hipSuccess; k++) {
+            float t = 0;
+            e = hipEventElapsedTime(&t, ev[k], ev[k + 1]);
+            float &best = k % 2 == 0 ? t_mf : t_reg;
+            if (e == hipSuccess) best = std::min(best, t);
+        }
         if (e != hipSuccess) st = device_fail(e, "ntt path timing");
     }
     for (hipEvent_t e : ev)
