@@ -13,17 +13,21 @@
 // with M = 64 W, N = 4 k, K = 64 m, followed by the exact recombination
 // sum_L 256^L Out[.][(i, L)] mod q.
 //
-// Kernel mapping (v_mfma_i32_16x16x64_i8: one ring element = one K step):
+// Kernel mapping (v_mfma_i32_32x32x32_i8: one ring element = two K steps):
 //   - a block of 4 waves owns ITW * 4 witnesses and a chunk of j; each wave
-//     owns ITW witnesses, all 64 output coefficients (4 row tiles) and 16*NT
-//     limb columns (NT column tiles), i.e. 4*NT accumulators per witness;
-//   - B fragments (limb planes, layout [j][col][c], 16 contiguous bytes per
-//     lane) come straight from HBM/L2 as 16-byte loads, reused across the
-//     wave's witnesses and row tiles;
+//     owns ITW witnesses, all 64 output coefficients (2 row blocks of 32) and
+//     32*NT limb columns (NT column blocks), i.e. 2*NT accumulators of 16
+//     registers per witness, 4*NT MFMAs per witness and ring element;
+//   - B fragments come straight from HBM/L2, reused across the wave's
+//     witnesses and row blocks.  The limb planes are stored in fragment order
+//     (see ajtai_limbs_kernel), so each fragment is one contiguous 1 KiB
+//     load: with a column-major layout every 16-byte lane load would touch
+//     its own cache line and the loads, not the MFMAs, would set the pace;
 //   - A fragments are never materialised in HBM: per (w, j) the wave writes
-//     R = (s reversed, then -s reversed) -- 128 bytes -- to LDS, and lane
-//     (rho, g) reads one 64-byte window of R; because S is Toeplitz, the four
-//     row-tile fragments are consecutive 16-byte slices of that window, each
+//     R = (s reversed, then -s reversed) -- 128 bytes -- to LDS.  S is
+//     Toeplitz (S[r][c] = R[63 - r + c]), so the fragment of row block rb and
+//     K half kb depends on kb - rb only: lane (rho, g) takes the three
+//     16-byte slices of R at 31 - rho + 16g + 32t, t = 0, 1, 2, each
 //     realigned with v_alignbyte;
 //   - every int8 digit is exact, -128 included.  -(-128) does not fit int8,
 //     so a digit vector is split as s = s1 + s2 with s1 = s where s != -128
@@ -47,10 +51,11 @@
 namespace lzx {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int I8_WAVES = 4;
 constexpr int I8_ITW = 2;          // witnesses per wave
-constexpr uint32_t I8_MAX_JCHUNK = 2032; // 2032 * 2^20 < 2^31
+constexpr uint32_t I8_MAX_JCHUNK = 2032; // 2032 * 2^20 < 2^31, a multiple of 16
 constexpr uint64_t I8_Q_LIMIT = (1ULL << 32) - (1ULL << 25);
 
 // Bytewise two's-complement negation (each byte of x must be != 0x80).
@@ -95,141 +100,149 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (chunk >= nchunks) return;
     // wave index as a scalar, so witness rows and their digit pointers stay in SGPRs
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const uint32_t rho = lane & 15, g = lane >> 4;
+    const uint32_t rho = lane & 31, g = lane >> 5;
     const uint32_t wbase = (grp * I8_WAVES + wave) * I8_ITW;
     const uint32_t j0 = chunk * jchunk, j1 = min(m, j0 + jchunk);
-    const uint32_t omin = 15 - rho + 16 * g; // window start in R
+    const uint32_t omin = 31 - rho + 16 * g; // first slice start in R (0..47); slices at omin + 32t
     const uint32_t d0 = omin >> 2, sh = omin & 3;
     const uint32_t e = lane >> 2, kq = lane & 3; // digit batch: ring element e, 16-byte chunk kq
 
-    // padding dwords 32..35 of every R slot stay zero
-    for (uint32_t idx = lane; idx < I8_ITW * JB; idx += 64) {
-        *reinterpret_cast<uint4 *>(&Rs[wave][idx / JB][idx % JB][32]) = make_uint4(0, 0, 0, 0);
-        *reinterpret_cast<uint4 *>(&Rs2[wave][idx / JB][idx % JB][32]) = make_uint4(0, 0, 0, 0);
-    }
-
-    v4i acc[I8_ITW][4][NT];
+    v16i acc[I8_ITW][2][NT];
 #pragma unroll
     for (int x = 0; x < I8_ITW; x++)
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-            for (int u = 0; u < NT; u++) acc[x][t][u] = v4i{0, 0, 0, 0};
+            for (int u = 0; u < NT; u++)
+#pragma unroll
+                for (int i = 0; i < 16; i++) acc[x][rb][u][i] = 0;
 
-    // uniform base pointers + 32-bit lane offsets (saddr-form global loads)
-    const uint32_t boff = (col0 + rho) * 64 + 16 * g;
-    const uint32_t doff = e * 64 + 16 * kq;
+    // uniform base pointers + 32-bit lane offsets (saddr-form global loads).
+    // Every load is unconditional, from a clamped (valid) address: a load on
+    // one path of a branch makes the compiler wait for all loads in flight
+    // where the paths meet, which would serialise the B prefetch.
+    const uint32_t boff = (col0 / 32) * 2048 + 16 * lane;
+    // (witnesses past W read row W - 1: their results are never stored)
     auto load_digits = [&](uint32_t jb, uint4 (&dst)[I8_ITW]) {
+        const uint32_t je = min(jb + e, m - 1);
 #pragma unroll
         for (int x = 0; x < I8_ITW; x++) {
             const uint32_t w = min(wbase + x, W - 1);
-            const int8_t *base = digits + ((size_t)w * m + jb) * 64;
-            dst[x] = (wbase + x < W && jb + e < j1) ? *reinterpret_cast<const uint4 *>(base + doff)
-                                                     : make_uint4(0, 0, 0, 0);
+            dst[x] = *reinterpret_cast<const uint4 *>(digits + ((size_t)w * m + je) * 64 + 16 * kq);
         }
     };
-    auto load_b = [&](uint32_t j, v4i (&bf)[NT]) {
-        const int8_t *base = limbs + (size_t)j * npad * 64;
+    // B fragment (u, kb): limb column col0 + 32u + rho, bytes 32kb + 16g .. +16, stored contiguously
+    auto load_b = [&](uint32_t j, v4i (&bf)[NT][2]) {
+        const int8_t *base = limbs + (size_t)min(j, m - 1) * npad * 64;
 #pragma unroll
-        for (int u = 0; u < NT; u++) bf[u] = *reinterpret_cast<const v4i *>(base + boff + 16 * 64 * u);
+        for (int u = 0; u < NT; u++)
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++)
+                bf[u][kb] = *reinterpret_cast<const v4i *>(base + boff + 2048 * u + 1024 * kb);
     };
 
-    // One window of R per (witness, ring element): the four row-tile fragments.
-    auto compute = [&](uint32_t (&Rw)[I8_ITW][JB][36], uint32_t jj, const v4i (&bf)[NT]) {
+    // One pass over R per (witness, ring element): three slices, 4*NT MFMAs.
+    auto compute = [&](uint32_t (&Rw)[I8_ITW][JB][36], uint32_t jj, const v4i (&bf)[NT][2]) {
 #pragma unroll
         for (int x = 0; x < I8_ITW; x++) {
             const uint32_t *R = Rw[x][jj];
-            uint32_t win[17];
+            uint32_t win[3][5];
 #pragma unroll
-            for (int q4 = 0; q4 < 17; q4++) win[q4] = R[d0 + q4];
-            // all four row-tile fragments first: each has its own registers, so
-            // realigning the next window never waits for an MFMA's operand reads
-            v4i af[4];
+            for (int t = 0; t < 3; t++)
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+                for (int q4 = 0; q4 < 5; q4++) win[t][q4] = R[d0 + 8 * t + q4];
+            // af[t] is the fragment for kb - rb = t - 1 (rows 32rb + rho, K bytes 32kb + 16g ..)
+            v4i af[3];
+#pragma unroll
+            for (int t = 0; t < 3; t++)
 #pragma unroll
                 for (int q4 = 0; q4 < 4; q4++)
-                    af[t][q4] = (int)__builtin_amdgcn_alignbyte(win[4 * (3 - t) + q4 + 1], win[4 * (3 - t) + q4], sh);
+                    af[t][q4] = (int)__builtin_amdgcn_alignbyte(win[t][q4 + 1], win[t][q4], sh);
+            // the two K halves of one accumulator are 2*NT MFMAs apart
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int kb = 0; kb < 2; kb++)
 #pragma unroll
-                for (int u = 0; u < NT; u++)
-                    acc[x][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[t], bf[u], acc[x][t][u], 0, 0, 0);
+                for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                    for (int u = 0; u < NT; u++)
+                        acc[x][rb][u] =
+                            __builtin_amdgcn_mfma_i32_32x32x32_i8(af[kb - rb + 1], bf[u][kb], acc[x][rb][u], 0, 0, 0);
         }
     };
 
     // B of ring element j lives in register set (j - j0) % 4 and is loaded four
     // ring elements ahead (an L2 round trip outlasts one ring element's
-    // MFMAs); digits are loaded one batch ahead.
+    // MFMAs); digits are loaded one batch ahead.  The batch loop is unrolled;
+    // ring elements past j1 (last chunk only) skip their MFMAs, while their
+    // loads still run (from clamped rows) so that every path issues the same
+    // loads.
+    v4i bs[4][NT][2];
+    auto batch = [&](bool split, uint32_t jb) {
+#pragma unroll
+        for (int s = 0; s < JB; s++) {
+            if (jb + s < j1) {
+                compute(Rs[wave], s, bs[s % 4]);
+                if (split) compute(Rs2[wave], s, bs[s % 4]);
+            }
+            load_b(jb + s + 4, bs[s % 4]);
+            __builtin_amdgcn_sched_barrier(0); // one ring element at a time: bounds register pressure
+        }
+    };
     uint4 dnext[I8_ITW];
     load_digits(j0, dnext);
-    v4i bs[4][NT];
 #pragma unroll
-    for (int s4 = 0; s4 < 4; s4++)
-        if (j0 + s4 < j1) load_b(j0 + s4, bs[s4]);
-    bool split = false; // the current batch holds a -128 digit (wave-uniform)
-    for (uint32_t j = j0; j < j1; j += 4) {
-        const uint32_t jj = (j - j0) % JB;
-        if (jj == 0) {
-            // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i]),
-            // built from s1; R2 the same from s2 when the batch has a -128 digit
-            uint32_t mins = 0;
+    for (int s4 = 0; s4 < 4; s4++) load_b(j0 + s4, bs[s4]);
+    for (uint32_t jb = j0; jb < j1; jb += JB) {
+        // R for the batch: R[15 - i] = bswap(s4[i]) (s reversed), R[31 - i] = -bswap(s4[i]),
+        // built from s1; R2 the same from s2 when the batch has a -128 digit
+        uint32_t mins = 0;
 #pragma unroll
-            for (int x = 0; x < I8_ITW; x++)
-                mins |= min_bytes(dnext[x].x) | min_bytes(dnext[x].y) | min_bytes(dnext[x].z) |
-                        min_bytes(dnext[x].w);
-            split = __ballot(mins != 0) != 0;
+        for (int x = 0; x < I8_ITW; x++)
+            mins |= min_bytes(dnext[x].x) | min_bytes(dnext[x].y) | min_bytes(dnext[x].z) | min_bytes(dnext[x].w);
+        const bool split = __ballot(mins != 0) != 0; // wave-uniform
 #pragma unroll
-            for (int x = 0; x < I8_ITW; x++) {
-                uint4 d = dnext[x];
-                uint32_t *R = Rs[wave][x][e];
-                if (split) {
-                    // s1 = s + 1 where s = -128 (0x80 -> 0x81, no carry); s2 = -1 there
-                    const uint32_t o0 = min_bytes(d.x) >> 7, o1 = min_bytes(d.y) >> 7, o2 = min_bytes(d.z) >> 7,
-                                   o3 = min_bytes(d.w) >> 7;
-                    d = make_uint4(d.x + o0, d.y + o1, d.z + o2, d.w + o3);
-                    const uint32_t t0 = __builtin_bswap32(o0), t1 = __builtin_bswap32(o1), t2 = __builtin_bswap32(o2),
-                                   t3 = __builtin_bswap32(o3);
-                    uint32_t *R2 = Rs2[wave][x][e];
-                    // s2 bytes are -1 (= 0x01 * 255), -s2 bytes are +1
-                    *reinterpret_cast<uint4 *>(R2 + 12 - 4 * kq) = make_uint4(t3 * 255u, t2 * 255u, t1 * 255u, t0 * 255u);
-                    *reinterpret_cast<uint4 *>(R2 + 28 - 4 * kq) = make_uint4(t3, t2, t1, t0);
-                }
-                const uint32_t r0 = __builtin_bswap32(d.x), r1 = __builtin_bswap32(d.y), r2 = __builtin_bswap32(d.z),
-                               r3 = __builtin_bswap32(d.w);
-                *reinterpret_cast<uint4 *>(R + 12 - 4 * kq) = make_uint4(r3, r2, r1, r0);
-                *reinterpret_cast<uint4 *>(R + 28 - 4 * kq) =
-                    make_uint4(neg_bytes(r3), neg_bytes(r2), neg_bytes(r1), neg_bytes(r0));
+        for (int x = 0; x < I8_ITW; x++) {
+            uint4 d = dnext[x];
+            uint32_t *R = Rs[wave][x][e];
+            if (split) {
+                // s1 = s + 1 where s = -128 (0x80 -> 0x81, no carry); s2 = -1 there
+                const uint32_t o0 = min_bytes(d.x) >> 7, o1 = min_bytes(d.y) >> 7, o2 = min_bytes(d.z) >> 7,
+                               o3 = min_bytes(d.w) >> 7;
+                d = make_uint4(d.x + o0, d.y + o1, d.z + o2, d.w + o3);
+                const uint32_t t0 = __builtin_bswap32(o0), t1 = __builtin_bswap32(o1), t2 = __builtin_bswap32(o2),
+                               t3 = __builtin_bswap32(o3);
+                uint32_t *R2 = Rs2[wave][x][e];
+                // s2 bytes are -1 (= 0x01 * 255), -s2 bytes are +1
+                *reinterpret_cast<uint4 *>(R2 + 12 - 4 * kq) = make_uint4(t3 * 255u, t2 * 255u, t1 * 255u, t0 * 255u);
+                *reinterpret_cast<uint4 *>(R2 + 28 - 4 * kq) = make_uint4(t3, t2, t1, t0);
             }
-            if (j + JB < j1) load_digits(j + JB, dnext);
+            const uint32_t r0 = __builtin_bswap32(d.x), r1 = __builtin_bswap32(d.y), r2 = __builtin_bswap32(d.z),
+                           r3 = __builtin_bswap32(d.w);
+            *reinterpret_cast<uint4 *>(R + 12 - 4 * kq) = make_uint4(r3, r2, r1, r0);
+            *reinterpret_cast<uint4 *>(R + 28 - 4 * kq) =
+                make_uint4(neg_bytes(r3), neg_bytes(r2), neg_bytes(r1), neg_bytes(r0));
         }
-#pragma unroll
-        for (int s4 = 0; s4 < 4; s4++) {
-            if (j + s4 < j1) {
-                compute(Rs[wave], jj + s4, bs[s4]);
-                if (split) compute(Rs2[wave], jj + s4, bs[s4]);
-                if (j + s4 + 4 < j1) load_b(j + s4 + 4, bs[s4]);
-            }
-        }
+        load_digits(jb + JB, dnext);
+        batch(split, jb);
     }
-    // D layout: lane (rho, g), register i -> row 4g + i of tile t, column 16u + rho = 4 * row_i + limb.
+    // D layout: lane (rho, g), register i -> row 32rb + 8(i/4) + 4g + i%4, column 32u + rho = 4 * row_i + limb.
     // Combine limbs across the lane quad, fold mod q, one u32 per (chunk, w, i, r).
 #pragma unroll
     for (int x = 0; x < I8_ITW; x++) {
         const uint32_t w = wbase + x;
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int rb = 0; rb < 2; rb++)
 #pragma unroll
             for (int u = 0; u < NT; u++)
 #pragma unroll
-                for (int ireg = 0; ireg < 4; ireg++) {
-                    const int64_t v = combine_limbs(acc[x][t][u][ireg]);
-                    const uint32_t irow = (col0 + 16 * u + rho) >> 2;
+                for (int ireg = 0; ireg < 16; ireg++) {
+                    const int64_t v = combine_limbs(acc[x][rb][u][ireg]);
+                    const uint32_t irow = (col0 + 32 * u + rho) >> 2;
                     if ((lane & 3) == 0 && w < W && irow < k) {
                         const uint64_t mag = (uint64_t)(v < 0 ? -v : v);
                         uint32_t res = reduce_u64(mag, q, barrett);
                         if (v < 0 && res) res = q - res;
-                        const uint32_t r = 16 * t + 4 * g + ireg;
+                        const uint32_t r = 32 * rb + 8 * (ireg / 4) + 4 * g + (ireg % 4);
                         partial[(((size_t)chunk * W + w) * k + irow) * 64 + r] = res;
                     }
                 }
@@ -247,17 +260,22 @@ __global__ void ajtai_i8_reduce_kernel(const uint32_t *__restrict__ partial, uin
     }
 }
 
-// limbs[j][col][c], col = 4i + L (columns >= 4k are zero): balanced base-256
-// digits of the centred coefficient of A[i][j][c].
+// Balanced base-256 digits of the centred coefficient of A[i][j][c], for limb
+// column col = 4i + L (columns >= 4k are zero), in MFMA fragment order: byte
+// (j, col, c) sits at j*npad*64 + ((2*(col/32) + c/32)*64 + lane)*16 + c%16
+// with lane = col%32 + 32*((c/16)%2), so the 64 lanes' 16-byte pieces of one
+// fragment (32 columns x 32 bytes of c) are one contiguous 1 KiB.
 __global__ void ajtai_limbs_kernel(const uint32_t *__restrict__ A, int8_t *__restrict__ limbs, uint32_t k,
                                    uint32_t m, uint32_t npad, uint32_t q)
 {
     const size_t total = (size_t)m * npad * 64;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t c = e & 63;
-        const size_t jc = e >> 6;
-        const uint32_t col = jc % npad;
-        const size_t j = jc / npad;
+        // e = j*npad*64 + (fragment f, lane, byte b); fragment f = 2*(col/32) + c/32
+        const uint32_t b = e & 15, lane = (e >> 4) & 63;
+        const uint32_t f = (uint32_t)((e >> 10) % (npad / 16));
+        const size_t j = (e >> 10) / (npad / 16);
+        const uint32_t col = 32 * (f >> 1) + (lane & 31);
+        const uint32_t c = 32 * (f & 1) + 16 * (lane >> 5) + b;
         const uint32_t i = col >> 2, L = col & 3;
         int8_t out = 0;
         if (i < k) {
@@ -276,7 +294,7 @@ __global__ void ajtai_limbs_kernel(const uint32_t *__restrict__ A, int8_t *__res
 
 bool ajtai_int8_supported(const lzx_ajtai_key *key) { return key->n == 64 && key->plan->q < I8_Q_LIMIT; }
 
-static uint32_t npad_of(size_t k) { return (uint32_t)(16 * ((4 * k + 15) / 16)); }
+static uint32_t npad_of(size_t k) { return (uint32_t)(32 * ((4 * k + 31) / 32)); }
 
 lzx_status ajtai_build_limbs(lzx_ajtai_key *key)
 {
@@ -313,7 +331,8 @@ static lzx_status i8_plan(const lzx_ajtai_key *key, size_t W, uint32_t max_chunk
     nchunks = std::min<uint32_t>(nchunks, max_chunks);
     nchunks = std::max<uint32_t>(nchunks, min_chunks);
     nchunks = std::max<uint32_t>(nchunks, 1);
-    p.jchunk = (p.m + nchunks - 1) / nchunks;
+    // whole digit batches of 16 ring elements (I8_MAX_JCHUNK is one too)
+    p.jchunk = std::min<uint32_t>(I8_MAX_JCHUNK, ((p.m + nchunks - 1) / nchunks + 15) / 16 * 16);
     p.nchunks = (p.m + p.jchunk - 1) / p.jchunk;
     p.padded = (p.nchunks + 7) / 8 * 8;
     if ((uint64_t)p.padded * p.groups > 0x7FFFFFFFull) return LZX_UNSUPPORTED;
@@ -330,17 +349,15 @@ static lzx_status i8_launch(const lzx_ajtai_key *key, const I8Plan &p, const int
     const uint32_t q = key->plan->q;
     const uint64_t barrett = key->plan->mod.barrett;
     for (uint32_t col0 = 0; col0 < p.npad; col0 += 64) {
-        const uint32_t nt = std::min<uint32_t>(4, (p.npad - col0) / 16);
+        const uint32_t nt = std::min<uint32_t>(2, (p.npad - col0) / 32);
         const dim3 grid(p.padded * p.groups);
 #define LZX_I8_LAUNCH(NT)                                                                                        \
     hipLaunchKernelGGL(ajtai_i8_kernel<NT>, grid, dim3(256), 0, stream, digits_dev, key->limbs_dev, partial,      \
                        (uint32_t)W, p.m, p.k, p.npad, col0, p.jchunk, p.nchunks, p.groups, q, barrett)
-        switch (nt) {
-        case 1: LZX_I8_LAUNCH(1); break;
-        case 2: LZX_I8_LAUNCH(2); break;
-        case 3: LZX_I8_LAUNCH(3); break;
-        default: LZX_I8_LAUNCH(4); break;
-        }
+        if (nt == 1)
+            LZX_I8_LAUNCH(1);
+        else
+            LZX_I8_LAUNCH(2);
 #undef LZX_I8_LAUNCH
         LZX_LAUNCH_CHECK("ajtai_i8_kernel");
     }
