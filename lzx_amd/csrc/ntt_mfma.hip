@@ -41,7 +41,13 @@
 //     LDS; each wave transforms whole tiles of 32 polynomials (8 KiB);
 //   - every global access is a whole 1 KiB per wave instruction (4
 //     polynomials), and the wave's next tile is loaded into VGPRs while the
-//     current one is multiplied.  The transposes between that layout and the
+//     current one is multiplied.  Loads and stores go through one buffer
+//     resource per tile sized to its polynomials: lanes of a ragged last tile
+//     read zeros and their stores are dropped by the hardware range check, so
+//     every tile issues the same 8 loads and 8 stores with no branch.  (With
+//     branch-skipped stores the compiler cannot count the stores that follow
+//     the prefetch, and its wait for the prefetched tile then also waits for
+//     the previous tile's first store.)  The transposes between that layout and the
 //     MFMA operand layout go through a per-wave 8 KiB LDS tile in a swizzled
 //     order, conflict-free on both sides (contiguous 16-byte operand loads of
 //     every polynomial at 64-byte granularity measured 10% slower in the
@@ -67,7 +73,7 @@
 namespace lzx {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) v4i g_v4i;
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(4))) uint32_t *c_u32p;
 
 constexpr int C0 = (int)0x80808080;
@@ -129,12 +135,14 @@ __global__ __launch_bounds__(64 * MF_WAVES) __attribute__((amdgpu_waves_per_eu(2
         np = min(TP, polys - p0);
         return (uint8_t *)(data + ((size_t)plan * polys + p0) * 64);
     };
-    auto fetch = [&](const uint8_t *base, uint32_t np, v4i (&X)[8]) {
+    // np polynomials from base: offsets past np * 256 bytes read zeros / drop stores
+    auto rsrc_of = [&](uint8_t *base, uint32_t np) {
+        return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)(np * 256), 0x00020000);
+    };
+    auto fetch = [&](__amdgpu_buffer_rsrc_t r, v4i (&X)[8]) {
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const uint32_t p = 4 * i + la;
-            X[i] = *(const g_v4i *)(base + (p < np ? 1024 * i + 16 * lane : 16 * ls));
-        }
+        for (int i = 0; i < 8; i++)
+            X[i] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, 1024 * i + 16 * lane, 0, 0));
     };
 
     for (uint32_t s = t0; s < t1;) {
@@ -144,23 +152,36 @@ __global__ __launch_bounds__(64 * MF_WAVES) __attribute__((amdgpu_waves_per_eu(2
         const c_u32p aux = (c_u32p)(mat + MF_MAT);
         const uint32_t q = aux[0], qinv = aux[1], b2 = aux[2];
         v4i X[8];
-        uint32_t tt = s + wave, np = 0;
-        uint8_t *base = nullptr;
+        uint32_t tt = s + wave;
         __syncthreads(); // every wave is done with the previous segment's matrices
         for (uint32_t c = wave; c < MF_DIR / 1024; c += WAVES)
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(mat + 1024 * c + 16 * lane),
                                              (__attribute__((address_space(3))) void *)(S + 1024 * c), 16, 0, 0);
-        if (tt < s_end) {
-            base = locate(tt, plan, np);
-            fetch(base, np, X);
-        }
+        // the resource of tile t (an empty one past the segment: no traffic)
+        auto tile_rsrc = [&](uint32_t t) {
+            uint32_t n = 0;
+            uint8_t *b = (uint8_t *)data;
+            if (t < s_end) b = locate(t, plan, n);
+            return rsrc_of(b, n);
+        };
+        // Pipeline per wave: the tile in T is multiplied while the next one is
+        // in flight into X; its results are stored only after X has been
+        // written to T.  vmcnt does not order loads against stores, so a wait
+        // for X also waits for every store in flight: issuing the stores after
+        // that wait leaves a whole tile's compute between each store and the
+        // next wait.
+        __amdgpu_buffer_rsrc_t rs = tile_rsrc(tt);
+        fetch(rs, X);
         __syncthreads();
+        // contiguous words -> swizzled tile
+#pragma unroll
+        for (int i = 0; i < 8; i++) *reinterpret_cast<v4i *>(T + mfc_pos(i, la, ls)) = X[i];
+        __amdgpu_buffer_rsrc_t rn = tile_rsrc(tt + WAVES);
+        fetch(rn, X);
+        mfc_wave_sync();
 
         for (; tt < s_end; tt += WAVES) {
-            // contiguous words -> swizzled tile -> operands
-#pragma unroll
-            for (int i = 0; i < 8; i++) *reinterpret_cast<v4i *>(T + mfc_pos(i, la, ls)) = X[i];
-            mfc_wave_sync();
+            // swizzled tile -> operands
             v4i B[2][2][2];
 #pragma unroll
             for (int pb = 0; pb < 2; pb++)
@@ -178,14 +199,6 @@ __global__ __launch_bounds__(64 * MF_WAVES) __attribute__((amdgpu_waves_per_eu(2
                         B[pb][1][kb] = (x + qq - y) ^ C0;
                     }
                 }
-            // the wave's next tile
-            const uint32_t tn = tt + WAVES;
-            uint32_t npn = 0;
-            uint8_t *basen = nullptr;
-            if (tn < s_end) {
-                basen = locate(tn, plan, npn);
-                fetch(basen, npn, X);
-            }
             const v4i i2 = {(int)b2, (int)b2, (int)b2, (int)b2};
             const v4i z = {0, 0, 0, 0};
             mfc_wave_sync(); // the operand reads precede the result writes
@@ -250,21 +263,20 @@ __global__ __launch_bounds__(64 * MF_WAVES) __attribute__((amdgpu_waves_per_eu(2
                 }
             }
             mfc_wave_sync();
-            // swizzled tile -> contiguous stores
+            // swizzled tile -> registers; the next tile -> T; then the stores
             v4i V[8];
 #pragma unroll
             for (int i = 0; i < 8; i++) V[i] = *reinterpret_cast<const v4i *>(T + mfc_pos(i, la, ls));
-            if (np == TP) {
+            mfc_wave_sync();
 #pragma unroll
-                for (int i = 0; i < 8; i++) *(g_v4i *)(base + 1024 * i + 16 * lane) = V[i];
-            } else {
+            for (int i = 0; i < 8; i++) *reinterpret_cast<v4i *>(T + mfc_pos(i, la, ls)) = X[i];
 #pragma unroll
-                for (int i = 0; i < 8; i++)
-                    if (4 * i + la < np) *(g_v4i *)(base + 1024 * i + 16 * lane) = V[i];
-            }
-            mfc_wave_sync(); // the tile is free for the next one
-            base = basen;
-            np = npn;
+            for (int i = 0; i < 8; i++)
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, V[i]), rs, 1024 * i + 16 * lane, 0, 0);
+            rs = rn;
+            rn = tile_rsrc(tt + 2 * WAVES);
+            fetch(rn, X);
+            mfc_wave_sync();
         }
         s = s_end;
     }
