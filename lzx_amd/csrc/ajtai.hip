@@ -6,7 +6,9 @@
 //   commit    (:118-153) forward NTT of each s_j, pointwise MAC of A^_ij * s^_j
 //                        into k accumulators, one inverse NTT per row.
 //
-// Device pipeline of a batched commit (W witnesses):
+// Device pipeline of a batched commit (W witnesses) where the key holds no
+// matrix-core fragments (ajtai_nttmm.hip computes the same sum on int8 MFMA
+// where it does):
 //   1. s^ = forward NTT of every witness element (scratch, [W][m][n]);
 //   2. ajtai_mac_kernel: split-K over j.  For each coefficient t the product
 //      is a (W x m) * (m x k) matrix product mod q; a thread owns one t and a
@@ -120,6 +122,11 @@ lzx_status ajtai_commit_ntt(const lzx_ajtai_key *key, const uint32_t *s_dev, con
                             uint32_t *out_dev, hipStream_t stream)
 {
     if (W == 0) return LZX_OK;
+    if (ajtai_nttmm_enabled(key)) {
+        // the same sum with the multiply-accumulate on the matrix cores
+        const lzx_status st = ajtai_commit_nttmm(key, s_dev, digits_dev, W, out_dev, stream);
+        if (st != LZX_UNSUPPORTED) return st;
+    }
     const size_t n = key->n, m = key->m, k = key->k;
     const uint32_t q = key->plan->q;
     const uint64_t barrett = key->plan->mod.barrett;
@@ -197,6 +204,10 @@ static lzx_status key_finish_device(lzx_ajtai_key *key)
     if (st != LZX_OK) return st;
     if (ajtai_int8_supported(key)) {
         st = ajtai_build_limbs(key);
+        if (st != LZX_OK) return st;
+    }
+    if (ajtai_nttmm_supported(key)) {
+        st = ajtai_build_nttmm(key);
         if (st != LZX_OK) return st;
     }
     LZX_HIP(hipDeviceSynchronize());
@@ -385,6 +396,8 @@ static lzx_status key_new(uint32_t q, uint32_t adic, uint32_t g, uint32_t log_n,
     kk->A_dev = kk->Ahat_dev = nullptr;
     kk->limbs_dev = nullptr;
     kk->limbs_bytes = 0;
+    kk->nm_afrag = nullptr;
+    kk->nm_corr = nullptr;
     kk->device = plan->device;
     *key = kk;
     return LZX_OK;
@@ -400,6 +413,8 @@ extern "C" void lzx_ajtai_key_destroy(lzx_ajtai_key *key)
     if (key->A_dev) (void)hipFree(key->A_dev);
     if (key->Ahat_dev) (void)hipFree(key->Ahat_dev);
     if (key->limbs_dev) (void)hipFree(key->limbs_dev);
+    if (key->nm_afrag) (void)hipFree(key->nm_afrag);
+    if (key->nm_corr) (void)hipFree(key->nm_corr);
     lzx_ntt_plan_destroy(key->plan);
     delete key;
 }

@@ -125,6 +125,67 @@ def test_batched_commit_paths_match_oracle(mods, shape):
         assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, n), want), f"digits path {path}"
 
 
+SMALL_Q = O.CRT_PRIMES[0]  # q < 2^31: the Shoup register transform on the witness side
+
+
+@pytest.mark.parametrize("mac", ["mfma", "valu"])
+@pytest.mark.parametrize("shape", [(6, 9, 77, 17), (4, 3, 40, 130), (5, 16, 33, 5), (6, 8, 1024, 1), (6, 16, 31, 8)])
+def test_ntt_path_multiply_accumulate_matches_oracle(mods, shape, mac, monkeypatch):
+    """The NTT-domain path's multiply-accumulate, on int8 MFMA (the per-
+    coefficient limb-split product) and on VALU (LZX_AJTAI_MAC=valu), equals
+    the oracle's cached-NTT commit (ajtai.rs:118-153): k leaving a row block
+    partly empty, m not a multiple of 32, W leaving a column block partly
+    empty, more than one column group (W = 130), digit and residue witnesses,
+    at q > 2^31 (FP64 witness transform) and q < 2^31."""
+    CM, D, R = mods
+    if mac == "valu":
+        monkeypatch.setenv("LZX_AJTAI_MAC", "valu")
+    else:
+        monkeypatch.delenv("LZX_AJTAI_MAC", raising=False)
+    log_n, k, m, W = shape
+    n = 1 << log_n
+    for mod in (O.Q_32, SMALL_Q):
+        q = mod[0]
+        seed = bytes([log_n + 3 * k + (q & 7)]) * 32
+        pk = CM.AjtaiPublicKey.from_seed(_params(mods, log_n, k, m, 8, mod=mod), seed)
+        _, Ah = O.ajtai_from_seed(mod, log_n, k, m, seed)
+        rng = np.random.default_rng(m * 7 + W + q % 1000)
+        digits = rng.integers(-128, 128, size=(W, m, n)).astype(np.int8)
+        digits[:, ::7] = 0
+        resid = np.mod(digits.astype(np.int64), q).astype(np.uint32)
+        ot = O.Ntt(*mod, log_n)
+        want = np.stack([O.ajtai_commit(ot, k, m, Ah, resid[w])[1] for w in range(W)])
+        out = D.empty(W * k * n, np.uint32)
+        pk.commit_digits_batch(D.to_device(digits), W, out, path=CM.COMMIT_NTT)
+        assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, n), want), (mod, "digits")
+        out = D.empty(W * k * n, np.uint32)
+        pk.commit_batch(D.to_device(resid), W, out)
+        assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, n), want), (mod, "residues")
+
+
+def test_ntt_path_limb_extremes(mods):
+    """NTT-domain operands at the limb split's extremes: the key's A^ entries
+    and the witness's s^ entries are drawn from {0, 1, (q-1)/2, (q+1)/2, q-2,
+    q-1} (A and s are their inverse transforms), the ring elements split
+    over several chunks; the commit equals the oracle's."""
+    CM, D, R = mods
+    q = O.Q_32[0]
+    k, m, W = 16, 2093, 9
+    rng = np.random.default_rng(5150)
+    ext = np.array([0, 1, (q - 1) // 2, (q + 1) // 2, q - 2, q - 1], dtype=np.uint64)
+    ot = O.Ntt(*O.Q_32, 6)
+    Ah = ext[rng.integers(0, ext.size, size=(k * m, 64))].astype(np.uint32)
+    A = ot.inverse(Ah)[1]
+    pk = CM.AjtaiPublicKey.from_matrix(_params(mods, 6, k, m, 8), A)
+    assert np.array_equal(pk.matrix_ntt(), Ah)
+    sh = ext[rng.integers(0, ext.size, size=(W * m, 64))].astype(np.uint32)
+    s = ot.inverse(sh)[1].reshape(W, m, 64)
+    want = np.stack([O.ajtai_commit(ot, k, m, Ah, s[w])[1] for w in range(W)])
+    out = D.empty(W * k * 64, np.uint32)
+    pk.commit_batch(D.to_device(s), W, out)
+    assert np.array_equal(D.to_host(out, np.uint32).reshape(W, k, 64), want)
+
+
 def test_int8_extreme_digits_and_limbs(mods):
     """Digits at +-127 and A entries at the centred extremes (0, (q-1)/2,
     (q+1)/2, q-1) exercise the limb split and int32 fold-back bound."""
