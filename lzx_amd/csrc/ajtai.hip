@@ -398,6 +398,7 @@ static lzx_status key_new(uint32_t q, uint32_t adic, uint32_t g, uint32_t log_n,
     kk->limbs_bytes = 0;
     kk->nm_afrag = nullptr;
     kk->nm_corr = nullptr;
+    kk->nm_psi = nullptr;
     kk->device = plan->device;
     *key = kk;
     return LZX_OK;
@@ -415,6 +416,7 @@ extern "C" void lzx_ajtai_key_destroy(lzx_ajtai_key *key)
     if (key->limbs_dev) (void)hipFree(key->limbs_dev);
     if (key->nm_afrag) (void)hipFree(key->nm_afrag);
     if (key->nm_corr) (void)hipFree(key->nm_corr);
+    if (key->nm_psi) (void)hipFree(key->nm_psi);
     lzx_ntt_plan_destroy(key->plan);
     delete key;
 }

@@ -15,6 +15,7 @@ struct lzx_ajtai_key {
     size_t limbs_bytes;
     int8_t *nm_afrag;   // NTT-domain MFMA product: limb fragments of the centred A^ (layout in ajtai_nttmm.hip)
     uint32_t *nm_corr;  // and its per-(row, coefficient) byte-offset constant
+    int8_t *nm_psi;     // n = 64: the forward-transform matrix as byte-limb fragments (digit witnesses)
     int device;
 };
 

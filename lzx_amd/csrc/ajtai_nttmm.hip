@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "ajtai.hpp"
 #include "ntt_regs.hpp"
@@ -112,14 +113,39 @@ __global__ void nm_key_corr_kernel(const uint64_t *__restrict__ part, uint32_t *
 
 // ---- witness side -----------------------------------------------------------
 
-// The byte Lb of each of four words, packed low to high.
-__device__ __forceinline__ uint32_t nm_bytes(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3, uint32_t Lb)
+// Every coefficient's B fragment of one block of 8 witnesses x 32 ring
+// elements, from the block's NTT-domain values in LDS (polynomial t at row t,
+// coefficient c at column (c + 2 (t / 32)) mod N, row stride N + 1 words).
+// A wave writes one whole fragment per store instruction (1 KiB contiguous):
+// lane s = 32 g + 4 w' + Lb takes byte Lb of the 16 values of witness
+// 8 cb + w', ring elements 32 jb + 16 g .. + 15.  The four lanes of a (w', g)
+// read the same words (broadcast), and the 16 (w', g) rows fall in distinct
+// banks.
+template <uint32_t N>
+__device__ __forceinline__ void nm_write_frags(const uint32_t *T, int8_t *__restrict__ frag, uint32_t jb, uint32_t cb,
+                                               uint32_t njb, uint32_t ncb, uint32_t t)
 {
+    constexpr uint32_t RS = N + 1;
+    const uint32_t lane = t & 63, wave = t >> 6;
+    const uint32_t Lb = lane & 3, wp = (lane >> 2) & 7, g = lane >> 5;
     // v_perm_b32(hi, lo, sel): selector bytes 0-3 pick lo's bytes, 4-7 hi's
     const uint32_t sel = Lb | (Lb + 4) << 8 | Lb << 16 | (Lb + 4) << 24;
-    const uint32_t lo = __builtin_amdgcn_perm(v1, v0, sel); // v0.Lb, v1.Lb, ..
-    const uint32_t hi = __builtin_amdgcn_perm(v3, v2, sel); // v2.Lb, v3.Lb, ..
-    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+    const uint32_t *src = T + (32 * wp + 16 * g) * RS;
+    for (uint32_t c = wave; c < N; c += 4) {
+        const uint32_t col = (c + 2 * wp) & (N - 1);
+        uint32_t v[16];
+#pragma unroll
+        for (uint32_t b = 0; b < 16; b++) v[b] = src[b * RS + col];
+        uint32_t d[4];
+#pragma unroll
+        for (uint32_t e = 0; e < 4; e++) {
+            const uint32_t lo = __builtin_amdgcn_perm(v[4 * e + 1], v[4 * e], sel); // v0.Lb, v1.Lb, ..
+            const uint32_t hi = __builtin_amdgcn_perm(v[4 * e + 3], v[4 * e + 2], sel);
+            d[e] = __builtin_amdgcn_perm(hi, lo, 0x05040100u) ^ 0x80808080u;
+        }
+        *reinterpret_cast<uint4 *>(frag + (((size_t)c * njb + jb) * ncb + cb) * 1024 + 16 * lane) =
+            make_uint4(d[0], d[1], d[2], d[3]);
+    }
 }
 
 // Forward NTT of 8 witnesses x 32 ring elements (one polynomial per thread, as
@@ -170,19 +196,100 @@ __global__ __launch_bounds__(256) void nm_witness_frag_kernel(const In *__restri
 #pragma unroll
     for (uint32_t c = 0; c < N; c++) T[t * RS + ((c + sw) & (N - 1))] = ntt_out<AR>(a[brv_bits(c, LOG_N)], tab, N);
     __syncthreads();
-    // item (c, r): r = 8 g + w' (witness 8 cb + w', ring elements 32 jb + 16 g ..) -> lane slots 4 r + Lb
-    for (uint32_t it = t; it < 16 * N; it += 256) {
-        const uint32_t c = it >> 4, r = it & 15, g = r >> 3, wp = r & 7;
-        const uint32_t col = (c + 2 * wp) & (N - 1);
-        uint32_t v[16];
+    nm_write_frags<N>(T, frag, jb, cb, njb, ncb, t);
+}
+
+// The same fragments for digit witnesses with n = 64, the forward NTT on the
+// matrix cores: s^[c] = sum_t psi^((2c+1) t) d_t (the forward transform of
+// the register kernels, natural output order) is the 64 x 64 matrix Psi times
+// the int8 digit vector.  Psi is split into four balanced byte limbs
+// (psi_frag[La][rb][kh][lane][16]: row c = 32 rb + lane % 32, column
+// t = 32 kh + 16 (lane / 32) + b), so each limb's product sums exactly in
+// int32 (64 * 128 * 128 = 2^20), and V = L + 2^16 H with L = P0 + 2^8 P1,
+// H = P2 + 2^8 P3 (|V| < 2^45) is exact in FP64: one rounded quotient and one
+// fused multiply-add give V mod q.  A wave transforms its two witnesses'
+// 32-element column blocks with 16 MFMAs each and writes the values to the
+// block's LDS tile; the fragments then leave as in the register kernel.  The
+// blocks are persistent (Psi, 16 KiB of fragments, stays in registers) and
+// load the next tile's digits while transforming the current one.
+__global__ __launch_bounds__(256) void nm_digits_frag_kernel(const int8_t *__restrict__ digits,
+                                                             int8_t *__restrict__ frag,
+                                                             const int8_t *__restrict__ psi_frag, uint32_t W,
+                                                             uint32_t m, uint32_t njb, uint32_t ncb, double qd,
+                                                             double qinvd)
+{
+    constexpr uint32_t N = 64, RS = N + 1;
+    __shared__ uint32_t T[256 * RS];
+    const uint32_t t = threadIdx.x, lane = t & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t ntiles = njb * ncb;
+    // persistent: Psi stays in registers across the block's tiles (jb fastest)
+    v4i P[4][2][2];
 #pragma unroll
-        for (uint32_t b = 0; b < 16; b++) v[b] = T[(32 * wp + 16 * g + b) * RS + col] ^ 0x80808080u;
-        int8_t *dst = frag + (((size_t)c * njb + jb) * ncb + cb) * 1024 + 64 * r;
+    for (int La = 0; La < 4; La++)
 #pragma unroll
-        for (uint32_t Lb = 0; Lb < 4; Lb++)
-            *reinterpret_cast<uint4 *>(dst + 16 * Lb) =
-                make_uint4(nm_bytes(v[0], v[1], v[2], v[3], Lb), nm_bytes(v[4], v[5], v[6], v[7], Lb),
-                           nm_bytes(v[8], v[9], v[10], v[11], Lb), nm_bytes(v[12], v[13], v[14], v[15], Lb));
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int kh = 0; kh < 2; kh++)
+                P[La][rb][kh] = *reinterpret_cast<const v4i *>(psi_frag + ((((La * 2 + rb) * 2 + kh) * 64) + lane) * 16);
+    const uint32_t jl = lane & 31, g = lane >> 5;
+    // the wave's two witnesses' digits of a tile, from clamped (valid) rows
+    auto load_b = [&](uint32_t tile, v4i (&B)[2][2]) {
+        tile = min(tile, ntiles - 1);
+        const uint32_t jb = tile % njb, cb = tile / njb;
+        const uint32_t j = min(32 * jb + jl, m - 1);
+#pragma unroll
+        for (uint32_t x = 0; x < 2; x++) {
+            const uint32_t w = min(8 * cb + 2 * wave + x, W - 1);
+            const int8_t *src = digits + ((size_t)w * m + j) * 64 + 16 * g;
+#pragma unroll
+            for (int kh = 0; kh < 2; kh++) B[x][kh] = *reinterpret_cast<const v4i *>(src + 32 * kh);
+        }
+    };
+    v4i Bn[2][2];
+    load_b(blockIdx.x, Bn);
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t jb = tile % njb, cb = tile / njb;
+        v4i B[2][2];
+#pragma unroll
+        for (uint32_t x = 0; x < 2; x++) {
+            // polynomials past W or m transform as zero
+            const bool live = 8 * cb + 2 * wave + x < W && 32 * jb + jl < m;
+#pragma unroll
+            for (int kh = 0; kh < 2; kh++) B[x][kh] = live ? Bn[x][kh] : v4i{0, 0, 0, 0};
+        }
+        load_b(tile + gridDim.x, Bn);
+#pragma unroll
+        for (uint32_t x = 0; x < 2; x++) {
+            const uint32_t wp = 2 * wave + x;
+            uint32_t *row = T + (32 * wp + jl) * RS;
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++) {
+                v16i acc[4];
+#pragma unroll
+                for (int La = 0; La < 4; La++) {
+#pragma unroll
+                    for (int r = 0; r < 16; r++) acc[La][r] = 0;
+#pragma unroll
+                    for (int kh = 0; kh < 2; kh++)
+                        acc[La] = __builtin_amdgcn_mfma_i32_32x32x32_i8(P[La][rb][kh], B[x][kh], acc[La], 0, 0, 0);
+                }
+                // D: register r of lane l holds row c = 32 rb + 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int32_t L = acc[0][r] + acc[1][r] * 256, H = acc[2][r] + acc[3][r] * 256;
+                    const double V = __builtin_fma((double)H, 65536.0, (double)L);
+                    const double qq = __builtin_rint(V * qinvd);
+                    double rr = __builtin_fma(-qq, qd, V); // exact, in (-q, q)
+                    rr = rr < 0.0 ? rr + qd : rr;
+                    const uint32_t c = 32 * rb + 8 * (r >> 2) + 4 * g + (r & 3);
+                    row[(c + 2 * wp) & (N - 1)] = (uint32_t)rr;
+                }
+            }
+        }
+        __syncthreads();
+        nm_write_frags<N>(T, frag, jb, cb, njb, ncb, t);
+        __syncthreads(); // the tile is free for the next one
     }
 }
 
@@ -190,9 +297,9 @@ __global__ __launch_bounds__(256) void nm_witness_frag_kernel(const In *__restri
 
 // Block (c + n * chunk, column group): NWV waves, each CBW column blocks of 32
 // (8 witnesses x 4 limbs) over NRB row blocks, for the chunk's ring elements.
-// Fragments are loaded PF K steps ahead, every load unconditional (clamped to
-// the chunk), so the waits for them are exact.  Epilogue: one u32 partial per
-// (chunk, c, i, w).
+// Fragments are loaded PF K steps ahead through buffer resources; the loop
+// has no branch, so the waits for them are exact.  Epilogue: one u32 partial
+// per (chunk, c, i, w).
 template <int NRB, int CBW>
 __global__ __launch_bounds__(512) void nm_gemm_kernel(const int8_t *__restrict__ afrag,
                                                       const int8_t *__restrict__ bfrag, uint32_t *__restrict__ partial,
@@ -206,11 +313,20 @@ __global__ __launch_bounds__(512) void nm_gemm_kernel(const int8_t *__restrict__
     const uint32_t cb0 = (blockIdx.y * nwv + wave) * CBW;
     const uint32_t jb0 = chunk * jbc, jb1 = min(njb, jb0 + jbc);
     if (jb0 >= jb1 || cb0 >= ncb) return; // uniform per wave
-    const int8_t *A = afrag + (size_t)c * njb * NRB * 1024 + 16 * lane;
-    const int8_t *B = bfrag + (size_t)c * njb * ncb * 1024 + 16 * lane;
-    uint32_t cbs[CBW];
+    // Buffer loads: one resource per operand over this chunk's K steps (SGPRs) and a 32-bit lane offset.
+    // Steps past the chunk's end are out of the resource's range and read zeros, so every step runs the
+    // same loads and MFMAs with no branch, and the waits for the prefetched steps stay exact.  (With
+    // per-lane 64-bit addresses the compiler recycled prefetch registers for address arithmetic, and a
+    // branch around the tail's MFMAs made every wait a full drain.)
+    const uint32_t nk = jb1 - jb0;
+    const __amdgpu_buffer_rsrc_t Ar = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(afrag + ((size_t)c * njb + jb0) * NRB * 1024), 0, (int)(nk * NRB * 1024), 0x00020000);
+    const __amdgpu_buffer_rsrc_t Br = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(bfrag + ((size_t)c * njb + jb0) * ncb * 1024), 0, (int)(nk * ncb * 1024), 0x00020000);
+    const uint32_t lo = 16 * lane;
+    uint32_t bo[CBW];
 #pragma unroll
-    for (int u = 0; u < CBW; u++) cbs[u] = min(cb0 + u, ncb - 1);
+    for (int u = 0; u < CBW; u++) bo[u] = lo + 1024 * min(cb0 + u, ncb - 1);
 
     v16i acc[NRB][CBW];
 #pragma unroll
@@ -220,26 +336,30 @@ __global__ __launch_bounds__(512) void nm_gemm_kernel(const int8_t *__restrict__
 #pragma unroll
             for (int x = 0; x < 16; x++) acc[rb][u][x] = 0;
     v4i fa[PF][NRB], fb[PF][CBW];
-    auto load = [&](uint32_t jb, v4i (&a)[NRB], v4i (&b)[CBW]) {
-        jb = min(jb, jb1 - 1);
+    auto load = [&](uint32_t st, v4i (&a)[NRB], v4i (&b)[CBW]) {
 #pragma unroll
-        for (int rb = 0; rb < NRB; rb++) a[rb] = *reinterpret_cast<const v4i *>(A + ((size_t)jb * NRB + rb) * 1024);
+        for (int rb = 0; rb < NRB; rb++)
+            a[rb] = __builtin_bit_cast(v4i,
+                                       __builtin_amdgcn_raw_buffer_load_b128(Ar, lo + 1024 * (st * NRB + rb), 0, 0));
 #pragma unroll
-        for (int u = 0; u < CBW; u++) b[u] = *reinterpret_cast<const v4i *>(B + ((size_t)jb * ncb + cbs[u]) * 1024);
+        for (int u = 0; u < CBW; u++)
+            b[u] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(Br, bo[u] + 1024 * st * ncb, 0, 0));
     };
 #pragma unroll
-    for (int s = 0; s < PF; s++) load(jb0 + s, fa[s], fb[s]);
-    for (uint32_t jb = jb0; jb < jb1; jb += PF) {
+    for (int s = 0; s < PF; s++) {
+        load(s, fa[s], fb[s]);
+        __builtin_amdgcn_sched_barrier(0); // stages in order, as the loop reloads them
+    }
+    for (uint32_t st = 0; st < nk; st += PF) {
 #pragma unroll
         for (int s = 0; s < PF; s++) {
-            if (jb + s < jb1) {
 #pragma unroll
-                for (int rb = 0; rb < NRB; rb++)
+            for (int rb = 0; rb < NRB; rb++)
 #pragma unroll
-                    for (int u = 0; u < CBW; u++)
-                        acc[rb][u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s][rb], fb[s][u], acc[rb][u], 0, 0, 0);
-            }
-            load(jb + s + PF, fa[s], fb[s]);
+                for (int u = 0; u < CBW; u++)
+                    acc[rb][u] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s][rb], fb[s][u], acc[rb][u], 0, 0, 0);
+            load(st + s + PF, fa[s], fb[s]);
+            __builtin_amdgcn_sched_barrier(0); // step by step: each wait covers only the oldest stage
         }
     }
     // D layout: lane l, register x -> row 32 rb + 8 (x / 4) + 4 (l / 32) + x % 4 = 4 i + La,
@@ -301,6 +421,33 @@ bool ajtai_nttmm_enabled(const lzx_ajtai_key *key)
     return key->nm_afrag && !(v && std::strcmp(v, "valu") == 0);
 }
 
+// Psi[c][t] = psi^((2c+1) t) mod q for n = 64, as balanced byte limbs in the
+// A-fragment order of nm_digits_frag_kernel (16 KiB).
+static std::vector<int8_t> nm_psi_table(const lzx_ntt_plan *p)
+{
+    const uint32_t n = p->n, q = p->q;
+    std::vector<uint64_t> pw(2 * n);
+    pw[0] = 1;
+    for (uint32_t e = 1; e < 2 * n; e++) pw[e] = pw[e - 1] * p->psi % q;
+    std::vector<int8_t> out(16 * 1024);
+    for (uint32_t La = 0; La < 4; La++)
+        for (uint32_t rb = 0; rb < 2; rb++)
+            for (uint32_t kh = 0; kh < 2; kh++)
+                for (uint32_t lane = 0; lane < 64; lane++)
+                    for (uint32_t b = 0; b < 16; b++) {
+                        const uint32_t c = 32 * rb + lane % 32, t = 32 * kh + 16 * (lane / 32) + b;
+                        const uint64_t v = pw[((2 * c + 1) * t) % (2 * n)];
+                        int64_t x = v <= (q - 1) / 2 ? (int64_t)v : (int64_t)v - (int64_t)q;
+                        int64_t d = 0;
+                        for (uint32_t L = 0; L <= La; L++) {
+                            d = ((x + 128) & 255) - 128;
+                            x = (x - d) >> 8;
+                        }
+                        out[(((La * 2 + rb) * 2 + kh) * 64 + lane) * 16 + b] = (int8_t)d;
+                    }
+    return out;
+}
+
 lzx_status ajtai_build_nttmm(lzx_ajtai_key *key)
 {
     const uint32_t k = (uint32_t)key->k, m = (uint32_t)key->m, n = (uint32_t)key->n, q = key->plan->q;
@@ -309,6 +456,11 @@ lzx_status ajtai_build_nttmm(lzx_ajtai_key *key)
     const size_t bytes = (size_t)n * njb * nrb * 1024;
     LZX_HIP(hipMalloc(&key->nm_afrag, bytes));
     LZX_HIP(hipMalloc(&key->nm_corr, (size_t)k * n * sizeof(uint32_t)));
+    if (n == 64) {
+        const std::vector<int8_t> psi = nm_psi_table(key->plan);
+        LZX_HIP(hipMalloc(&key->nm_psi, psi.size()));
+        LZX_HIP(hipMemcpy(key->nm_psi, psi.data(), psi.size(), hipMemcpyHostToDevice));
+    }
     hipLaunchKernelGGL(nm_key_frag_kernel, dim3(nm_grid(bytes / 16)), dim3(256), 0, nullptr, key->Ahat_dev,
                        key->nm_afrag, k, m, n, njb, nrb, q);
     LZX_LAUNCH_CHECK("nm_key_frag_kernel");
@@ -345,6 +497,9 @@ lzx_status ajtai_commit_nttmm(const lzx_ajtai_key *key, const uint32_t *s_dev, c
     const uint32_t njb = (m + 31) / 32, nrb = (4 * k + 31) / 32;
     const uint32_t ncb = (uint32_t)((4 * W + 31) / 32);
     const uint32_t W32 = (uint32_t)W;
+    // each coefficient's fragments are one buffer resource (< 2^31 bytes)
+    if ((uint64_t)njb * ncb * 1024 > 0x7FFFFFFFull || (uint64_t)njb * nrb * 1024 > 0x7FFFFFFFull)
+        return LZX_UNSUPPORTED;
     // chunks: about two blocks per CU, each chunk within the int32 bound
     const uint32_t cus = (uint32_t)std::max(1, key->plan->cus);
     uint32_t nchunks = std::max<uint32_t>((njb + NM_MAX_JBC - 1) / NM_MAX_JBC, (2 * cus + n - 1) / n);
@@ -376,10 +531,17 @@ lzx_status ajtai_commit_nttmm(const lzx_ajtai_key *key, const uint32_t *s_dev, c
             hipLaunchKernelGGL((nm_witness_frag_kernel<LN, AR, uint32_t>), fgrid, dim3(256), 0, stream, s_dev,     \
                                bfrag, key->plan->dev_tab, W32, m, njb, ncb);                                       \
     } while (0)
-    switch (key->plan->log_n) {
-    case 4: if (big) LZX_NM_FRAG(4, ARITH_F64); else LZX_NM_FRAG(4, ARITH_SHOUP); break;
-    case 5: if (big) LZX_NM_FRAG(5, ARITH_F64); else LZX_NM_FRAG(5, ARITH_SHOUP); break;
-    default: if (big) LZX_NM_FRAG(6, ARITH_F64); else LZX_NM_FRAG(6, ARITH_SHOUP); break;
+    if (digits_dev && n == 64 && key->nm_psi) {
+        // digit witnesses: the forward transform on the matrix cores
+        const dim3 pgrid(std::min<uint32_t>(njb * ncb, 2 * cus));
+        hipLaunchKernelGGL(nm_digits_frag_kernel, pgrid, dim3(256), 0, stream, digits_dev, bfrag, key->nm_psi, W32,
+                           m, njb, ncb, (double)q, 1.0 / (double)q);
+    } else {
+        switch (key->plan->log_n) {
+        case 4: if (big) LZX_NM_FRAG(4, ARITH_F64); else LZX_NM_FRAG(4, ARITH_SHOUP); break;
+        case 5: if (big) LZX_NM_FRAG(5, ARITH_F64); else LZX_NM_FRAG(5, ARITH_SHOUP); break;
+        default: if (big) LZX_NM_FRAG(6, ARITH_F64); else LZX_NM_FRAG(6, ARITH_SHOUP); break;
+        }
     }
 #undef LZX_NM_FRAG
     e = hipGetLastError();
