@@ -113,34 +113,38 @@ __global__ void nm_key_corr_kernel(const uint64_t *__restrict__ part, uint32_t *
 
 // ---- witness side -----------------------------------------------------------
 
+// The block's LDS tile of NTT-domain values, coefficient-major: value c of
+// polynomial t (witness w' = t / 32, ring element jj = t % 32) at word
+// (16 c + v) * 20 + jj % 16 with v = 2 w' + jj / 16, so the 16 ring elements a
+// fragment lane needs are one 64-byte run (four 16-byte reads); rows are padded
+// to 20 words so that the 16 runs of one coefficient start in different banks.
+constexpr uint32_t NM_ROW = 20;
+__device__ __forceinline__ uint32_t nm_tidx(uint32_t c, uint32_t t)
+{
+    return (16 * c + 2 * (t >> 5) + ((t & 31) >> 4)) * NM_ROW + (t & 15);
+}
+
 // Every coefficient's B fragment of one block of 8 witnesses x 32 ring
-// elements, from the block's NTT-domain values in LDS (polynomial t at row t,
-// coefficient c at column (c + 2 (t / 32)) mod N, row stride N + 1 words).
-// A wave writes one whole fragment per store instruction (1 KiB contiguous):
-// lane s = 32 g + 4 w' + Lb takes byte Lb of the 16 values of witness
-// 8 cb + w', ring elements 32 jb + 16 g .. + 15.  The four lanes of a (w', g)
-// read the same words (broadcast), and the 16 (w', g) rows fall in distinct
-// banks.
+// elements, from the tile.  A wave writes one whole fragment per store
+// instruction (1 KiB contiguous): lane s = 32 g + 4 w' + Lb takes byte Lb of
+// the 16 values of witness 8 cb + w', ring elements 32 jb + 16 g .. + 15 (the
+// four lanes of a (w', g) read the same run).
 template <uint32_t N>
 __device__ __forceinline__ void nm_write_frags(const uint32_t *T, int8_t *__restrict__ frag, uint32_t jb, uint32_t cb,
                                                uint32_t njb, uint32_t ncb, uint32_t t)
 {
-    constexpr uint32_t RS = N + 1;
     const uint32_t lane = t & 63, wave = t >> 6;
-    const uint32_t Lb = lane & 3, wp = (lane >> 2) & 7, g = lane >> 5;
+    const uint32_t Lb = lane & 3, v = 2 * ((lane >> 2) & 7) + (lane >> 5);
     // v_perm_b32(hi, lo, sel): selector bytes 0-3 pick lo's bytes, 4-7 hi's
     const uint32_t sel = Lb | (Lb + 4) << 8 | Lb << 16 | (Lb + 4) << 24;
-    const uint32_t *src = T + (32 * wp + 16 * g) * RS;
     for (uint32_t c = wave; c < N; c += 4) {
-        const uint32_t col = (c + 2 * wp) & (N - 1);
-        uint32_t v[16];
-#pragma unroll
-        for (uint32_t b = 0; b < 16; b++) v[b] = src[b * RS + col];
+        const uint4 *run = reinterpret_cast<const uint4 *>(T + (16 * c + v) * NM_ROW);
         uint32_t d[4];
 #pragma unroll
         for (uint32_t e = 0; e < 4; e++) {
-            const uint32_t lo = __builtin_amdgcn_perm(v[4 * e + 1], v[4 * e], sel); // v0.Lb, v1.Lb, ..
-            const uint32_t hi = __builtin_amdgcn_perm(v[4 * e + 3], v[4 * e + 2], sel);
+            const uint4 x = run[e];
+            const uint32_t lo = __builtin_amdgcn_perm(x.y, x.x, sel); // x.Lb, y.Lb, ..
+            const uint32_t hi = __builtin_amdgcn_perm(x.w, x.z, sel);
             d[e] = __builtin_amdgcn_perm(hi, lo, 0x05040100u) ^ 0x80808080u;
         }
         *reinterpret_cast<uint4 *>(frag + (((size_t)c * njb + jb) * ncb + cb) * 1024 + 16 * lane) =
@@ -158,10 +162,8 @@ __global__ __launch_bounds__(256) void nm_witness_frag_kernel(const In *__restri
                                                               const uint32_t *__restrict__ tab, uint32_t W,
                                                               uint32_t m, uint32_t njb, uint32_t ncb)
 {
-    constexpr uint32_t N = 1u << LOG_N, RS = N + 1;
-    // polynomial t at row t; its coefficient c at column (c + 2 (t / 32)) mod N, so the fragment reads (16
-    // ring elements of one column, for 16 column/half pairs at once) touch 32 distinct banks per 32 lanes
-    __shared__ uint32_t T[256 * RS];
+    constexpr uint32_t N = 1u << LOG_N;
+    __shared__ __attribute__((aligned(16))) uint32_t T[16 * N * NM_ROW];
     const uint32_t t = threadIdx.x;
     const uint32_t jb = blockIdx.x % njb, cb = blockIdx.x / njb;
     const uint32_t w = 8 * cb + (t >> 5), j = 32 * jb + (t & 31);
@@ -192,9 +194,8 @@ __global__ __launch_bounds__(256) void nm_witness_frag_kernel(const In *__restri
         for (uint32_t i = 0; i < N; i++) a[i] = 0;
     }
     ntt_regs_any<LOG_N, AR, NTT_FWD>(a, tab);
-    const uint32_t sw = 2 * (t >> 5);
 #pragma unroll
-    for (uint32_t c = 0; c < N; c++) T[t * RS + ((c + sw) & (N - 1))] = ntt_out<AR>(a[brv_bits(c, LOG_N)], tab, N);
+    for (uint32_t c = 0; c < N; c++) T[nm_tidx(c, t)] = ntt_out<AR>(a[brv_bits(c, LOG_N)], tab, N);
     __syncthreads();
     nm_write_frags<N>(T, frag, jb, cb, njb, ncb, t);
 }
@@ -218,8 +219,8 @@ __global__ __launch_bounds__(256) void nm_digits_frag_kernel(const int8_t *__res
                                                              uint32_t m, uint32_t njb, uint32_t ncb, double qd,
                                                              double qinvd)
 {
-    constexpr uint32_t N = 64, RS = N + 1;
-    __shared__ uint32_t T[256 * RS];
+    constexpr uint32_t N = 64;
+    __shared__ __attribute__((aligned(16))) uint32_t T[16 * N * NM_ROW];
     const uint32_t t = threadIdx.x, lane = t & 63;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const uint32_t ntiles = njb * ncb;
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(256) void nm_digits_frag_kernel(const int8_t *__res
 #pragma unroll
         for (uint32_t x = 0; x < 2; x++) {
             const uint32_t wp = 2 * wave + x;
-            uint32_t *row = T + (32 * wp + jl) * RS;
+            const uint32_t tp = 32 * wp + jl; // this lane's polynomial in the tile
 #pragma unroll
             for (int rb = 0; rb < 2; rb++) {
                 v16i acc[4];
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(256) void nm_digits_frag_kernel(const int8_t *__res
                     double rr = __builtin_fma(-qq, qd, V); // exact, in (-q, q)
                     rr = rr < 0.0 ? rr + qd : rr;
                     const uint32_t c = 32 * rb + 8 * (r >> 2) + 4 * g + (r & 3);
-                    row[(c + 2 * wp) & (N - 1)] = (uint32_t)rr;
+                    T[nm_tidx(c, tp)] = (uint32_t)rr;
                 }
             }
         }
