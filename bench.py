@@ -225,9 +225,14 @@ def achievable_int8_tops(torch, dim: int = 8192, reps: int = 10):
 
 def bench_ajtai(reps: int = 5):
     """Ajtai commit, d=64, k=16, m=2^16, W=64 small-digit witnesses (digits in
-    [-8, 8), P(0) = 17/32), A from seed [0x43; 32]: NTT-domain path vs int8
-    MFMA path.  2^26 ring-MACs per commit; the int8 path performs 2^40 int8
-    MACs (= 2.2e12 int8 ops)."""
+    [-8, 8), P(0) = 17/32), A from seed [0x43; 32], all legs bit-identical:
+      - plain_ntt_path: the NTT-domain commit with the multiply-accumulate on
+        the VALU (LZX_AJTAI_MAC=valu), the plain drop-in;
+      - ntt_path: the same NTT-domain commit with the multiply-accumulate (and
+        the digit transform) on int8 MFMA, 2^36 int8 MACs;
+      - int8_path: the coefficient-domain Toeplitz product on int8 MFMA, 2^40
+        int8 MACs (2.2e12 int8 ops).
+    2^26 ring-MACs per commit."""
     import torch
 
     from lzx_amd import commitment as CM
@@ -240,22 +245,39 @@ def bench_ajtai(reps: int = 5):
     pk = CM.AjtaiPublicKey.from_seed(CM.AjtaiParams(R.RingConfig(R.Modulus32.Q_32, 6), k, m, 8), bytes([0x43]) * 32)
     key_s = time.perf_counter() - t0
     digits = D.to_device(W.small_digits(0xC3, nw * m * 64).reshape(nw, m, 64))
+    o0 = D.empty(nw * k * 64, np.uint32)
     o1 = D.empty(nw * k * 64, np.uint32)
     o2 = D.empty(nw * k * 64, np.uint32)
+    prev = os.environ.get("LZX_AJTAI_MAC")
+    os.environ["LZX_AJTAI_MAC"] = "valu"  # read by the library on every commit call
+    try:
+        ms_plain = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o0, path=CM.COMMIT_NTT), reps)
+    finally:
+        if prev is None:
+            del os.environ["LZX_AJTAI_MAC"]
+        else:
+            os.environ["LZX_AJTAI_MAC"] = prev
     ms_ntt = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o1, path=CM.COMMIT_NTT), reps)
     ms_i8 = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o2, path=CM.COMMIT_INT8), reps)
-    same = bool(torch.equal(o1, o2))
+    same = bool(torch.equal(o1, o2)) and bool(torch.equal(o0, o1))
     o3 = D.empty(nw * k * 64, np.uint32)
     ms_auto = _median_gpu_ms(torch, lambda: pk.commit_digits_batch(digits, nw, o3, path=CM.COMMIT_AUTO), reps)
     auto_choice = {CM.COMMIT_NTT: "ntt_path", CM.COMMIT_INT8: "int8_path"}.get(pk.commit_path_choice(nw), "not measured")
     rmacs = k * m * nw
     i8_ops = 2.0 * rmacs * 64 * 64 * 4
+    nm_ops = 2.0 * rmacs * 64 * 16
     return {"workload": "Ajtai commit d=64 k=16 m=2^16 W=64, q=3221225473, digits in [-8,8)",
             "ring_macs_per_commit": rmacs, "key_from_seed_s": round(key_s, 3),
-            "ntt_path": {"ms": round(ms_ntt, 4), "ring_macs_per_s": rmacs / (ms_ntt * 1e-3)},
+            "plain_ntt_path": {"ms": round(ms_plain, 4), "ring_macs_per_s": rmacs / (ms_plain * 1e-3),
+                               "multiply_accumulate": "VALU (LZX_AJTAI_MAC=valu)"},
+            "ntt_path": {"ms": round(ms_ntt, 4), "ring_macs_per_s": rmacs / (ms_ntt * 1e-3),
+                         "multiply_accumulate": "int8 MFMA, NTT domain (1024 int8 MACs per ring-MAC)",
+                         "int8_tops": round(nm_ops / (ms_ntt * 1e-3) / 1e12, 1)},
             "int8_path": {"ms": round(ms_i8, 4), "ring_macs_per_s": rmacs / (ms_i8 * 1e-3),
+                          "multiply_accumulate": "int8 MFMA, coefficient domain (16384 int8 MACs per ring-MAC)",
                           "int8_tops": round(i8_ops / (ms_i8 * 1e-3) / 1e12, 1),
                           "int8_mfma_frac_of_5000_tops": round(i8_ops / (ms_i8 * 1e-3) / 5e15, 4)},
+            "int8_over_plain_ntt": round(ms_plain / ms_i8, 3), "ntt_mfma_over_plain_ntt": round(ms_plain / ms_ntt, 3),
             "int8_over_ntt": round(ms_ntt / ms_i8, 3), "paths_bit_identical": same,
             "auto": {"ms": round(ms_auto, 4), "choice": auto_choice, "equals_paths": bool(torch.equal(o1, o3)),
                      "faster_path": "int8_path" if ms_i8 <= ms_ntt else "ntt_path"},
@@ -587,7 +609,15 @@ def bench_c1(reps: int = 50):
            "ring_macs": k * m,
            "host_call": {"entry": "lzx_ajtai_commit_host (AjtaiPublicKey::commit signature, PCIe in and out)",
                          "us": round(t_host * 1e6, 2), "ring_macs_per_s": k * m / t_host}}
-    for name, path in (("ntt_path", CM.COMMIT_NTT), ("int8_path", CM.COMMIT_INT8), ("auto", CM.COMMIT_AUTO)):
+    prev = os.environ.get("LZX_AJTAI_MAC")
+    for name, path in (("plain_ntt_path", CM.COMMIT_NTT), ("ntt_path", CM.COMMIT_NTT), ("int8_path", CM.COMMIT_INT8),
+                       ("auto", CM.COMMIT_AUTO)):
+        if name == "plain_ntt_path":
+            os.environ["LZX_AJTAI_MAC"] = "valu"  # the NTT-domain commit with the VALU multiply-accumulate
+        elif prev is None:
+            os.environ.pop("LZX_AJTAI_MAC", None)
+        else:
+            os.environ["LZX_AJTAI_MAC"] = prev
         out = D.empty(k * 64, np.uint32)
         t = _timed_events(torch, lambda: pk.commit_digits_batch(dd, 1, out, path=path), 20)
         same = bool(np.array_equal(D.to_host(out, np.uint32).reshape(k, 64), want))
