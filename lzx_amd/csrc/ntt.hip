@@ -145,14 +145,14 @@ __global__ __launch_bounds__(64) void ntt_pipe_kernel(uint32_t *__restrict__ dat
 #pragma unroll
             for (uint32_t i = 0; i < C; i++) {
                 const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
-                *reinterpret_cast<u32x4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const u32x4 *>(T + 4 * s);
+                tile_store_wt(g, 64 * N * 4, (p * C + c) * 16, *reinterpret_cast<const u32x4 *>(T + 4 * s));
             }
         } else {
 #pragma unroll
             for (uint32_t i = 0; i < C; i++) {
                 const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
                 if (p < np)
-                    *reinterpret_cast<u32x4 *>(g + (p * C + c) * 4) = *reinterpret_cast<const u32x4 *>(T + 4 * s);
+                    tile_store_wt(g, np * N * 4, (p * C + c) * 16, *reinterpret_cast<const u32x4 *>(T + 4 * s));
             }
             __builtin_amdgcn_s_waitcnt(WAIT_VM0);
         }
@@ -272,12 +272,15 @@ __device__ __forceinline__ void lazy_run(const LazyTile<1u << LOG_N> &L, uint32_
         if (np == 64) {
 #pragma unroll
             for (uint32_t i = 0; i < C; i++)
-                *L.full(g, i) = *reinterpret_cast<const u32x4 *>(T + L.stage + ROW * PPI * i);
+                tile_store_wt((uint8_t *)g, 64 * N * 4, L.voff[i / 4] + 1024 * (i % 4),
+                              *reinterpret_cast<const u32x4 *>(T + L.stage + ROW * PPI * i));
         } else {
 #pragma unroll
             for (uint32_t i = 0; i < C; i++) {
                 const uint32_t p = PPI * i + L.lane / C;
-                if (p < np) *L.part(g, i, p) = *reinterpret_cast<const u32x4 *>(T + L.stage + ROW * PPI * i);
+                if (p < np)
+                    tile_store_wt((uint8_t *)g, np * N * 4, 16 * (C * p + L.lane % C),
+                                  *reinterpret_cast<const u32x4 *>(T + L.stage + ROW * PPI * i));
             }
             __builtin_amdgcn_s_waitcnt(WAIT_VM0);
         }

@@ -366,6 +366,16 @@ __device__ __forceinline__ void wave_sync()
 // every load is unconditional and the array stays in VGPRs.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4))); // promotable to VGPRs across loop iterations
 
+// 16-byte store at lane byte offset off from the wave-uniform tile base g,
+// through a buffer resource over the tile's bytes with system-scope cache
+// policy (sc0 sc1, operand 0x11 on gfx950): the store writes through L2, so a
+// launch ends with no dirty lines for its release to write back.
+__device__ __forceinline__ void tile_store_wt(void *g, uint32_t bytes, uint32_t off, u32x4 v)
+{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(g, 0, (int)bytes, 0x00020000);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0x11);
+}
+
 template <uint32_t C>
 __device__ __forceinline__ void tile_fetch(const uint32_t *__restrict__ g, uint32_t np, u32x4 (&r)[C], uint32_t lane)
 {
