@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE (each scaled by the reading for a 64 MiB copy kernel), per kernel
 # class of the plans; the rocprofv3 summaries are kept under profiles/.
 NTT_FWD_KERNEL = {"mfma": "ntt_mfma_kernel<NTT_FWD>", "register": "ntt_lazy_kernel<6,NTT_FWD>"}
-NTT_FWD_TRAFFIC_BYTES = {"mfma": (66670 + 65536) * 1024, "register": (65812 + 65536) * 1024}
+NTT_FWD_TRAFFIC_BYTES = {"mfma": (66657 + 65536) * 1024, "register": (65812 + 65536) * 1024}
 POLYS = 1 << 16
 LOG_N = 6
 N = 1 << LOG_N
@@ -69,10 +69,12 @@ def bench_ntt(steps: int, warmup: int):
     total_ms = start.elapsed_time(end)
     # per-launch durations for the roofline: K forward launches back to back
     # between one event pair, then K inverse launches likewise (each launch is
-    # a full transform of the buffer, so K inverses restore the input)
+    # a full transform of the buffer, so K inverses restore the input), queued
+    # behind a spin kernel so that the first one does not start on an idle GPU
     fwd_ms, inv_ms = [], []
     for _ in range(3):
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        _queue_ahead(torch, stream)
         e0.record(stream)
         for _ in range(steps):
             R.forward_multi(tabs, data, POLYS, stream)
@@ -174,6 +176,7 @@ def _median_gpu_ms(torch, fn, reps, warmup=1):
     times = []
     for _ in range(reps):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        _queue_ahead(torch, stream)
         a.record(stream)
         fn()
         b.record(stream)
@@ -193,6 +196,7 @@ def achievable_copy_gbs(torch, nbytes: int = 134217728, reps: int = 20) -> float
     ts = []
     for _ in range(reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        _queue_ahead(torch, stream)
         e0.record(stream)
         b.copy_(a)
         e1.record(stream)
@@ -213,6 +217,7 @@ def achievable_int8_tops(torch, dim: int = 8192, reps: int = 10):
         ts = []
         for _ in range(reps):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            _queue_ahead(torch, stream)
             e0.record(stream)
             torch._int_mm(a, b)
             e1.record(stream)
@@ -463,17 +468,29 @@ def bench_ntt_large(reps: int = 10):
     return out
 
 
+def _queue_ahead(torch, stream):
+    """A ~1 ms spin kernel on the stream ahead of a timed region: the host
+    queues the timed launches while it runs, so the first timed launch starts
+    from a busy queue, as in a pipeline of such calls, not from an idle GPU
+    waking up (which added about 2 us per launch to 50 back-to-back 25 us NTT
+    launches; with the spin kernel ahead, event times match the kernel trace)."""
+    with torch.cuda.stream(stream):
+        torch.cuda._sleep(2_000_000)
+
+
 def _timed_events(torch, fn, reps):
     """Seconds per call of fn: reps back-to-back calls between two HIP events on
-    the launch stream (torch's current stream), after one warm-up call.  Calls
-    queue behind each other, so the host's launch latency is hidden as it is
-    in a pipeline of such calls; the median of three such runs."""
+    the launch stream (torch's current stream), after one warm-up call and with
+    a spin kernel queued ahead (_queue_ahead).  Calls queue behind each other,
+    so the host's launch latency is hidden as it is in a pipeline of such
+    calls; the median of three such runs."""
     stream = torch.cuda.current_stream()
     fn()
     torch.cuda.synchronize()
     runs = []
     for _ in range(3):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        _queue_ahead(torch, stream)
         e0.record(stream)
         for _ in range(reps):
             fn()
@@ -564,6 +581,7 @@ def bench_uneven_batch(reps: int = 5, autotune: bool = True):
     times = []
     for _ in range(reps):
         a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        _queue_ahead(torch, stream)
         a.record(stream)
         b.run()
         e.record(stream)
