@@ -175,6 +175,17 @@ __device__ __forceinline__ double rq2_swap_d(double v)
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// The grid is persistent (the resident blocks, each taking every gridDim.x-th
+// tile).  Norm and inverse (PF) load a block's next tile of a into r right
+// after the current one is staged, so it arrives while the current tile is
+// transformed, and stage it into the LDS tile before the current results are
+// stored, so the wait for it has no stores in flight.  Past the last tile the
+// prefetch reads row 0 of the current tile (in bounds, never used), so every
+// load is unconditional.  Tile t + gridDim.x belongs to this block alone, so
+// the early read is safe wherever the host allows aliasing (out == a exactly).
+// The product loads a at the top of each tile: its b is already in flight
+// during a's transform, and a prefetch of the next a kept r live beside a's
+// parked transform, whose extra register moves cost more than it hid.
 template <int LOG_N, int MODE>
 __global__ __launch_bounds__(64) void rq2_f64_kernel(const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
                                                      const uint32_t *__restrict__ tab, size_t count)
@@ -183,20 +194,37 @@ __global__ __launch_bounds__(64) void rq2_f64_kernel(const uint32_t *a, const ui
     __shared__ __attribute__((aligned(16))) uint32_t T[64 * N]; // one tile: row l = lane l's polynomial
     const uint32_t lane = threadIdx.x, odd = lane & 1;
     const size_t tiles = (count + RQ2_TILE - 1) / RQ2_TILE;
+    auto rows_of = [&](size_t tt) -> uint32_t {
+        const size_t e = tt * RQ2_TILE;
+        return 2 * (count - e < RQ2_TILE ? (uint32_t)(count - e) : RQ2_TILE);
+    };
+    // tile tn's a, or row 0 of tile t when tn is past the end
+    auto fetch_next_a = [&](size_t t, size_t tn, u32x4(&r)[C]) {
+        const bool in = tn < tiles;
+        tile_fetch<C>(a + (in ? tn : t) * RQ2_TILE * 2 * N, in ? rows_of(tn) : 0, r, lane);
+    };
+    constexpr bool PF = MODE != RQ2_MUL;
     u32x4 r[C];
+    if (PF && blockIdx.x < tiles) {
+        tile_fetch<C>(a + (size_t)blockIdx.x * RQ2_TILE * 2 * N, rows_of(blockIdx.x), r, lane);
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
+    }
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint32_t *tb = tab;
         asm volatile("" : "+s"(tb));
         const __attribute__((address_space(4))) double *dt =
             (const __attribute__((address_space(4))) double *)(tb + TabLayout::dbl(N));
         const double q = dt[0], qinv = dt[1];
-        const size_t e0 = t * RQ2_TILE;
+        const size_t e0 = t * RQ2_TILE, tn = t + gridDim.x;
         const uint32_t ne = count - e0 < RQ2_TILE ? (uint32_t)(count - e0) : RQ2_TILE;
         const uint32_t rows = 2 * ne;
-        tile_fetch<C>(a + e0 * 2 * N, rows, r, lane);
+        if constexpr (!PF) {
+            tile_fetch<C>(a + e0 * 2 * N, rows, r, lane);
 #pragma unroll
-        for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
-        wave_sync();
+            for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
+        }
+        wave_sync(); // a's tile is staged
         // every lane computes (rows past 2 ne hold stale values and are not stored),
         // so the partner exchange always sees both lanes of a pair
         double x[N];
@@ -205,7 +233,10 @@ __global__ __launch_bounds__(64) void rq2_f64_kernel(const uint32_t *a, const ui
             const u32x4 v = *reinterpret_cast<const u32x4 *>(T + 4 * tile_slot<C>(lane, c4));
             x[4 * c4] = v.x, x[4 * c4 + 1] = v.y, x[4 * c4 + 2] = v.z, x[4 * c4 + 3] = v.w;
         }
-        if (MODE == RQ2_MUL) tile_fetch<C>(b + e0 * 2 * N, rows, r, lane); // in flight during a's transform
+        if (MODE == RQ2_MUL)
+            tile_fetch<C>(b + e0 * 2 * N, rows, r, lane); // in flight during a's transform
+        else
+            fetch_next_a(t, tn, r);
         ntt_regs_f64<LOG_N, NTT_FWD>(x, tb); // x[j]: evaluation brv(j), the order the inverse reads
         bool good = true;
         if (MODE == RQ2_MUL) {
@@ -231,6 +262,7 @@ __global__ __launch_bounds__(64) void rq2_f64_kernel(const uint32_t *a, const ui
                 const double rr = f64_mul(odd ? xs : pa, pb, q, qinv);
                 x[j] = odd ? p + rr : p - rr;
             }
+
         } else if (MODE == RQ2_NORM) {
 #pragma unroll
             for (uint32_t j = 0; j < N; j++) {
@@ -278,12 +310,20 @@ __global__ __launch_bounds__(64) void rq2_f64_kernel(const uint32_t *a, const ui
         wave_sync();
         const uint32_t orows = MODE == RQ2_NORM ? ne : rows;
         uint32_t *go = out + e0 * (MODE == RQ2_NORM ? N : 2 * N);
+        u32x4 v[C];
+#pragma unroll
+        for (uint32_t i = 0; i < C; i++) v[i] = *reinterpret_cast<const u32x4 *>(T + 4 * (64 * i + lane));
+        if constexpr (PF) {
+            wave_sync(); // every lane has read its results: the next tile's a goes into T
+#pragma unroll
+            for (uint32_t i = 0; i < C; i++) *reinterpret_cast<u32x4 *>(T + 4 * (64 * i + lane)) = r[i];
+        }
 #pragma unroll
         for (uint32_t i = 0; i < C; i++) {
             const uint32_t s = 64 * i + lane, p = s / C, c = ((s % C) - p) & (C - 1);
-            if (p < orows) *reinterpret_cast<u32x4 *>(go + (p * C + c) * 4) = *reinterpret_cast<const u32x4 *>(T + 4 * s);
+            if (p < orows) *reinterpret_cast<u32x4 *>(go + (p * C + c) * 4) = v[i];
         }
-        wave_sync(); // the stores have read the tile before the next one is staged
+        if constexpr (!PF) wave_sync(); // the results are read before the next tile is staged
     }
 }
 
@@ -376,20 +416,41 @@ static void launch_fused(int mode, const uint32_t *a, const uint32_t *b, uint32_
                            plan->dev_tab, bar, count);
 }
 
+// resident blocks of one rq2_f64_kernel instance on a CU (queried once)
+template <int LOG_N, int MODE>
+static uint32_t rq2_f64_per_cu()
+{
+    static const uint32_t per_cu = [] {
+        int v = 0;
+        const hipError_t e =
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, reinterpret_cast<const void *>(&rq2_f64_kernel<LOG_N, MODE>), 64, 0);
+        return (uint32_t)(e == hipSuccess && v > 0 ? v : 4);
+    }();
+    return per_cu;
+}
+
+template <int LOG_N, int MODE>
+static void launch_f64_mode(const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok, const lzx_ntt_plan *plan,
+                            size_t count, hipStream_t s)
+{
+    // persistent: every resident block takes every gridDim.x-th tile
+    const size_t tiles = (count + RQ2_TILE - 1) / RQ2_TILE;
+    const size_t resident = (size_t)rq2_f64_per_cu<LOG_N, MODE>() * (size_t)std::max(1, plan->cus);
+    const uint32_t blocks = (uint32_t)std::max<size_t>(1, std::min<size_t>(tiles, resident));
+    hipLaunchKernelGGL((rq2_f64_kernel<LOG_N, MODE>), dim3(blocks), dim3(64), 0, s, a, b, out, ok, plan->dev_tab,
+                       count);
+}
+
 template <int LOG_N>
 static void launch_fused_f64(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
                              const lzx_ntt_plan *plan, size_t count, hipStream_t s)
 {
-    const uint32_t blocks = (uint32_t)std::min<size_t>((count + RQ2_TILE - 1) / RQ2_TILE, RQ2_MAX_BLOCKS);
     if (mode == RQ2_MUL)
-        hipLaunchKernelGGL((rq2_f64_kernel<LOG_N, RQ2_MUL>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
-                           plan->dev_tab, count);
+        launch_f64_mode<LOG_N, RQ2_MUL>(a, b, out, ok, plan, count, s);
     else if (mode == RQ2_NORM)
-        hipLaunchKernelGGL((rq2_f64_kernel<LOG_N, RQ2_NORM>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
-                           plan->dev_tab, count);
+        launch_f64_mode<LOG_N, RQ2_NORM>(a, b, out, ok, plan, count, s);
     else
-        hipLaunchKernelGGL((rq2_f64_kernel<LOG_N, RQ2_INV>), dim3(blocks), dim3(64), 0, s, a, b, out, ok,
-                           plan->dev_tab, count);
+        launch_f64_mode<LOG_N, RQ2_INV>(a, b, out, ok, plan, count, s);
 }
 
 static void dispatch_fused_f64(int mode, const uint32_t *a, const uint32_t *b, uint32_t *out, uint8_t *ok,
