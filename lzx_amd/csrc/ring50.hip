@@ -264,12 +264,23 @@ __global__ __launch_bounds__(QS_BLOCK) void quad_kernel(const uint64_t *a, const
         if (MODE == QS_INV || MODE == QS_MUL) quad_inverse<LOG_N>(x0, x1, itw, T, lane);
         const uint64_t r0 = d50_to_u64(d50_canonical(x0, T.q, T.qinv));
         const uint64_t r1 = d50_to_u64(d50_canonical(x1, T.q, T.qinv));
+        // stores through a resource over the tile's in-range words, at system
+        // scope (sc0 sc1: written through L2, so the launch ends with no dirty
+        // lines to write back); lanes past len fall outside it and are dropped
+        const uint64_t ob = (uint64_t)(out + base);
+        const uint64_t obu = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)ob) |
+                             (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(ob >> 32)) << 32;
+        const uint32_t nw = (uint32_t)__builtin_amdgcn_readfirstlane(
+            (uint32_t)(len - base < (size_t)QS_TILE ? len - base : (size_t)QS_TILE));
+        const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc((void *)obu, 0, (int)(8 * nw), 0x00020000);
         if (MODE == QS_INV || MODE == QS_MUL) {
-            if (base + e0 < len) out[base + e0] = r0, out[base + e1] = r1;
-        } else if (base + 2 * lane < len) {
-            u64x2 v;
-            v[0] = r0, v[1] = r1;
-            *(u64x2 *)(out + base + 2 * lane) = v;
+            typedef unsigned qs_v2u __attribute__((ext_vector_type(2)));
+            __builtin_amdgcn_raw_buffer_store_b64(qs_v2u{(unsigned)r0, (unsigned)(r0 >> 32)}, ro, 8 * e0, 0, 0x11);
+            __builtin_amdgcn_raw_buffer_store_b64(qs_v2u{(unsigned)r1, (unsigned)(r1 >> 32)}, ro, 8 * e1, 0, 0x11);
+        } else {
+            typedef unsigned qs_v4u __attribute__((ext_vector_type(4)));
+            const qs_v4u v = {(unsigned)r0, (unsigned)(r0 >> 32), (unsigned)r1, (unsigned)(r1 >> 32)};
+            __builtin_amdgcn_raw_buffer_store_b128(v, ro, 16 * lane, 0, 0x11);
         }
     }
 }
