@@ -262,7 +262,6 @@ __global__ __launch_bounds__(64) void rq2_f64_kernel(const uint32_t *a, const ui
                 const double rr = f64_mul(odd ? xs : pa, pb, q, qinv);
                 x[j] = odd ? p + rr : p - rr;
             }
-
         } else if (MODE == RQ2_NORM) {
 #pragma unroll
             for (uint32_t j = 0; j < N; j++) {
